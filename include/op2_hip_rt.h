@@ -1,0 +1,156 @@
+/*
+ * op2_hip_rt.h — the runtime ABI that generated loop stubs call.
+ *
+ * It replaces what the reference's HIP stubs take from its CUDA runtime
+ * (hip/loop_host.hpp.jinja:188-307: op_plan_get_stage, reallocConstArrays,
+ * reallocReductArrays, mv{Const,Reduct}ArraysTo{Device,Host}, OP_reduct_h/d,
+ * op_mpi_halo_exchanges_grouped, op_mpi_set_dirtybit_cuda, hipDeviceSynchronize):
+ *   - loop lifecycle: argument checks, upload of host-dirty dats
+ *     (op_cuda_rt_support.cpp:398-452 semantics), dirty marking
+ *     (:454-462), per-loop timing and the synchronisation policy;
+ *   - device data layout (AoS or SoA with a wave64-padded stride);
+ *   - execution plans: the block partition and two-level colouring of
+ *     op_plan_core(OP_STAGE_ALL) (op_rt_support.cpp:301-1126), the flat
+ *     colouring of OP_COLOR2, and the owner-compute inverse map;
+ *   - gbl reduction slots and their delivery to host pointers, immediately or
+ *     at the end of a captured HIP graph.
+ * All pointers in the plan structs are device pointers unless named *_h.
+ */
+#ifndef OP2_HIP_RT_H
+#define OP2_HIP_RT_H
+
+#include "op2_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OP2H_MAX_ARGS 32
+#define OP2H_MAX_GBL 8
+
+typedef struct {
+  int kernel_idx;
+  const char *name;
+  op_set set;
+  int plan;        /* OP2H_PLAN_* resolved for this call */
+  int block_size;  /* threads per workgroup */
+  int part_size;   /* elements per plan block (staged plan) */
+  int capturing;   /* 1 while a graph is being recorded */
+  void *stream;    /* hipStream_t */
+  double wall0;
+  int nargs;
+  op_arg *args;
+  /* gbl reductions of this call */
+  int ngbl;
+  int gbl_arg[OP2H_MAX_GBL];
+  void *gbl_slot_d[OP2H_MAX_GBL];
+  void *gbl_slot_h[OP2H_MAX_GBL];
+} op2h_loop_ctx;
+
+/* Loop lifecycle.  op2h_loop_begin checks the arguments (OP_diags > 0),
+ * uploads dats whose host copy is newer, resolves plan and block size
+ * (op2h_set_loop_plan, else defaults `def_plan`/`def_block`), and starts the
+ * timers.  op2h_loop_end marks written dats as device-newer, delivers gbl
+ * results (now, or at graph sync when capturing), synchronises per the sync
+ * mode and records timing.  `is_indirect` selects the default plan. */
+void op2h_loop_begin(op2h_loop_ctx *ctx, int kernel_idx, const char *name,
+                     op_set set, int nargs, op_arg *args, int def_plan,
+                     int def_block);
+void op2h_loop_end(op2h_loop_ctx *ctx);
+
+/* Kernel timing hooks around each launch (no-ops unless enabled). */
+void op2h_kernel_timer_start(op2h_loop_ctx *ctx);
+void op2h_kernel_timer_stop(op2h_loop_ctx *ctx);
+
+/* Device layout of a dat: 0 AoS, else the SoA stride in elements. */
+int op2h_dat_soa_stride(op_dat dat);
+
+/* Device result slot for gbl arg `argi` of this call (dim*size bytes of the
+ * arg, plus the linked info arg for OP_MIN/OP_MAX when `info_argi` >= 0).
+ * The kernel writes the loop's own reduction over the set (identity-started)
+ * into the slot; op2h_loop_end combines it with the host value. */
+void *op2h_gbl_slot(op2h_loop_ctx *ctx, int argi, int info_argi);
+
+/* Device scratch for per-workgroup partials and the zero-initialised
+ * completion counter of the single-pass (last-workgroup-reduces) reduction.
+ * Kernels must leave the counter at zero when they finish. */
+void *op2h_reduct_scratch(size_t bytes);
+unsigned int *op2h_reduct_counter(void);
+
+/* ---- execution plans ---- */
+
+/* Block partition + two-level colouring (OP_STAGE_ALL semantics), with
+ * indirect staging lists deduplicated per (map) rather than per dat. */
+typedef struct {
+  int set_size, part_size, nblocks, ncolors;
+  int nmaps;                   /* distinct maps among indirect args */
+  int *ncolblk_h;              /* [ncolors] blocks per colour (host) */
+  int *colstart_h;             /* [ncolors+1] first index into blkmap (host) */
+  int *max_nthrcol_h;          /* [1] */
+  int max_ind_h[OP2H_MAX_ARGS];/* per map: max unique targets in a block */
+  int *blkmap;                 /* [nblocks] blocks ordered by colour */
+  int *offset, *nelems;        /* [nblocks] */
+  int *nthrcol;                /* [nblocks] */
+  unsigned char *thrcol;       /* [set_size] thread colour within its block */
+  int *ind_offs[OP2H_MAX_ARGS];  /* per map: [nblocks] start in ind_map */
+  int *ind_sizes[OP2H_MAX_ARGS]; /* per map: [nblocks] unique targets */
+  int *ind_map[OP2H_MAX_ARGS];   /* per map: concatenated unique targets */
+  unsigned short *loc_map[OP2H_MAX_ARGS]; /* per (map, map index) in arg
+                                   order of first use: [set_size] local ids */
+  int map_of_arg[OP2H_MAX_ARGS]; /* arg -> staged map slot, -1 if direct */
+  int loc_of_arg[OP2H_MAX_ARGS]; /* arg -> loc_map slot, -1 if direct */
+  int built_on_device;         /* 1 if the colouring was computed on the GPU */
+  double build_ms;
+} op2h_staged_plan;
+
+/* Flat colouring of all elements (OP_COLOR2): elements permuted by colour. */
+typedef struct {
+  int set_size, ncolors;
+  int *col_offsets_h;  /* [ncolors+1] */
+  int *col_reord;      /* [set_size] element ids grouped by colour */
+  int built_on_device;
+  double build_ms;
+} op2h_color_plan;
+
+/* Owner-compute inverse of an increment map: for each target element, the
+ * (element * dim + slot) entries that increment it, in ascending element
+ * order.  `targets` lists the target elements with at least one entry. */
+typedef struct {
+  int from_size, to_size, dim, ntargets;
+  int *offsets;   /* [ntargets+1] */
+  int *entries;   /* [from_size*dim] element*dim+slot */
+  int *targets;   /* [ntargets] */
+  double build_ms;
+} op2h_gather_plan;
+
+const op2h_staged_plan *op2h_plan_staged(op2h_loop_ctx *ctx);
+const op2h_color_plan *op2h_plan_color(op2h_loop_ctx *ctx);
+const op2h_gather_plan *op2h_plan_gather(op_map map);
+
+/* Copy a plan's colouring back to the host for checking: thread colours,
+ * block colours (per block), and block offsets/sizes.  Returns 0. */
+int op2h_staged_plan_download(const op2h_staged_plan *p, int *thrcol,
+                              int *blkcol, int *offset, int *nelems);
+int op2h_color_plan_download(const op2h_color_plan *p, int *color_of_elem);
+
+/* Build the plans of a loop without running it (for inspection and tests):
+ * descriptors as in op_par_loop, on loop name `name`. */
+const op2h_staged_plan *op2h_plan_staged_for(const char *name, op_set set,
+                                             int part_size, int nargs,
+                                             op_arg *args);
+const op2h_color_plan *op2h_plan_color_for(const char *name, op_set set,
+                                           int nargs, op_arg *args);
+
+/* Compulsory ("algorithmic") bytes of one call of a loop: every dat element
+ * and map entry the loop touches, counted once; RW and INC count a read and a
+ * write.  Indirectly accessed dats count the distinct targets of their map. */
+double op2h_algorithmic_bytes(op_set set, int nargs, const op_arg *args);
+
+/* Where plans are coloured: 1 on the GPU (default), 0 on the host. */
+void op2h_set_plan_builder(int on_device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,259 @@
+// The airfoil application on this library's OP2 API
+// (apps/c/airfoil/airfoil_plain/dp/airfoil.cpp:72-315), as a callable entry
+// point for tests and benchmarks.
+#include "op2amd_airfoil.h"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "op2_hip_rt.h"
+
+namespace {
+
+double now() {
+  return std::chrono::duration<double>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Airfoil {
+  op_set nodes, edges, bedges, cells;
+  op_map pedge, pecell, pbedge, pbecell, pcell;
+  op_dat p_bound, p_x, p_q, p_qold, p_adt, p_res;
+  double rms = 0, maxerr = 0, rms_k0 = 0;
+  int errloc = 0;
+
+  // airfoil.cpp:223-278
+  void step() {
+    op_par_loop_airfoil_1_save_soln("save_soln", cells,
+                                    op_arg_dat(p_q, -1, OP_ID, 4, "double", OP_READ),
+                                    op_arg_dat(p_qold, -1, OP_ID, 4, "double", OP_WRITE));
+    for (int k = 0; k < 2; k++) {
+      op_par_loop_airfoil_2_adt_calc("adt_calc", cells,
+                                     op_arg_dat(p_x, 0, pcell, 2, "double", OP_READ),
+                                     op_arg_dat(p_x, 1, pcell, 2, "double", OP_READ),
+                                     op_arg_dat(p_x, 2, pcell, 2, "double", OP_READ),
+                                     op_arg_dat(p_x, 3, pcell, 2, "double", OP_READ),
+                                     op_arg_dat(p_q, -1, OP_ID, 4, "double", OP_READ),
+                                     op_arg_dat(p_adt, -1, OP_ID, 1, "double", OP_WRITE));
+      op_par_loop_airfoil_3_res_calc("res_calc", edges,
+                                     op_arg_dat(p_x, 0, pedge, 2, "double", OP_READ),
+                                     op_arg_dat(p_x, 1, pedge, 2, "double", OP_READ),
+                                     op_arg_dat(p_q, 0, pecell, 4, "double", OP_READ),
+                                     op_arg_dat(p_q, 1, pecell, 4, "double", OP_READ),
+                                     op_arg_dat(p_adt, 0, pecell, 1, "double", OP_READ),
+                                     op_arg_dat(p_adt, 1, pecell, 1, "double", OP_READ),
+                                     op_arg_dat(p_res, 0, pecell, 4, "double", OP_INC),
+                                     op_arg_dat(p_res, 1, pecell, 4, "double", OP_INC));
+      op_par_loop_airfoil_4_bres_calc("bres_calc", bedges,
+                                      op_arg_dat(p_x, 0, pbedge, 2, "double", OP_READ),
+                                      op_arg_dat(p_x, 1, pbedge, 2, "double", OP_READ),
+                                      op_arg_dat(p_q, 0, pbecell, 4, "double", OP_READ),
+                                      op_arg_dat(p_adt, 0, pbecell, 1, "double", OP_READ),
+                                      op_arg_dat(p_res, 0, pbecell, 4, "double", OP_INC),
+                                      op_arg_dat(p_bound, -1, OP_ID, 1, "int", OP_READ));
+      rms = 0.0;
+      maxerr = 0.0;
+      errloc = 0;
+      op_par_loop_airfoil_5_update("update", cells,
+                                   op_arg_dat(p_qold, -1, OP_ID, 4, "double", OP_READ),
+                                   op_arg_dat(p_q, -1, OP_ID, 4, "double", OP_WRITE),
+                                   op_arg_dat(p_res, -1, OP_ID, 4, "double", OP_RW),
+                                   op_arg_dat(p_adt, -1, OP_ID, 1, "double", OP_READ),
+                                   op_arg_gbl(&rms, 1, "double", OP_INC),
+                                   op_arg_gbl(&maxerr, 1, "double", OP_MAX),
+                                   op_arg_idx(-1, OP_ID),
+                                   op_arg_info(&errloc, 1, "int", 5));
+      if (k == 0) rms_k0 = rms;
+    }
+  }
+};
+
+void record(const op2amd_airfoil_result *r, int it, double rms_raw, double rms_k0,
+            double maxerr, int errloc, int ncell) {
+  if (r->rms) r->rms[it] = std::sqrt(rms_raw / (double)ncell);
+  if (r->rms_k0) r->rms_k0[it] = rms_k0;
+  if (r->maxerr) r->maxerr[it] = maxerr;
+  if (r->errloc) r->errloc[it] = errloc;
+}
+
+}  // namespace
+
+extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opts *o,
+                                  op2amd_airfoil_result *r) {
+  if (!m || !o || !r || o->niter < 1 || o->warmup < 0 || o->warmup > o->niter)
+    return 1;
+  double t_setup0 = now();
+  std::string soa_tok = "OP_AUTO_SOA";
+  char prog[] = "airfoil";
+  char *argv[2] = {prog, &soa_tok[0]};
+  op_init(o->soa ? 2 : 1, argv, 0);
+  op2h_set_plan_builder(o->plan_on_device);
+  op2h_enable_kernel_timing(o->kernel_timing);
+
+  // constants (airfoil.cpp:149-164); <math.h> in C++ resolves atan(1.0f) to
+  // the float overload, reproduced with std::atan(float)
+  double gam = 1.4f, gm1 = gam - 1.0f, cfl = 0.9f, eps = 0.05f;
+  double mach = 0.4f;
+  double alpha = 3.0f * std::atan(1.0f) / 45.0f;
+  double p = 1.0f, rr = 1.0f;
+  double u = sqrt(gam * p / rr) * mach;
+  double e = p / (rr * gm1) + 0.5f * u * u;
+  double qinf[4] = {rr, rr * u, 0.0f, rr * e};
+  if (o->alpha_mode == 1) {
+    qinf[1] = rr * u * cos(alpha);
+    qinf[2] = rr * u * sin(alpha);
+  }
+  std::vector<double> q(4 * (size_t)m->ncell), res(4 * (size_t)m->ncell, 0.0);
+  for (int n = 0; n < m->ncell; n++)
+    for (int k = 0; k < 4; k++) q[4 * (size_t)n + k] = qinf[k];
+
+  Airfoil A;
+  A.nodes = op_decl_set(m->nnode, "nodes");
+  A.edges = op_decl_set(m->nedge, "edges");
+  A.bedges = op_decl_set(m->nbedge, "bedges");
+  A.cells = op_decl_set(m->ncell, "cells");
+  A.pedge = op_decl_map(A.edges, A.nodes, 2, m->edge, "pedge");
+  A.pecell = op_decl_map(A.edges, A.cells, 2, m->ecell, "pecell");
+  A.pbedge = op_decl_map(A.bedges, A.nodes, 2, m->bedge, "pbedge");
+  A.pbecell = op_decl_map(A.bedges, A.cells, 1, m->becell, "pbecell");
+  A.pcell = op_decl_map(A.cells, A.nodes, 4, m->cell, "pcell");
+  A.p_bound = op_decl_dat(A.bedges, 1, "int", m->bound, "p_bound");
+  A.p_x = op_decl_dat(A.nodes, 2, "double", m->x, "p_x");
+  A.p_q = op_decl_dat(A.cells, 4, "double", q.data(), "p_q");
+  A.p_qold = op_decl_dat(A.cells, 4, "double", q.data(), "p_qold");
+  A.p_adt = op_decl_dat_temp(A.cells, 1, "double", (double *)nullptr, "p_adt");
+  A.p_res = op_decl_dat(A.cells, 4, "double", res.data(), "p_res");
+  op_decl_const2("gam", 1, "double", &gam);
+  op_decl_const2("gm1", 1, "double", &gm1);
+  op_decl_const2("cfl", 1, "double", &cfl);
+  op_decl_const2("eps", 1, "double", &eps);
+  op_decl_const2("mach", 1, "double", &mach);
+  op_decl_const2("alpha", 1, "double", &alpha);
+  op_decl_const2("qinf", 4, "double", qinf);
+
+  // compulsory bytes per loop call (same argument lists as step())
+  {
+    op_arg a1[2] = {op_arg_dat(A.p_q, -1, OP_ID, 4, "double", OP_READ),
+                    op_arg_dat(A.p_qold, -1, OP_ID, 4, "double", OP_WRITE)};
+    op_arg a2[6] = {op_arg_dat(A.p_x, 0, A.pcell, 2, "double", OP_READ),
+                    op_arg_dat(A.p_x, 1, A.pcell, 2, "double", OP_READ),
+                    op_arg_dat(A.p_x, 2, A.pcell, 2, "double", OP_READ),
+                    op_arg_dat(A.p_x, 3, A.pcell, 2, "double", OP_READ),
+                    op_arg_dat(A.p_q, -1, OP_ID, 4, "double", OP_READ),
+                    op_arg_dat(A.p_adt, -1, OP_ID, 1, "double", OP_WRITE)};
+    op_arg a3[8] = {op_arg_dat(A.p_x, 0, A.pedge, 2, "double", OP_READ),
+                    op_arg_dat(A.p_x, 1, A.pedge, 2, "double", OP_READ),
+                    op_arg_dat(A.p_q, 0, A.pecell, 4, "double", OP_READ),
+                    op_arg_dat(A.p_q, 1, A.pecell, 4, "double", OP_READ),
+                    op_arg_dat(A.p_adt, 0, A.pecell, 1, "double", OP_READ),
+                    op_arg_dat(A.p_adt, 1, A.pecell, 1, "double", OP_READ),
+                    op_arg_dat(A.p_res, 0, A.pecell, 4, "double", OP_INC),
+                    op_arg_dat(A.p_res, 1, A.pecell, 4, "double", OP_INC)};
+    op_arg a4[6] = {op_arg_dat(A.p_x, 0, A.pbedge, 2, "double", OP_READ),
+                    op_arg_dat(A.p_x, 1, A.pbedge, 2, "double", OP_READ),
+                    op_arg_dat(A.p_q, 0, A.pbecell, 4, "double", OP_READ),
+                    op_arg_dat(A.p_adt, 0, A.pbecell, 1, "double", OP_READ),
+                    op_arg_dat(A.p_res, 0, A.pbecell, 4, "double", OP_INC),
+                    op_arg_dat(A.p_bound, -1, OP_ID, 1, "int", OP_READ)};
+    op_arg a5[4] = {op_arg_dat(A.p_qold, -1, OP_ID, 4, "double", OP_READ),
+                    op_arg_dat(A.p_q, -1, OP_ID, 4, "double", OP_WRITE),
+                    op_arg_dat(A.p_res, -1, OP_ID, 4, "double", OP_RW),
+                    op_arg_dat(A.p_adt, -1, OP_ID, 1, "double", OP_READ)};
+    r->loop_bytes[0] = op2h_algorithmic_bytes(A.cells, 2, a1);
+    r->loop_bytes[1] = op2h_algorithmic_bytes(A.cells, 6, a2);
+    r->loop_bytes[2] = op2h_algorithmic_bytes(A.edges, 8, a3);
+    r->loop_bytes[3] = op2h_algorithmic_bytes(A.bedges, 6, a4);
+    r->loop_bytes[4] = op2h_algorithmic_bytes(A.cells, 4, a5);
+  }
+
+  if (o->mode == OP2AMD_MODE_PLAIN) {
+    op2h_set_sync_mode(OP2H_SYNC_EVERY_LOOP);
+    op2h_set_loop_plan(nullptr, OP2H_PLAN_AUTO, 0);
+  } else {
+    op2h_set_sync_mode(OP2H_SYNC_GBL_ONLY);
+    op2h_set_loop_plan("res_calc", o->plan_res, o->block_res);
+    op2h_set_loop_plan("bres_calc", o->plan_bres, o->block_bres);
+    op2h_set_loop_plan("save_soln", OP2H_PLAN_AUTO, o->block_direct);
+    op2h_set_loop_plan("update", OP2H_PLAN_AUTO, o->block_direct);
+    op2h_set_loop_plan("adt_calc", OP2H_PLAN_AUTO, o->block_adt);
+  }
+
+  double t0 = 0.0;
+  const int ncell = m->ncell;
+  if (o->mode != OP2AMD_MODE_GRAPHED) {
+    for (int it = 1; it <= o->niter; ++it) {
+      if (it == o->warmup + 1) {
+        op2h_device_sync();
+        if (it == 1) r->setup_seconds = now() - t_setup0;
+        op2h_reset_stats();
+        t0 = now();
+      }
+      A.step();
+      record(r, it - 1, A.rms, A.rms_k0, A.maxerr, A.errloc, ncell);
+      if (it == o->warmup) r->setup_seconds = now() - t_setup0;
+    }
+    op2h_device_sync();
+  } else {
+    // iteration 1 eagerly: builds every plan and sizes every buffer
+    if (o->warmup == 0) {
+      op2h_device_sync();
+      op2h_reset_stats();
+      t0 = now();
+    }
+    A.step();
+    record(r, 0, A.rms, A.rms_k0, A.maxerr, A.errloc, ncell);
+    if (op2h_graph_begin() != 0) return 2;
+    A.step();
+    if (op2h_graph_end() != 0) return 3;
+    for (int it = 2; it <= o->niter; ++it) {
+      if (it == o->warmup + 1 && o->warmup > 0) {
+        op2h_device_sync();
+        r->setup_seconds = now() - t_setup0;
+        op2h_reset_stats();
+        t0 = now();
+      }
+      op2h_graph_launch(1);
+    }
+    if (o->warmup == 1 && o->niter == 1) t0 = now();
+    op2h_device_sync();
+    double t1 = now();
+    // per-iteration gbl results from the device history:
+    // [rms_k0][max_k0][loc_k0][rms_k1][max_k1][loc_k1] per launch
+    int nrec = 0, nl = o->niter - 1;
+    std::vector<double> hist(std::max(nl, 1) * 8);
+    int words = op2h_graph_history(nl, hist.data(), &nrec);
+    if (nl > 0 && (words != 6 || nrec != nl)) {
+      printf("op2amd_airfoil_run: graph history has %d records of %d words\n",
+             nrec, words);
+      op_exit();
+      return 4;
+    }
+    for (int i = 0; i < nrec; ++i) {
+      const double *h = &hist[(size_t)i * words];
+      double mx = h[4] > 0.0 ? h[4] : 0.0;
+      int loc = 0;
+      if (h[4] > 0.0) std::memcpy(&loc, &h[5], sizeof(int));
+      record(r, i + 1, 0.0 + h[3], 0.0 + h[0], mx, loc, ncell);
+    }
+    op2h_graph_sync();
+    r->seconds = t1 - t0;
+  }
+  if (o->mode != OP2AMD_MODE_GRAPHED) r->seconds = now() - t0;
+  const char *names[5] = {"save_soln", "adt_calc", "res_calc", "bres_calc", "update"};
+  for (int l = 0; l < 5; ++l) {
+    r->loop_calls[l] = 0;
+    r->loop_kernel_ms[l] = 0.0;
+    op2h_loop_stats(names[l], &r->loop_calls[l], &r->loop_kernel_ms[l]);
+  }
+  if (r->q_out) op_fetch_data(A.p_q, r->q_out);
+  op2h_graph_destroy();
+  op_exit();
+  return 0;
+}

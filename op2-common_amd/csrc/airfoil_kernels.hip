@@ -1,0 +1,989 @@
+// airfoil_kernels.hip — the HIP-only target's output for the airfoil
+// application (apps/c/airfoil/airfoil_plain/dp/airfoil.cpp:225-277): one
+// op_par_loop_airfoil_<i>_<kernel> stub per loop, its gfx950 kernels and the
+// application's device constants.
+//
+// User kernels (save_soln.h, adt_calc.h, res_calc.h, bres_calc.h, update.h) are
+// device functions with the reference's expression trees; the whole library is
+// compiled with -ffp-contract=off, so each multiply and add rounds as in the
+// reference's host build.  Loops with indirect increments (res_calc,
+// bres_calc) have four plans:
+//   ATOMIC  one thread per edge, FP64 atomics on res;
+//   COLOR   flat colouring, one launch per colour, plain read-modify-write;
+//   STAGED  plan blocks staged in LDS, increments applied colour by colour
+//           inside the block, blocks launched colour by colour;
+//   GATHER  one thread per cell, the edges of the cell visited in edge order
+//           and res accumulated in registers — the sequential executor's
+//           summation order, so results match it bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "op2_hip_rt.h"
+
+#define KCHECK(x)                                                            \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess) {                                                  \
+      printf("op2hip: %s failed at %s:%d: %s\n", #x, __FILE__, __LINE__,     \
+             hipGetErrorString(e_));                                         \
+      exit(-1);                                                              \
+    }                                                                        \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// application constants (op_decl_const, airfoil.cpp:204-210)
+// ---------------------------------------------------------------------------
+__constant__ double gam_d;
+__constant__ double gm1_d;
+__constant__ double cfl_d;
+__constant__ double eps_d;
+__constant__ double mach_d;
+__constant__ double alpha_d;
+__constant__ double qinf_d[4];
+
+extern "C" void op_decl_const_char(int dim, char const *type, int size,
+                                   char *dat, char const *name) {
+  (void)type;
+  op2h_device_sync();
+  size_t bytes = (size_t)dim * size;
+  if (!strcmp(name, "gam")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(gam_d), dat, bytes));
+  else if (!strcmp(name, "gm1")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(gm1_d), dat, bytes));
+  else if (!strcmp(name, "cfl")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cfl_d), dat, bytes));
+  else if (!strcmp(name, "eps")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(eps_d), dat, bytes));
+  else if (!strcmp(name, "mach")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(mach_d), dat, bytes));
+  else if (!strcmp(name, "alpha")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(alpha_d), dat, bytes));
+  else if (!strcmp(name, "qinf")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(qinf_d), dat, bytes));
+  else {
+    printf("op_decl_const error -- unknown constant %s\n", name);
+    exit(-1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// user kernels (apps/c/airfoil/airfoil_plain/dp/*.h)
+// ---------------------------------------------------------------------------
+namespace op2_k {
+
+__device__ __forceinline__ void adt_calc(const double *x1, const double *x2,
+                                         const double *x3, const double *x4,
+                                         const double *q, double *adt) {
+  double dx, dy, ri, u, v, c;
+  ri = 1.0 / q[0];
+  u = ri * q[1];
+  v = ri * q[2];
+  c = sqrt(gam_d * gm1_d * (ri * q[3] - 0.5 * (u * u + v * v)));
+  dx = x2[0] - x1[0];
+  dy = x2[1] - x1[1];
+  *adt = fabs(u * dy - v * dx) + c * sqrt(dx * dx + dy * dy);
+  dx = x3[0] - x2[0];
+  dy = x3[1] - x2[1];
+  *adt += fabs(u * dy - v * dx) + c * sqrt(dx * dx + dy * dy);
+  dx = x4[0] - x3[0];
+  dy = x4[1] - x3[1];
+  *adt += fabs(u * dy - v * dx) + c * sqrt(dx * dx + dy * dy);
+  dx = x1[0] - x4[0];
+  dy = x1[1] - x4[1];
+  *adt += fabs(u * dy - v * dx) + c * sqrt(dx * dx + dy * dy);
+  *adt = (*adt) / cfl_d;
+}
+
+__device__ __forceinline__ void res_calc(const double *x1, const double *x2,
+                                         const double *q1, const double *q2,
+                                         const double *adt1, const double *adt2,
+                                         double *res1, double *res2) {
+  double dx, dy, mu, ri, p1, vol1, p2, vol2, f;
+  dx = x1[0] - x2[0];
+  dy = x1[1] - x2[1];
+  ri = 1.0 / q1[0];
+  p1 = gm1_d * (q1[3] - 0.5 * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
+  vol1 = ri * (q1[1] * dy - q1[2] * dx);
+  ri = 1.0 / q2[0];
+  p2 = gm1_d * (q2[3] - 0.5 * ri * (q2[1] * q2[1] + q2[2] * q2[2]));
+  vol2 = ri * (q2[1] * dy - q2[2] * dx);
+  mu = 0.5 * ((*adt1) + (*adt2)) * eps_d;
+  f = 0.5 * (vol1 * q1[0] + vol2 * q2[0]) + mu * (q1[0] - q2[0]);
+  res1[0] += f;
+  res2[0] -= f;
+  f = 0.5 * (vol1 * q1[1] + p1 * dy + vol2 * q2[1] + p2 * dy) +
+      mu * (q1[1] - q2[1]);
+  res1[1] += f;
+  res2[1] -= f;
+  f = 0.5 * (vol1 * q1[2] - p1 * dx + vol2 * q2[2] - p2 * dx) +
+      mu * (q1[2] - q2[2]);
+  res1[2] += f;
+  res2[2] -= f;
+  f = 0.5 * (vol1 * (q1[3] + p1) + vol2 * (q2[3] + p2)) + mu * (q1[3] - q2[3]);
+  res1[3] += f;
+  res2[3] -= f;
+}
+
+__device__ __forceinline__ void bres_calc(const double *x1, const double *x2,
+                                          const double *q1, const double *adt1,
+                                          double *res1, const int *bound) {
+  double dx, dy, mu, ri, p1, vol1, p2, vol2, f;
+  dx = x1[0] - x2[0];
+  dy = x1[1] - x2[1];
+  ri = 1.0 / q1[0];
+  p1 = gm1_d * (q1[3] - 0.5 * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
+  if (*bound == 1) {
+    res1[1] += +p1 * dy;
+    res1[2] += -p1 * dx;
+  } else {
+    vol1 = ri * (q1[1] * dy - q1[2] * dx);
+    ri = 1.0 / qinf_d[0];
+    p2 = gm1_d * (qinf_d[3] - 0.5 * ri * (qinf_d[1] * qinf_d[1] + qinf_d[2] * qinf_d[2]));
+    vol2 = ri * (qinf_d[1] * dy - qinf_d[2] * dx);
+    mu = (*adt1) * eps_d;
+    f = 0.5 * (vol1 * q1[0] + vol2 * qinf_d[0]) + mu * (q1[0] - qinf_d[0]);
+    res1[0] += f;
+    f = 0.5 * (vol1 * q1[1] + p1 * dy + vol2 * qinf_d[1] + p2 * dy) +
+        mu * (q1[1] - qinf_d[1]);
+    res1[1] += f;
+    f = 0.5 * (vol1 * q1[2] - p1 * dx + vol2 * qinf_d[2] - p2 * dx) +
+        mu * (q1[2] - qinf_d[2]);
+    res1[2] += f;
+    f = 0.5 * (vol1 * (q1[3] + p1) + vol2 * (qinf_d[3] + p2)) +
+        mu * (q1[3] - qinf_d[3]);
+    res1[3] += f;
+  }
+}
+
+__device__ __forceinline__ void update(const double *qold, double *q,
+                                       double *res, const double *adt,
+                                       double *rms, double *maxerr,
+                                       const int *idx, int *errloc) {
+  double del, adti;
+  adti = 1.0 / (*adt);
+  for (int n = 0; n < 4; n++) {
+    del = adti * res[n];
+    q[n] = qold[n] - del;
+    res[n] = 0.0;
+    double sqdel = del * del;
+    *rms += sqdel;
+    if (sqdel > *maxerr) {
+      *maxerr = sqdel;
+      *errloc = *idx;
+    }
+  }
+}
+
+}  // namespace op2_k
+
+// ---------------------------------------------------------------------------
+// dat access: AoS (element-contiguous, 16-byte vector loads) or SoA
+// (component rows `stride` elements apart)
+// ---------------------------------------------------------------------------
+namespace {
+
+template <bool SOA>
+__device__ __forceinline__ void ld2(const double *__restrict p, int n, int s,
+                                    double *v) {
+  if (SOA) {
+    v[0] = p[n];
+    v[1] = p[(size_t)s + n];
+  } else {
+    double2 a = *reinterpret_cast<const double2 *>(p + 2 * (size_t)n);
+    v[0] = a.x;
+    v[1] = a.y;
+  }
+}
+
+template <bool SOA>
+__device__ __forceinline__ void ld4(const double *__restrict p, int n, int s,
+                                    double *v) {
+  if (SOA) {
+    v[0] = p[n];
+    v[1] = p[(size_t)s + n];
+    v[2] = p[2 * (size_t)s + n];
+    v[3] = p[3 * (size_t)s + n];
+  } else {
+    double2 a = *reinterpret_cast<const double2 *>(p + 4 * (size_t)n);
+    double2 b = *reinterpret_cast<const double2 *>(p + 4 * (size_t)n + 2);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+  }
+}
+
+template <bool SOA>
+__device__ __forceinline__ void st4(double *__restrict p, int n, int s,
+                                    const double *v) {
+  if (SOA) {
+    p[n] = v[0];
+    p[(size_t)s + n] = v[1];
+    p[2 * (size_t)s + n] = v[2];
+    p[3 * (size_t)s + n] = v[3];
+  } else {
+    *reinterpret_cast<double2 *>(p + 4 * (size_t)n) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2 *>(p + 4 * (size_t)n + 2) = make_double2(v[2], v[3]);
+  }
+}
+
+template <bool SOA>
+__device__ __forceinline__ void add4(double *__restrict p, int n, int s,
+                                     const double *v) {
+  double o[4];
+  ld4<SOA>(p, n, s, o);
+  o[0] += v[0];
+  o[1] += v[1];
+  o[2] += v[2];
+  o[3] += v[3];
+  st4<SOA>(p, n, s, o);
+}
+
+template <bool SOA>
+__device__ __forceinline__ void atomic_add4(double *p, int n, int s,
+                                            const double *v) {
+  size_t st = SOA ? (size_t)s : 1, base = SOA ? (size_t)n : 4 * (size_t)n;
+  unsafeAtomicAdd(p + base, v[0]);
+  unsafeAtomicAdd(p + base + st, v[1]);
+  unsafeAtomicAdd(p + base + 2 * st, v[2]);
+  unsafeAtomicAdd(p + base + 3 * st, v[3]);
+}
+
+int grid_for(int n, int bs) { return n > 0 ? (n + bs - 1) / bs : 0; }
+
+// ---------------------------------------------------------------------------
+// save_soln: qold = q over the whole dat, 16-byte vectors (direct copy)
+// ---------------------------------------------------------------------------
+__global__ void k_save_soln(const double2 *__restrict q, double2 *__restrict qold,
+                            int n2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n2) qold[i] = q[i];
+}
+
+// ---------------------------------------------------------------------------
+// adt_calc: one thread per cell, nodes gathered through pcell
+// ---------------------------------------------------------------------------
+template <bool SOA>
+__global__ void k_adt_calc(const double *__restrict x, const double *__restrict q,
+                           double *__restrict adt, const int *__restrict pcell,
+                           int n, int sx, int sq) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  int4 cn = reinterpret_cast<const int4 *>(pcell)[c];
+  double x1[2], x2[2], x3[2], x4[2], qc[4], a;
+  ld2<SOA>(x, cn.x, sx, x1);
+  ld2<SOA>(x, cn.y, sx, x2);
+  ld2<SOA>(x, cn.z, sx, x3);
+  ld2<SOA>(x, cn.w, sx, x4);
+  ld4<SOA>(q, c, sq, qc);
+  op2_k::adt_calc(x1, x2, x3, x4, qc, &a);
+  adt[c] = a;
+}
+
+// ---------------------------------------------------------------------------
+// res_calc
+// ---------------------------------------------------------------------------
+template <bool SOA>
+__device__ __forceinline__ void res_edge(const double *__restrict x,
+                                         const double *__restrict q,
+                                         const double *__restrict adt,
+                                         int2 en, int2 ec, int sx, int sq,
+                                         double *r1, double *r2) {
+  double x1[2], x2[2], q1[4], q2[4];
+  ld2<SOA>(x, en.x, sx, x1);
+  ld2<SOA>(x, en.y, sx, x2);
+  ld4<SOA>(q, ec.x, sq, q1);
+  ld4<SOA>(q, ec.y, sq, q2);
+  double a1 = adt[ec.x], a2 = adt[ec.y];
+  for (int d = 0; d < 4; ++d) {
+    r1[d] = 0.0;
+    r2[d] = 0.0;
+  }
+  op2_k::res_calc(x1, x2, q1, q2, &a1, &a2, r1, r2);
+}
+
+template <bool SOA>
+__global__ void k_res_atomic(const double *__restrict x, const double *__restrict q,
+                             const double *__restrict adt, double *res,
+                             const int2 *__restrict pedge,
+                             const int2 *__restrict pecell, int n, int sx, int sq) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int2 en = pedge[e], ec = pecell[e];
+  double r1[4], r2[4];
+  res_edge<SOA>(x, q, adt, en, ec, sx, sq, r1, r2);
+  atomic_add4<SOA>(res, ec.x, sq, r1);
+  atomic_add4<SOA>(res, ec.y, sq, r2);
+}
+
+template <bool SOA>
+__global__ void k_res_color(const double *__restrict x, const double *__restrict q,
+                            const double *__restrict adt, double *res,
+                            const int2 *__restrict pedge,
+                            const int2 *__restrict pecell,
+                            const int *__restrict col_reord, int start, int end,
+                            int sx, int sq) {
+  int i = start + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  int e = col_reord[i];
+  int2 en = pedge[e], ec = pecell[e];
+  double r1[4], r2[4];
+  res_edge<SOA>(x, q, adt, en, ec, sx, sq, r1, r2);
+  add4<SOA>(res, ec.x, sq, r1);
+  add4<SOA>(res, ec.y, sq, r2);
+}
+
+// Owner-compute: one thread per cell; entries are edge*2+slot in edge order.
+template <bool SOA>
+__global__ void k_res_gather(const double *__restrict x, const double *__restrict q,
+                             const double *__restrict adt, double *res,
+                             const int2 *__restrict pedge,
+                             const int2 *__restrict pecell,
+                             const int *__restrict offsets,
+                             const int *__restrict entries,
+                             const int *__restrict targets, int ntargets, int sx,
+                             int sq) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntargets) return;
+  int cell = targets[t];
+  double acc[4];
+  ld4<SOA>(res, cell, sq, acc);
+  int k0 = offsets[t], k1 = offsets[t + 1];
+  for (int k = k0; k < k1; ++k) {
+    int ent = entries[k];
+    int e = ent >> 1;
+    int2 en = pedge[e], ec = pecell[e];
+    double r1[4], r2[4];
+    res_edge<SOA>(x, q, adt, en, ec, sx, sq, r1, r2);
+    const double *r = (ent & 1) ? r2 : r1;
+    acc[0] += r[0];
+    acc[1] += r[1];
+    acc[2] += r[2];
+    acc[3] += r[3];
+  }
+  st4<SOA>(res, cell, sq, acc);
+}
+
+struct StagedArgs {
+  const int *blkmap, *offset, *nelems, *nthrcol;
+  const unsigned char *thrcol;
+  const int *ind_offs_n, *ind_sizes_n, *ind_map_n;  // node map slot
+  const int *ind_offs_c, *ind_sizes_c, *ind_map_c;  // cell map slot
+  const unsigned short *loc_n0, *loc_n1, *loc_c0, *loc_c1;
+  int max_n, max_c;
+};
+
+// Two-level coloured, LDS-staged res_calc: one workgroup per plan block.
+// LDS: x of the block's nodes [max_n][2], q [max_c][4], adt [max_c],
+// res increments [max_c][4].
+template <bool SOA>
+__global__ void __launch_bounds__(1024)
+    k_res_staged(const double *__restrict x, const double *__restrict q,
+                 const double *__restrict adt, double *__restrict res,
+                 StagedArgs P, int block_offset, int sx, int sq) {
+  extern __shared__ double lds[];
+  const int b = P.blkmap[blockIdx.x + block_offset];
+  const int off = P.offset[b], nel = P.nelems[b];
+  const int nn = P.ind_sizes_n[b], nbase = P.ind_offs_n[b];
+  const int nc = P.ind_sizes_c[b], cbase = P.ind_offs_c[b];
+  double *lx = lds;
+  double *lq = lx + 2 * P.max_n;
+  double *la = lq + 4 * P.max_c;
+  double *lr = la + P.max_c;
+  for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+    double v[2];
+    ld2<SOA>(x, P.ind_map_n[nbase + k], sx, v);
+    lx[2 * k] = v[0];
+    lx[2 * k + 1] = v[1];
+  }
+  for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+    int cell = P.ind_map_c[cbase + k];
+    double v[4];
+    ld4<SOA>(q, cell, sq, v);
+    lq[4 * k] = v[0];
+    lq[4 * k + 1] = v[1];
+    lq[4 * k + 2] = v[2];
+    lq[4 * k + 3] = v[3];
+    la[k] = adt[cell];
+    lr[4 * k] = 0.0;
+    lr[4 * k + 1] = 0.0;
+    lr[4 * k + 2] = 0.0;
+    lr[4 * k + 3] = 0.0;
+  }
+  __syncthreads();
+  double r1[4] = {0.0, 0.0, 0.0, 0.0}, r2[4] = {0.0, 0.0, 0.0, 0.0};
+  int col = -1, l0 = 0, l1 = 0;
+  if ((int)threadIdx.x < nel) {
+    const int e = off + threadIdx.x;
+    const int a0 = P.loc_n0[e], a1 = P.loc_n1[e];
+    l0 = P.loc_c0[e];
+    l1 = P.loc_c1[e];
+    op2_k::res_calc(lx + 2 * a0, lx + 2 * a1, lq + 4 * l0, lq + 4 * l1, la + l0,
+                    la + l1, r1, r2);
+    col = P.thrcol[e];
+  }
+  const int ncol = P.nthrcol[b];
+  for (int c = 0; c < ncol; ++c) {
+    if (col == c) {
+      for (int d = 0; d < 4; ++d) {
+        lr[4 * l0 + d] += r1[d];
+        lr[4 * l1 + d] += r2[d];
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < nc; k += blockDim.x)
+    add4<SOA>(res, P.ind_map_c[cbase + k], sq, lr + 4 * k);
+}
+
+// ---------------------------------------------------------------------------
+// bres_calc
+// ---------------------------------------------------------------------------
+template <bool SOA>
+__device__ __forceinline__ void bres_edge(const double *__restrict x,
+                                          const double *__restrict q,
+                                          const double *__restrict adt,
+                                          int2 bn, int cell, int bound, int sx,
+                                          int sq, double *r) {
+  double x1[2], x2[2], q1[4];
+  ld2<SOA>(x, bn.x, sx, x1);
+  ld2<SOA>(x, bn.y, sx, x2);
+  ld4<SOA>(q, cell, sq, q1);
+  double a1 = adt[cell];
+  for (int d = 0; d < 4; ++d) r[d] = 0.0;
+  op2_k::bres_calc(x1, x2, q1, &a1, r, &bound);
+}
+
+template <bool SOA>
+__global__ void k_bres_atomic(const double *__restrict x, const double *__restrict q,
+                              const double *__restrict adt, double *res,
+                              const int2 *__restrict pbedge,
+                              const int *__restrict pbecell,
+                              const int *__restrict bound, int n, int sx, int sq) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int cell = pbecell[e];
+  double r[4];
+  bres_edge<SOA>(x, q, adt, pbedge[e], cell, bound[e], sx, sq, r);
+  atomic_add4<SOA>(res, cell, sq, r);
+}
+
+template <bool SOA>
+__global__ void k_bres_color(const double *__restrict x, const double *__restrict q,
+                             const double *__restrict adt, double *res,
+                             const int2 *__restrict pbedge,
+                             const int *__restrict pbecell,
+                             const int *__restrict bound,
+                             const int *__restrict col_reord, int start, int end,
+                             int sx, int sq) {
+  int i = start + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  int e = col_reord[i];
+  int cell = pbecell[e];
+  double r[4];
+  bres_edge<SOA>(x, q, adt, pbedge[e], cell, bound[e], sx, sq, r);
+  add4<SOA>(res, cell, sq, r);
+}
+
+template <bool SOA>
+__global__ void k_bres_gather(const double *__restrict x, const double *__restrict q,
+                              const double *__restrict adt, double *res,
+                              const int2 *__restrict pbedge,
+                              const int *__restrict bound,
+                              const int *__restrict offsets,
+                              const int *__restrict entries,
+                              const int *__restrict targets, int ntargets, int sx,
+                              int sq) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntargets) return;
+  int cell = targets[t];
+  double acc[4];
+  ld4<SOA>(res, cell, sq, acc);
+  for (int k = offsets[t]; k < offsets[t + 1]; ++k) {
+    int e = entries[k];  // map dim 1: entry = bedge
+    double r[4];
+    bres_edge<SOA>(x, q, adt, pbedge[e], cell, bound[e], sx, sq, r);
+    acc[0] += r[0];
+    acc[1] += r[1];
+    acc[2] += r[2];
+    acc[3] += r[3];
+  }
+  st4<SOA>(res, cell, sq, acc);
+}
+
+struct BStagedArgs {
+  const int *blkmap, *offset, *nelems, *nthrcol;
+  const unsigned char *thrcol;
+  const int *ind_offs_n, *ind_sizes_n, *ind_map_n;
+  const int *ind_offs_c, *ind_sizes_c, *ind_map_c;
+  const unsigned short *loc_n0, *loc_n1, *loc_c0;
+  int max_n, max_c;
+};
+
+template <bool SOA>
+__global__ void __launch_bounds__(1024)
+    k_bres_staged(const double *__restrict x, const double *__restrict q,
+                  const double *__restrict adt, double *__restrict res,
+                  const int *__restrict bound, BStagedArgs P, int block_offset,
+                  int sx, int sq) {
+  extern __shared__ double lds[];
+  const int b = P.blkmap[blockIdx.x + block_offset];
+  const int off = P.offset[b], nel = P.nelems[b];
+  const int nn = P.ind_sizes_n[b], nbase = P.ind_offs_n[b];
+  const int nc = P.ind_sizes_c[b], cbase = P.ind_offs_c[b];
+  double *lx = lds;
+  double *lq = lx + 2 * P.max_n;
+  double *la = lq + 4 * P.max_c;
+  double *lr = la + P.max_c;
+  for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+    double v[2];
+    ld2<SOA>(x, P.ind_map_n[nbase + k], sx, v);
+    lx[2 * k] = v[0];
+    lx[2 * k + 1] = v[1];
+  }
+  for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+    int cell = P.ind_map_c[cbase + k];
+    double v[4];
+    ld4<SOA>(q, cell, sq, v);
+    for (int d = 0; d < 4; ++d) {
+      lq[4 * k + d] = v[d];
+      lr[4 * k + d] = 0.0;
+    }
+    la[k] = adt[cell];
+  }
+  __syncthreads();
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  int col = -1, l0 = 0;
+  if ((int)threadIdx.x < nel) {
+    const int e = off + threadIdx.x;
+    const int a0 = P.loc_n0[e], a1 = P.loc_n1[e];
+    l0 = P.loc_c0[e];
+    int bd = bound[e];
+    op2_k::bres_calc(lx + 2 * a0, lx + 2 * a1, lq + 4 * l0, la + l0, r, &bd);
+    col = P.thrcol[e];
+  }
+  const int ncol = P.nthrcol[b];
+  for (int c = 0; c < ncol; ++c) {
+    if (col == c)
+      for (int d = 0; d < 4; ++d) lr[4 * l0 + d] += r[d];
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < nc; k += blockDim.x)
+    add4<SOA>(res, P.ind_map_c[cbase + k], sq, lr + 4 * k);
+}
+
+// ---------------------------------------------------------------------------
+// update, with the rms (INC) and maxerr (MAX, errloc argmax) reductions.
+// Each workgroup reduces its elements (waves by DPP-free shuffles, then LDS),
+// writes one partial; the last workgroup to finish combines the partials in
+// workgroup order and writes the slots, then clears the completion counter.
+// ---------------------------------------------------------------------------
+struct MaxLoc {
+  double v;
+  int loc;
+};
+
+__device__ __forceinline__ MaxLoc maxloc_combine(MaxLoc a, MaxLoc b) {
+  // larger value wins; equal values keep the lower element index, so the
+  // result is the first element in set order reaching the maximum
+  if (b.v > a.v || (b.v == a.v && b.loc < a.loc)) return b;
+  return a;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ MaxLoc wave_maxloc(MaxLoc m) {
+  for (int o = 32; o > 0; o >>= 1) {
+    MaxLoc t;
+    t.v = __shfl_xor(m.v, o, 64);
+    t.loc = __shfl_xor(m.loc, o, 64);
+    m = maxloc_combine(m, t);
+  }
+  return m;
+}
+
+template <bool SOA>
+__global__ void __launch_bounds__(1024)
+    k_update(const double *__restrict qold, double *__restrict q,
+             double *__restrict res, const double *__restrict adt, int n, int s,
+             double *__restrict part_sum, double *__restrict part_max,
+             int *__restrict part_loc, unsigned int *counter,
+             double *__restrict slot_rms, double *__restrict slot_max) {
+  __shared__ double s_sum[16];
+  __shared__ double s_max[16];
+  __shared__ int s_loc[16];
+  __shared__ bool s_last;
+  double rms = 0.0;
+  MaxLoc ml{-INFINITY, 0x7fffffff};
+  const int stride = gridDim.x * blockDim.x;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+    double qo[4], qn[4], r[4], a = adt[c], mx = ml.v;
+    int loc = ml.loc;
+    ld4<SOA>(qold, c, s, qo);
+    ld4<SOA>(res, c, s, r);
+    op2_k::update(qo, qn, r, &a, &rms, &mx, &c, &loc);
+    ml.v = mx;
+    ml.loc = loc;
+    st4<SOA>(q, c, s, qn);
+    st4<SOA>(res, c, s, r);
+  }
+  rms = wave_sum(rms);
+  ml = wave_maxloc(ml);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_sum[wave] = rms;
+    s_max[wave] = ml.v;
+    s_loc[wave] = ml.loc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bs = 0.0;
+    MaxLoc bm{-INFINITY, 0x7fffffff};
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) {
+      bs += s_sum[w];
+      bm = maxloc_combine(bm, MaxLoc{s_max[w], s_loc[w]});
+    }
+    part_sum[blockIdx.x] = bs;
+    part_max[blockIdx.x] = bm.v;
+    part_loc[blockIdx.x] = bm.loc;
+    __threadfence();
+    unsigned int done = atomicAdd(counter, 1u);
+    s_last = (done == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last || wave != 0) return;
+  __threadfence();
+  double tot = 0.0;
+  MaxLoc tm{-INFINITY, 0x7fffffff};
+  for (int b = lane; b < (int)gridDim.x; b += 64) {
+    tot += __hip_atomic_load(&part_sum[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    MaxLoc t{__hip_atomic_load(&part_max[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+             __hip_atomic_load(&part_loc[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+    tm = maxloc_combine(tm, t);
+  }
+  tot = wave_sum(tot);
+  tm = wave_maxloc(tm);
+  if (lane == 0) {
+    slot_rms[0] = tot;
+    slot_max[0] = tm.v;
+    reinterpret_cast<int *>(slot_max + 1)[0] = tm.loc;
+    *counter = 0u;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// loop stubs
+// ===========================================================================
+namespace {
+
+bool loop_soa(const op_arg *args, int nargs, int *stride) {
+  int soa = -1;
+  *stride = 0;
+  for (int m = 0; m < nargs; ++m) {
+    const op_arg &a = args[m];
+    if (a.argtype != OP_ARG_DAT || !a.opt || a.dat->dim <= 1) continue;
+    int s = op2h_dat_soa_stride(a.dat);
+    int this_soa = s > 0;
+    if (soa >= 0 && this_soa != soa) {
+      printf("op2hip: loop mixes AoS and SoA dats; declare all dats of a loop "
+             "with the same layout\n");
+      exit(-1);
+    }
+    soa = this_soa;
+  }
+  return soa == 1;
+}
+
+int set_stride(const op_arg &a) {
+  int s = op2h_dat_soa_stride(a.dat);
+  return s;
+}
+
+void account(int kernel_idx, op_set set, int nargs, const op_arg *args) {
+  OP_kernels[kernel_idx].transfer += (float)op2h_algorithmic_bytes(set, nargs, args);
+}
+
+}  // namespace
+
+extern "C" void op_par_loop_airfoil_1_save_soln(char const *name, op_set set,
+                                                op_arg arg0, op_arg arg1) {
+  op_arg args[2] = {arg0, arg1};
+  op2h_loop_ctx ctx;
+  op2h_loop_begin(&ctx, 0, name, set, 2, args, OP2H_PLAN_ATOMIC, 256);
+  hipStream_t st = (hipStream_t)ctx.stream;
+  if (set->size > 0) {
+    int stride = 0;
+    bool soa = loop_soa(args, 2, &stride);
+    // identical layouts: the copy covers the whole device allocation
+    long long ndouble = soa ? 4LL * set->stride : 4LL * set->size;
+    int n2 = (int)(ndouble / 2);
+    op2h_kernel_timer_start(&ctx);
+    k_save_soln<<<grid_for(n2, ctx.block_size), ctx.block_size, 0, st>>>(
+        (const double2 *)args[0].data_d, (double2 *)args[1].data_d, n2);
+    op2h_kernel_timer_stop(&ctx);
+    KCHECK(hipGetLastError());
+  }
+  account(0, set, 2, args);
+  op2h_loop_end(&ctx);
+}
+
+extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
+                                               op_arg arg0, op_arg arg1,
+                                               op_arg arg2, op_arg arg3,
+                                               op_arg arg4, op_arg arg5) {
+  op_arg args[6] = {arg0, arg1, arg2, arg3, arg4, arg5};
+  op2h_loop_ctx ctx;
+  op2h_loop_begin(&ctx, 1, name, set, 6, args, OP2H_PLAN_ATOMIC, 256);
+  hipStream_t st = (hipStream_t)ctx.stream;
+  if (set->size > 0) {
+    int dummy;
+    bool soa = loop_soa(args, 6, &dummy);
+    int sx = set_stride(args[0]), sq = set_stride(args[4]);
+    const double *x = (const double *)args[0].data_d;
+    const double *q = (const double *)args[4].data_d;
+    double *adt = (double *)args[5].data_d;
+    const int *pcell = args[0].map->map_d;
+    int g = grid_for(set->size, ctx.block_size);
+    op2h_kernel_timer_start(&ctx);
+    if (soa)
+      k_adt_calc<true><<<g, ctx.block_size, 0, st>>>(x, q, adt, pcell, set->size, sx, sq);
+    else
+      k_adt_calc<false><<<g, ctx.block_size, 0, st>>>(x, q, adt, pcell, set->size, sx, sq);
+    op2h_kernel_timer_stop(&ctx);
+    KCHECK(hipGetLastError());
+  }
+  account(1, set, 6, args);
+  op2h_loop_end(&ctx);
+}
+
+extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
+                                               op_arg arg0, op_arg arg1,
+                                               op_arg arg2, op_arg arg3,
+                                               op_arg arg4, op_arg arg5,
+                                               op_arg arg6, op_arg arg7) {
+  op_arg args[8] = {arg0, arg1, arg2, arg3, arg4, arg5, arg6, arg7};
+  op2h_loop_ctx ctx;
+  op2h_loop_begin(&ctx, 2, name, set, 8, args, OP2H_PLAN_STAGED, 256);
+  hipStream_t st = (hipStream_t)ctx.stream;
+  if (set->size > 0) {
+    int dummy;
+    bool soa = loop_soa(args, 8, &dummy);
+    int sx = set_stride(args[0]), sq = set_stride(args[2]);
+    const double *x = (const double *)args[0].data_d;
+    const double *q = (const double *)args[2].data_d;
+    const double *adt = (const double *)args[4].data_d;
+    double *res = (double *)args[6].data_d;
+    const int2 *pedge = (const int2 *)args[0].map->map_d;
+    const int2 *pecell = (const int2 *)args[2].map->map_d;
+    const int bs = ctx.block_size;
+    switch (ctx.plan) {
+      case OP2H_PLAN_ATOMIC: {
+        op2h_kernel_timer_start(&ctx);
+        int g = grid_for(set->size, bs);
+        if (soa) k_res_atomic<true><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, set->size, sx, sq);
+        else k_res_atomic<false><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, set->size, sx, sq);
+        op2h_kernel_timer_stop(&ctx);
+        break;
+      }
+      case OP2H_PLAN_COLOR: {
+        const op2h_color_plan *p = op2h_plan_color(&ctx);
+        op2h_kernel_timer_start(&ctx);
+        for (int c = 0; c < p->ncolors; ++c) {
+          int s0 = p->col_offsets_h[c], s1 = p->col_offsets_h[c + 1];
+          int g = grid_for(s1 - s0, bs);
+          if (!g) continue;
+          if (soa) k_res_color<true><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->col_reord, s0, s1, sx, sq);
+          else k_res_color<false><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->col_reord, s0, s1, sx, sq);
+        }
+        op2h_kernel_timer_stop(&ctx);
+        break;
+      }
+      case OP2H_PLAN_GATHER: {
+        const op2h_gather_plan *p = op2h_plan_gather(args[6].map);
+        op2h_kernel_timer_start(&ctx);
+        int g = grid_for(p->ntargets, bs);
+        if (soa) k_res_gather<true><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+        else k_res_gather<false><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+        op2h_kernel_timer_stop(&ctx);
+        break;
+      }
+      default: {  // STAGED
+        ctx.part_size = bs;
+        const op2h_staged_plan *p = op2h_plan_staged(&ctx);
+        int sn = p->map_of_arg[0], sc = p->map_of_arg[2];
+        StagedArgs A;
+        A.blkmap = p->blkmap;
+        A.offset = p->offset;
+        A.nelems = p->nelems;
+        A.nthrcol = p->nthrcol;
+        A.thrcol = p->thrcol;
+        A.ind_offs_n = p->ind_offs[sn];
+        A.ind_sizes_n = p->ind_sizes[sn];
+        A.ind_map_n = p->ind_map[sn];
+        A.ind_offs_c = p->ind_offs[sc];
+        A.ind_sizes_c = p->ind_sizes[sc];
+        A.ind_map_c = p->ind_map[sc];
+        A.loc_n0 = p->loc_map[p->loc_of_arg[0]];
+        A.loc_n1 = p->loc_map[p->loc_of_arg[1]];
+        A.loc_c0 = p->loc_map[p->loc_of_arg[2]];
+        A.loc_c1 = p->loc_map[p->loc_of_arg[3]];
+        A.max_n = p->max_ind_h[sn];
+        A.max_c = p->max_ind_h[sc];
+        size_t lds = sizeof(double) * (2 * (size_t)A.max_n + 9 * (size_t)A.max_c);
+        if (lds > 160 * 1024) {
+          printf("op2hip: res_calc staged plan needs %zu bytes of LDS\n", lds);
+          exit(-1);
+        }
+        op2h_kernel_timer_start(&ctx);
+        for (int c = 0; c < p->ncolors; ++c) {
+          int nb = p->ncolblk_h[c];
+          if (!nb) continue;
+          if (soa) k_res_staged<true><<<nb, bs, lds, st>>>(x, q, adt, res, A, p->colstart_h[c], sx, sq);
+          else k_res_staged<false><<<nb, bs, lds, st>>>(x, q, adt, res, A, p->colstart_h[c], sx, sq);
+        }
+        op2h_kernel_timer_stop(&ctx);
+        break;
+      }
+    }
+    KCHECK(hipGetLastError());
+  }
+  account(2, set, 8, args);
+  op2h_loop_end(&ctx);
+}
+
+extern "C" void op_par_loop_airfoil_4_bres_calc(char const *name, op_set set,
+                                                op_arg arg0, op_arg arg1,
+                                                op_arg arg2, op_arg arg3,
+                                                op_arg arg4, op_arg arg5) {
+  op_arg args[6] = {arg0, arg1, arg2, arg3, arg4, arg5};
+  op2h_loop_ctx ctx;
+  op2h_loop_begin(&ctx, 3, name, set, 6, args, OP2H_PLAN_STAGED, 256);
+  hipStream_t st = (hipStream_t)ctx.stream;
+  if (set->size > 0) {
+    int dummy;
+    bool soa = loop_soa(args, 6, &dummy);
+    int sx = set_stride(args[0]), sq = set_stride(args[2]);
+    const double *x = (const double *)args[0].data_d;
+    const double *q = (const double *)args[2].data_d;
+    const double *adt = (const double *)args[3].data_d;
+    double *res = (double *)args[4].data_d;
+    const int *bound = (const int *)args[5].data_d;
+    const int2 *pbedge = (const int2 *)args[0].map->map_d;
+    const int *pbecell = args[2].map->map_d;
+    const int bs = ctx.block_size;
+    switch (ctx.plan) {
+      case OP2H_PLAN_ATOMIC: {
+        op2h_kernel_timer_start(&ctx);
+        int g = grid_for(set->size, bs);
+        if (soa) k_bres_atomic<true><<<g, bs, 0, st>>>(x, q, adt, res, pbedge, pbecell, bound, set->size, sx, sq);
+        else k_bres_atomic<false><<<g, bs, 0, st>>>(x, q, adt, res, pbedge, pbecell, bound, set->size, sx, sq);
+        op2h_kernel_timer_stop(&ctx);
+        break;
+      }
+      case OP2H_PLAN_COLOR: {
+        const op2h_color_plan *p = op2h_plan_color(&ctx);
+        op2h_kernel_timer_start(&ctx);
+        for (int c = 0; c < p->ncolors; ++c) {
+          int s0 = p->col_offsets_h[c], s1 = p->col_offsets_h[c + 1];
+          int g = grid_for(s1 - s0, bs);
+          if (!g) continue;
+          if (soa) k_bres_color<true><<<g, bs, 0, st>>>(x, q, adt, res, pbedge, pbecell, bound, p->col_reord, s0, s1, sx, sq);
+          else k_bres_color<false><<<g, bs, 0, st>>>(x, q, adt, res, pbedge, pbecell, bound, p->col_reord, s0, s1, sx, sq);
+        }
+        op2h_kernel_timer_stop(&ctx);
+        break;
+      }
+      case OP2H_PLAN_GATHER: {
+        const op2h_gather_plan *p = op2h_plan_gather(args[4].map);
+        op2h_kernel_timer_start(&ctx);
+        int g = grid_for(p->ntargets, bs);
+        if (soa) k_bres_gather<true><<<g, bs, 0, st>>>(x, q, adt, res, pbedge, bound, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+        else k_bres_gather<false><<<g, bs, 0, st>>>(x, q, adt, res, pbedge, bound, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+        op2h_kernel_timer_stop(&ctx);
+        break;
+      }
+      default: {  // STAGED
+        ctx.part_size = bs;
+        const op2h_staged_plan *p = op2h_plan_staged(&ctx);
+        int sn = p->map_of_arg[0], sc = p->map_of_arg[2];
+        BStagedArgs A;
+        A.blkmap = p->blkmap;
+        A.offset = p->offset;
+        A.nelems = p->nelems;
+        A.nthrcol = p->nthrcol;
+        A.thrcol = p->thrcol;
+        A.ind_offs_n = p->ind_offs[sn];
+        A.ind_sizes_n = p->ind_sizes[sn];
+        A.ind_map_n = p->ind_map[sn];
+        A.ind_offs_c = p->ind_offs[sc];
+        A.ind_sizes_c = p->ind_sizes[sc];
+        A.ind_map_c = p->ind_map[sc];
+        A.loc_n0 = p->loc_map[p->loc_of_arg[0]];
+        A.loc_n1 = p->loc_map[p->loc_of_arg[1]];
+        A.loc_c0 = p->loc_map[p->loc_of_arg[2]];
+        A.max_n = p->max_ind_h[sn];
+        A.max_c = p->max_ind_h[sc];
+        size_t lds = sizeof(double) * (2 * (size_t)A.max_n + 9 * (size_t)A.max_c);
+        if (lds > 160 * 1024) {
+          printf("op2hip: bres_calc staged plan needs %zu bytes of LDS\n", lds);
+          exit(-1);
+        }
+        op2h_kernel_timer_start(&ctx);
+        for (int c = 0; c < p->ncolors; ++c) {
+          int nb = p->ncolblk_h[c];
+          if (!nb) continue;
+          if (soa) k_bres_staged<true><<<nb, bs, lds, st>>>(x, q, adt, res, bound, A, p->colstart_h[c], sx, sq);
+          else k_bres_staged<false><<<nb, bs, lds, st>>>(x, q, adt, res, bound, A, p->colstart_h[c], sx, sq);
+        }
+        op2h_kernel_timer_stop(&ctx);
+        break;
+      }
+    }
+    KCHECK(hipGetLastError());
+  }
+  account(3, set, 6, args);
+  op2h_loop_end(&ctx);
+}
+
+extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
+                                             op_arg arg0, op_arg arg1,
+                                             op_arg arg2, op_arg arg3,
+                                             op_arg arg4, op_arg arg5,
+                                             op_arg arg6, op_arg arg7) {
+  op_arg args[8] = {arg0, arg1, arg2, arg3, arg4, arg5, arg6, arg7};
+  op2h_loop_ctx ctx;
+  op2h_loop_begin(&ctx, 4, name, set, 8, args, OP2H_PLAN_ATOMIC, 256);
+  hipStream_t st = (hipStream_t)ctx.stream;
+  double *slot_rms = (double *)op2h_gbl_slot(&ctx, 4, -1);
+  double *slot_max = (double *)op2h_gbl_slot(&ctx, 5, 7);
+  int dummy;
+  bool soa = loop_soa(args, 8, &dummy);
+  int s = set_stride(args[0]);
+  const int bs = ctx.block_size;
+  // enough workgroups to fill 256 CUs several times over; each thread
+  // strides over the set when it is larger
+  int g = grid_for(set->size, bs);
+  g = std::max(1, std::min(g, 2048));
+  char *scratch = (char *)op2h_reduct_scratch((size_t)g * 24);
+  double *part_sum = (double *)scratch;
+  double *part_max = part_sum + g;
+  int *part_loc = (int *)(part_max + g);
+  op2h_kernel_timer_start(&ctx);
+  if (soa)
+    k_update<true><<<g, bs, 0, st>>>((const double *)args[0].data_d, (double *)args[1].data_d,
+                                     (double *)args[2].data_d, (const double *)args[3].data_d,
+                                     set->size, s, part_sum, part_max, part_loc,
+                                     op2h_reduct_counter(), slot_rms, slot_max);
+  else
+    k_update<false><<<g, bs, 0, st>>>((const double *)args[0].data_d, (double *)args[1].data_d,
+                                      (double *)args[2].data_d, (const double *)args[3].data_d,
+                                      set->size, s, part_sum, part_max, part_loc,
+                                      op2h_reduct_counter(), slot_rms, slot_max);
+  op2h_kernel_timer_stop(&ctx);
+  KCHECK(hipGetLastError());
+  account(4, set, 8, args);
+  op2h_loop_end(&ctx);
+}

@@ -1,0 +1,90 @@
+// Plan construction for the runtime (internal header).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "op2_hip_rt.h"
+
+namespace op2h {
+
+struct DeviceAllocs {
+  std::vector<void *> ptrs;
+  template <class T> T *upload(const std::vector<T> &v, hipStream_t s);
+  template <class T> T *alloc(size_t n);
+  ~DeviceAllocs();
+};
+
+struct StagedPlanRec {
+  std::string signature;
+  op2h_staged_plan plan{};
+  std::vector<int> ncolblk_h, colstart_h, max_nthrcol_h;
+  DeviceAllocs mem;
+};
+
+struct ColorPlanRec {
+  std::string signature;
+  op2h_color_plan plan{};
+  std::vector<int> col_offsets_h;
+  DeviceAllocs mem;
+};
+
+struct GatherPlanRec {
+  op2h_gather_plan plan{};
+  DeviceAllocs mem;
+};
+
+// Host-side description of the indirect structure of a loop.
+struct LoopShape {
+  int set_size = 0;
+  int nargs = 0;
+  std::vector<int> map_slot;  // per arg: staged map slot or -1
+  std::vector<int> loc_slot;  // per arg: loc-map slot or -1
+  std::vector<int> dat_slot;  // per arg: conflict (dat) slot or -1
+  std::vector<bool> conflict; // per arg: INC/RW through a map
+  std::vector<op_map> maps;   // per map slot
+  std::vector<std::pair<op_map, int>> locs;  // per loc slot: (map, idx)
+  std::vector<int> dat_to_size;              // per dat slot
+  std::vector<std::pair<op_map, int>> arg_map;  // per arg (map, idx)
+};
+
+LoopShape loop_shape(op_set set, int nargs, const op_arg *args);
+
+// Greedy first-fit colouring in element order with 32 colours per pass
+// (op_rt_support.cpp:684-754 for elements, :798-884 for blocks).
+// Returns colours per item and the number of colours.
+int greedy_block_thread_colours(const LoopShape &sh, int lo, int hi,
+                                std::vector<std::vector<unsigned>> &work,
+                                int *thrcol);
+int greedy_block_colours(const LoopShape &sh, int nblocks, const int *offset,
+                         const int *nelems,
+                         std::vector<std::vector<unsigned>> &work, int *blkcol);
+
+std::unique_ptr<StagedPlanRec> build_staged_plan(op_set set, int part_size,
+                                                 int nargs, const op_arg *args,
+                                                 int on_device,
+                                                 hipStream_t stream);
+std::unique_ptr<ColorPlanRec> build_color_plan(op_set set, int nargs,
+                                               const op_arg *args,
+                                               int on_device,
+                                               hipStream_t stream);
+std::unique_ptr<GatherPlanRec> build_gather_plan(op_map map, hipStream_t stream);
+
+// GPU colouring (op2h_plan_gpu.hip).  Colours elements [0,n) so that no two
+// elements sharing an increment target get the same colour, on the device;
+// returns the number of colours and leaves colours in `col_d`.
+int gpu_colour_elements(const LoopShape &sh, int n, int *col_d,
+                        hipStream_t stream);
+int gpu_colour_blocks_and_threads(const LoopShape &sh, int nblocks,
+                                  int part_size, int *blkcol_d,
+                                  unsigned char *thrcol_d, int *nthrcol_d,
+                                  hipStream_t stream);
+
+int download_staged(const op2h_staged_plan *p, int *thrcol, int *blkcol,
+                    int *offset, int *nelems);
+int download_color(const op2h_color_plan *p, int *color_of_elem);
+
+}  // namespace op2h

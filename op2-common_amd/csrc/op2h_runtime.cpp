@@ -1,0 +1,1398 @@
+// op2h_runtime.cpp — host side of the MI355X-native OP2 runtime.
+//
+// Registry, declarations and argument constructors keep the semantics of
+// op2/src/core/op_lib_core.cpp (cited per function); the device side is this
+// backend's own: one HIP stream, dats AoS or SoA with a stride padded to a
+// multiple of 64 elements (one wave64 per 64 consecutive elements), plans
+// uploaded once and cached per loop signature, gbl reductions delivered from
+// device result slots, and optional HIP-graph capture of whole time steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "op2_hip_rt.h"
+#include "op2h_plan_build.h"
+
+// ---------------------------------------------------------------------------
+// global state (op_lib_core.cpp:55-86)
+// ---------------------------------------------------------------------------
+int OP_diags = 0, OP_part_size = 0, OP_block_size = 256, OP_auto_soa = 0;
+int OP_hybrid_gpu = 0, OP_cache_line_size = 128, OP_maps_base_index = 0;
+int OP_kern_max = 0, OP_kern_curr = 0;
+op_kernel *OP_kernels = nullptr;
+
+#define HIPCHECK(x)                                                          \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess) {                                                  \
+      printf("op2hip: %s failed at %s:%d: %s\n", #x, __FILE__, __LINE__,     \
+             hipGetErrorString(e_));                                         \
+      exit(-1);                                                              \
+    }                                                                        \
+  } while (0)
+
+namespace {
+
+double wall_now() {
+  return std::chrono::duration<double>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+char *copy_str(const char *s) {
+  size_t n = std::strlen(s) + 1;
+  char *p = (char *)std::malloc(n);
+  std::memcpy(p, s, n);
+  return p;
+}
+
+struct LoopConfig {
+  int plan = OP2H_PLAN_AUTO;
+  int block_size = 0;
+};
+
+struct LoopStats {
+  int calls = 0;
+  double kernel_ms = 0.0;
+};
+
+struct PendingTimer {
+  int kernel_idx;
+  hipEvent_t a, b;
+};
+
+// A gbl result recorded inside a captured graph, delivered at graph sync.
+struct DeferredGbl {
+  op_access acc;
+  int type;  // 0 double, 1 float, 2 int
+  int dim;
+  char *host_ptr;
+  char *info_ptr;  // linked info arg for MIN/MAX (may be null)
+  int info_size;
+  std::vector<char> snapshot;       // host value at record time
+  std::vector<char> info_snapshot;  // host info value at record time
+  char *slot_h;                     // pinned copy written by the graph
+  size_t bytes;                     // slot bytes (value + info)
+};
+
+struct Runtime {
+  bool initialised = false;
+  hipStream_t stream = nullptr;
+  std::vector<op_set> sets;
+  std::vector<op_map> maps;
+  std::vector<op_dat> dats;
+  std::map<std::string, LoopConfig> loop_cfg;
+  LoopConfig all_cfg;
+  std::map<int, LoopStats> stats;
+  std::vector<PendingTimer> pending;
+  std::vector<hipEvent_t> event_pool;
+  bool kernel_timing = false;
+  int sync_mode = OP2H_SYNC_EVERY_LOOP;
+  int plan_on_device = 0;
+  // plans
+  std::vector<std::unique_ptr<op2h::StagedPlanRec>> staged;
+  std::vector<std::unique_ptr<op2h::ColorPlanRec>> color;
+  std::map<op_map, std::unique_ptr<op2h::GatherPlanRec>> gather;
+  // reductions
+  void *red_scratch = nullptr;
+  size_t red_scratch_bytes = 0;
+  unsigned int *red_counter = nullptr;
+  // gbl slots: persistent per (kernel, arg) outside capture
+  std::map<std::pair<int, int>, std::pair<void *, void *>> slots;
+  std::vector<std::pair<void *, void *>> capture_slots;  // (dev, pinned) pool
+  int capture_slot_next = 0;
+  // graph capture
+  bool capturing = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<DeferredGbl> deferred;
+  std::vector<std::pair<void *, size_t>> history_sources;  // device slots
+  double *history = nullptr;
+  unsigned long long *history_count = nullptr;
+  int history_words = 0;   // words per launch
+  int history_capacity = 0;  // launches
+  std::map<op_map, long long> distinct_targets;  // per map, for byte counts
+  // profiling
+  std::vector<std::pair<std::string, double>> prof_stack;
+  std::map<std::string, double> prof_totals;
+};
+
+Runtime &rt() {
+  static Runtime R;
+  return R;
+}
+
+void ensure_init() {
+  Runtime &R = rt();
+  if (R.initialised) return;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    printf("op2hip: no HIP device available; this backend has no CPU "
+           "fallback\n");
+    exit(-1);
+  }
+  HIPCHECK(hipSetDevice(0));
+  HIPCHECK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
+  HIPCHECK(hipMalloc(&R.red_counter, 64 * sizeof(unsigned int)));
+  HIPCHECK(hipMemset(R.red_counter, 0, 64 * sizeof(unsigned int)));
+  OP_hybrid_gpu = 1;
+  R.initialised = true;
+}
+
+int type_code(const char *t) {
+  if (!std::strncmp(t, "double", 6) || !std::strcmp(t, "r8")) return 0;
+  if (!std::strncmp(t, "float", 5) || !std::strcmp(t, "r4")) return 1;
+  if (!std::strncmp(t, "int", 3) || !std::strcmp(t, "i4")) return 2;
+  if (!std::strcmp(t, "bool")) return 3;
+  return -1;
+}
+
+int elem_bytes_of(const char *t, int fallback) {
+  switch (type_code(t)) {
+    case 0: return 8;
+    case 1: return 4;
+    case 2: return 4;
+    case 3: return 1;
+  }
+  return fallback;
+}
+
+bool dat_is_soa(op_dat d) {
+  if (d->dim <= 1) return false;
+  if (std::strstr(d->type, ":soa")) return true;
+  return OP_auto_soa != 0;
+}
+
+int round64(int n) { return (n + 63) & ~63; }
+
+// --- layout transforms (op_cuda_decl.cpp:71-99, op_cuda_rt_support.cpp:373-396) ---
+void upload_dat(op_dat d) {
+  Runtime &R = rt();
+  op_set s = d->set;
+  size_t n = (size_t)s->size;
+  size_t bytes = (size_t)d->size * (size_t)std::max(s->stride, 1);
+  if (!d->data_d) {
+    HIPCHECK(hipMalloc(&d->data_d, std::max(bytes, (size_t)64)));
+    HIPCHECK(hipMemset(d->data_d, 0, std::max(bytes, (size_t)64)));
+  }
+  if (!dat_is_soa(d)) {
+    HIPCHECK(hipMemcpyAsync(d->data_d, d->data, (size_t)d->size * n,
+                            hipMemcpyHostToDevice, R.stream));
+  } else {
+    int eb = d->size / d->dim;
+    std::vector<char> tmp(bytes);
+    for (size_t i = 0; i < n; ++i)
+      for (int k = 0; k < d->dim; ++k)
+        std::memcpy(&tmp[((size_t)k * s->stride + i) * eb],
+                    d->data + (i * d->dim + k) * eb, eb);
+    HIPCHECK(hipMemcpyAsync(d->data_d, tmp.data(), bytes, hipMemcpyHostToDevice,
+                            R.stream));
+  }
+  HIPCHECK(hipStreamSynchronize(R.stream));
+  d->dirty_hd = 0;
+}
+
+void download_dat(op_dat d) {
+  Runtime &R = rt();
+  if (!d->data_d) return;
+  op_set s = d->set;
+  size_t n = (size_t)s->size;
+  if (!dat_is_soa(d)) {
+    HIPCHECK(hipMemcpyAsync(d->data, d->data_d, (size_t)d->size * n,
+                            hipMemcpyDeviceToHost, R.stream));
+    HIPCHECK(hipStreamSynchronize(R.stream));
+  } else {
+    int eb = d->size / d->dim;
+    size_t bytes = (size_t)d->size * s->stride;
+    std::vector<char> tmp(bytes);
+    HIPCHECK(hipMemcpyAsync(tmp.data(), d->data_d, bytes, hipMemcpyDeviceToHost,
+                            R.stream));
+    HIPCHECK(hipStreamSynchronize(R.stream));
+    for (size_t i = 0; i < n; ++i)
+      for (int k = 0; k < d->dim; ++k)
+        std::memcpy(d->data + (i * d->dim + k) * eb,
+                    &tmp[((size_t)k * s->stride + i) * eb], eb);
+  }
+  d->dirty_hd = 0;
+}
+
+hipEvent_t take_event() {
+  Runtime &R = rt();
+  if (!R.event_pool.empty()) {
+    hipEvent_t e = R.event_pool.back();
+    R.event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIPCHECK(hipEventCreate(&e));
+  return e;
+}
+
+void resolve_timers() {
+  Runtime &R = rt();
+  if (R.pending.empty()) return;
+  HIPCHECK(hipStreamSynchronize(R.stream));
+  for (auto &p : R.pending) {
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    R.stats[p.kernel_idx].kernel_ms += ms;
+    R.stats[p.kernel_idx].calls += 1;
+    R.event_pool.push_back(p.a);
+    R.event_pool.push_back(p.b);
+  }
+  R.pending.clear();
+}
+
+// Combine a device result `res` into host value `host` (dim elements).
+template <class T>
+bool combine(op_access acc, T *host, const T *res, int dim, bool *improved) {
+  for (int d = 0; d < dim; ++d) {
+    if (acc == OP_INC) {
+      host[d] += res[d];
+    } else if (acc == OP_MAX) {
+      if (res[d] > host[d]) {
+        host[d] = res[d];
+        if (improved) improved[d] = true;
+      }
+    } else if (acc == OP_MIN) {
+      if (res[d] < host[d]) {
+        host[d] = res[d];
+        if (improved) improved[d] = true;
+      }
+    } else if (acc == OP_WRITE || acc == OP_RW) {
+      host[d] = res[d];
+    }
+  }
+  return true;
+}
+
+void combine_gbl(op_access acc, int type, int dim, char *host, const char *res,
+                 char *info_host, const char *info_res, int info_size) {
+  bool improved[64] = {false};
+  int dd = std::min(dim, 64);
+  switch (type) {
+    case 0: combine(acc, (double *)host, (const double *)res, dd, improved); break;
+    case 1: combine(acc, (float *)host, (const float *)res, dd, improved); break;
+    case 2: combine(acc, (int *)host, (const int *)res, dd, improved); break;
+    default: break;
+  }
+  if (info_host && info_res) {
+    int per = info_size / std::max(dim, 1);
+    for (int d = 0; d < dd; ++d)
+      if (improved[d]) std::memcpy(info_host + d * per, info_res + d * per, per);
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// lifecycle and declarations
+// ===========================================================================
+extern "C" void op_init(int argc, char **argv, int diags) {
+  OP_diags = diags;
+  // argv tokens as op_set_args (op_lib_core.cpp:205-281)
+  for (int n = 1; n < argc; ++n) {
+    const char *a = argv[n];
+    if (!std::strncmp(a, "OP_BLOCK_SIZE=", 14)) OP_block_size = atoi(a + 14);
+    else if (!std::strncmp(a, "OP_PART_SIZE=", 13)) OP_part_size = atoi(a + 13);
+    else if (!std::strncmp(a, "OP_CACHE_LINE_SIZE=", 19)) OP_cache_line_size = atoi(a + 19);
+    else if (!std::strcmp(a, "OP_AUTO_SOA")) OP_auto_soa = 1;
+    else if (!std::strncmp(a, "OP_MAPS_BASE_INDEX=", 19)) OP_maps_base_index = atoi(a + 19);
+    else if (!std::strncmp(a, "OP_HIP_PLAN=", 12)) {
+      // OP_HIP_PLAN=<loop>:<plan>[:<block>]
+      std::string s(a + 12);
+      size_t c1 = s.find(':');
+      if (c1 == std::string::npos) continue;
+      std::string loop = s.substr(0, c1), rest = s.substr(c1 + 1);
+      size_t c2 = rest.find(':');
+      std::string plan = rest.substr(0, c2);
+      int bs = c2 == std::string::npos ? 0 : atoi(rest.c_str() + c2 + 1);
+      int p = plan == "atomic" ? OP2H_PLAN_ATOMIC
+              : plan == "color" ? OP2H_PLAN_COLOR
+              : plan == "staged" ? OP2H_PLAN_STAGED
+              : plan == "gather" ? OP2H_PLAN_GATHER : OP2H_PLAN_AUTO;
+      op2h_set_loop_plan(loop == "*" ? nullptr : loop.c_str(), p, bs);
+    }
+  }
+  ensure_init();
+}
+
+extern "C" void op_init_soa(int argc, char **argv, int diags, int soa) {
+  OP_auto_soa = soa;
+  op_init(argc, argv, diags);
+}
+
+extern "C" void op_exit(void) {
+  Runtime &R = rt();
+  if (!R.initialised) return;
+  resolve_timers();
+  HIPCHECK(hipStreamSynchronize(R.stream));
+  op2h_graph_destroy();
+  R.staged.clear();
+  R.color.clear();
+  R.gather.clear();
+  for (op_dat d : R.dats) {
+    if (!d) continue;
+    if (d->data_d) (void)hipFree(d->data_d);
+    if (!d->user_managed) std::free(d->data);
+    std::free((void *)d->name);
+    std::free((void *)d->type);
+    std::free(d);
+  }
+  for (op_map m : R.maps) {
+    if (m->map_d) (void)hipFree(m->map_d);
+    std::free(m->map);
+    std::free((void *)m->name);
+    std::free(m);
+  }
+  for (op_set s : R.sets) {
+    std::free((void *)s->name);
+    std::free(s);
+  }
+  for (auto &kv : R.slots) {
+    (void)hipFree(kv.second.first);
+    (void)hipHostFree(kv.second.second);
+  }
+  if (R.red_scratch) (void)hipFree(R.red_scratch);
+  if (R.red_counter) (void)hipFree(R.red_counter);
+  for (auto e : R.event_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(R.stream);
+  for (int k = 0; k < OP_kern_max; ++k) std::free(OP_kernels[k].times);
+  std::free(OP_kernels);
+  OP_kernels = nullptr;
+  OP_kern_max = 0;
+  R.sets.clear();
+  R.maps.clear();
+  R.dats.clear();
+  R.slots.clear();
+  R.capture_slots.clear();
+  R.stats.clear();
+  R.distinct_targets.clear();
+  R.loop_cfg.clear();
+  R.all_cfg = LoopConfig();
+  R.event_pool.clear();
+  R.red_scratch = nullptr;
+  R.red_scratch_bytes = 0;
+  R.red_counter = nullptr;
+  R.stream = nullptr;
+  R.initialised = false;
+}
+
+extern "C" op_set op_decl_set(idx_l_t size, char const *name) {
+  // op_lib_core.cpp:391-418
+  ensure_init();
+  if (size < 0) {
+    printf(" op_decl_set error -- negative/zero size for set: %s\n", name);
+    exit(-1);
+  }
+  op_set s = (op_set)std::calloc(1, sizeof(op_set_core));
+  s->index = (int)rt().sets.size();
+  s->size = size;
+  s->core_size = size;
+  s->name = copy_str(name);
+  s->stride = round64(std::max(size, 1));
+  rt().sets.push_back(s);
+  return s;
+}
+
+extern "C" op_map op_decl_map(op_set from, op_set to, int dim, idx_l_t *imap,
+                              char const *name) {
+  // op_lib_core.cpp:419-500 (copies the table; optional 1-based conversion)
+  ensure_init();
+  if (!from) {
+    printf(" op_decl_map error -- invalid 'from' set for map %s\n", name);
+    exit(-1);
+  }
+  if (!to) {
+    printf("op_decl_map error -- invalid 'to' set for map %s\n", name);
+    exit(-1);
+  }
+  if (dim <= 0) {
+    printf("op_decl_map error -- negative/zero dimension for map %s\n", name);
+    exit(-1);
+  }
+  size_t n = (size_t)from->size * dim;
+  op_map m = (op_map)std::calloc(1, sizeof(op_map_core));
+  m->index = (int)rt().maps.size();
+  m->from = from;
+  m->to = to;
+  m->dim = dim;
+  m->name = copy_str(name);
+  m->map = (int *)std::malloc(std::max(n, (size_t)1) * sizeof(int));
+  if (imap) std::memcpy(m->map, imap, n * sizeof(int));
+  else std::memset(m->map, 0, n * sizeof(int));
+  if (OP_maps_base_index == 1 && imap)
+    for (size_t i = 0; i < n; ++i) m->map[i]--;
+  // a map entry outside the target set would make every kernel that reads
+  // through it access memory out of bounds: refuse it here
+  for (size_t i = 0; i < n; ++i)
+    if (m->map[i] < 0 || m->map[i] >= to->size) {
+      printf("op_decl_map error -- invalid data for map %s\n", name);
+      printf("element = %zu, dimension = %zu, map = %d\n", i / dim, i % dim,
+             m->map[i]);
+      exit(-1);
+    }
+  HIPCHECK(hipMalloc(&m->map_d, std::max(n, (size_t)1) * sizeof(int)));
+  HIPCHECK(hipMemcpy(m->map_d, m->map, n * sizeof(int), hipMemcpyHostToDevice));
+  rt().maps.push_back(m);
+  return m;
+}
+
+extern "C" op_map op_decl_map_long(op_set from, op_set to, int dim,
+                                   idx_g_t *imap, char const *name) {
+  std::vector<int> tmp((size_t)from->size * dim);
+  for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (int)imap[i];
+  return op_decl_map(from, to, dim, tmp.data(), name);
+}
+
+extern "C" op_dat op_decl_dat_char(op_set set, int dim, char const *type,
+                                   int size, char *data, char const *name) {
+  // op_lib_core.cpp:502-576 (always copies: OP_NO_REALLOC is not supported)
+  ensure_init();
+  if (!set) {
+    printf("op_decl_dat error -- invalid set for data: %s\n", name);
+    exit(-1);
+  }
+  if (dim <= 0) {
+    printf("op_decl_dat error -- negative/zero dimension for data: %s\n", name);
+    exit(-1);
+  }
+  op_dat d = (op_dat)std::calloc(1, sizeof(op_dat_core));
+  d->index = (int)rt().dats.size();
+  d->set = set;
+  d->dim = dim;
+  d->size = dim * size;
+  size_t bytes = (size_t)d->size * set->size;
+  d->data = (char *)std::calloc(std::max(bytes, (size_t)1), 1);
+  if (data) std::memcpy(d->data, data, bytes);
+  d->name = copy_str(name);
+  d->type = copy_str(type);
+  d->dirtybit = 1;
+  rt().dats.push_back(d);
+  upload_dat(d);
+  return d;
+}
+
+extern "C" op_dat op_decl_dat_temp_char(op_set set, int dim, char const *type,
+                                        int size, char const *name) {
+  return op_decl_dat_char(set, dim, type, size, nullptr, name);
+}
+
+extern "C" int op_free_dat_temp_char(op_dat dat) {
+  Runtime &R = rt();
+  for (auto &d : R.dats)
+    if (d == dat) {
+      HIPCHECK(hipStreamSynchronize(R.stream));
+      if (d->data_d) (void)hipFree(d->data_d);
+      std::free(d->data);
+      std::free((void *)d->name);
+      std::free((void *)d->type);
+      std::free(d);
+      d = nullptr;
+      return 1;
+    }
+  return 0;
+}
+
+// ===========================================================================
+// argument constructors (op_lib_core.cpp:832-1060)
+// ===========================================================================
+namespace {
+const char *canon_type(const char *typ) {
+  if (!std::strcmp(typ, "double") || !std::strcmp(typ, "r8") ||
+      !std::strcmp(typ, "real*8") || !std::strcmp(typ, "real(8)"))
+    return "double";
+  if (!std::strcmp(typ, "float") || !std::strcmp(typ, "r4") ||
+      !std::strcmp(typ, "real*4") || !std::strcmp(typ, "real(4)"))
+    return "float";
+  if (!std::strcmp(typ, "int") || !std::strcmp(typ, "i4") ||
+      !std::strcmp(typ, "integer*4") || !std::strcmp(typ, "integer(4)"))
+    return "int";
+  if (!std::strcmp(typ, "bool")) return "bool";
+  return typ;
+}
+}  // namespace
+
+extern "C" op_arg op_opt_arg_dat(int opt, op_dat dat, int idx, op_map map,
+                                 int dim, char const *typ, op_access acc) {
+  op_arg a;
+  std::memset(&a, 0, sizeof(a));
+  a.index = -1;
+  a.opt = opt;
+  a.argtype = OP_ARG_DAT;
+  a.dat = dat;
+  a.map = map;
+  a.dim = dim;
+  a.idx = idx;
+  if (dat) {
+    a.size = dat->size;
+    a.data = dat->data;
+    a.data_d = dat->data_d;
+  } else {
+    a.size = -1;
+  }
+  a.map_data = map ? map->map : nullptr;
+  a.map_data_d = map ? map->map_d : nullptr;
+  a.type = canon_type(typ);
+  a.acc = acc;
+  return a;
+}
+
+extern "C" op_arg op_arg_dat(op_dat dat, int idx, op_map map, int dim,
+                             char const *typ, op_access acc) {
+  op_arg a = op_opt_arg_dat(1, dat, idx, map, dim, typ, acc);
+  if (idx == -1) {
+    a.map_data = nullptr;
+    a.map_data_d = nullptr;
+  }
+  return a;
+}
+
+extern "C" op_arg op_opt_arg_gbl_char(int opt, char *data, int dim,
+                                      const char *typ, int size,
+                                      op_access acc) {
+  op_arg a;
+  std::memset(&a, 0, sizeof(a));
+  a.argtype = OP_ARG_GBL;
+  a.opt = opt;
+  a.dim = dim;
+  a.idx = -1;
+  a.size = dim * size;
+  a.data = data;
+  a.type = canon_type(typ);
+  a.acc = acc;
+  a.index = -1;
+  if (!data) a.opt = 0;
+  return a;
+}
+
+extern "C" op_arg op_arg_gbl_char(char *data, int dim, const char *typ,
+                                  int size, op_access acc) {
+  return op_opt_arg_gbl_char(1, data, dim, typ, size, acc);
+}
+
+extern "C" op_arg op_arg_info_char(char *data, int dim, const char *typ,
+                                   int size, int ref) {
+  op_arg a = op_opt_arg_gbl_char(1, data, dim, typ, size, ref);
+  a.argtype = OP_ARG_INFO;
+  return a;
+}
+
+extern "C" op_arg op_arg_idx(int idx, op_map map) {
+  op_arg a;
+  std::memset(&a, 0, sizeof(a));
+  a.index = -1;
+  a.opt = 1;
+  a.argtype = OP_ARG_IDX;
+  a.map = map;
+  a.idx = idx;
+  a.dim = 1;
+  a.size = sizeof(int);
+  a.type = "int";
+  a.acc = OP_READ;
+  a.map_data = (idx == -1 || !map) ? nullptr : map->map;
+  a.map_data_d = (idx == -1 || !map) ? nullptr : map->map_d;
+  return a;
+}
+
+extern "C" void op_arg_check(op_set set, int m, op_arg arg, int *ninds,
+                             const char *name) {
+  // op_lib_core.cpp:754-829: prints, never exits
+  auto err = [&](const char *msg) {
+    printf("error: arg %d in kernel \"%s\"\n", m, name);
+    printf("%s \n", msg);
+  };
+  if (arg.opt == 0) return;
+  if (arg.argtype == OP_ARG_DAT) {
+    if (!set) err("invalid set");
+    if (!arg.map && arg.dat->set != set)
+      printf("dataset dat %s on set %s does not match loop set %s\n",
+             arg.dat->name, arg.dat->set->name, set ? set->name : "?");
+    if (arg.map) {
+      if (arg.map->from != set) {
+        err("mapping error");
+        printf("map from set %s does not match set \n", arg.map->from->name);
+      }
+      if (arg.map->to != arg.dat->set) {
+        err("mapping error");
+        printf("map %s to set %s does not match dat %s set %s\n", arg.map->name,
+               arg.map->to->name, arg.dat->name, arg.dat->set->name);
+      }
+    }
+    if ((!arg.map && arg.idx != -1) ||
+        (arg.map && (arg.idx >= arg.map->dim || arg.idx < -1 * arg.map->dim)))
+      err("invalid index");
+    if (arg.dat->dim != arg.dim) err("dataset dim does not match declared dim");
+    if (std::strcmp(canon_type(arg.dat->type), arg.type) &&
+        std::strncmp(arg.dat->type, arg.type, std::strlen(arg.type)))
+      err("dataset type does not match declared type");
+    if (arg.idx >= 0) (*ninds)++;
+  }
+  if (arg.argtype == OP_ARG_GBL || arg.argtype == OP_ARG_INFO) {
+    if (!std::strcmp(arg.type, "error")) err("datatype does not match declared type");
+    if (arg.dim < 0) err("dimension should be positive");
+    if (!arg.data) err("NULL pointer for global data");
+  }
+  if (arg.argtype == OP_ARG_IDX) {
+    if (arg.idx == -1 && arg.map) err("op_arg_idx with -1 index but non-OP_ID map");
+    if (arg.idx != -1 && !arg.map) err("op_arg_idx map index not -1, but no valid map given");
+  }
+}
+
+// ===========================================================================
+// data access (op_cuda_rt_support.cpp:276-307,373-396; op_lib_core.cpp:1283-1369)
+// ===========================================================================
+extern "C" void op_upload_dat(op_dat dat) { upload_dat(dat); }
+extern "C" void op_download_dat(op_dat dat) { download_dat(dat); }
+extern "C" void op_upload_all(void) {
+  for (op_dat d : rt().dats)
+    if (d && d->dirty_hd == 1) upload_dat(d);
+}
+
+extern "C" void op_fetch_data_char(op_dat dat, char *usr_ptr) {
+  if (rt().capturing) {
+    printf("op_fetch_data error -- not allowed during graph capture\n");
+    exit(-1);
+  }
+  if (dat->dirty_hd == 2) download_dat(dat);
+  std::memcpy(usr_ptr, dat->data, (size_t)dat->size * dat->set->size);
+}
+
+extern "C" void op_fetch_data_idx_char(op_dat dat, char *usr_ptr, int low,
+                                       int high) {
+  if (low < 0 || high >= dat->set->size || low > high) {
+    printf("op_fetch_data_idx error -- range %d..%d outside set %s\n", low, high,
+           dat->set->name);
+    exit(-1);
+  }
+  if (dat->dirty_hd == 2) download_dat(dat);
+  std::memcpy(usr_ptr, dat->data + (size_t)low * dat->size,
+              (size_t)(high - low + 1) * dat->size);
+}
+
+extern "C" idx_g_t op_get_size(op_set set) { return set->size; }
+
+extern "C" void op_print_dat_to_binfile(op_dat dat, const char *file_name) {
+  // op_lib_core.cpp:1283-1307: int count, int dim, then raw AoS data
+  if (dat->dirty_hd == 2) download_dat(dat);
+  FILE *fp = std::fopen(file_name, "wb");
+  if (!fp) {
+    printf("can't open file %s\n", file_name);
+    exit(2);
+  }
+  int count = dat->set->size, dim = dat->dim;
+  if (std::fwrite(&count, sizeof(int), 1, fp) < 1 ||
+      std::fwrite(&dim, sizeof(int), 1, fp) < 1 ||
+      std::fwrite(dat->data, dat->size, count, fp) < (size_t)count) {
+    printf("error writing to %s\n", file_name);
+    exit(2);
+  }
+  std::fclose(fp);
+}
+
+extern "C" void op_print_dat_to_txtfile(op_dat dat, const char *file_name) {
+  // op_lib_core.cpp:1309-1369
+  if (dat->dirty_hd == 2) download_dat(dat);
+  FILE *fp = std::fopen(file_name, "w");
+  if (!fp) {
+    printf("can't open file %s\n", file_name);
+    exit(2);
+  }
+  std::fprintf(fp, "%d %d\n", dat->set->size, dat->dim);
+  int tc = type_code(dat->type);
+  for (int i = 0; i < dat->set->size; i++) {
+    for (int j = 0; j < dat->dim; j++) {
+      size_t k = (size_t)i * dat->dim + j;
+      if (tc == 0) {
+        double v = ((double *)dat->data)[k];
+        if (v == -0.0) v = +0.0;
+        std::fprintf(fp, " %+2.15lE", v);
+      } else if (tc == 1) {
+        std::fprintf(fp, " %+f", ((float *)dat->data)[k]);
+      } else if (tc == 2) {
+        std::fprintf(fp, " %+d", ((int *)dat->data)[k]);
+      } else {
+        printf("Unknown type %s, cannot be written to file %s\n", dat->type,
+               file_name);
+        exit(2);
+      }
+    }
+    std::fprintf(fp, "\n");
+  }
+  std::fclose(fp);
+}
+
+// ===========================================================================
+// timing and diagnostics (op_lib_core.cpp:1067-1167)
+// ===========================================================================
+extern "C" void op_timers_core(double *cpu, double *et) {
+  *cpu = (double)clock() / CLOCKS_PER_SEC;
+  *et = wall_now();
+}
+extern "C" void op_timers(double *cpu, double *et) {
+  op2h_device_sync();
+  op_timers_core(cpu, et);
+}
+
+extern "C" void op_timing_realloc(int kernel) {
+  if (kernel < OP_kern_max) return;
+  int old = OP_kern_max;
+  OP_kern_max = kernel + 10;
+  OP_kernels = (op_kernel *)std::realloc(OP_kernels, OP_kern_max * sizeof(op_kernel));
+  for (int n = old; n < OP_kern_max; ++n) {
+    std::memset(&OP_kernels[n], 0, sizeof(op_kernel));
+    OP_kernels[n].name = "unused";
+    OP_kernels[n].times = (double *)std::calloc(1, sizeof(double));
+    OP_kernels[n].ntimes = 1;
+  }
+}
+
+extern "C" void op_timing_output(void) {
+  resolve_timers();
+  if (OP_kern_max == 0) return;
+  op_printf("\n  count   plan time     time(host)   kernel ms      GB/s      "
+            "kernel name \n");
+  op_printf(" -------------------------------------------------------------"
+            "-------------------\n");
+  for (int n = 0; n < OP_kern_max; ++n) {
+    if (OP_kernels[n].count == 0) continue;
+    auto it = rt().stats.find(n);
+    double kms = it == rt().stats.end() ? 0.0 : it->second.kernel_ms;
+    double secs = kms > 0 ? kms * 1e-3 : OP_kernels[n].time;
+    double gbs = secs > 0 ? OP_kernels[n].transfer / secs * 1e-9 : 0.0;
+    op_printf(" %6d;  %8.4f;  %10.4f;  %10.4f;  %8.2f;   %s \n", OP_kernels[n].count,
+              OP_kernels[n].plan_time, OP_kernels[n].time, kms, gbs,
+              OP_kernels[n].name);
+  }
+}
+
+extern "C" void op_printf(const char *format, ...) {
+  va_list ap;
+  va_start(ap, format);
+  vprintf(format, ap);
+  va_end(ap);
+}
+
+extern "C" void op_diagnostic_output(void) {
+  if (OP_diags <= 1) return;
+  printf("\n  OP diagnostic output\n");
+  printf("  --------------------\n");
+  printf("\n       set       size\n");
+  for (op_set s : rt().sets) printf("%10s %10d\n", s->name, s->size);
+  printf("\n       map        dim       from         to\n");
+  for (op_map m : rt().maps)
+    printf("%10s %10d %10s %10s\n", m->name, m->dim, m->from->name, m->to->name);
+  printf("\n       dat        dim        set     layout\n");
+  for (op_dat d : rt().dats)
+    if (d)
+      printf("%10s %10d %10s %10s\n", d->name, d->dim, d->set->name,
+             dat_is_soa(d) ? "soa" : "aos");
+  printf("\n");
+}
+
+extern "C" int op_is_root(void) { return 1; }
+extern "C" void op_rank(int *rank) { *rank = 0; }
+
+// ---- profiling C API: a flat per-name wall-clock tree (op_profile.h:197-212) ----
+extern "C" void op_profile_start(const char *name) { op_profile_enter(name); }
+extern "C" void op_profile_enter(const char *name) {
+  op2h_device_sync();
+  rt().prof_stack.push_back({name, wall_now()});
+}
+extern "C" void op_profile_enter_kernel(const char *name, const char *,
+                                        const char *) {
+  op_profile_enter(name);
+}
+extern "C" void op_profile_next(const char *name) {
+  op_profile_exit();
+  op_profile_enter(name);
+}
+extern "C" void op_profile_exit(void) {
+  Runtime &R = rt();
+  if (R.prof_stack.empty()) return;
+  op2h_device_sync();
+  auto top = R.prof_stack.back();
+  R.prof_stack.pop_back();
+  std::string path;
+  for (auto &p : R.prof_stack) path += p.first + "/";
+  R.prof_totals[path + top.first] += wall_now() - top.second;
+}
+extern "C" void op_profile_end(void) {
+  while (!rt().prof_stack.empty()) op_profile_exit();
+}
+extern "C" void op_profile_output(void) {
+  for (auto &kv : rt().prof_totals)
+    op_printf("%-40s %10.4f s\n", kv.first.c_str(), kv.second);
+}
+
+// ---- single-rank no-ops (op_dummy_singlenode.cpp:43-208) ----
+extern "C" void op_partition(const char *, const char *, op_set, op_map, op_dat) {}
+extern "C" void op_renumber(op_map) {}
+extern "C" int op_mpi_halo_exchanges(op_set set, int, op_arg *) { return set->size; }
+extern "C" int op_mpi_halo_exchanges_cuda(op_set set, int, op_arg *) { return set->size; }
+extern "C" int op_mpi_halo_exchanges_grouped(op_set set, int, op_arg *, int) {
+  return set->size;
+}
+extern "C" void op_mpi_set_dirtybit(int, op_arg *) {}
+extern "C" void op_mpi_set_dirtybit_cuda(int, op_arg *) {}
+extern "C" void op_mpi_wait_all(int, op_arg *) {}
+extern "C" void op_mpi_wait_all_cuda(int, op_arg *) {}
+extern "C" void op_mpi_wait_all_grouped(int, op_arg *, int) {}
+extern "C" void op_mpi_reduce_float(op_arg *, float *) {}
+extern "C" void op_mpi_reduce_double(op_arg *, double *) {}
+extern "C" void op_mpi_reduce_int(op_arg *, int *) {}
+extern "C" void op_mpi_reduce_bool(op_arg *, bool *) {}
+extern "C" void op_mpi_barrier(void) {}
+
+// ===========================================================================
+// backend controls
+// ===========================================================================
+extern "C" int op2h_set_loop_plan(const char *name, int plan, int block_size) {
+  if (plan < 0 || plan >= OP2H_PLAN_COUNT) return -1;
+  if (block_size < 0 || block_size > 1024 || block_size % 64) return -1;
+  LoopConfig c;
+  c.plan = plan;
+  c.block_size = block_size;
+  if (!name) {
+    rt().all_cfg = c;
+    rt().loop_cfg.clear();
+  } else {
+    rt().loop_cfg[name] = c;
+  }
+  return 0;
+}
+
+extern "C" int op2h_get_loop_plan(const char *name, int *plan, int *block_size) {
+  LoopConfig c = rt().all_cfg;
+  auto it = name ? rt().loop_cfg.find(name) : rt().loop_cfg.end();
+  if (it != rt().loop_cfg.end()) c = it->second;
+  if (plan) *plan = c.plan;
+  if (block_size) *block_size = c.block_size;
+  return 0;
+}
+
+extern "C" void op2h_set_sync_mode(int mode) { rt().sync_mode = mode; }
+extern "C" int op2h_get_sync_mode(void) { return rt().sync_mode; }
+extern "C" void op2h_set_plan_builder(int on_device) { rt().plan_on_device = on_device; }
+
+extern "C" void op2h_enable_kernel_timing(int on) { rt().kernel_timing = on != 0; }
+
+extern "C" int op2h_loop_stats(const char *name, int *calls, double *kernel_ms) {
+  resolve_timers();
+  for (int n = 0; n < OP_kern_max; ++n)
+    if (OP_kernels[n].count && !std::strcmp(OP_kernels[n].name, name)) {
+      auto it = rt().stats.find(n);
+      if (calls) *calls = it == rt().stats.end() ? 0 : it->second.calls;
+      if (kernel_ms) *kernel_ms = it == rt().stats.end() ? 0.0 : it->second.kernel_ms;
+      return 0;
+    }
+  return -1;
+}
+
+extern "C" void op2h_reset_stats(void) {
+  resolve_timers();
+  rt().stats.clear();
+  for (int n = 0; n < OP_kern_max; ++n) {
+    OP_kernels[n].count = 0;
+    OP_kernels[n].time = 0;
+    OP_kernels[n].transfer = 0;
+    OP_kernels[n].transfer2 = 0;
+  }
+}
+
+extern "C" void *op2h_stream(void) {
+  ensure_init();
+  return rt().stream;
+}
+
+extern "C" void op2h_device_sync(void) {
+  if (!rt().initialised || rt().capturing) return;
+  HIPCHECK(hipStreamSynchronize(rt().stream));
+}
+
+extern "C" const char *op2h_version(void) { return "op2hip 0.1 gfx950"; }
+
+// ===========================================================================
+// loop lifecycle
+// ===========================================================================
+extern "C" void op2h_loop_begin(op2h_loop_ctx *ctx, int kernel_idx,
+                                const char *name, op_set set, int nargs,
+                                op_arg *args, int def_plan, int def_block) {
+  Runtime &R = rt();
+  ensure_init();
+  std::memset(ctx, 0, sizeof(*ctx));
+  ctx->kernel_idx = kernel_idx;
+  ctx->name = name;
+  ctx->set = set;
+  ctx->nargs = nargs;
+  ctx->args = args;
+  ctx->stream = R.stream;
+  ctx->capturing = R.capturing ? 1 : 0;
+  op_timing_realloc(kernel_idx);
+  OP_kernels[kernel_idx].name = name;
+  OP_kernels[kernel_idx].count += 1;
+  ctx->wall0 = wall_now();
+  if (OP_diags > 0) {
+    int ninds = 0;
+    for (int m = 0; m < nargs; ++m) op_arg_check(set, m, args[m], &ninds, name);
+  }
+  if (OP_diags > 2) printf(" kernel routine: %s\n", name);
+  for (int m = 0; m < nargs; ++m) {
+    op_arg &a = args[m];
+    if (a.argtype == OP_ARG_DAT && a.opt && a.dat && a.dat->dirty_hd == 1) {
+      if (R.capturing) {
+        printf("op2hip: dat %s changed on the host during graph capture\n",
+               a.dat->name);
+        exit(-1);
+      }
+      upload_dat(a.dat);
+    }
+    if (a.argtype == OP_ARG_DAT && a.dat) a.data_d = a.dat->data_d;
+  }
+  LoopConfig c = R.all_cfg;
+  auto it = R.loop_cfg.find(name);
+  if (it != R.loop_cfg.end()) c = it->second;
+  ctx->plan = c.plan == OP2H_PLAN_AUTO ? def_plan : c.plan;
+  ctx->block_size = c.block_size ? c.block_size
+                                 : (OP_block_size > 0 && OP_block_size % 64 == 0 &&
+                                            OP_block_size != 256
+                                        ? OP_block_size
+                                        : def_block);
+  ctx->part_size = OP_part_size > 0 ? OP_part_size : ctx->block_size;
+}
+
+extern "C" void op2h_kernel_timer_start(op2h_loop_ctx *ctx) {
+  Runtime &R = rt();
+  if (!R.kernel_timing || R.capturing) return;
+  PendingTimer t{ctx->kernel_idx, take_event(), take_event()};
+  HIPCHECK(hipEventRecord(t.a, R.stream));
+  R.pending.push_back(t);
+}
+
+extern "C" void op2h_kernel_timer_stop(op2h_loop_ctx *ctx) {
+  Runtime &R = rt();
+  if (!R.kernel_timing || R.capturing || R.pending.empty()) return;
+  (void)ctx;
+  HIPCHECK(hipEventRecord(R.pending.back().b, R.stream));
+}
+
+extern "C" void op2h_loop_end(op2h_loop_ctx *ctx) {
+  Runtime &R = rt();
+  for (int m = 0; m < ctx->nargs; ++m) {
+    op_arg &a = ctx->args[m];
+    if (a.argtype == OP_ARG_DAT && a.opt && a.acc != OP_READ) a.dat->dirty_hd = 2;
+  }
+  // gbl results
+  bool need_sync = R.sync_mode == OP2H_SYNC_EVERY_LOOP;
+  for (int g = 0; g < ctx->ngbl; ++g) {
+    op_arg &a = ctx->args[ctx->gbl_arg[g]];
+    int info = -1;
+    for (int m = 0; m < ctx->nargs; ++m)
+      if (ctx->args[m].argtype == OP_ARG_INFO && ctx->args[m].opt) info = m;
+    bool with_info = (a.acc == OP_MIN || a.acc == OP_MAX) && info >= 0;
+    size_t bytes = (size_t)a.size + (with_info ? ctx->args[info].size : 0);
+    HIPCHECK(hipMemcpyAsync(ctx->gbl_slot_h[g], ctx->gbl_slot_d[g], bytes,
+                            hipMemcpyDeviceToHost, R.stream));
+    if (R.capturing) {
+      DeferredGbl d;
+      d.acc = a.acc;
+      d.type = type_code(a.type);
+      d.dim = a.dim;
+      d.host_ptr = a.data;
+      d.snapshot.assign(a.data, a.data + a.size);
+      d.info_ptr = with_info ? ctx->args[info].data : nullptr;
+      d.info_size = with_info ? ctx->args[info].size : 0;
+      if (with_info)
+        d.info_snapshot.assign(ctx->args[info].data,
+                               ctx->args[info].data + d.info_size);
+      d.slot_h = (char *)ctx->gbl_slot_h[g];
+      d.bytes = bytes;
+      R.deferred.push_back(std::move(d));
+      R.history_sources.push_back({ctx->gbl_slot_d[g], bytes});
+    } else {
+      need_sync = true;
+    }
+  }
+  if (!R.capturing) {
+    if (need_sync) HIPCHECK(hipStreamSynchronize(R.stream));
+    for (int g = 0; g < ctx->ngbl; ++g) {
+      op_arg &a = ctx->args[ctx->gbl_arg[g]];
+      int info = -1;
+      for (int m = 0; m < ctx->nargs; ++m)
+        if (ctx->args[m].argtype == OP_ARG_INFO && ctx->args[m].opt) info = m;
+      bool with_info = (a.acc == OP_MIN || a.acc == OP_MAX) && info >= 0;
+      const char *res = (const char *)ctx->gbl_slot_h[g];
+      combine_gbl(a.acc, type_code(a.type), a.dim, a.data, res,
+                  with_info ? ctx->args[info].data : nullptr,
+                  with_info ? res + a.size : nullptr,
+                  with_info ? ctx->args[info].size : 0);
+    }
+  }
+  double t1 = wall_now();
+  OP_kernels[ctx->kernel_idx].time += t1 - ctx->wall0;
+  OP_kernels[ctx->kernel_idx].times[0] += t1 - ctx->wall0;
+}
+
+extern "C" int op2h_dat_soa_stride(op_dat dat) {
+  return dat_is_soa(dat) ? dat->set->stride : 0;
+}
+
+extern "C" void *op2h_gbl_slot(op2h_loop_ctx *ctx, int argi, int info_argi) {
+  Runtime &R = rt();
+  if (ctx->ngbl >= OP2H_MAX_GBL) {
+    printf("op2hip: too many gbl reductions in loop %s\n", ctx->name);
+    exit(-1);
+  }
+  size_t bytes = (size_t)ctx->args[argi].size +
+                 (info_argi >= 0 ? (size_t)ctx->args[info_argi].size : 0);
+  bytes = (bytes + 63) & ~(size_t)63;
+  std::pair<void *, void *> slot;
+  if (bytes > 256) {
+    printf("op2hip: gbl argument too large (%zu bytes)\n", bytes);
+    exit(-1);
+  }
+  if (R.capturing) {
+    if (R.capture_slot_next >= (int)R.capture_slots.size()) {
+      printf("op2hip: too many gbl results in one captured graph\n");
+      exit(-1);
+    }
+    slot = R.capture_slots[R.capture_slot_next++];
+  } else {
+    auto key = std::make_pair(ctx->kernel_idx, argi);
+    auto it = R.slots.find(key);
+    if (it == R.slots.end()) {
+      HIPCHECK(hipMalloc(&slot.first, 256));
+      HIPCHECK(hipHostMalloc(&slot.second, 256, hipHostMallocDefault));
+      R.slots[key] = slot;
+    } else {
+      slot = it->second;
+    }
+  }
+  int g = ctx->ngbl++;
+  ctx->gbl_arg[g] = argi;
+  ctx->gbl_slot_d[g] = slot.first;
+  ctx->gbl_slot_h[g] = slot.second;
+  return slot.first;
+}
+
+extern "C" void *op2h_reduct_scratch(size_t bytes) {
+  Runtime &R = rt();
+  if (bytes > R.red_scratch_bytes) {
+    if (R.capturing) {
+      printf("op2hip: reduction scratch must be sized before graph capture\n");
+      exit(-1);
+    }
+    if (R.red_scratch) {
+      HIPCHECK(hipStreamSynchronize(R.stream));
+      HIPCHECK(hipFree(R.red_scratch));
+    }
+    size_t nb = std::max(bytes, (size_t)1 << 16);
+    HIPCHECK(hipMalloc(&R.red_scratch, nb));
+    R.red_scratch_bytes = nb;
+  }
+  return R.red_scratch;
+}
+
+extern "C" unsigned int *op2h_reduct_counter(void) { return rt().red_counter; }
+
+extern "C" double op2h_algorithmic_bytes(op_set set, int nargs,
+                                         const op_arg *args) {
+  std::map<op_map, long long> &distinct = rt().distinct_targets;
+  double bytes = 0.0;
+  std::vector<op_dat> seen;
+  std::vector<std::pair<op_map, int>> seen_maps;
+  for (int m = 0; m < nargs; ++m) {
+    const op_arg &a = args[m];
+    if (a.argtype != OP_ARG_DAT || !a.opt || !a.dat) continue;
+    if (a.map && a.idx >= 0 &&
+        std::find(seen_maps.begin(), seen_maps.end(), std::make_pair(a.map, a.idx)) ==
+            seen_maps.end()) {
+      seen_maps.push_back({a.map, a.idx});
+      bytes += 4.0 * a.map->from->size;
+    }
+    if (std::find(seen.begin(), seen.end(), a.dat) != seen.end()) continue;
+    seen.push_back(a.dat);
+    double factor = (a.acc == OP_INC || a.acc == OP_RW) ? 2.0 : 1.0;
+    long long count = set->size;
+    if (a.map) {
+      auto it = distinct.find(a.map);
+      if (it == distinct.end()) {
+        std::vector<char> hit(a.map->to->size, 0);
+        long long c = 0;
+        for (size_t i = 0; i < (size_t)a.map->from->size * a.map->dim; ++i)
+          if (!hit[a.map->map[i]]) {
+            hit[a.map->map[i]] = 1;
+            ++c;
+          }
+        it = distinct.emplace(a.map, c).first;
+      }
+      count = it->second;
+    }
+    bytes += factor * (double)count * a.dat->size;
+  }
+  return bytes;
+}
+
+// ===========================================================================
+// plans
+// ===========================================================================
+namespace {
+std::string arg_signature(const char *name, op_set set, int nargs,
+                          const op_arg *args, int part) {
+  std::string s = std::string(name) + "|" + std::to_string(set->index) + "|" +
+                  std::to_string(part);
+  for (int m = 0; m < nargs; ++m) {
+    const op_arg &a = args[m];
+    s += "|" + std::to_string(a.argtype) + "," +
+         std::to_string(a.dat ? a.dat->index : -1) + "," +
+         std::to_string(a.map ? a.map->index : -1) + "," + std::to_string(a.idx) +
+         "," + std::to_string(a.acc) + "," + std::to_string(a.opt);
+  }
+  return s;
+}
+}  // namespace
+
+extern "C" const op2h_staged_plan *op2h_plan_staged_for(const char *name,
+                                                        op_set set,
+                                                        int part_size, int nargs,
+                                                        op_arg *args) {
+  Runtime &R = rt();
+  std::string sig = arg_signature(name, set, nargs, args, part_size);
+  for (auto &p : R.staged)
+    if (p->signature == sig) return &p->plan;
+  if (R.capturing) {
+    printf("op2hip: plan for %s must be built before graph capture\n", name);
+    exit(-1);
+  }
+  double t0 = wall_now();
+  auto rec = op2h::build_staged_plan(set, part_size, nargs, args,
+                                     R.plan_on_device, R.stream);
+  rec->signature = sig;
+  rec->plan.build_ms = (wall_now() - t0) * 1e3;
+  R.staged.push_back(std::move(rec));
+  return &R.staged.back()->plan;
+}
+
+extern "C" const op2h_staged_plan *op2h_plan_staged(op2h_loop_ctx *ctx) {
+  const op2h_staged_plan *p = op2h_plan_staged_for(
+      ctx->name, ctx->set, ctx->part_size, ctx->nargs, ctx->args);
+  OP_kernels[ctx->kernel_idx].plan_time += 0.0f;
+  return p;
+}
+
+extern "C" const op2h_color_plan *op2h_plan_color_for(const char *name,
+                                                      op_set set, int nargs,
+                                                      op_arg *args) {
+  Runtime &R = rt();
+  std::string sig = arg_signature(name, set, nargs, args, -1);
+  for (auto &p : R.color)
+    if (p->signature == sig) return &p->plan;
+  if (R.capturing) {
+    printf("op2hip: plan for %s must be built before graph capture\n", name);
+    exit(-1);
+  }
+  double t0 = wall_now();
+  auto rec = op2h::build_color_plan(set, nargs, args, R.plan_on_device, R.stream);
+  rec->signature = sig;
+  rec->plan.build_ms = (wall_now() - t0) * 1e3;
+  R.color.push_back(std::move(rec));
+  return &R.color.back()->plan;
+}
+
+extern "C" const op2h_color_plan *op2h_plan_color(op2h_loop_ctx *ctx) {
+  return op2h_plan_color_for(ctx->name, ctx->set, ctx->nargs, ctx->args);
+}
+
+extern "C" const op2h_gather_plan *op2h_plan_gather(op_map map) {
+  Runtime &R = rt();
+  auto it = R.gather.find(map);
+  if (it != R.gather.end()) return &it->second->plan;
+  if (R.capturing) {
+    printf("op2hip: plan for map %s must be built before graph capture\n",
+           map->name);
+    exit(-1);
+  }
+  double t0 = wall_now();
+  auto rec = op2h::build_gather_plan(map, R.stream);
+  rec->plan.build_ms = (wall_now() - t0) * 1e3;
+  auto *p = &rec->plan;
+  R.gather[map] = std::move(rec);
+  return p;
+}
+
+extern "C" int op2h_staged_plan_download(const op2h_staged_plan *p, int *thrcol,
+                                         int *blkcol, int *offset, int *nelems) {
+  return op2h::download_staged(p, thrcol, blkcol, offset, nelems);
+}
+
+extern "C" int op2h_color_plan_download(const op2h_color_plan *p,
+                                        int *color_of_elem) {
+  return op2h::download_color(p, color_of_elem);
+}
+
+// ===========================================================================
+// graph capture
+// ===========================================================================
+namespace {
+constexpr int kSlotPool = 64;
+constexpr int kHistSources = 16;
+constexpr size_t kHistoryBytes = 64u << 20;
+
+struct HistSources {
+  const unsigned long long *p[kHistSources];
+  int words[kHistSources];
+  int n;
+};
+
+// Appends the gbl result slots recorded in a graph to the device history:
+// record r of the history holds all slot words of launch r, in record order.
+// One workgroup; the launch counter is read by every lane before the barrier
+// and advanced by lane 0 after it.
+__global__ void history_append(HistSources src, double *hist,
+                               unsigned long long *count, int capacity,
+                               int total_words) {
+  unsigned long long c = *count;
+  size_t base = (size_t)(c % (unsigned long long)capacity) * total_words;
+  int w = 0;
+  for (int s = 0; s < src.n; ++s) {
+    for (int k = threadIdx.x; k < src.words[s]; k += blockDim.x)
+      hist[base + w + k] = __longlong_as_double((long long)src.p[s][k]);
+    w += src.words[s];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *count = c + 1;
+}
+}  // namespace
+
+extern "C" int op2h_graph_begin(void) {
+  Runtime &R = rt();
+  ensure_init();
+  if (R.capturing) return -1;
+  op2h_graph_destroy();
+  resolve_timers();
+  HIPCHECK(hipStreamSynchronize(R.stream));
+  R.deferred.clear();
+  R.history_sources.clear();
+  // everything a capture needs is allocated before capture starts
+  for (int i = 0; i < kSlotPool; ++i) {
+    std::pair<void *, void *> slot;
+    HIPCHECK(hipMalloc(&slot.first, 256));
+    HIPCHECK(hipHostMalloc(&slot.second, 256, hipHostMallocDefault));
+    R.capture_slots.push_back(slot);
+  }
+  R.capture_slot_next = 0;
+  HIPCHECK(hipMalloc(&R.history, kHistoryBytes));
+  HIPCHECK(hipMalloc(&R.history_count, sizeof(unsigned long long)));
+  HIPCHECK(hipMemset(R.history_count, 0, sizeof(unsigned long long)));
+  HIPCHECK(hipStreamSynchronize(R.stream));
+  HIPCHECK(hipDeviceSynchronize());
+  HIPCHECK(hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal));
+  R.capturing = true;
+  return 0;
+}
+
+extern "C" int op2h_graph_end(void) {
+  Runtime &R = rt();
+  if (!R.capturing) return -1;
+  size_t words = 0;
+  HistSources src;
+  std::memset(&src, 0, sizeof(src));
+  if ((int)R.history_sources.size() > kHistSources) {
+    printf("op2hip: more than %d gbl results in one captured graph\n",
+           kHistSources);
+    exit(-1);
+  }
+  for (auto &s : R.history_sources) {
+    src.p[src.n] = (const unsigned long long *)s.first;
+    src.words[src.n] = (int)((s.second + 7) / 8);
+    words += src.words[src.n];
+    src.n++;
+  }
+  R.history_words = (int)words;
+  R.history_capacity =
+      words ? std::max(1, (int)(kHistoryBytes / (words * sizeof(double)))) : 0;
+  if (words)
+    history_append<<<1, 64, 0, R.stream>>>(src, R.history, R.history_count,
+                                           R.history_capacity, (int)words);
+  hipGraph_t g;
+  HIPCHECK(hipStreamEndCapture(R.stream, &g));
+  R.capturing = false;
+  R.graph = g;
+  HIPCHECK(hipGraphInstantiate(&R.graph_exec, g, nullptr, nullptr, 0));
+  return 0;
+}
+
+extern "C" int op2h_graph_launch(int count) {
+  Runtime &R = rt();
+  if (!R.graph_exec || R.capturing) return -1;
+  for (int i = 0; i < count; ++i) HIPCHECK(hipGraphLaunch(R.graph_exec, R.stream));
+  return 0;
+}
+
+extern "C" int op2h_graph_sync(void) {
+  Runtime &R = rt();
+  if (R.capturing) return -1;
+  HIPCHECK(hipStreamSynchronize(R.stream));
+  for (auto &d : R.deferred) {
+    std::memcpy(d.host_ptr, d.snapshot.data(), d.snapshot.size());
+    if (d.info_ptr)
+      std::memcpy(d.info_ptr, d.info_snapshot.data(), d.info_snapshot.size());
+    combine_gbl(d.acc, d.type, d.dim, d.host_ptr, d.slot_h, d.info_ptr,
+                d.info_ptr ? d.slot_h + d.snapshot.size() : nullptr, d.info_size);
+  }
+  return 0;
+}
+
+extern "C" int op2h_graph_history(int max_records, double *out, int *nrecords) {
+  Runtime &R = rt();
+  if (nrecords) *nrecords = 0;
+  if (!R.history || !R.history_words || R.capturing) return 0;
+  HIPCHECK(hipStreamSynchronize(R.stream));
+  unsigned long long count = 0;
+  HIPCHECK(hipMemcpy(&count, R.history_count, sizeof(count), hipMemcpyDeviceToHost));
+  unsigned long long avail = std::min<unsigned long long>(count, R.history_capacity);
+  unsigned long long n = std::min<unsigned long long>(avail, (unsigned long long)std::max(max_records, 0));
+  for (unsigned long long r = 0; r < n; ++r) {
+    unsigned long long rec = count - n + r;
+    size_t pos = (size_t)(rec % R.history_capacity) * R.history_words;
+    HIPCHECK(hipMemcpy(out + r * R.history_words, R.history + pos,
+                       R.history_words * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (nrecords) *nrecords = (int)n;
+  return R.history_words;
+}
+
+extern "C" void op2h_graph_destroy(void) {
+  Runtime &R = rt();
+  if (!R.initialised) return;
+  if (R.graph_exec || R.history) HIPCHECK(hipStreamSynchronize(R.stream));
+  if (R.graph_exec) (void)hipGraphExecDestroy(R.graph_exec);
+  if (R.graph) (void)hipGraphDestroy(R.graph);
+  R.graph_exec = nullptr;
+  R.graph = nullptr;
+  for (auto &s : R.capture_slots) {
+    (void)hipFree(s.first);
+    (void)hipHostFree(s.second);
+  }
+  R.capture_slots.clear();
+  R.capture_slot_next = 0;
+  R.deferred.clear();
+  R.history_sources.clear();
+  if (R.history) (void)hipFree(R.history);
+  if (R.history_count) (void)hipFree(R.history_count);
+  R.history = nullptr;
+  R.history_count = nullptr;
+  R.history_words = 0;
+  R.history_capacity = 0;
+}

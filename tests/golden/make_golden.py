@@ -1,0 +1,132 @@
+"""Writes the golden fixtures of tests/golden/ from the REFERENCE's own code.
+
+Run in the development container only (it needs /root/reference):
+    python tests/golden/make_golden.py
+
+It compiles the reference's sequential executor and airfoil app from their
+sources (oracle/Makefile, target `ref`), generates the synthetic meshes with
+this library's generator, runs
+  - oracle/_ref/ref_harness  (reference op_seq.h executor + reference airfoil
+    kernels, per-iteration rms/maxerr/errloc at full precision, final q, and
+    the state after the first loop calls),
+  - oracle/_ref/airfoil_seq  (the unmodified reference airfoil.cpp),
+  - oracle/_ref/ref_plan_dump (the reference's op_plan_core(OP_STAGE_ALL)),
+and stores their outputs as .npz fixtures.  The fixtures are data only.
+"""
+import hashlib
+import os
+import pathlib
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+REPO = HERE.parent.parent
+sys.path.insert(0, str(REPO / "op2-common_amd"))
+
+import op2amd  # noqa: E402
+
+REF = REPO / "oracle" / "_ref"
+
+
+def run(cmd, cwd):
+    subprocess.run(cmd, cwd=cwd, check=True, stdout=subprocess.DEVNULL)
+
+
+def harness_case(name, I, J, ordering, niter, alpha_mode, keep_first):
+    mesh = op2amd.mesh.generate(I, J, 0, ordering)
+    with tempfile.TemporaryDirectory() as d:
+        grid = pathlib.Path(d) / "new_grid.dat"
+        mesh.write(grid)
+        sha = hashlib.sha256(grid.read_bytes()).hexdigest()
+        run([str(REF / "ref_harness"), str(grid), str(niter), str(alpha_mode), "out"], d)
+        hist = np.loadtxt(pathlib.Path(d) / "out_hist.txt", ndmin=2)
+        q = np.fromfile(pathlib.Path(d) / "out_q.bin", dtype=np.float64).reshape(-1, 4)
+        arrays = dict(
+            I=I, J=J, ordering=ordering, niter=niter, alpha_mode=alpha_mode,
+            grid_sha256=sha,
+            rms_k0_raw=hist[:, 1], rms_raw=hist[:, 2], rms=hist[:, 3],
+            maxerr=hist[:, 4], errloc=hist[:, 5].astype(np.int32), q=q)
+        if keep_first:
+            nc = mesh.ncell
+            first = np.fromfile(pathlib.Path(d) / "out_first.bin", dtype=np.float64)
+            arrays["first_adt"] = first[:nc]
+            arrays["first_res_after_res"] = first[nc:5 * nc].reshape(nc, 4)
+            arrays["first_res_after_bres"] = first[5 * nc:9 * nc].reshape(nc, 4)
+            arrays["first_q_after_update"] = first[9 * nc:13 * nc].reshape(nc, 4)
+    np.savez_compressed(HERE / f"{name}.npz", **arrays)
+    print("wrote", name, "sha", sha[:12], "rms[-1]", arrays["rms"][-1])
+
+
+def unmodified_app_case(name, I, J, ordering):
+    """The unmodified reference app (alpha unused, 1000 iterations fixed)."""
+    mesh = op2amd.mesh.generate(I, J, 0, ordering)
+    with tempfile.TemporaryDirectory() as d:
+        mesh.write(pathlib.Path(d) / "new_grid.dat")
+        run([str(REF / "airfoil_seq")], d)
+        raw = (pathlib.Path(d) / "out_grid_seq.bin").read_bytes()
+        count, dim = np.frombuffer(raw[:8], dtype=np.int32)
+        q = np.frombuffer(raw[8:], dtype=np.float64).reshape(count, dim)
+    np.savez_compressed(HERE / f"{name}.npz", I=I, J=J, ordering=ordering,
+                        niter=1000, alpha_mode=0, q=q)
+    print("wrote", name)
+
+
+def plan_case(name, I, J, ordering, part_size):
+    mesh = op2amd.mesh.generate(I, J, 0, ordering)
+    out = {"I": I, "J": J, "ordering": ordering, "part_size": part_size}
+    with tempfile.TemporaryDirectory() as d:
+        mesh.write(pathlib.Path(d) / "new_grid.dat")
+        run([str(REF / "ref_plan_dump"), "new_grid.dat", str(part_size), "plan.bin"], d)
+        raw = np.fromfile(pathlib.Path(d) / "plan.bin", dtype=np.int32)
+    pos = 0
+
+    def take(k):
+        nonlocal pos
+        v = raw[pos:pos + k].copy()
+        pos += k
+        return v
+    for loop, nargs, staged in (("adt_calc", 6, [0, 1, 2, 3]),
+                                ("res_calc", 8, list(range(8))),
+                                ("bres_calc", 6, [0, 1, 2, 3, 4])):
+        set_size, nblocks, ncolors, _ = take(4)
+        out[f"{loop}_offset"] = take(nblocks)
+        out[f"{loop}_nelems"] = take(nblocks)
+        out[f"{loop}_nthrcol"] = take(nblocks)
+        out[f"{loop}_thrcol"] = take(set_size)
+        out[f"{loop}_ncolblk"] = take(ncolors)
+        out[f"{loop}_blkmap"] = take(nblocks)
+        ninds = int(take(1)[0])
+        nind = take(ninds)
+        out[f"{loop}_nindirect"] = nind
+        out[f"{loop}_ind_sizes"] = take(nblocks * ninds)
+        out[f"{loop}_ind_offs"] = take(nblocks * ninds)
+        for i in range(ninds):
+            out[f"{loop}_ind_map{i}"] = take(int(nind[i]))
+        for m in staged:
+            out[f"{loop}_loc_map{m}"] = take(set_size)
+        out[f"{loop}_ncolors"] = ncolors
+        out[f"{loop}_nblocks"] = nblocks
+    assert pos == raw.size, (pos, raw.size)
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+    print("wrote", name)
+
+
+def main():
+    if not pathlib.Path("/root/reference").exists():
+        sys.exit("the reference tree is not available here")
+    subprocess.run(["make", "-s", "-C", str(REPO / "oracle"), "ref"], check=True)
+    harness_case("airfoil_120x60_random_a1", 120, 60, "random", 1000, 1, False)
+    harness_case("airfoil_120x60_rcm_a1", 120, 60, "rcm", 1000, 1, False)
+    harness_case("airfoil_24x12_random_a1", 24, 12, "random", 300, 1, True)
+    harness_case("airfoil_24x12_structured_a0", 24, 12, "structured", 50, 0, False)
+    unmodified_app_case("airfoil_24x12_random_app_a0", 24, 12, "random")
+    plan_case("plan_60x30_random_p0", 60, 30, "random", 0)
+    plan_case("plan_60x30_rcm_p64", 60, 30, "rcm", 64)
+
+
+if __name__ == "__main__":
+    main()
