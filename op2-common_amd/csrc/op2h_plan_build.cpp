@@ -218,24 +218,39 @@ std::unique_ptr<StagedPlanRec> build_staged_plan(op_set set, int part_size,
                                                            sh.locs[l].second]];
     }
   }
+  // staging lists and partition go to the device first: the device
+  // colouring reads them
+  P.offset = rec->mem.upload(offset, stream);
+  P.nelems = rec->mem.upload(nelems, stream);
+  for (int s = 0; s < nm; ++s) {
+    P.ind_offs[s] = rec->mem.upload(ind_offs[s], stream);
+    P.ind_sizes[s] = rec->mem.upload(ind_sizes[s], stream);
+    P.ind_map[s] = rec->mem.upload(ind_map[s], stream);
+  }
+  for (int l = 0; l < nl; ++l) P.loc_map[l] = rec->mem.upload(loc_map[l], stream);
   // colouring
-  std::vector<int> thrcol_i(std::max(n, 1)), nthrcol(nblocks), blkcol(nblocks);
-  std::vector<unsigned char> thrcol(std::max(n, 1));
-  int ncolors = 0;
+  std::vector<int> nthrcol(nblocks), blkcol(nblocks);
+  int ncolors = -1;
   unsigned char *thrcol_d = rec->mem.alloc<unsigned char>(std::max(n, 1));
-  int *nthrcol_d = nullptr;
+  int *nthrcol_d = rec->mem.alloc<int>(std::max(nblocks, 1));
   if (on_device) {
     int *blkcol_d = rec->mem.alloc<int>(std::max(nblocks, 1));
-    nthrcol_d = rec->mem.alloc<int>(std::max(nblocks, 1));
-    ncolors = gpu_colour_blocks_and_threads(sh, nblocks, bs, blkcol_d, thrcol_d,
+    ncolors = gpu_colour_blocks_and_threads(sh, nblocks, bs, P, blkcol_d, thrcol_d,
                                             nthrcol_d, stream);
-    PCHECK(hipMemcpyAsync(blkcol.data(), blkcol_d, nblocks * sizeof(int),
-                          hipMemcpyDeviceToHost, stream));
-    PCHECK(hipMemcpyAsync(nthrcol.data(), nthrcol_d, nblocks * sizeof(int),
-                          hipMemcpyDeviceToHost, stream));
-    PCHECK(hipStreamSynchronize(stream));
-    P.built_on_device = 1;
-  } else {
+    if (ncolors >= 0) {
+      PCHECK(hipMemcpyAsync(blkcol.data(), blkcol_d, nblocks * sizeof(int),
+                            hipMemcpyDeviceToHost, stream));
+      PCHECK(hipMemcpyAsync(nthrcol.data(), nthrcol_d, nblocks * sizeof(int),
+                            hipMemcpyDeviceToHost, stream));
+      PCHECK(hipStreamSynchronize(stream));
+      for (int b = 0; b < nblocks; ++b)
+        if (nthrcol[b] > 255) ncolors = -1;
+      P.built_on_device = ncolors >= 0;
+    }
+  }
+  if (ncolors < 0) {
+    std::vector<int> thrcol_i(std::max(n, 1));
+    std::vector<unsigned char> thrcol(std::max(n, 1));
     std::vector<std::vector<unsigned>> work(sh.dat_to_size.size());
     for (size_t d = 0; d < work.size(); ++d)
       work[d].assign(std::max(sh.dat_to_size[d], 1), 0u);
@@ -251,7 +266,10 @@ std::unique_ptr<StagedPlanRec> build_staged_plan(op_set set, int part_size,
     ncolors = greedy_block_colours(sh, nblocks, offset.data(), nelems.data(), work,
                                    blkcol.data());
     PCHECK(hipMemcpyAsync(thrcol_d, thrcol.data(), n, hipMemcpyHostToDevice, stream));
-    nthrcol_d = rec->mem.upload(nthrcol, stream);
+    PCHECK(hipMemcpyAsync(nthrcol_d, nthrcol.data(), nblocks * sizeof(int),
+                          hipMemcpyHostToDevice, stream));
+    PCHECK(hipStreamSynchronize(stream));
+    P.built_on_device = 0;
   }
   P.ncolors = ncolors;
   rec->ncolblk_h.assign(std::max(ncolors, 1), 0);
@@ -268,16 +286,8 @@ std::unique_ptr<StagedPlanRec> build_staged_plan(op_set set, int part_size,
   P.colstart_h = rec->colstart_h.data();
   P.max_nthrcol_h = rec->max_nthrcol_h.data();
   P.blkmap = rec->mem.upload(blkmap, stream);
-  P.offset = rec->mem.upload(offset, stream);
-  P.nelems = rec->mem.upload(nelems, stream);
   P.nthrcol = nthrcol_d;
   P.thrcol = thrcol_d;
-  for (int s = 0; s < nm; ++s) {
-    P.ind_offs[s] = rec->mem.upload(ind_offs[s], stream);
-    P.ind_sizes[s] = rec->mem.upload(ind_sizes[s], stream);
-    P.ind_map[s] = rec->mem.upload(ind_map[s], stream);
-  }
-  for (int l = 0; l < nl; ++l) P.loc_map[l] = rec->mem.upload(loc_map[l], stream);
   PCHECK(hipStreamSynchronize(stream));
   return rec;
 }
@@ -293,14 +303,18 @@ std::unique_ptr<ColorPlanRec> build_color_plan(op_set set, int nargs,
   P.set_size = n;
   std::vector<int> col(std::max(n, 1));
   int ncolors = 0;
+  ncolors = -1;
   if (on_device) {
     int *col_d = rec->mem.alloc<int>(std::max(n, 1));
     ncolors = gpu_colour_elements(sh, n, col_d, stream);
-    PCHECK(hipMemcpyAsync(col.data(), col_d, n * sizeof(int), hipMemcpyDeviceToHost,
-                          stream));
-    PCHECK(hipStreamSynchronize(stream));
-    P.built_on_device = 1;
-  } else {
+    if (ncolors >= 0) {
+      PCHECK(hipMemcpyAsync(col.data(), col_d, n * sizeof(int),
+                            hipMemcpyDeviceToHost, stream));
+      PCHECK(hipStreamSynchronize(stream));
+      P.built_on_device = 1;
+    }
+  }
+  if (ncolors < 0) {
     // OP_COLOR2: one block spanning the set, greedy thread colouring
     std::vector<std::vector<unsigned>> work(sh.dat_to_size.size());
     for (size_t d = 0; d < work.size(); ++d)

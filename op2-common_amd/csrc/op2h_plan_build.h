@@ -73,15 +73,18 @@ std::unique_ptr<ColorPlanRec> build_color_plan(op_set set, int nargs,
                                                hipStream_t stream);
 std::unique_ptr<GatherPlanRec> build_gather_plan(op_map map, hipStream_t stream);
 
-// GPU colouring (op2h_plan_gpu.hip).  Colours elements [0,n) so that no two
-// elements sharing an increment target get the same colour, on the device;
-// returns the number of colours and leaves colours in `col_d`.
+// GPU colouring (op2h_plan_gpu.hip).  gpu_colour_elements colours elements
+// [0,n) so that no two elements sharing an increment target get the same
+// colour; gpu_colour_blocks_and_threads colours the threads of each plan block
+// and the blocks, from the plan's uploaded staging lists.  Both return the
+// number of colours, or -1 when the device path does not apply (the caller
+// then colours on the host).
 int gpu_colour_elements(const LoopShape &sh, int n, int *col_d,
                         hipStream_t stream);
 int gpu_colour_blocks_and_threads(const LoopShape &sh, int nblocks,
-                                  int part_size, int *blkcol_d,
-                                  unsigned char *thrcol_d, int *nthrcol_d,
-                                  hipStream_t stream);
+                                  int part_size, const op2h_staged_plan &P,
+                                  int *blkcol_d, unsigned char *thrcol_d,
+                                  int *nthrcol_d, hipStream_t stream);
 
 int download_staged(const op2h_staged_plan *p, int *thrcol, int *blkcol,
                     int *offset, int *nelems);
