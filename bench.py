@@ -1,20 +1,22 @@
 """Airfoil benchmark: steps/s at 720k cells, tuned + graphed vs plain HIP.
 
 One step is one airfoil time step (save_soln, then twice adt_calc, res_calc,
-bres_calc, update with the rms reduction) over the whole mesh, FP64, with all
-data resident in HBM.
+bres_calc, update with the rms reduction) over the whole 1200x600 mesh
+(720,000 cells, 1,438,200 edges), FP64, with all data resident in HBM.
 
   plain    the reference's execution model on this backend: LDS-staged,
            two-level-coloured indirect loops, device sync and gbl read-back
            after every loop;
-  tuned    per-loop plan and block size picked by timing each variant on the
+  tuned    per-loop plan and block size picked by timing every variant on the
            same mesh (exhaustive sweep);
   graphed  the tuned plans with each time step replayed from one HIP graph,
            rms/maxerr/errloc read back from a device history at the end.
 
-The headline value is the graphed steps/s.  Parity: the graphed rms
-trajectory against the plain one (every iteration) and against the oracle's
-OpenMP run for the first iterations, each as max relative error.
+Two input orderings are measured (both named by the workload): the headline
+uses the RCM-renumbered mesh; the randomly numbered mesh is reported beside
+it, as given and after the runtime's op_renumber.  Parity: the graphed rms
+trajectory against the plain one at every iteration, and against the
+oracle's OpenMP run for the first iterations (max relative error).
 """
 import argparse
 import json
@@ -31,6 +33,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "airfoil steps/s, 720k cells: tuned/graphed vs plain HIP at rms rel err ≤1e-10"
+PLANS = ("atomic", "color", "staged", "gather")
 
 
 def log(msg):
@@ -38,8 +41,84 @@ def log(msg):
 
 
 def rel(a, b):
-    a, b = np.asarray(a), np.asarray(b)
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
     return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+def sweep(op2amd, mesh, iters, renumber):
+    """Exhaustive sweep: per indirect loop, every plan x block size, timed
+    per call with HIP events; then the block size of the direct loops."""
+    best = {}
+    for loop in ("res_calc", "bres_calc"):
+        times = {}
+        for plan in PLANS:
+            for bs in (128, 256, 512):
+                kw = {"plan_res": "atomic", "plan_bres": "atomic"}
+                kw["plan_res" if loop == "res_calc" else "plan_bres"] = plan
+                kw["block_res" if loop == "res_calc" else "block_bres"] = bs
+                r = op2amd.airfoil.run(mesh, iters, mode="tuned", warmup=2,
+                                       kernel_timing=True, renumber=renumber, **kw)
+                times[(plan, bs)] = r["loop_kernel_ms"][loop] / max(r["loop_calls"][loop], 1)
+        best[loop] = min(times, key=times.get)
+        log(f"  sweep {loop}: " + ", ".join(
+            f"{p}/{b}={t * 1e3:.1f}us" for (p, b), t in sorted(times.items(), key=lambda kv: kv[1])))
+    direct = {}
+    for bs in (128, 256, 512, 1024):
+        r = op2amd.airfoil.run(mesh, iters, mode="tuned", warmup=2, kernel_timing=True,
+                               renumber=renumber, plan_res=best["res_calc"][0],
+                               block_res=best["res_calc"][1], block_direct=bs, block_adt=bs)
+        direct[bs] = sum(r["loop_kernel_ms"][lp] / max(r["loop_calls"][lp], 1)
+                         for lp in ("save_soln", "adt_calc", "update"))
+    bd = min(direct, key=direct.get)
+    return dict(plan_res=best["res_calc"][0], block_res=best["res_calc"][1],
+                plan_bres=best["bres_calc"][0], block_bres=best["bres_calc"][1],
+                block_direct=bd, block_adt=bd)
+
+
+def parse_plans(text):
+    # "res=color:256,bres=atomic:128,direct=1024"
+    kw = {}
+    for part in text.split(","):
+        k, v = part.split("=")
+        if k in ("res", "bres"):
+            p, b = v.split(":")
+            kw[f"plan_{k}"] = p
+            kw[f"block_{k}"] = int(b)
+        elif k == "direct":
+            kw["block_direct"] = kw["block_adt"] = int(v)
+    return kw
+
+
+def measure(op2amd, mesh, K, W, args, renumber=False, plans=None, plain_too=True):
+    n = K + W
+    out = {}
+    if plain_too:
+        plain = op2amd.airfoil.run(mesh, n, mode="plain", warmup=W, renumber=renumber)
+        out["plain_steps_per_s"] = K / plain["seconds"]
+        log(f"  plain: {out['plain_steps_per_s']:.1f} steps/s")
+    kw = plans or sweep(op2amd, mesh, args.sweep_iters, renumber)
+    out["tuned_plans"] = kw
+    graphed = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, renumber=renumber, **kw)
+    out["graphed_steps_per_s"] = K / graphed["seconds"]
+    log(f"  graphed {kw}: {out['graphed_steps_per_s']:.1f} steps/s")
+    if plain_too:
+        out["speedup_vs_plain"] = out["graphed_steps_per_s"] / out["plain_steps_per_s"]
+        out["rms_rel_err_graphed_vs_plain"] = rel(graphed["rms"], plain["rms"])
+        out["q_rel_err_graphed_vs_plain"] = float(
+            np.max(np.abs(graphed["q"] - plain["q"])) / np.max(np.abs(plain["q"])))
+    timed = op2amd.airfoil.run(mesh, min(K, 100) + 2, mode="tuned", warmup=2,
+                               kernel_timing=True, renumber=renumber, **kw)
+    per_loop = {}
+    for lp in op2amd.airfoil.LOOPS:
+        us = 1e3 * timed["loop_kernel_ms"][lp] / max(timed["loop_calls"][lp], 1)
+        b = timed["loop_bytes"][lp]
+        per_loop[lp] = {"avg_us": round(us, 3), "bytes_per_call": b,
+                        "achieved_GBs": round(b / (us * 1e-6) / 1e9, 1) if us > 0 else 0.0}
+    out["per_loop"] = per_loop
+    out["step_bytes"] = sum(timed["loop_bytes"][lp] * (1 if lp == "save_soln" else 2)
+                            for lp in op2amd.airfoil.LOOPS)
+    out["_graphed"] = graphed
+    return out
 
 
 def main():
@@ -48,7 +127,11 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--mesh", default="1200x600")
-    ap.add_argument("--ordering", default="random", choices=["random", "rcm"])
+    ap.add_argument("--ordering", default="rcm", choices=["random", "rcm"],
+                    help="input ordering of the headline measurement")
+    ap.add_argument("--plans", default=None,
+                    help="skip the sweep, e.g. res=color:256,bres=atomic:128,direct=1024")
+    ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-iters", type=int, default=6)
     args = ap.parse_args()
@@ -58,87 +141,62 @@ def main():
         sys.exit(2)
 
     import op2amd
-    I, J = (int(v) for v in args.mesh.split("x"))
-    t0 = time.time()
-    mesh = op2amd.mesh.generate(I, J, 0, args.ordering)
-    log(f"mesh {I}x{J} {args.ordering}: {mesh.ncell} cells, {mesh.nedge} edges "
-        f"({time.time() - t0:.1f}s)")
-    K, W = args.steps, args.warmup
-    n = K + W
-
-    # ---- plain HIP -------------------------------------------------------
-    plain = op2amd.airfoil.run(mesh, n, mode="plain", warmup=W)
-    plain_sps = K / plain["seconds"]
-    log(f"plain: {plain_sps:.1f} steps/s")
-
-    # ---- exhaustive sweep of indirect-loop plans (kernel time per call) ----
-    cands = [(p, b) for p in ("atomic", "color", "staged", "gather") for b in (128, 256, 512)]
-    best = {}
-    for loop in ("res_calc", "bres_calc"):
-        times = {}
-        for plan, bs in cands:
-            kw = {"plan_res": "atomic", "plan_bres": "atomic"}
-            kw["plan_res" if loop == "res_calc" else "plan_bres"] = plan
-            kw["block_res" if loop == "res_calc" else "block_bres"] = bs
-            r = op2amd.airfoil.run(mesh, args.sweep_iters, mode="tuned", warmup=2,
-                                   kernel_timing=True, **kw)
-            times[(plan, bs)] = r["loop_kernel_ms"][loop] / max(r["loop_calls"][loop], 1)
-        best[loop] = min(times, key=times.get)
-        log(f"sweep {loop}: " + ", ".join(f"{p}/{b}={t * 1e3:.1f}us" for (p, b), t in
-                                          sorted(times.items(), key=lambda kv: kv[1])))
-    direct_times = {}
-    for bs in (128, 256, 512, 1024):
-        r = op2amd.airfoil.run(mesh, args.sweep_iters, mode="tuned", warmup=2,
-                               kernel_timing=True, plan_res=best["res_calc"][0],
-                               block_res=best["res_calc"][1], block_direct=bs, block_adt=bs)
-        direct_times[bs] = sum(r["loop_kernel_ms"][lp] / max(r["loop_calls"][lp], 1)
-                               for lp in ("save_soln", "adt_calc", "update"))
-    best_direct = min(direct_times, key=direct_times.get)
-    tuned_kw = dict(plan_res=best["res_calc"][0], block_res=best["res_calc"][1],
-                    plan_bres=best["bres_calc"][0], block_bres=best["bres_calc"][1],
-                    block_direct=best_direct, block_adt=best_direct)
-    log(f"tuned plans: {tuned_kw}")
-
-    # ---- graphed (headline) -----------------------------------------------
-    graphed = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, **tuned_kw)
-    value = K / graphed["seconds"]
-    log(f"graphed: {value:.1f} steps/s")
-
-    # ---- per-kernel HIP-event timing of the tuned plans (same stream) -----
-    timed = op2amd.airfoil.run(mesh, min(K, 100) + 2, mode="tuned", warmup=2,
-                               kernel_timing=True, **tuned_kw)
-    per_loop = {lp: {
-        "avg_us": 1e3 * timed["loop_kernel_ms"][lp] / max(timed["loop_calls"][lp], 1),
-        "bytes_per_call": timed["loop_bytes"][lp],
-    } for lp in op2amd.airfoil.LOOPS}
-    for lp, d in per_loop.items():
-        d["achieved_GBs"] = d["bytes_per_call"] / (d["avg_us"] * 1e-6) / 1e9 if d["avg_us"] > 0 else 0.0
-    dom = per_loop["res_calc"]
-
-    # ---- parity ---------------------------------------------------------
-    err_vs_plain = rel(graphed["rms"], plain["rms"])
-    q_err_vs_plain = float(np.max(np.abs(graphed["q"] - plain["q"])) / np.max(np.abs(plain["q"])))
     from oracle import oracle
-    ncheck = 5
+    I, J = (int(v) for v in args.mesh.split("x"))
+    K, W = args.steps, args.warmup
+    plans = parse_plans(args.plans) if args.plans else None
+
+    meshes = {}
+    for ordering in ("rcm", "random"):
+        if args.only_headline and ordering != args.ordering:
+            continue
+        t0 = time.time()
+        meshes[ordering] = op2amd.mesh.generate(I, J, 0, ordering)
+        log(f"mesh {I}x{J} {ordering}: {meshes[ordering].ncell} cells "
+            f"({time.time() - t0:.1f}s)")
+    mesh = meshes[args.ordering]
+
+    log(f"headline: {args.ordering} ordering")
+    head = measure(op2amd, mesh, K, W, args, plans=plans)
+    value = head["graphed_steps_per_s"]
+    graphed = head.pop("_graphed")
+
+    others = {}
+    if not args.only_headline:
+        other = "random" if args.ordering == "rcm" else "rcm"
+        log(f"{other} ordering, as given")
+        others[f"{other}_ordering"] = measure(op2amd, meshes[other], K, W, args)
+        others[f"{other}_ordering"].pop("_graphed")
+        if other == "random":
+            log("random ordering after op_renumber(pecell)")
+            r = measure(op2amd, meshes["random"], K, W, args, renumber=True)
+            r.pop("_graphed")
+            others["random_ordering_op_renumber"] = r
+
+    # parity of the headline against the oracle's OpenMP executor
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    ncheck = 5
     orc = oracle.airfoil_run(mesh, ncheck, 1, "openmp", nthreads=nthreads)
     err_vs_oracle = rel(graphed["rms"][:ncheck], orc["rms"])
 
-    # ---- CPU baseline: the oracle's OpenMP executor on the host cores -----
     cpu = None
     if not args.no_cpu:
         t = time.time()
         oracle.airfoil_run(mesh, 1, 1, "openmp", nthreads=nthreads)
         one = max(time.time() - t, 1e-3)
-        iters = int(max(2, min(100, 15.0 / one)))
+        iters = int(max(2, min(200, 15.0 / one)))
         t = time.time()
         oracle.airfoil_run(mesh, iters, 1, "openmp", nthreads=nthreads)
         secs = time.time() - t
-        cpu = {"value": iters / secs, "unit": "steps/s", "cores": nthreads, "kind": "port",
-               "sample": f"{iters} airfoil steps on the same {I}x{J} mesh ({secs:.1f} s), "
-                         "oracle OpenMP executor, g++ -O3 -fopenmp"}
+        cpu = {"value": round(iters / secs, 3), "unit": "steps/s", "cores": nthreads,
+               "kind": "port",
+               "sample": f"{iters} airfoil steps on the same {I}x{J} {args.ordering} mesh "
+                         f"({secs:.1f} s): the oracle's OpenMP executor (block colours "
+                         "in sequence, blocks in parallel), g++ -O3 -fopenmp"}
         log(f"cpu baseline: {cpu['value']:.2f} steps/s on {nthreads} threads")
 
+    dom = head["per_loop"]["res_calc"]
+    kw = head["tuned_plans"]
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -154,23 +212,17 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"airfoil {I}x{J} ({mesh.ncell} cells, {mesh.nedge} edges), "
-                        f"{args.ordering} ordering, FP64, one step = save_soln + 2 x "
+                        f"{args.ordering} ordering, FP64; one step = save_soln + 2 x "
                         "(adt_calc, res_calc, bres_calc, update)",
             "mesh": args.mesh, "ordering": args.ordering,
-            "plain_steps_per_s": round(plain_sps, 2),
-            "speedup_vs_plain": round(value / plain_sps, 3),
-            "tuned_plans": {k: (list(v) if isinstance(v, tuple) else v) for k, v in tuned_kw.items()},
-            "rms_rel_err_graphed_vs_plain": err_vs_plain,
-            "q_rel_err_graphed_vs_plain": q_err_vs_plain,
             "rms_rel_err_vs_oracle_first_steps": err_vs_oracle,
-            "per_loop": per_loop,
-            "step_bytes": sum(timed["loop_bytes"][lp] * (1 if lp == "save_soln" else 2)
-                              for lp in op2amd.airfoil.LOOPS),
+            **{k: v for k, v in head.items()},
+            **others,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"res_calc ({tuned_kw['plan_res']}, block {tuned_kw['block_res']})",
-            "achieved": round(dom["achieved_GBs"], 1),
+            "kernel": f"res_calc ({kw['plan_res']}, block {kw['block_res']})",
+            "achieved": dom["achieved_GBs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom["achieved_GBs"] / HBM_PEAK_GBS, 4),

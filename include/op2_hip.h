@@ -204,10 +204,21 @@ void op_profile_exit(void);
 void op_profile_end(void);
 void op_profile_output(void);
 
+/* ---- renumbering (op2/src/externlib/op_renumber.cpp:224-505) ----
+ * Reorders every set reachable from base->to for locality: base->to by
+ * reverse Cuthill-McKee on the graph "two elements share a base->from
+ * element" (the reference uses Scotch's ordering, not available here), the
+ * other sets by the reference's propagation rules (op_renumber.cpp:72-151).
+ * Host and device data, maps and plans are all moved to the new numbering,
+ * which the application then sees (as after the reference's op_renumber). */
+void op_renumber(op_map base);
+/* The composed new_of_old permutation applied to `set` by op_renumber calls
+ * (identity if none); returns 1 if the set was renumbered. */
+int op2h_set_permutation(op_set set, int *new_of_old);
+
 /* ---- single-rank no-ops kept for link compatibility (op_dummy_singlenode.cpp:43-208) ---- */
 void op_partition(const char *lib_name, const char *lib_routine,
                   op_set prime_set, op_map prime_map, op_dat coords);
-void op_renumber(op_map base);
 int op_mpi_halo_exchanges(op_set set, int nargs, op_arg *args);
 int op_mpi_halo_exchanges_cuda(op_set set, int nargs, op_arg *args);
 int op_mpi_halo_exchanges_grouped(op_set set, int nargs, op_arg *args,
@@ -250,6 +261,23 @@ int op2h_set_loop_plan(const char *name, int plan, int block_size);
 int op2h_get_loop_plan(const char *name, int *plan, int *block_size);
 void op2h_set_sync_mode(int mode);
 int op2h_get_sync_mode(void);
+
+/* Online autotuning.  With autotuning on, every loop whose plan is AUTO runs
+ * its first calls through each candidate variant — (plan, block size) for
+ * loops with indirect increments, block size for the others — `trials` calls
+ * each, timed with HIP events, then keeps the fastest.  Tuning happens in
+ * eager calls only; a loop still tuning when a graph is captured is recorded
+ * with its best variant so far.  Each timed call is kept as a record. */
+typedef struct {
+  char loop[32];
+  int set_size;
+  int plan, block_size;
+  double kernel_ms;   /* one call */
+} op2h_tuning_record;
+void op2h_set_autotune(int on, int trials);
+int op2h_tuning_done(void);  /* 1 when every seen loop has chosen */
+int op2h_tuning_records(int max, op2h_tuning_record *out);
+int op2h_tuning_choice(const char *name, int *plan, int *block_size);
 
 /* Step capture into a HIP graph.  Between op2h_graph_begin() and
  * op2h_graph_end() every op_par_loop is recorded instead of run; gbl

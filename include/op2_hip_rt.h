@@ -36,6 +36,7 @@ typedef struct {
   int block_size;  /* threads per workgroup */
   int part_size;   /* elements per plan block (staged plan) */
   int capturing;   /* 1 while a graph is being recorded */
+  int tuning;      /* 1 while this call times an autotuning candidate */
   void *stream;    /* hipStream_t */
   double wall0;
   int nargs;
@@ -117,6 +118,7 @@ typedef struct {
  * order.  `targets` lists the target elements with at least one entry. */
 typedef struct {
   int from_size, to_size, dim, ntargets;
+  int max_count;  /* most entries of one target */
   int *offsets;   /* [ntargets+1] */
   int *entries;   /* [from_size*dim] element*dim+slot */
   int *targets;   /* [ntargets] */
@@ -140,6 +142,12 @@ const op2h_staged_plan *op2h_plan_staged_for(const char *name, op_set set,
                                              op_arg *args);
 const op2h_color_plan *op2h_plan_color_for(const char *name, op_set set,
                                            int nargs, op_arg *args);
+
+/* Device buffer of `bytes` private to a generated stub (e.g. a loop-specific
+ * packing of a plan), cached under `key` with the plans: freed by op_exit,
+ * dropped by op_renumber.  *fresh is 1 when the caller must fill it (before
+ * any graph capture). */
+void *op2h_plan_user(const char *key, size_t bytes, int *fresh);
 
 /* Compulsory ("algorithmic") bytes of one call of a loop: every dat element
  * and map entry the loop touches, counted once; RW and INC count a read and a

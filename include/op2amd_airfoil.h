@@ -43,6 +43,16 @@ typedef struct {
   int block_adt;
   int kernel_timing;           /* 1: per-loop HIP event timing */
   int plan_on_device;          /* 1: colour plans on the GPU */
+  int renumber;                /* 1: op_renumber(pecell) after declarations,
+                                  as airfoil_hdf5/dp/airfoil.cpp:155 does;
+                                  results are reported in the input numbering */
+  int mixed;                   /* 1: p_adt and p_res declared "float": fluxes
+                                  and adt in FP32, q/qold/update/rms in FP64 */
+  int autotune;                /* >0: runtime autotuning with this many timed
+                                  calls per variant (loops left on AUTO); in
+                                  graphed mode iterations run eagerly until
+                                  every loop has chosen, then the step is
+                                  captured */
 } op2amd_airfoil_opts;
 
 typedef struct {
@@ -58,6 +68,8 @@ typedef struct {
   double loop_kernel_ms[5];  /* per loop: summed kernel time (kernel_timing) */
   int loop_calls[5];
   double loop_bytes[5];      /* compulsory bytes of one call of each loop */
+  int chosen_plan[5], chosen_block[5];  /* autotuner's choices (-1 if none) */
+  int eager_iterations;      /* iterations run before graph capture */
 } op2amd_airfoil_result;
 
 /* Run the airfoil time-marching loop on mesh `m`.  Returns 0 on success. */

@@ -23,6 +23,7 @@ double now() {
 }
 
 struct Airfoil {
+  const char *rt = "double";  // storage type of p_adt and p_res
   op_set nodes, edges, bedges, cells;
   op_map pedge, pecell, pbedge, pbecell, pcell;
   op_dat p_bound, p_x, p_q, p_qold, p_adt, p_res;
@@ -41,22 +42,22 @@ struct Airfoil {
                                      op_arg_dat(p_x, 2, pcell, 2, "double", OP_READ),
                                      op_arg_dat(p_x, 3, pcell, 2, "double", OP_READ),
                                      op_arg_dat(p_q, -1, OP_ID, 4, "double", OP_READ),
-                                     op_arg_dat(p_adt, -1, OP_ID, 1, "double", OP_WRITE));
+                                     op_arg_dat(p_adt, -1, OP_ID, 1, rt, OP_WRITE));
       op_par_loop_airfoil_3_res_calc("res_calc", edges,
                                      op_arg_dat(p_x, 0, pedge, 2, "double", OP_READ),
                                      op_arg_dat(p_x, 1, pedge, 2, "double", OP_READ),
                                      op_arg_dat(p_q, 0, pecell, 4, "double", OP_READ),
                                      op_arg_dat(p_q, 1, pecell, 4, "double", OP_READ),
-                                     op_arg_dat(p_adt, 0, pecell, 1, "double", OP_READ),
-                                     op_arg_dat(p_adt, 1, pecell, 1, "double", OP_READ),
-                                     op_arg_dat(p_res, 0, pecell, 4, "double", OP_INC),
-                                     op_arg_dat(p_res, 1, pecell, 4, "double", OP_INC));
+                                     op_arg_dat(p_adt, 0, pecell, 1, rt, OP_READ),
+                                     op_arg_dat(p_adt, 1, pecell, 1, rt, OP_READ),
+                                     op_arg_dat(p_res, 0, pecell, 4, rt, OP_INC),
+                                     op_arg_dat(p_res, 1, pecell, 4, rt, OP_INC));
       op_par_loop_airfoil_4_bres_calc("bres_calc", bedges,
                                       op_arg_dat(p_x, 0, pbedge, 2, "double", OP_READ),
                                       op_arg_dat(p_x, 1, pbedge, 2, "double", OP_READ),
                                       op_arg_dat(p_q, 0, pbecell, 4, "double", OP_READ),
-                                      op_arg_dat(p_adt, 0, pbecell, 1, "double", OP_READ),
-                                      op_arg_dat(p_res, 0, pbecell, 4, "double", OP_INC),
+                                      op_arg_dat(p_adt, 0, pbecell, 1, rt, OP_READ),
+                                      op_arg_dat(p_res, 0, pbecell, 4, rt, OP_INC),
                                       op_arg_dat(p_bound, -1, OP_ID, 1, "int", OP_READ));
       rms = 0.0;
       maxerr = 0.0;
@@ -64,8 +65,8 @@ struct Airfoil {
       op_par_loop_airfoil_5_update("update", cells,
                                    op_arg_dat(p_qold, -1, OP_ID, 4, "double", OP_READ),
                                    op_arg_dat(p_q, -1, OP_ID, 4, "double", OP_WRITE),
-                                   op_arg_dat(p_res, -1, OP_ID, 4, "double", OP_RW),
-                                   op_arg_dat(p_adt, -1, OP_ID, 1, "double", OP_READ),
+                                   op_arg_dat(p_res, -1, OP_ID, 4, rt, OP_RW),
+                                   op_arg_dat(p_adt, -1, OP_ID, 1, rt, OP_READ),
                                    op_arg_gbl(&rms, 1, "double", OP_INC),
                                    op_arg_gbl(&maxerr, 1, "double", OP_MAX),
                                    op_arg_idx(-1, OP_ID),
@@ -115,6 +116,7 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
     for (int k = 0; k < 4; k++) q[4 * (size_t)n + k] = qinf[k];
 
   Airfoil A;
+  if (o->mixed) A.rt = "float";
   A.nodes = op_decl_set(m->nnode, "nodes");
   A.edges = op_decl_set(m->nedge, "edges");
   A.bedges = op_decl_set(m->nbedge, "bedges");
@@ -128,8 +130,14 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
   A.p_x = op_decl_dat(A.nodes, 2, "double", m->x, "p_x");
   A.p_q = op_decl_dat(A.cells, 4, "double", q.data(), "p_q");
   A.p_qold = op_decl_dat(A.cells, 4, "double", q.data(), "p_qold");
-  A.p_adt = op_decl_dat_temp(A.cells, 1, "double", (double *)nullptr, "p_adt");
-  A.p_res = op_decl_dat(A.cells, 4, "double", res.data(), "p_res");
+  std::vector<float> resf(4 * (size_t)m->ncell, 0.0f);
+  if (o->mixed) {
+    A.p_adt = op_decl_dat_temp(A.cells, 1, "float", (float *)nullptr, "p_adt");
+    A.p_res = op_decl_dat(A.cells, 4, "float", resf.data(), "p_res");
+  } else {
+    A.p_adt = op_decl_dat_temp(A.cells, 1, "double", (double *)nullptr, "p_adt");
+    A.p_res = op_decl_dat(A.cells, 4, "double", res.data(), "p_res");
+  }
   op_decl_const2("gam", 1, "double", &gam);
   op_decl_const2("gm1", 1, "double", &gm1);
   op_decl_const2("cfl", 1, "double", &cfl);
@@ -137,6 +145,11 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
   op_decl_const2("mach", 1, "double", &mach);
   op_decl_const2("alpha", 1, "double", &alpha);
   op_decl_const2("qinf", 4, "double", qinf);
+  if (o->renumber) op_renumber(A.pecell);
+  std::vector<int> cell_new_of_old(m->ncell);
+  op2h_set_permutation(A.cells, cell_new_of_old.data());
+  std::vector<int> cell_old_of_new(m->ncell);
+  for (int c = 0; c < m->ncell; ++c) cell_old_of_new[cell_new_of_old[c]] = c;
 
   // compulsory bytes per loop call (same argument lists as step())
   {
@@ -147,25 +160,25 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
                     op_arg_dat(A.p_x, 2, A.pcell, 2, "double", OP_READ),
                     op_arg_dat(A.p_x, 3, A.pcell, 2, "double", OP_READ),
                     op_arg_dat(A.p_q, -1, OP_ID, 4, "double", OP_READ),
-                    op_arg_dat(A.p_adt, -1, OP_ID, 1, "double", OP_WRITE)};
+                    op_arg_dat(A.p_adt, -1, OP_ID, 1, A.rt, OP_WRITE)};
     op_arg a3[8] = {op_arg_dat(A.p_x, 0, A.pedge, 2, "double", OP_READ),
                     op_arg_dat(A.p_x, 1, A.pedge, 2, "double", OP_READ),
                     op_arg_dat(A.p_q, 0, A.pecell, 4, "double", OP_READ),
                     op_arg_dat(A.p_q, 1, A.pecell, 4, "double", OP_READ),
-                    op_arg_dat(A.p_adt, 0, A.pecell, 1, "double", OP_READ),
-                    op_arg_dat(A.p_adt, 1, A.pecell, 1, "double", OP_READ),
-                    op_arg_dat(A.p_res, 0, A.pecell, 4, "double", OP_INC),
-                    op_arg_dat(A.p_res, 1, A.pecell, 4, "double", OP_INC)};
+                    op_arg_dat(A.p_adt, 0, A.pecell, 1, A.rt, OP_READ),
+                    op_arg_dat(A.p_adt, 1, A.pecell, 1, A.rt, OP_READ),
+                    op_arg_dat(A.p_res, 0, A.pecell, 4, A.rt, OP_INC),
+                    op_arg_dat(A.p_res, 1, A.pecell, 4, A.rt, OP_INC)};
     op_arg a4[6] = {op_arg_dat(A.p_x, 0, A.pbedge, 2, "double", OP_READ),
                     op_arg_dat(A.p_x, 1, A.pbedge, 2, "double", OP_READ),
                     op_arg_dat(A.p_q, 0, A.pbecell, 4, "double", OP_READ),
-                    op_arg_dat(A.p_adt, 0, A.pbecell, 1, "double", OP_READ),
-                    op_arg_dat(A.p_res, 0, A.pbecell, 4, "double", OP_INC),
+                    op_arg_dat(A.p_adt, 0, A.pbecell, 1, A.rt, OP_READ),
+                    op_arg_dat(A.p_res, 0, A.pbecell, 4, A.rt, OP_INC),
                     op_arg_dat(A.p_bound, -1, OP_ID, 1, "int", OP_READ)};
     op_arg a5[4] = {op_arg_dat(A.p_qold, -1, OP_ID, 4, "double", OP_READ),
                     op_arg_dat(A.p_q, -1, OP_ID, 4, "double", OP_WRITE),
-                    op_arg_dat(A.p_res, -1, OP_ID, 4, "double", OP_RW),
-                    op_arg_dat(A.p_adt, -1, OP_ID, 1, "double", OP_READ)};
+                    op_arg_dat(A.p_res, -1, OP_ID, 4, A.rt, OP_RW),
+                    op_arg_dat(A.p_adt, -1, OP_ID, 1, A.rt, OP_READ)};
     r->loop_bytes[0] = op2h_algorithmic_bytes(A.cells, 2, a1);
     r->loop_bytes[1] = op2h_algorithmic_bytes(A.cells, 6, a2);
     r->loop_bytes[2] = op2h_algorithmic_bytes(A.edges, 8, a3);
@@ -176,6 +189,10 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
   if (o->mode == OP2AMD_MODE_PLAIN) {
     op2h_set_sync_mode(OP2H_SYNC_EVERY_LOOP);
     op2h_set_loop_plan(nullptr, OP2H_PLAN_AUTO, 0);
+  } else if (o->autotune > 0) {
+    op2h_set_sync_mode(OP2H_SYNC_GBL_ONLY);
+    op2h_set_loop_plan(nullptr, OP2H_PLAN_AUTO, 0);
+    op2h_set_autotune(1, o->autotune);
   } else {
     op2h_set_sync_mode(OP2H_SYNC_GBL_ONLY);
     op2h_set_loop_plan("res_calc", o->plan_res, o->block_res);
@@ -209,10 +226,23 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
     }
     A.step();
     record(r, 0, A.rms, A.rms_k0, A.maxerr, A.errloc, ncell);
+    int it0 = 2;  // first iteration replayed from the graph
+    while (o->autotune > 0 && !op2h_tuning_done() && it0 <= o->niter) {
+      if (it0 == o->warmup + 1) {
+        op2h_device_sync();
+        r->setup_seconds = now() - t_setup0;
+        op2h_reset_stats();
+        t0 = now();
+      }
+      A.step();
+      record(r, it0 - 1, A.rms, A.rms_k0, A.maxerr, A.errloc, ncell);
+      ++it0;
+    }
+    r->eager_iterations = it0 - 1;
     if (op2h_graph_begin() != 0) return 2;
     A.step();
     if (op2h_graph_end() != 0) return 3;
-    for (int it = 2; it <= o->niter; ++it) {
+    for (int it = it0; it <= o->niter; ++it) {
       if (it == o->warmup + 1 && o->warmup > 0) {
         op2h_device_sync();
         r->setup_seconds = now() - t_setup0;
@@ -226,7 +256,7 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
     double t1 = now();
     // per-iteration gbl results from the device history:
     // [rms_k0][max_k0][loc_k0][rms_k1][max_k1][loc_k1] per launch
-    int nrec = 0, nl = o->niter - 1;
+    int nrec = 0, nl = o->niter - it0 + 1;
     std::vector<double> hist(std::max(nl, 1) * 8);
     int words = op2h_graph_history(nl, hist.data(), &nrec);
     if (nl > 0 && (words != 6 || nrec != nl)) {
@@ -240,7 +270,7 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
       double mx = h[4] > 0.0 ? h[4] : 0.0;
       int loc = 0;
       if (h[4] > 0.0) std::memcpy(&loc, &h[5], sizeof(int));
-      record(r, i + 1, 0.0 + h[3], 0.0 + h[0], mx, loc, ncell);
+      record(r, it0 - 1 + i, 0.0 + h[3], 0.0 + h[0], mx, loc, ncell);
     }
     op2h_graph_sync();
     r->seconds = t1 - t0;
@@ -248,11 +278,23 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
   if (o->mode != OP2AMD_MODE_GRAPHED) r->seconds = now() - t0;
   const char *names[5] = {"save_soln", "adt_calc", "res_calc", "bres_calc", "update"};
   for (int l = 0; l < 5; ++l) {
+    r->chosen_plan[l] = r->chosen_block[l] = -1;
+    op2h_tuning_choice(names[l], &r->chosen_plan[l], &r->chosen_block[l]);
     r->loop_calls[l] = 0;
     r->loop_kernel_ms[l] = 0.0;
     op2h_loop_stats(names[l], &r->loop_calls[l], &r->loop_kernel_ms[l]);
   }
-  if (r->q_out) op_fetch_data(A.p_q, r->q_out);
+  if (r->q_out) {  // back to the input numbering
+    std::vector<double> qn(4 * (size_t)m->ncell);
+    op_fetch_data(A.p_q, qn.data());
+    for (int c = 0; c < m->ncell; ++c)
+      std::memcpy(r->q_out + 4 * (size_t)c, &qn[4 * (size_t)cell_new_of_old[c]],
+                  4 * sizeof(double));
+  }
+  if (r->errloc && o->renumber)
+    for (int it = 0; it < o->niter; ++it)
+      if (r->errloc[it] >= 0 && r->errloc[it] < m->ncell)
+        r->errloc[it] = cell_old_of_new[r->errloc[it]];
   op2h_graph_destroy();
   op_exit();
   return 0;

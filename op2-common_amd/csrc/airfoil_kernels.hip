@@ -44,6 +44,12 @@ __constant__ double eps_d;
 __constant__ double mach_d;
 __constant__ double alpha_d;
 __constant__ double qinf_d[4];
+// FP32 copies for the mixed-precision flux kernels
+__constant__ float gam_f;
+__constant__ float gm1_f;
+__constant__ float cfl_f;
+__constant__ float eps_f;
+__constant__ float qinf_f[4];
 
 extern "C" void op_decl_const_char(int dim, char const *type, int size,
                                    char *dat, char const *name) {
@@ -61,6 +67,15 @@ extern "C" void op_decl_const_char(int dim, char const *type, int size,
     printf("op_decl_const error -- unknown constant %s\n", name);
     exit(-1);
   }
+  // FP32 copies of the flux constants, rounded once on the host
+  float f[4];
+  for (int i = 0; i < dim && i < 4; ++i)
+    f[i] = size == 8 ? (float)((const double *)dat)[i] : ((const float *)dat)[i];
+  if (!strcmp(name, "gam")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(gam_f), f, sizeof(float)));
+  else if (!strcmp(name, "gm1")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(gm1_f), f, sizeof(float)));
+  else if (!strcmp(name, "cfl")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cfl_f), f, sizeof(float)));
+  else if (!strcmp(name, "eps")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(eps_f), f, sizeof(float)));
+  else if (!strcmp(name, "qinf")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(qinf_f), f, 4 * sizeof(float)));
 }
 
 // ---------------------------------------------------------------------------
@@ -173,6 +188,92 @@ __device__ __forceinline__ void update(const double *qold, double *q,
 
 }  // namespace op2_k
 
+// FP32 user kernels of the mixed-precision mode: the reference's
+// single-precision airfoil kernels (apps/c/airfoil/airfoil_plain/sp/
+// adt_calc.h, res_calc.h, bres_calc.h) on FP32 copies of q and x.
+namespace op2_kf {
+
+__device__ __forceinline__ void adt_calc(const float *x1, const float *x2,
+                                         const float *x3, const float *x4,
+                                         const float *q, float *adt) {
+  float dx, dy, ri, u, v, c;
+  ri = 1.0f / q[0];
+  u = ri * q[1];
+  v = ri * q[2];
+  c = sqrtf(gam_f * gm1_f * (ri * q[3] - 0.5f * (u * u + v * v)));
+  dx = x2[0] - x1[0];
+  dy = x2[1] - x1[1];
+  *adt = fabsf(u * dy - v * dx) + c * sqrtf(dx * dx + dy * dy);
+  dx = x3[0] - x2[0];
+  dy = x3[1] - x2[1];
+  *adt += fabsf(u * dy - v * dx) + c * sqrtf(dx * dx + dy * dy);
+  dx = x4[0] - x3[0];
+  dy = x4[1] - x3[1];
+  *adt += fabsf(u * dy - v * dx) + c * sqrtf(dx * dx + dy * dy);
+  dx = x1[0] - x4[0];
+  dy = x1[1] - x4[1];
+  *adt += fabsf(u * dy - v * dx) + c * sqrtf(dx * dx + dy * dy);
+  *adt = (*adt) / cfl_f;
+}
+
+__device__ __forceinline__ void res_calc(const float *x1, const float *x2,
+                                         const float *q1, const float *q2,
+                                         const float *adt1, const float *adt2,
+                                         float *res1, float *res2) {
+  float dx, dy, mu, ri, p1, vol1, p2, vol2, f;
+  dx = x1[0] - x2[0];
+  dy = x1[1] - x2[1];
+  ri = 1.0f / q1[0];
+  p1 = gm1_f * (q1[3] - 0.5f * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
+  vol1 = ri * (q1[1] * dy - q1[2] * dx);
+  ri = 1.0f / q2[0];
+  p2 = gm1_f * (q2[3] - 0.5f * ri * (q2[1] * q2[1] + q2[2] * q2[2]));
+  vol2 = ri * (q2[1] * dy - q2[2] * dx);
+  mu = 0.5f * ((*adt1) + (*adt2)) * eps_f;
+  f = 0.5f * (vol1 * q1[0] + vol2 * q2[0]) + mu * (q1[0] - q2[0]);
+  res1[0] += f;
+  res2[0] -= f;
+  f = 0.5f * (vol1 * q1[1] + p1 * dy + vol2 * q2[1] + p2 * dy) + mu * (q1[1] - q2[1]);
+  res1[1] += f;
+  res2[1] -= f;
+  f = 0.5f * (vol1 * q1[2] - p1 * dx + vol2 * q2[2] - p2 * dx) + mu * (q1[2] - q2[2]);
+  res1[2] += f;
+  res2[2] -= f;
+  f = 0.5f * (vol1 * (q1[3] + p1) + vol2 * (q2[3] + p2)) + mu * (q1[3] - q2[3]);
+  res1[3] += f;
+  res2[3] -= f;
+}
+
+__device__ __forceinline__ void bres_calc(const float *x1, const float *x2,
+                                          const float *q1, const float *adt1,
+                                          float *res1, const int *bound) {
+  float dx, dy, mu, ri, p1, vol1, p2, vol2, f;
+  dx = x1[0] - x2[0];
+  dy = x1[1] - x2[1];
+  ri = 1.0f / q1[0];
+  p1 = gm1_f * (q1[3] - 0.5f * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
+  if (*bound == 1) {
+    res1[1] += +p1 * dy;
+    res1[2] += -p1 * dx;
+  } else {
+    vol1 = ri * (q1[1] * dy - q1[2] * dx);
+    ri = 1.0f / qinf_f[0];
+    p2 = gm1_f * (qinf_f[3] - 0.5f * ri * (qinf_f[1] * qinf_f[1] + qinf_f[2] * qinf_f[2]));
+    vol2 = ri * (qinf_f[1] * dy - qinf_f[2] * dx);
+    mu = (*adt1) * eps_f;
+    f = 0.5f * (vol1 * q1[0] + vol2 * qinf_f[0]) + mu * (q1[0] - qinf_f[0]);
+    res1[0] += f;
+    f = 0.5f * (vol1 * q1[1] + p1 * dy + vol2 * qinf_f[1] + p2 * dy) + mu * (q1[1] - qinf_f[1]);
+    res1[1] += f;
+    f = 0.5f * (vol1 * q1[2] - p1 * dx + vol2 * qinf_f[2] - p2 * dx) + mu * (q1[2] - qinf_f[2]);
+    res1[2] += f;
+    f = 0.5f * (vol1 * (q1[3] + p1) + vol2 * (qinf_f[3] + p2)) + mu * (q1[3] - qinf_f[3]);
+    res1[3] += f;
+  }
+}
+
+}  // namespace op2_kf
+
 // ---------------------------------------------------------------------------
 // dat access: AoS (element-contiguous, 16-byte vector loads) or SoA
 // (component rows `stride` elements apart)
@@ -246,7 +347,22 @@ __device__ __forceinline__ void atomic_add4(double *p, int n, int s,
   unsafeAtomicAdd(p + base + 3 * st, v[3]);
 }
 
-int grid_for(int n, int bs) { return n > 0 ? (n + bs - 1) / bs : 0; }
+int grid_for(long long n, int bs) { return n > 0 ? (int)((n + bs - 1) / bs) : 0; }
+
+// Workgroups are dispatched round-robin over the XCDs (blockIdx % nxcd).
+// Logical block ids are assigned so that each XCD runs one contiguous range
+// of them: neighbours in a locally numbered mesh then stay in one XCD's L2.
+// The map is a bijection of [0, gridDim.x).
+__device__ __forceinline__ int xcd_block(int nxcd) {
+  const int b = blockIdx.x, n = gridDim.x;
+  if (nxcd <= 1 || n < nxcd) return b;
+  const int q = n / nxcd, r = n % nxcd;
+  const int x = b % nxcd, i = b / nxcd;
+  const int start = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  return start + i;
+}
+
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
 
 // ---------------------------------------------------------------------------
 // save_soln: qold = q over the whole dat, 16-byte vectors (direct copy)
@@ -264,7 +380,7 @@ template <bool SOA>
 __global__ void k_adt_calc(const double *__restrict x, const double *__restrict q,
                            double *__restrict adt, const int *__restrict pcell,
                            int n, int sx, int sq) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = xcd_block(kXcds) * blockDim.x + threadIdx.x;
   if (c >= n) return;
   int4 cn = reinterpret_cast<const int4 *>(pcell)[c];
   double x1[2], x2[2], x3[2], x4[2], qc[4], a;
@@ -304,7 +420,7 @@ __global__ void k_res_atomic(const double *__restrict x, const double *__restric
                              const double *__restrict adt, double *res,
                              const int2 *__restrict pedge,
                              const int2 *__restrict pecell, int n, int sx, int sq) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_block(kXcds) * blockDim.x + threadIdx.x;
   if (e >= n) return;
   int2 en = pedge[e], ec = pecell[e];
   double r1[4], r2[4];
@@ -359,6 +475,113 @@ __global__ void k_res_gather(const double *__restrict x, const double *__restric
     acc[3] += r[3];
   }
   st4<SOA>(res, cell, sq, acc);
+}
+
+// Owner-compute with K lanes per cell: lane k of a cell's group computes the
+// flux of the cell's k-th edge (in edge order); the group's first lane adds
+// the K results to res in edge order, so the sum is the sequential
+// executor's.  Every lane of a wave reaches every shuffle.
+template <bool SOA, int K>
+__global__ void __launch_bounds__(1024)
+    k_res_gather_lanes(const double *__restrict x, const double *__restrict q,
+                       const double *__restrict adt, double *res,
+                       const int2 *__restrict pedge, const int2 *__restrict pecell,
+                       const int *__restrict offsets, const int *__restrict entries,
+                       const int *__restrict targets, int ntargets, int sx, int sq) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = g / K, k = g & (K - 1);
+  const bool valid = t < ntargets;
+  int k0 = 0, cnt = 0;
+  if (valid) {
+    k0 = offsets[t];
+    cnt = offsets[t + 1] - k0;
+  }
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  if (k < cnt) {
+    int ent = entries[k0 + k];
+    int e = ent >> 1;
+    double r1[4], r2[4];
+    res_edge<SOA>(x, q, adt, pedge[e], pecell[e], sx, sq, r1, r2);
+    for (int d = 0; d < 4; ++d) r[d] = (ent & 1) ? r2[d] : r1[d];
+  }
+  const int lead = (threadIdx.x & 63) & ~(K - 1);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  int cell = 0;
+  if (k == 0 && valid) {
+    cell = targets[t];
+    ld4<SOA>(res, cell, sq, acc);
+  }
+  for (int j = 0; j < K; ++j) {
+    double v[4];
+    for (int d = 0; d < 4; ++d) v[d] = __shfl(r[d], lead + j, 64);
+    if (j < cnt)
+      for (int d = 0; d < 4; ++d) acc[d] += v[d];
+  }
+  if (k == 0 && valid) st4<SOA>(res, cell, sq, acc);
+}
+
+// Gather with a packed table: rec[t*K + k] = {n0, n1, other cell, slot} of
+// the k-th edge of target t in edge order, {-1, -1, -1, 0} past its last edge.
+template <int K>
+__global__ void k_pack_res_gather(const int2 *__restrict pedge,
+                                  const int2 *__restrict pecell,
+                                  const int *__restrict offsets,
+                                  const int *__restrict entries, int ntargets,
+                                  int4 *__restrict rec) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)ntargets * K) return;
+  int t = (int)(i / K), k = (int)(i % K);
+  int k0 = offsets[t], cnt = offsets[t + 1] - k0;
+  int4 r = make_int4(-1, -1, -1, 0);
+  if (k < cnt) {
+    int ent = entries[k0 + k];
+    int e = ent >> 1, slot = ent & 1;
+    int2 en = pedge[e], ec = pecell[e];
+    r = make_int4(en.x, en.y, slot ? ec.x : ec.y, slot);
+  }
+  rec[i] = r;
+}
+
+template <bool SOA, int K>
+__global__ void __launch_bounds__(1024)
+    k_res_gather_packed(const double *__restrict x, const double *__restrict q,
+                        const double *__restrict adt, double *res,
+                        const int4 *__restrict rec, const int *__restrict targets,
+                        int ntargets, int sx, int sq) {
+  const int g = xcd_block(kXcds) * blockDim.x + threadIdx.x;
+  const int t = g / K, k = g & (K - 1);
+  const bool valid = t < ntargets;
+  const int cell = valid ? targets[t] : 0;
+  const int4 rr = valid ? rec[g] : make_int4(-1, -1, -1, 0);
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  const int has = rr.x >= 0;
+  if (has) {
+    double x1[2], x2[2], qa[4], qb[4];
+    ld2<SOA>(x, rr.x, sx, x1);
+    ld2<SOA>(x, rr.y, sx, x2);
+    ld4<SOA>(q, cell, sq, qa);
+    ld4<SOA>(q, rr.z, sq, qb);
+    double aa = adt[cell], ab = adt[rr.z];
+    double r1[4] = {0.0, 0.0, 0.0, 0.0}, r2[4] = {0.0, 0.0, 0.0, 0.0};
+    if (rr.w == 0) {
+      op2_k::res_calc(x1, x2, qa, qb, &aa, &ab, r1, r2);
+      for (int d = 0; d < 4; ++d) r[d] = r1[d];
+    } else {
+      op2_k::res_calc(x1, x2, qb, qa, &ab, &aa, r1, r2);
+      for (int d = 0; d < 4; ++d) r[d] = r2[d];
+    }
+  }
+  const int lead = (threadIdx.x & 63) & ~(K - 1);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (k == 0 && valid) ld4<SOA>(res, cell, sq, acc);
+  for (int j = 0; j < K; ++j) {
+    double v[4];
+    for (int d = 0; d < 4; ++d) v[d] = __shfl(r[d], lead + j, 64);
+    int hv = __shfl(has, lead + j, 64);
+    if (hv)
+      for (int d = 0; d < 4; ++d) acc[d] += v[d];
+  }
+  if (k == 0 && valid) st4<SOA>(res, cell, sq, acc);
 }
 
 struct StagedArgs {
@@ -571,9 +794,11 @@ __global__ void __launch_bounds__(1024)
 
 // ---------------------------------------------------------------------------
 // update, with the rms (INC) and maxerr (MAX, errloc argmax) reductions.
-// Each workgroup reduces its elements (waves by DPP-free shuffles, then LDS),
-// writes one partial; the last workgroup to finish combines the partials in
-// workgroup order and writes the slots, then clears the completion counter.
+// Each workgroup reduces its elements (waves by shuffles, then LDS) and writes
+// one partial; a one-workgroup kernel then combines the partials in
+// workgroup order and writes the result slots.  The kernel boundary makes the
+// partials visible across XCDs, so no device-scope fence is needed inside the
+// streaming kernel (on a multi-XCD gfx950 such a fence writes back L2).
 // ---------------------------------------------------------------------------
 struct MaxLoc {
   double v;
@@ -607,12 +832,10 @@ __global__ void __launch_bounds__(1024)
     k_update(const double *__restrict qold, double *__restrict q,
              double *__restrict res, const double *__restrict adt, int n, int s,
              double *__restrict part_sum, double *__restrict part_max,
-             int *__restrict part_loc, unsigned int *counter,
-             double *__restrict slot_rms, double *__restrict slot_max) {
+             int *__restrict part_loc) {
   __shared__ double s_sum[16];
   __shared__ double s_max[16];
   __shared__ int s_loc[16];
-  __shared__ bool s_last;
   double rms = 0.0;
   MaxLoc ml{-INFINITY, 0x7fffffff};
   const int stride = gridDim.x * blockDim.x;
@@ -647,28 +870,272 @@ __global__ void __launch_bounds__(1024)
     part_sum[blockIdx.x] = bs;
     part_max[blockIdx.x] = bm.v;
     part_loc[blockIdx.x] = bm.loc;
-    __threadfence();
-    unsigned int done = atomicAdd(counter, 1u);
-    s_last = (done == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!s_last || wave != 0) return;
-  __threadfence();
+}
+
+// One workgroup of 256: lane l combines partials l, l+256, ... in order, then
+// the wave trees and the four wave results in wave order.
+__global__ void __launch_bounds__(256)
+    k_update_finish(const double *__restrict part_sum,
+                    const double *__restrict part_max,
+                    const int *__restrict part_loc, int nparts,
+                    double *__restrict slot_rms, double *__restrict slot_max) {
+  __shared__ double s_sum[4];
+  __shared__ double s_max[4];
+  __shared__ int s_loc[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double tot = 0.0;
   MaxLoc tm{-INFINITY, 0x7fffffff};
-  for (int b = lane; b < (int)gridDim.x; b += 64) {
-    tot += __hip_atomic_load(&part_sum[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    MaxLoc t{__hip_atomic_load(&part_max[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-             __hip_atomic_load(&part_loc[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
-    tm = maxloc_combine(tm, t);
+  for (int b = tid; b < nparts; b += 256) {
+    tot += part_sum[b];
+    tm = maxloc_combine(tm, MaxLoc{part_max[b], part_loc[b]});
   }
   tot = wave_sum(tot);
   tm = wave_maxloc(tm);
   if (lane == 0) {
-    slot_rms[0] = tot;
-    slot_max[0] = tm.v;
-    reinterpret_cast<int *>(slot_max + 1)[0] = tm.loc;
-    *counter = 0u;
+    s_sum[wave] = tot;
+    s_max[wave] = tm.v;
+    s_loc[wave] = tm.loc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    MaxLoc m{-INFINITY, 0x7fffffff};
+    for (int w = 0; w < 4; ++w) {
+      t += s_sum[w];
+      m = maxloc_combine(m, MaxLoc{s_max[w], s_loc[w]});
+    }
+    slot_rms[0] = t;
+    slot_max[0] = m.v;
+    reinterpret_cast<int *>(slot_max + 1)[0] = m.loc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: adt and res stored in FP32, fluxes and adt computed in FP32
+// from FP32 copies of q and x; q, qold, the update and the rms/maxerr
+// reductions stay FP64.
+// ---------------------------------------------------------------------------
+template <bool SOA>
+__device__ __forceinline__ void ld2f(const double *__restrict p, int n, int s, float *v) {
+  double t[2];
+  ld2<SOA>(p, n, s, t);
+  v[0] = (float)t[0];
+  v[1] = (float)t[1];
+}
+
+template <bool SOA>
+__device__ __forceinline__ void ld4f(const double *__restrict p, int n, int s, float *v) {
+  double t[4];
+  ld4<SOA>(p, n, s, t);
+  for (int d = 0; d < 4; ++d) v[d] = (float)t[d];
+}
+
+template <bool SOA>
+__device__ __forceinline__ void ld4s(const float *__restrict p, int n, int s, float *v) {
+  if (SOA) {
+    for (int d = 0; d < 4; ++d) v[d] = p[(size_t)d * s + n];
+  } else {
+    float4 a = *reinterpret_cast<const float4 *>(p + 4 * (size_t)n);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = a.z;
+    v[3] = a.w;
+  }
+}
+
+template <bool SOA>
+__device__ __forceinline__ void st4s(float *__restrict p, int n, int s, const float *v) {
+  if (SOA) {
+    for (int d = 0; d < 4; ++d) p[(size_t)d * s + n] = v[d];
+  } else {
+    *reinterpret_cast<float4 *>(p + 4 * (size_t)n) = make_float4(v[0], v[1], v[2], v[3]);
+  }
+}
+
+template <bool SOA>
+__device__ __forceinline__ void atomic_add4s(float *p, int n, int s, const float *v) {
+  size_t st = SOA ? (size_t)s : 1, base = SOA ? (size_t)n : 4 * (size_t)n;
+  for (int d = 0; d < 4; ++d) unsafeAtomicAdd(p + base + d * st, v[d]);
+}
+
+template <bool SOA>
+__global__ void k_adt_calc_mixed(const double *__restrict x, const double *__restrict q,
+                                 float *__restrict adt, const int *__restrict pcell,
+                                 int n, int sx, int sq) {
+  int c = xcd_block(kXcds) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  int4 cn = reinterpret_cast<const int4 *>(pcell)[c];
+  float x1[2], x2[2], x3[2], x4[2], qc[4], a;
+  ld2f<SOA>(x, cn.x, sx, x1);
+  ld2f<SOA>(x, cn.y, sx, x2);
+  ld2f<SOA>(x, cn.z, sx, x3);
+  ld2f<SOA>(x, cn.w, sx, x4);
+  ld4f<SOA>(q, c, sq, qc);
+  op2_kf::adt_calc(x1, x2, x3, x4, qc, &a);
+  adt[c] = a;
+}
+
+template <bool SOA>
+__global__ void k_res_atomic_mixed(const double *__restrict x, const double *__restrict q,
+                                   const float *__restrict adt, float *res,
+                                   const int2 *__restrict pedge,
+                                   const int2 *__restrict pecell, int n, int sx, int sq) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int2 en = pedge[e], ec = pecell[e];
+  float x1[2], x2[2], q1[4], q2[4];
+  ld2f<SOA>(x, en.x, sx, x1);
+  ld2f<SOA>(x, en.y, sx, x2);
+  ld4f<SOA>(q, ec.x, sq, q1);
+  ld4f<SOA>(q, ec.y, sq, q2);
+  float a1 = adt[ec.x], a2 = adt[ec.y];
+  float r1[4] = {0.f, 0.f, 0.f, 0.f}, r2[4] = {0.f, 0.f, 0.f, 0.f};
+  op2_kf::res_calc(x1, x2, q1, q2, &a1, &a2, r1, r2);
+  atomic_add4s<SOA>(res, ec.x, sq, r1);
+  atomic_add4s<SOA>(res, ec.y, sq, r2);
+}
+
+template <bool SOA, int K>
+__global__ void __launch_bounds__(1024)
+    k_res_gather_packed_mixed(const double *__restrict x, const double *__restrict q,
+                              const float *__restrict adt, float *res,
+                              const int4 *__restrict rec, const int *__restrict targets,
+                              int ntargets, int sx, int sq) {
+  const int g = xcd_block(kXcds) * blockDim.x + threadIdx.x;
+  const int t = g / K, k = g & (K - 1);
+  const bool valid = t < ntargets;
+  const int cell = valid ? targets[t] : 0;
+  const int4 rr = valid ? rec[g] : make_int4(-1, -1, -1, 0);
+  float r[4] = {0.f, 0.f, 0.f, 0.f};
+  const int has = rr.x >= 0;
+  if (has) {
+    float x1[2], x2[2], qa[4], qb[4];
+    ld2f<SOA>(x, rr.x, sx, x1);
+    ld2f<SOA>(x, rr.y, sx, x2);
+    ld4f<SOA>(q, cell, sq, qa);
+    ld4f<SOA>(q, rr.z, sq, qb);
+    float aa = adt[cell], ab = adt[rr.z];
+    float r1[4] = {0.f, 0.f, 0.f, 0.f}, r2[4] = {0.f, 0.f, 0.f, 0.f};
+    if (rr.w == 0) {
+      op2_kf::res_calc(x1, x2, qa, qb, &aa, &ab, r1, r2);
+      for (int d = 0; d < 4; ++d) r[d] = r1[d];
+    } else {
+      op2_kf::res_calc(x1, x2, qb, qa, &ab, &aa, r1, r2);
+      for (int d = 0; d < 4; ++d) r[d] = r2[d];
+    }
+  }
+  const int lead = (threadIdx.x & 63) & ~(K - 1);
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  if (k == 0 && valid) ld4s<SOA>(res, cell, sq, acc);
+  for (int j = 0; j < K; ++j) {
+    float v[4];
+    for (int d = 0; d < 4; ++d) v[d] = __shfl(r[d], lead + j, 64);
+    int hv = __shfl(has, lead + j, 64);
+    if (hv)
+      for (int d = 0; d < 4; ++d) acc[d] += v[d];
+  }
+  if (k == 0 && valid) st4s<SOA>(res, cell, sq, acc);
+}
+
+template <bool SOA>
+__device__ __forceinline__ void bres_edge_mixed(const double *__restrict x,
+                                                const double *__restrict q,
+                                                const float *__restrict adt, int2 bn,
+                                                int cell, int bound, int sx, int sq,
+                                                float *r) {
+  float x1[2], x2[2], q1[4];
+  ld2f<SOA>(x, bn.x, sx, x1);
+  ld2f<SOA>(x, bn.y, sx, x2);
+  ld4f<SOA>(q, cell, sq, q1);
+  float a1 = adt[cell];
+  for (int d = 0; d < 4; ++d) r[d] = 0.f;
+  op2_kf::bres_calc(x1, x2, q1, &a1, r, &bound);
+}
+
+template <bool SOA>
+__global__ void k_bres_atomic_mixed(const double *__restrict x, const double *__restrict q,
+                                    const float *__restrict adt, float *res,
+                                    const int2 *__restrict pbedge,
+                                    const int *__restrict pbecell,
+                                    const int *__restrict bound, int n, int sx, int sq) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int cell = pbecell[e];
+  float r[4];
+  bres_edge_mixed<SOA>(x, q, adt, pbedge[e], cell, bound[e], sx, sq, r);
+  atomic_add4s<SOA>(res, cell, sq, r);
+}
+
+template <bool SOA>
+__global__ void k_bres_gather_mixed(const double *__restrict x, const double *__restrict q,
+                                    const float *__restrict adt, float *res,
+                                    const int2 *__restrict pbedge,
+                                    const int *__restrict bound,
+                                    const int *__restrict offsets,
+                                    const int *__restrict entries,
+                                    const int *__restrict targets, int ntargets, int sx,
+                                    int sq) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntargets) return;
+  int cell = targets[t];
+  float acc[4];
+  ld4s<SOA>(res, cell, sq, acc);
+  for (int k = offsets[t]; k < offsets[t + 1]; ++k) {
+    int e = entries[k];
+    float r[4];
+    bres_edge_mixed<SOA>(x, q, adt, pbedge[e], cell, bound[e], sx, sq, r);
+    for (int d = 0; d < 4; ++d) acc[d] += r[d];
+  }
+  st4s<SOA>(res, cell, sq, acc);
+}
+
+template <bool SOA>
+__global__ void __launch_bounds__(1024)
+    k_update_mixed(const double *__restrict qold, double *__restrict q,
+                   float *__restrict res, const float *__restrict adt, int n, int s,
+                   double *__restrict part_sum, double *__restrict part_max,
+                   int *__restrict part_loc) {
+  __shared__ double s_sum[16];
+  __shared__ double s_max[16];
+  __shared__ int s_loc[16];
+  double rms = 0.0;
+  MaxLoc ml{-INFINITY, 0x7fffffff};
+  const int stride = gridDim.x * blockDim.x;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+    double qo[4], qn[4], r[4], a = (double)adt[c], mx = ml.v;
+    float rf[4];
+    int loc = ml.loc;
+    ld4<SOA>(qold, c, s, qo);
+    ld4s<SOA>(res, c, s, rf);
+    for (int d = 0; d < 4; ++d) r[d] = (double)rf[d];
+    op2_k::update(qo, qn, r, &a, &rms, &mx, &c, &loc);
+    ml.v = mx;
+    ml.loc = loc;
+    st4<SOA>(q, c, s, qn);
+    const float z[4] = {0.f, 0.f, 0.f, 0.f};
+    st4s<SOA>(res, c, s, z);
+  }
+  rms = wave_sum(rms);
+  ml = wave_maxloc(ml);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_sum[wave] = rms;
+    s_max[wave] = ml.v;
+    s_loc[wave] = ml.loc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bs = 0.0;
+    MaxLoc bm{-INFINITY, 0x7fffffff};
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) {
+      bs += s_sum[w];
+      bm = maxloc_combine(bm, MaxLoc{s_max[w], s_loc[w]});
+    }
+    part_sum[blockIdx.x] = bs;
+    part_max[blockIdx.x] = bm.v;
+    part_loc[blockIdx.x] = bm.loc;
   }
 }
 
@@ -696,6 +1163,8 @@ bool loop_soa(const op_arg *args, int nargs, int *stride) {
   }
   return soa == 1;
 }
+
+bool is_float(const op_arg &a) { return a.dat && !strncmp(a.dat->type, "float", 5); }
 
 int set_stride(const op_arg &a) {
   int s = op2h_dat_soa_stride(a.dat);
@@ -748,7 +1217,11 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
     const int *pcell = args[0].map->map_d;
     int g = grid_for(set->size, ctx.block_size);
     op2h_kernel_timer_start(&ctx);
-    if (soa)
+    if (is_float(args[5])) {
+      float *adtf = (float *)args[5].data_d;
+      if (soa) k_adt_calc_mixed<true><<<g, ctx.block_size, 0, st>>>(x, q, adtf, pcell, set->size, sx, sq);
+      else k_adt_calc_mixed<false><<<g, ctx.block_size, 0, st>>>(x, q, adtf, pcell, set->size, sx, sq);
+    } else if (soa)
       k_adt_calc<true><<<g, ctx.block_size, 0, st>>>(x, q, adt, pcell, set->size, sx, sq);
     else
       k_adt_calc<false><<<g, ctx.block_size, 0, st>>>(x, q, adt, pcell, set->size, sx, sq);
@@ -779,6 +1252,44 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
     const int2 *pedge = (const int2 *)args[0].map->map_d;
     const int2 *pecell = (const int2 *)args[2].map->map_d;
     const int bs = ctx.block_size;
+    const bool mixed = is_float(args[6]);
+    if (mixed != is_float(args[4])) {
+      printf("op2hip: res_calc needs adt and res both FP32 (mixed mode) or both FP64\n");
+      exit(-1);
+    }
+    if (mixed) {
+      // mixed precision: atomic plan, else the owner-compute plan
+      const float *adtf = (const float *)args[4].data_d;
+      float *resf = (float *)args[6].data_d;
+      op2h_kernel_timer_start(&ctx);
+      if (ctx.plan == OP2H_PLAN_ATOMIC) {
+        int g = grid_for(set->size, bs);
+        if (soa) k_res_atomic_mixed<true><<<g, bs, 0, st>>>(x, q, adtf, resf, pedge, pecell, set->size, sx, sq);
+        else k_res_atomic_mixed<false><<<g, bs, 0, st>>>(x, q, adtf, resf, pedge, pecell, set->size, sx, sq);
+      } else {
+        const op2h_gather_plan *p = op2h_plan_gather(args[6].map);
+        if (p->max_count > 4) {
+          printf("op2hip: mixed-precision res_calc supports at most 4 edges per cell\n");
+          exit(-1);
+        }
+        char key[96];
+        snprintf(key, sizeof key, "res_calc.gather4.%d.%d", args[0].map->index,
+                 args[2].map->index);
+        int fresh = 0;
+        int4 *rec = (int4 *)op2h_plan_user(key, sizeof(int4) * 4 * (size_t)p->ntargets, &fresh);
+        if (fresh)
+          k_pack_res_gather<4><<<grid_for(4 * (long long)p->ntargets, 256), 256, 0, st>>>(
+              pedge, pecell, p->offsets, p->entries, p->ntargets, rec);
+        int g = grid_for(4 * (long long)p->ntargets, bs);
+        if (soa) k_res_gather_packed_mixed<true, 4><<<g, bs, 0, st>>>(x, q, adtf, resf, rec, p->targets, p->ntargets, sx, sq);
+        else k_res_gather_packed_mixed<false, 4><<<g, bs, 0, st>>>(x, q, adtf, resf, rec, p->targets, p->ntargets, sx, sq);
+      }
+      op2h_kernel_timer_stop(&ctx);
+      KCHECK(hipGetLastError());
+      account(2, set, 8, args);
+      op2h_loop_end(&ctx);
+      return;
+    }
     switch (ctx.plan) {
       case OP2H_PLAN_ATOMIC: {
         op2h_kernel_timer_start(&ctx);
@@ -804,9 +1315,28 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       case OP2H_PLAN_GATHER: {
         const op2h_gather_plan *p = op2h_plan_gather(args[6].map);
         op2h_kernel_timer_start(&ctx);
-        int g = grid_for(p->ntargets, bs);
-        if (soa) k_res_gather<true><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
-        else k_res_gather<false><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+        if (p->max_count <= 4) {
+          // packed per-(cell, edge) records, built once per plan
+          char key[96];
+          snprintf(key, sizeof key, "res_calc.gather4.%d.%d", args[0].map->index,
+                   args[2].map->index);
+          int fresh = 0;
+          int4 *rec = (int4 *)op2h_plan_user(key, sizeof(int4) * 4 * (size_t)p->ntargets, &fresh);
+          if (fresh)
+            k_pack_res_gather<4><<<grid_for(4 * (long long)p->ntargets, 256), 256, 0, st>>>(
+                pedge, pecell, p->offsets, p->entries, p->ntargets, rec);
+          int g = grid_for(4 * (long long)p->ntargets, bs);
+          if (soa) k_res_gather_packed<true, 4><<<g, bs, 0, st>>>(x, q, adt, res, rec, p->targets, p->ntargets, sx, sq);
+          else k_res_gather_packed<false, 4><<<g, bs, 0, st>>>(x, q, adt, res, rec, p->targets, p->ntargets, sx, sq);
+        } else if (p->max_count <= 8) {
+          int g = grid_for(8 * (long long)p->ntargets, bs);
+          if (soa) k_res_gather_lanes<true, 8><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+          else k_res_gather_lanes<false, 8><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+        } else {
+          int g = grid_for(p->ntargets, bs);
+          if (soa) k_res_gather<true><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+          else k_res_gather<false><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+        }
         op2h_kernel_timer_stop(&ctx);
         break;
       }
@@ -874,6 +1404,26 @@ extern "C" void op_par_loop_airfoil_4_bres_calc(char const *name, op_set set,
     const int2 *pbedge = (const int2 *)args[0].map->map_d;
     const int *pbecell = args[2].map->map_d;
     const int bs = ctx.block_size;
+    if (is_float(args[4])) {
+      const float *adtf = (const float *)args[3].data_d;
+      float *resf = (float *)args[4].data_d;
+      op2h_kernel_timer_start(&ctx);
+      if (ctx.plan == OP2H_PLAN_ATOMIC) {
+        int g = grid_for(set->size, bs);
+        if (soa) k_bres_atomic_mixed<true><<<g, bs, 0, st>>>(x, q, adtf, resf, pbedge, pbecell, bound, set->size, sx, sq);
+        else k_bres_atomic_mixed<false><<<g, bs, 0, st>>>(x, q, adtf, resf, pbedge, pbecell, bound, set->size, sx, sq);
+      } else {
+        const op2h_gather_plan *p = op2h_plan_gather(args[4].map);
+        int g = grid_for(p->ntargets, bs);
+        if (soa) k_bres_gather_mixed<true><<<g, bs, 0, st>>>(x, q, adtf, resf, pbedge, bound, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+        else k_bres_gather_mixed<false><<<g, bs, 0, st>>>(x, q, adtf, resf, pbedge, bound, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+      }
+      op2h_kernel_timer_stop(&ctx);
+      KCHECK(hipGetLastError());
+      account(3, set, 6, args);
+      op2h_loop_end(&ctx);
+      return;
+    }
     switch (ctx.plan) {
       case OP2H_PLAN_ATOMIC: {
         op2h_kernel_timer_start(&ctx);
@@ -972,16 +1522,28 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
   double *part_max = part_sum + g;
   int *part_loc = (int *)(part_max + g);
   op2h_kernel_timer_start(&ctx);
-  if (soa)
+  if (is_float(args[2]) != is_float(args[3])) {
+    printf("op2hip: update needs res and adt both FP32 (mixed mode) or both FP64\n");
+    exit(-1);
+  }
+  if (is_float(args[2])) {
+    if (soa)
+      k_update_mixed<true><<<g, bs, 0, st>>>((const double *)args[0].data_d, (double *)args[1].data_d,
+                                             (float *)args[2].data_d, (const float *)args[3].data_d,
+                                             set->size, s, part_sum, part_max, part_loc);
+    else
+      k_update_mixed<false><<<g, bs, 0, st>>>((const double *)args[0].data_d, (double *)args[1].data_d,
+                                              (float *)args[2].data_d, (const float *)args[3].data_d,
+                                              set->size, s, part_sum, part_max, part_loc);
+  } else if (soa)
     k_update<true><<<g, bs, 0, st>>>((const double *)args[0].data_d, (double *)args[1].data_d,
                                      (double *)args[2].data_d, (const double *)args[3].data_d,
-                                     set->size, s, part_sum, part_max, part_loc,
-                                     op2h_reduct_counter(), slot_rms, slot_max);
+                                     set->size, s, part_sum, part_max, part_loc);
   else
     k_update<false><<<g, bs, 0, st>>>((const double *)args[0].data_d, (double *)args[1].data_d,
                                       (double *)args[2].data_d, (const double *)args[3].data_d,
-                                      set->size, s, part_sum, part_max, part_loc,
-                                      op2h_reduct_counter(), slot_rms, slot_max);
+                                      set->size, s, part_sum, part_max, part_loc);
+  k_update_finish<<<1, 256, 0, st>>>(part_sum, part_max, part_loc, g, slot_rms, slot_max);
   op2h_kernel_timer_stop(&ctx);
   KCHECK(hipGetLastError());
   account(4, set, 8, args);

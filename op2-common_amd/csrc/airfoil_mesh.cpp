@@ -2,6 +2,7 @@
 // Conventions follow apps/mesh_generators/naca0012.m:217-303 (orientation,
 // bound codes) and the reader in apps/c/airfoil/airfoil_plain/dp/airfoil.cpp:94-141.
 #include "op2amd_mesh.h"
+#include "op2h_rcm.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -91,34 +92,6 @@ void cell_graph(const op2amd_mesh *m, std::vector<int> &off,
   }
   for (int c = 0; c < m->ncell; ++c)
     std::sort(adj.begin() + off[c], adj.begin() + off[c + 1]);
-}
-
-// Breadth-first level structure from `root` restricted to unvisited vertices;
-// returns the eccentricity and writes the last level.
-int bfs_levels(int root, const std::vector<int> &off, const std::vector<int> &adj,
-               const std::vector<char> &done, std::vector<int> &level,
-               std::vector<int> &last) {
-  std::vector<int> frontier{root}, next;
-  std::fill(level.begin(), level.end(), -1);
-  level[root] = 0;
-  int depth = 0;
-  last = frontier;
-  while (!frontier.empty()) {
-    next.clear();
-    for (int v : frontier)
-      for (int k = off[v]; k < off[v + 1]; ++k) {
-        int w = adj[k];
-        if (!done[w] && level[w] < 0) {
-          level[w] = depth + 1;
-          next.push_back(w);
-        }
-      }
-    if (next.empty()) break;
-    last = next;
-    frontier.swap(next);
-    ++depth;
-  }
-  return depth;
 }
 
 }  // namespace
@@ -279,48 +252,9 @@ extern "C" int op2amd_mesh_renumber_rcm(op2amd_mesh *m, int *cell_perm,
   const int nc = m->ncell;
   std::vector<int> off, adj;
   cell_graph(m, off, adj);
-  auto deg = [&](int v) { return off[v + 1] - off[v]; };
-  auto by_deg = [&](int a, int b) {
-    return deg(a) != deg(b) ? deg(a) < deg(b) : a < b;
-  };
-
-  std::vector<char> done(nc, 0);
-  std::vector<int> cm;  // Cuthill-McKee order
-  cm.reserve(nc);
-  std::vector<int> level(nc), last, nb;
-  while ((int)cm.size() < nc) {
-    int root = -1;
-    for (int v = 0; v < nc; ++v)
-      if (!done[v] && (root < 0 || by_deg(v, root))) root = v;
-    // George-Liu pseudo-peripheral vertex search.
-    int ecc = bfs_levels(root, off, adj, done, level, last);
-    for (int it = 0; it < 16; ++it) {
-      int cand = *std::min_element(last.begin(), last.end(), by_deg);
-      std::vector<int> last2;
-      int ecc2 = bfs_levels(cand, off, adj, done, level, last2);
-      if (ecc2 <= ecc) break;
-      root = cand;
-      ecc = ecc2;
-      last.swap(last2);
-    }
-    size_t head = cm.size();
-    cm.push_back(root);
-    done[root] = 1;
-    while (head < cm.size()) {
-      int v = cm[head++];
-      nb.clear();
-      for (int k = off[v]; k < off[v + 1]; ++k)
-        if (!done[adj[k]]) nb.push_back(adj[k]);
-      std::sort(nb.begin(), nb.end(), by_deg);
-      for (int w : nb) {
-        done[w] = 1;
-        cm.push_back(w);
-      }
-    }
-  }
-  std::vector<int> cnew(nc);
-  for (int k = 0; k < nc; ++k) cnew[cm[nc - 1 - k]] = k;
-
+  std::vector<int> cnew = op2h::rcm_order(nc, off, adj);
+  std::vector<int> cm(nc);  // cells in new order
+  for (int c = 0; c < nc; ++c) cm[cnew[c]] = c;
   // edges: stable sort by new index of ecell[e][0]
   std::vector<int> eorder(m->nedge);
   std::iota(eorder.begin(), eorder.end(), 0);
@@ -338,7 +272,7 @@ extern "C" int op2amd_mesh_renumber_rcm(op2amd_mesh *m, int *cell_perm,
   std::vector<int> nnew(m->nnode, -1);
   int counter = 0;
   for (int k = 0; k < nc; ++k) {
-    int c = cm[nc - 1 - k];
+    int c = cm[k];
     for (int d = 0; d < 4; ++d) {
       int v = m->cell[4 * c + d];
       if (nnew[v] < 0) nnew[v] = counter++;

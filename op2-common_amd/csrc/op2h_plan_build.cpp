@@ -357,6 +357,9 @@ std::unique_ptr<GatherPlanRec> build_gather_plan(op_map map, hipStream_t stream)
   for (size_t i = 0; i < targets.size(); ++i) offsets[i] = count[targets[i]];
   offsets[targets.size()] = count[to];
   P.ntargets = (int)targets.size();
+  P.max_count = 0;
+  for (size_t i = 0; i < targets.size(); ++i)
+    P.max_count = std::max(P.max_count, offsets[i + 1] - offsets[i]);
   P.offsets = rec->mem.upload(offsets, stream);
   P.entries = rec->mem.upload(entries, stream);
   P.targets = rec->mem.upload(targets, stream);

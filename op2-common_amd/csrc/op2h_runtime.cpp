@@ -23,6 +23,7 @@
 
 #include "op2_hip_rt.h"
 #include "op2h_plan_build.h"
+#include "op2h_rcm.h"
 
 // ---------------------------------------------------------------------------
 // global state (op_lib_core.cpp:55-86)
@@ -60,6 +61,15 @@ char *copy_str(const char *s) {
 struct LoopConfig {
   int plan = OP2H_PLAN_AUTO;
   int block_size = 0;
+};
+
+struct TuneState {
+  std::vector<std::pair<int, int>> cands;  // (plan, block)
+  std::vector<double> best_ms;             // per candidate, min over trials
+  size_t next = 0;
+  int trials_done = 0;
+  bool done = false;
+  int plan = 0, block = 0;
 };
 
 struct LoopStats {
@@ -123,6 +133,14 @@ struct Runtime {
   int history_words = 0;   // words per launch
   int history_capacity = 0;  // launches
   std::map<op_map, long long> distinct_targets;  // per map, for byte counts
+  std::vector<std::vector<int>> set_new_of_old;  // composed op_renumber perms
+  std::map<std::string, std::pair<void *, size_t>> user_plans;
+  // autotuning
+  bool autotune = false;
+  int tune_trials = 2;
+  std::map<std::string, TuneState> tune;
+  std::vector<op2h_tuning_record> tune_records;
+  hipEvent_t tune_ev[2] = {nullptr, nullptr};
   // profiling
   std::vector<std::pair<std::string, double>> prof_stack;
   std::map<std::string, double> prof_totals;
@@ -342,6 +360,8 @@ extern "C" void op_exit(void) {
   R.staged.clear();
   R.color.clear();
   R.gather.clear();
+  for (auto &kv : R.user_plans) (void)hipFree(kv.second.first);
+  R.user_plans.clear();
   for (op_dat d : R.dats) {
     if (!d) continue;
     if (d->data_d) (void)hipFree(d->data_d);
@@ -379,6 +399,12 @@ extern "C" void op_exit(void) {
   R.capture_slots.clear();
   R.stats.clear();
   R.distinct_targets.clear();
+  R.set_new_of_old.clear();
+  R.tune.clear();
+  R.autotune = false;
+  if (R.tune_ev[0]) (void)hipEventDestroy(R.tune_ev[0]);
+  if (R.tune_ev[1]) (void)hipEventDestroy(R.tune_ev[1]);
+  R.tune_ev[0] = R.tune_ev[1] = nullptr;
   R.loop_cfg.clear();
   R.all_cfg = LoopConfig();
   R.event_pool.clear();
@@ -838,7 +864,6 @@ extern "C" void op_profile_output(void) {
 
 // ---- single-rank no-ops (op_dummy_singlenode.cpp:43-208) ----
 extern "C" void op_partition(const char *, const char *, op_set, op_map, op_dat) {}
-extern "C" void op_renumber(op_map) {}
 extern "C" int op_mpi_halo_exchanges(op_set set, int, op_arg *) { return set->size; }
 extern "C" int op_mpi_halo_exchanges_cuda(op_set set, int, op_arg *) { return set->size; }
 extern "C" int op_mpi_halo_exchanges_grouped(op_set set, int, op_arg *, int) {
@@ -854,6 +879,180 @@ extern "C" void op_mpi_reduce_double(op_arg *, double *) {}
 extern "C" void op_mpi_reduce_int(op_arg *, int *) {}
 extern "C" void op_mpi_reduce_bool(op_arg *, bool *) {}
 extern "C" void op_mpi_barrier(void) {}
+
+// ===========================================================================
+// renumbering (op2/src/externlib/op_renumber.cpp:72-198,224-505)
+// ===========================================================================
+namespace {
+
+struct Renumber {
+  std::vector<std::vector<int>> perm, iperm;  // per set: new_of_old, old_of_new
+
+  // op_renumber.cpp:72-151: give `to` an order derived from `from`, then
+  // recurse over the maps touching `to`, in declaration order
+  void propagate(op_set from, op_set to) {
+    Runtime &R = rt();
+    if (to->size == 0) return;
+    std::vector<int> &pt = perm[to->index];
+    if (pt.empty()) {  // a map to->from: stable sort by new index of its first column
+      for (op_map m : R.maps)
+        if (m->to == from && m->from == to) {
+          std::vector<int> order(to->size);
+          for (int i = 0; i < to->size; ++i) order[i] = i;
+          const std::vector<int> &pf = perm[from->index];
+          std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return pf[m->map[(size_t)m->dim * a]] < pf[m->map[(size_t)m->dim * b]];
+          });
+          pt.assign(to->size, 0);
+          for (int k = 0; k < to->size; ++k) pt[order[k]] = k;
+          break;
+        }
+    }
+    if (pt.empty()) {  // a map from->to that is onto: first visit in new order
+      for (op_map m : R.maps)
+        if (m->to == to && m->from == from) {
+          std::vector<int> cand(to->size, -1);
+          int counter = 0;
+          for (int i = 0; i < from->size; ++i)
+            for (int d = 0; d < m->dim; ++d) {
+              int idx = m->map[(size_t)iperm[from->index][i] * m->dim + d];
+              if (cand[idx] == -1) cand[idx] = counter++;
+            }
+          if (counter == to->size) {
+            pt.swap(cand);
+            break;
+          }
+        }
+    }
+    if (pt.empty()) return;
+    iperm[to->index].assign(to->size, 0);
+    for (int i = 0; i < to->size; ++i) iperm[to->index][pt[i]] = i;
+    for (op_map m : R.maps)
+      if (m->to == to && perm[m->from->index].empty()) propagate(to, m->from);
+    for (op_map m : R.maps)
+      if (m->from == to && perm[m->to->index].empty()) propagate(to, m->to);
+  }
+};
+
+}  // namespace
+
+extern "C" void op_renumber(op_map base) {
+  Runtime &R = rt();
+  if (R.capturing) {
+    printf("op_renumber error -- not allowed during graph capture\n");
+    exit(-1);
+  }
+  if (!base || base->dim < 2) {
+    printf("op_renumber error -- the base map must have dimension >= 2\n");
+    exit(-1);
+  }
+  op2h_device_sync();
+  for (op_dat d : R.dats)
+    if (d && d->dirty_hd == 2) download_dat(d);
+  // graph of base->to: two elements are adjacent when one element of
+  // base->from references both
+  op_set S = base->to;
+  const int n = S->size;
+  std::vector<std::vector<int>> nbr(n);
+  for (int e = 0; e < base->from->size; ++e)
+    for (int a = 0; a < base->dim; ++a)
+      for (int b = 0; b < base->dim; ++b) {
+        int u = base->map[(size_t)e * base->dim + a], v = base->map[(size_t)e * base->dim + b];
+        if (u != v) nbr[u].push_back(v);
+      }
+  std::vector<int> off(n + 1, 0), adj;
+  for (int v = 0; v < n; ++v) {
+    std::sort(nbr[v].begin(), nbr[v].end());
+    nbr[v].erase(std::unique(nbr[v].begin(), nbr[v].end()), nbr[v].end());
+    off[v + 1] = off[v] + (int)nbr[v].size();
+  }
+  adj.reserve(off[n]);
+  for (int v = 0; v < n; ++v) adj.insert(adj.end(), nbr[v].begin(), nbr[v].end());
+  auto bandwidth = [&](double *avg) {
+    long long sum = 0;
+    int mx = 0;
+    for (int e = 0; e < base->from->size; ++e) {
+      int dist = 0;
+      for (int a = 0; a < base->dim; ++a)
+        for (int b = 0; b < base->dim; ++b)
+          dist = std::max(dist, std::abs(base->map[(size_t)e * base->dim + a] -
+                                         base->map[(size_t)e * base->dim + b]));
+      mx = std::max(mx, dist);
+      sum += dist;
+    }
+    *avg = base->from->size ? (double)sum / base->from->size : 0.0;
+    return mx;
+  };
+  double avg0, avg1;
+  int max0 = bandwidth(&avg0);
+
+  Renumber rn;
+  rn.perm.assign(R.sets.size(), {});
+  rn.iperm.assign(R.sets.size(), {});
+  rn.perm[S->index] = op2h::rcm_order(n, off, adj);
+  rn.propagate(S, S);
+
+  // reorder_set (op_renumber.cpp:153-198): rows of maps from the set and of
+  // dats on it move; entries of maps into the set are renumbered
+  for (op_set s : R.sets) {
+    const std::vector<int> &p = rn.perm[s->index];
+    if (p.empty() || s->size == 0) continue;
+    for (op_map m : R.maps) {
+      if (m->from == s) {
+        std::vector<int> tmp(m->map, m->map + (size_t)s->size * m->dim);
+        for (int i = 0; i < s->size; ++i)
+          std::copy(tmp.begin() + (size_t)i * m->dim, tmp.begin() + (size_t)(i + 1) * m->dim,
+                    m->map + (size_t)p[i] * m->dim);
+      }
+    }
+    for (op_map m : R.maps)
+      if (m->to == s)
+        for (size_t i = 0; i < (size_t)m->from->size * m->dim; ++i) m->map[i] = p[m->map[i]];
+    for (op_dat d : R.dats) {
+      if (!d || d->set != s) continue;
+      std::vector<char> tmp(d->data, d->data + (size_t)d->size * s->size);
+      for (int i = 0; i < s->size; ++i)
+        std::memcpy(d->data + (size_t)p[i] * d->size, tmp.data() + (size_t)i * d->size,
+                    d->size);
+    }
+    if (R.set_new_of_old.size() < R.sets.size()) R.set_new_of_old.resize(R.sets.size());
+    std::vector<int> &acc = R.set_new_of_old[s->index];
+    if (acc.empty()) {
+      acc = p;
+    } else {
+      for (int &v : acc) v = p[v];
+    }
+  }
+  for (op_map m : R.maps)
+    HIPCHECK(hipMemcpy(m->map_d, m->map, (size_t)m->from->size * m->dim * sizeof(int),
+                       hipMemcpyHostToDevice));
+  for (op_dat d : R.dats)
+    if (d) upload_dat(d);
+  R.staged.clear();
+  R.color.clear();
+  R.gather.clear();
+  for (auto &kv : R.user_plans) (void)hipFree(kv.second.first);
+  R.user_plans.clear();
+  R.distinct_targets.clear();
+  int max1 = bandwidth(&avg1);
+  if (OP_diags > 0) {
+    op_printf("Renumbering using base map %s (reverse Cuthill-McKee)\n", base->name);
+    op_printf("Before renumbering: maximum bandwidth = %d average bandwidth = %.1f\n", max0, avg0);
+    op_printf("After renumbering: maximum bandwidth = %d average bandwidth = %.1f\n", max1, avg1);
+  }
+}
+
+extern "C" int op2h_set_permutation(op_set set, int *new_of_old) {
+  Runtime &R = rt();
+  if ((size_t)set->index < R.set_new_of_old.size() &&
+      !R.set_new_of_old[set->index].empty()) {
+    std::copy(R.set_new_of_old[set->index].begin(), R.set_new_of_old[set->index].end(),
+              new_of_old);
+    return 1;
+  }
+  for (int i = 0; i < set->size; ++i) new_of_old[i] = i;
+  return 0;
+}
 
 // ===========================================================================
 // backend controls
@@ -887,6 +1086,32 @@ extern "C" int op2h_get_sync_mode(void) { return rt().sync_mode; }
 extern "C" void op2h_set_plan_builder(int on_device) { rt().plan_on_device = on_device; }
 
 extern "C" void op2h_enable_kernel_timing(int on) { rt().kernel_timing = on != 0; }
+
+extern "C" void op2h_set_autotune(int on, int trials) {
+  rt().autotune = on != 0;
+  rt().tune_trials = std::max(1, trials);
+}
+
+extern "C" int op2h_tuning_done(void) {
+  for (auto &kv : rt().tune)
+    if (!kv.second.done) return 0;
+  return 1;
+}
+
+extern "C" int op2h_tuning_records(int max, op2h_tuning_record *out) {
+  int n = std::min((int)rt().tune_records.size(), std::max(max, 0));
+  if (out)
+    for (int i = 0; i < n; ++i) out[i] = rt().tune_records[i];
+  return (int)rt().tune_records.size();
+}
+
+extern "C" int op2h_tuning_choice(const char *name, int *plan, int *block_size) {
+  auto it = rt().tune.find(name);
+  if (it == rt().tune.end() || !it->second.done) return -1;
+  if (plan) *plan = it->second.plan;
+  if (block_size) *block_size = it->second.block;
+  return 0;
+}
 
 extern "C" int op2h_loop_stats(const char *name, int *calls, double *kernel_ms) {
   resolve_timers();
@@ -963,6 +1188,38 @@ extern "C" void op2h_loop_begin(op2h_loop_ctx *ctx, int kernel_idx,
   LoopConfig c = R.all_cfg;
   auto it = R.loop_cfg.find(name);
   if (it != R.loop_cfg.end()) c = it->second;
+  if (R.autotune && c.plan == OP2H_PLAN_AUTO && c.block_size == 0 && set->size > 0) {
+    TuneState &ts = R.tune[name];
+    if (ts.cands.empty()) {
+      if (def_plan == OP2H_PLAN_STAGED) {
+        for (int p : {OP2H_PLAN_ATOMIC, OP2H_PLAN_COLOR, OP2H_PLAN_STAGED, OP2H_PLAN_GATHER})
+          for (int b : {128, 256, 512}) ts.cands.push_back({p, b});
+      } else {
+        for (int b : {128, 256, 512, 1024}) ts.cands.push_back({def_plan, b});
+      }
+      ts.best_ms.assign(ts.cands.size(), 1e300);
+    }
+    if (!ts.done && !R.capturing) {
+      ctx->plan = ts.cands[ts.next].first;
+      ctx->block_size = ts.cands[ts.next].second;
+      ctx->part_size = ctx->block_size;
+      ctx->tuning = 1;
+      return;
+    }
+    if (!ts.done) {  // capturing before tuning finished: best so far
+      size_t bi = 0;
+      for (size_t i = 0; i < ts.best_ms.size(); ++i)
+        if (ts.best_ms[i] < ts.best_ms[bi]) bi = i;
+      ctx->plan = ts.cands[bi].first;
+      ctx->block_size = ts.cands[bi].second;
+      ctx->part_size = ctx->block_size;
+      return;
+    }
+    ctx->plan = ts.plan;
+    ctx->block_size = ts.block;
+    ctx->part_size = ctx->block_size;
+    return;
+  }
   ctx->plan = c.plan == OP2H_PLAN_AUTO ? def_plan : c.plan;
   ctx->block_size = c.block_size ? c.block_size
                                  : (OP_block_size > 0 && OP_block_size % 64 == 0 &&
@@ -974,6 +1231,14 @@ extern "C" void op2h_loop_begin(op2h_loop_ctx *ctx, int kernel_idx,
 
 extern "C" void op2h_kernel_timer_start(op2h_loop_ctx *ctx) {
   Runtime &R = rt();
+  if (ctx->tuning) {
+    if (!R.tune_ev[0]) {
+      HIPCHECK(hipEventCreate(&R.tune_ev[0]));
+      HIPCHECK(hipEventCreate(&R.tune_ev[1]));
+    }
+    HIPCHECK(hipEventRecord(R.tune_ev[0], R.stream));
+    return;
+  }
   if (!R.kernel_timing || R.capturing) return;
   PendingTimer t{ctx->kernel_idx, take_event(), take_event()};
   HIPCHECK(hipEventRecord(t.a, R.stream));
@@ -982,6 +1247,40 @@ extern "C" void op2h_kernel_timer_start(op2h_loop_ctx *ctx) {
 
 extern "C" void op2h_kernel_timer_stop(op2h_loop_ctx *ctx) {
   Runtime &R = rt();
+  if (ctx->tuning) {
+    HIPCHECK(hipEventRecord(R.tune_ev[1], R.stream));
+    HIPCHECK(hipEventSynchronize(R.tune_ev[1]));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, R.tune_ev[0], R.tune_ev[1]));
+    TuneState &ts = R.tune[ctx->name];
+    op2h_tuning_record rec;
+    std::memset(&rec, 0, sizeof(rec));
+    std::snprintf(rec.loop, sizeof rec.loop, "%s", ctx->name);
+    rec.set_size = ctx->set->size;
+    rec.plan = ctx->plan;
+    rec.block_size = ctx->block_size;
+    rec.kernel_ms = ms;
+    // the first call of a variant also builds its plan: it is not timed
+    if (ts.trials_done > 0) {
+      R.tune_records.push_back(rec);
+      ts.best_ms[ts.next] = std::min(ts.best_ms[ts.next], (double)ms);
+    }
+    if (++ts.trials_done > R.tune_trials) {
+      ts.trials_done = 0;
+      if (++ts.next == ts.cands.size()) {
+        size_t bi = 0;
+        for (size_t i = 0; i < ts.best_ms.size(); ++i)
+          if (ts.best_ms[i] < ts.best_ms[bi]) bi = i;
+        ts.done = true;
+        ts.plan = ts.cands[bi].first;
+        ts.block = ts.cands[bi].second;
+        if (OP_diags > 1)
+          printf(" autotune %s: plan %d block %d (%.2f us)\n", ctx->name, ts.plan,
+                 ts.block, ts.best_ms[bi] * 1e3);
+      }
+    }
+    return;
+  }
   if (!R.kernel_timing || R.capturing || R.pending.empty()) return;
   (void)ctx;
   HIPCHECK(hipEventRecord(R.pending.back().b, R.stream));
@@ -1002,8 +1301,11 @@ extern "C" void op2h_loop_end(op2h_loop_ctx *ctx) {
       if (ctx->args[m].argtype == OP_ARG_INFO && ctx->args[m].opt) info = m;
     bool with_info = (a.acc == OP_MIN || a.acc == OP_MAX) && info >= 0;
     size_t bytes = (size_t)a.size + (with_info ? ctx->args[info].size : 0);
-    HIPCHECK(hipMemcpyAsync(ctx->gbl_slot_h[g], ctx->gbl_slot_d[g], bytes,
-                            hipMemcpyDeviceToHost, R.stream));
+    // inside a captured graph the slots reach the host through the history
+    // record of the launch (no per-loop copy node)
+    if (!R.capturing)
+      HIPCHECK(hipMemcpyAsync(ctx->gbl_slot_h[g], ctx->gbl_slot_d[g], bytes,
+                              hipMemcpyDeviceToHost, R.stream));
     if (R.capturing) {
       DeferredGbl d;
       d.acc = a.acc;
@@ -1231,6 +1533,29 @@ extern "C" const op2h_gather_plan *op2h_plan_gather(op_map map) {
   return p;
 }
 
+extern "C" void *op2h_plan_user(const char *key, size_t bytes, int *fresh) {
+  Runtime &R = rt();
+  auto it = R.user_plans.find(key);
+  if (it != R.user_plans.end() && it->second.second == bytes) {
+    *fresh = 0;
+    return it->second.first;
+  }
+  if (R.capturing) {
+    printf("op2hip: plan %s must be built before graph capture\n", key);
+    exit(-1);
+  }
+  if (it != R.user_plans.end()) {
+    HIPCHECK(hipStreamSynchronize(R.stream));
+    (void)hipFree(it->second.first);
+    R.user_plans.erase(it);
+  }
+  void *p = nullptr;
+  HIPCHECK(hipMalloc(&p, std::max(bytes, (size_t)16)));
+  R.user_plans[key] = {p, bytes};
+  *fresh = 1;
+  return p;
+}
+
 extern "C" int op2h_staged_plan_download(const op2h_staged_plan *p, int *thrcol,
                                          int *blkcol, int *offset, int *nelems) {
   return op2h::download_staged(p, thrcol, blkcol, offset, nelems);
@@ -1344,6 +1669,20 @@ extern "C" int op2h_graph_sync(void) {
   Runtime &R = rt();
   if (R.capturing) return -1;
   HIPCHECK(hipStreamSynchronize(R.stream));
+  // the latest launch's record holds every slot of the graph, in record order
+  unsigned long long count = 0;
+  if (R.history_count)
+    HIPCHECK(hipMemcpy(&count, R.history_count, sizeof(count), hipMemcpyDeviceToHost));
+  if (count == 0) return 0;  // graph never launched: host values stay as they are
+  std::vector<double> rec(R.history_words);
+  size_t pos = (size_t)((count - 1) % R.history_capacity) * R.history_words;
+  HIPCHECK(hipMemcpy(rec.data(), R.history + pos, R.history_words * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  size_t w = 0;
+  for (auto &d : R.deferred) {
+    std::memcpy(d.slot_h, &rec[w], d.bytes);
+    w += (d.bytes + 7) / 8;
+  }
   for (auto &d : R.deferred) {
     std::memcpy(d.host_ptr, d.snapshot.data(), d.snapshot.size());
     if (d.info_ptr)

@@ -18,7 +18,8 @@ class Opts(C.Structure):
                 ("block_res", C.c_int), ("plan_bres", C.c_int),
                 ("block_bres", C.c_int), ("block_direct", C.c_int),
                 ("block_adt", C.c_int), ("kernel_timing", C.c_int),
-                ("plan_on_device", C.c_int)]
+                ("plan_on_device", C.c_int), ("renumber", C.c_int),
+                ("mixed", C.c_int), ("autotune", C.c_int)]
 
 
 class Result(C.Structure):
@@ -26,7 +27,9 @@ class Result(C.Structure):
                 ("maxerr", C.POINTER(C.c_double)), ("errloc", C.POINTER(C.c_int)),
                 ("q_out", C.POINTER(C.c_double)), ("seconds", C.c_double),
                 ("setup_seconds", C.c_double), ("loop_kernel_ms", C.c_double * 5),
-                ("loop_calls", C.c_int * 5), ("loop_bytes", C.c_double * 5)]
+                ("loop_calls", C.c_int * 5), ("loop_bytes", C.c_double * 5),
+                ("chosen_plan", C.c_int * 5), ("chosen_block", C.c_int * 5),
+                ("eager_iterations", C.c_int)]
 
 
 lib.op2amd_airfoil_run.restype = C.c_int
@@ -36,11 +39,13 @@ lib.op2amd_airfoil_run.argtypes = [C.POINTER(op2amd_mesh), C.POINTER(Opts),
 
 def run(mesh, niter, mode="plain", warmup=0, alpha_mode=1, soa=False,
         plan_res="auto", block_res=0, plan_bres="auto", block_bres=0,
-        block_direct=0, block_adt=0, kernel_timing=False, plan_on_device=False):
+        block_direct=0, block_adt=0, kernel_timing=False, plan_on_device=False,
+        renumber=False, mixed=False, autotune=0):
     """Run the airfoil app; returns a dict of numpy arrays and timings."""
     o = Opts(niter, warmup, alpha_mode, MODES[mode], int(soa),
              PLAN_NAMES[plan_res], block_res, PLAN_NAMES[plan_bres], block_bres,
-             block_direct, block_adt, int(kernel_timing), int(plan_on_device))
+             block_direct, block_adt, int(kernel_timing), int(plan_on_device),
+             int(renumber), int(mixed), int(autotune))
     rms = np.zeros(niter)
     rms_k0 = np.zeros(niter)
     maxerr = np.zeros(niter)
@@ -61,4 +66,6 @@ def run(mesh, niter, mode="plain", warmup=0, alpha_mode=1, soa=False,
         "loop_kernel_ms": dict(zip(LOOPS, list(r.loop_kernel_ms))),
         "loop_calls": dict(zip(LOOPS, list(r.loop_calls))),
         "loop_bytes": dict(zip(LOOPS, list(r.loop_bytes))),
+        "chosen": {lp: (r.chosen_plan[i], r.chosen_block[i]) for i, lp in enumerate(LOOPS)},
+        "eager_iterations": r.eager_iterations,
     }

@@ -24,6 +24,9 @@ CASES = [
     dict(mode="graphed", plan_res="gather", plan_bres="gather"),
     dict(mode="plain", soa=True),
     dict(mode="graphed", plan_res="atomic", plan_bres="gather", soa=True),
+    dict(mode="tuned", plan_res="staged", plan_bres="staged", plan_on_device=True),
+    dict(mode="tuned", plan_res="color", plan_bres="color", plan_on_device=True),
+    dict(mode="graphed", plan_res="staged", plan_bres="atomic", renumber=True),
 ]
 
 

@@ -148,3 +148,28 @@ def test_direct_loops_and_reductions(soa):
     assert abs(rms.value - rms_o) <= 1e-14 * rms_o
     assert maxerr.value == mx_o and errloc.value == loc_o
     assert big.value == 1e300 and loc2.value == -3 and r2.value == 0.0
+
+
+def test_op_renumber_matches_generator_rcm():
+    """op_renumber(pecell) on the randomly numbered mesh gives the numbering
+    of the generator's RCM ordering: same RCM rule, same propagation rules."""
+    rnd = op2amd.mesh.generate(40, 20, 5, "random")
+    rcm = op2amd.mesh.generate(40, 20, 5, "rcm")
+    q, res = inputs(rnd, 0)
+    app = App(rnd, q, res)
+    try:
+        api.lib.op_renumber.argtypes = [api.op_map]
+        api.lib.op_renumber(app.pecell)
+        x = app.fetch(app.x, (rnd.nnode, 2))
+
+        def table(m, rows, dim):
+            return np.ctypeslib.as_array(m.contents.map, shape=(rows * dim,)).reshape(rows, dim).copy()
+        ecell = table(app.pecell, rnd.nedge, 2)
+        edge = table(app.pedge, rnd.nedge, 2)
+        cell = table(app.pcell, rnd.ncell, 4)
+        becell = table(app.pbecell, rnd.nbedge, 1).ravel()
+    finally:
+        app.close()
+    assert np.array_equal(x, rcm.x)
+    assert np.array_equal(ecell, rcm.ecell) and np.array_equal(edge, rcm.edge)
+    assert np.array_equal(cell, rcm.cell) and np.array_equal(becell, rcm.becell)

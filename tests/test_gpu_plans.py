@@ -72,7 +72,7 @@ def test_host_staged_plan_equals_reference_greedy(part):
 
 
 @pytest.mark.parametrize("I,J", [(60, 30), (1200, 600)])
-@pytest.mark.parametrize("builder", [0])
+@pytest.mark.parametrize("builder", [0, 1])
 def test_staged_and_flat_colourings_conflict_free(I, J, builder):
     mesh = op2amd.mesh.generate(I, J, 0, "random")
     q, res = inputs(mesh, 0)
@@ -94,3 +94,26 @@ def test_staged_and_flat_colourings_conflict_free(I, J, builder):
     assert block_conflicts(mesh.ecell, blk[block_of_edge], block_of_edge) == 0
     assert thread_conflicts(mesh.ecell, col, np.zeros(mesh.nedge, np.int64)) == 0
     assert ncol_flat <= 8
+
+
+@pytest.mark.parametrize("part", [64, 256])
+def test_device_thread_colours_equal_reference_greedy(part):
+    """The device builder colours threads with the reference's sequential
+    greedy (one lane per block); its block colouring is its own but valid."""
+    mesh = op2amd.mesh.generate(60, 30, 0, "rcm")
+    q, res = inputs(mesh, 0)
+    app = App(mesh, q, res)
+    try:
+        api.op2h_set_plan_builder(1)
+        p = api.op2h_plan_staged_for(b"res_calc", app.edges, part, 8, res_args(app))
+        assert p.contents.built_on_device == 1
+        thr, blk, off, nel = download_staged(p, mesh.nedge)
+        ncolors = p.contents.ncolors
+    finally:
+        app.close()
+    n, args = oracle.airfoil_plan_args(mesh, "res_calc")
+    ref = oracle.plan_build(n, args, part)
+    assert np.array_equal(thr, ref["thrcol"])
+    block_of_edge = np.repeat(np.arange(len(nel)), nel)
+    assert block_conflicts(mesh.ecell, blk[block_of_edge], block_of_edge) == 0
+    assert ncolors <= 2 * ref["ncolors"] + 2
