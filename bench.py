@@ -121,8 +121,93 @@ def measure(op2amd, mesh, K, W, args, renumber=False, plans=None, plain_too=True
     return out
 
 
+def workload_large(op2amd, args):
+    """4800x2400 (11.5M cells): FP64 vs FP32 mixed mode, both tuned+graphed.
+    Error reference: the FP64 owner-compute run, whose q, maxerr and errloc
+    equal the sequential executor's bit for bit (tests/test_gpu_airfoil.py)."""
+    I, J = (int(v) for v in args.large_mesh.split("x"))
+    mesh = op2amd.mesh.generate(I, J, 0, args.ordering)
+    n, W = args.large_iters, 20
+    out = {}
+    common = dict(plan_res="gather", block_res=512, plan_bres="atomic", block_bres=128,
+                  block_direct=1024, block_adt=1024)
+    ref = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, **common)
+    out["fp64_steps_per_s"] = (n - W) / ref["seconds"]
+    mixed = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, mixed=True, **common)
+    out["mixed_steps_per_s"] = (n - W) / mixed["seconds"]
+    out["mixed_rms_rel_err_last"] = float(abs(mixed["rms"][-1] / ref["rms"][-1] - 1))
+    out["mixed_rms_rel_err_max"] = rel(mixed["rms"], ref["rms"])
+    out["mixed_q_rel_err"] = float(np.max(np.abs(mixed["q"] - ref["q"])) / np.max(np.abs(ref["q"])))
+    for label, kw in (("fp64", {}), ("mixed", {"mixed": True})):
+        t = op2amd.airfoil.run(mesh, 12, mode="tuned", warmup=2, kernel_timing=True, **common, **kw)
+        out[f"{label}_per_loop"] = {
+            lp: {"avg_us": round(1e3 * t["loop_kernel_ms"][lp] / max(t["loop_calls"][lp], 1), 2),
+                 "bytes_per_call": t["loop_bytes"][lp],
+                 "achieved_GBs": round(t["loop_bytes"][lp] / (1e-3 * t["loop_kernel_ms"][lp] /
+                                                              max(t["loop_calls"][lp], 1)) / 1e9, 1)}
+            for lp in op2amd.airfoil.LOOPS}
+    dom = out["fp64_per_loop"]["res_calc"]
+    line = {
+        "metric": "airfoil steps/s, 11.5M cells, FP64 vs FP32-flux mixed mode",
+        "value": round(out["fp64_steps_per_s"], 2), "unit": "steps/s", "n_gpus": 1,
+        "steps": n - W, "warmup": W, "ms_per_step": round(1e3 / out["fp64_steps_per_s"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"airfoil {I}x{J} ({mesh.ncell} cells), {args.ordering} "
+                               f"ordering, {n} iterations", **out},
+        "roofline": {"bound": "hbm", "kernel": "res_calc (gather, block 512)",
+                     "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def workload_ensemble(op2amd, args):
+    """256 uneven meshes (40x20 .. 640x320, per-member Mach and alpha):
+    one launch per loop over all members, graph-captured, vs one launch per
+    loop per member."""
+    sp = op2amd.ensemble.spec(nmesh=args.ensemble_size)
+    n, W = args.ensemble_iters, 5
+    batched = op2amd.ensemble.run(sp, n, warmup=W, batched=True, graphed=True)
+    looped = op2amd.ensemble.run(sp, min(n, 30), warmup=W, batched=False)
+    timing = op2amd.ensemble.run(sp, 12, warmup=2, batched=False, member_timing=True)
+    nc = batched["total_cells"]
+    b_ups = nc * (n - W) / batched["seconds"]
+    l_ups = nc * (min(n, 30) - W) / looped["seconds"]
+    per = timing["member_step_ms"]
+    same = bool(np.array_equal(batched["rms"][:min(n, 30)], looped["rms"][:min(n, 30)]))
+    line = {
+        "metric": "ensemble cell-updates/s (256 uneven airfoil meshes, FP64)",
+        "value": round(b_ups, 1), "unit": "cell-steps/s", "n_gpus": 1, "steps": n - W,
+        "warmup": W, "ms_per_step": round(1e3 * batched["seconds"] / (n - W), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{sp.nmesh} members, {nc} cells, {batched['total_edges']} edges",
+                   "batched_graphed_cell_steps_per_s": b_ups,
+                   "looped_cell_steps_per_s": l_ups,
+                   "speedup_batched_vs_looped": b_ups / l_ups,
+                   "batched_step_ms": 1e3 * batched["seconds"] / (n - W),
+                   "looped_step_ms": 1e3 * looped["seconds"] / (min(n, 30) - W),
+                   "looped_member_step_ms_p50": float(np.percentile(per, 50)),
+                   "looped_member_step_ms_p99": float(np.percentile(per, 99)),
+                   "looped_member_step_ms_sum": float(per.sum()),
+                   "rms_identical_batched_vs_looped": same},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="airfoil",
+                    choices=["airfoil", "airfoil-large", "ensemble"],
+                    help="airfoil: the headline 720k line (default); the others "
+                         "measure the 11.5M-cell FP64/mixed config and the ensemble")
+    ap.add_argument("--large-mesh", default="4800x2400")
+    ap.add_argument("--large-iters", type=int, default=2000)
+    ap.add_argument("--ensemble-size", type=int, default=256)
+    ap.add_argument("--ensemble-iters", type=int, default=200)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
@@ -141,6 +226,10 @@ def main():
         sys.exit(2)
 
     import op2amd
+    if args.workload == "airfoil-large":
+        return workload_large(op2amd, args)
+    if args.workload == "ensemble":
+        return workload_ensemble(op2amd, args)
     from oracle import oracle
     I, J = (int(v) for v in args.mesh.split("x"))
     K, W = args.steps, args.warmup

@@ -1,0 +1,82 @@
+/*
+ * Batched ensembles of independent airfoil meshes.
+ *
+ * An ensemble is many airfoil problems of different sizes and free streams
+ * advanced in lock step.  All member meshes are concatenated into one set of
+ * nodes/cells/edges/bedges with per-member offset tables; each airfoil loop
+ * is then ONE launch over every member (no edge crosses members, so the
+ * concatenation is exact), the per-member rms/maxerr/errloc reductions run
+ * through a tile table whose tiles never straddle two members, and the whole
+ * time step is captured in one HIP graph.  The baseline launches every loop
+ * once per member.  The arithmetic per element is the single-mesh airfoil's
+ * (owner-compute res_calc/bres_calc), so each member's trajectory equals its
+ * own sequential run (airfoil.cpp:221-297) bit for bit in q.
+ *
+ * Members: member m (0-based) has scale s = 1 + (max_I/min_I - 1) u, mesh
+ * size round(min_I s) x round(min_J s), Mach = mach_lo + (mach_hi - mach_lo) u',
+ * alpha = alpha_lo + (alpha_hi - alpha_lo) u'' degrees, with u, u', u'' drawn
+ * in that order per member from one SplitMix64 stream seeded with `seed`; its
+ * mesh is op2amd_mesh_generate(I, J, seed + 1 + m, ordering).  The free stream
+ * is qinf = (r, r u cos a, r u sin a, r e) with the constants of
+ * airfoil.cpp:149-159 (gam, gm1, p, r as float literals) and u = sqrt(gam p/r)
+ * Mach, e = p/(r gm1) + 0.5 u^2, Mach and a in double.
+ */
+#ifndef OP2AMD_ENSEMBLE_H
+#define OP2AMD_ENSEMBLE_H
+
+#include <stdint.h>
+
+#include "op2amd_mesh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int nmesh;
+  int min_I, min_J, max_I, max_J;
+  uint64_t seed;
+  double mach_lo, mach_hi, alpha_lo_deg, alpha_hi_deg;
+  int ordering; /* OP2AMD_ORDER_* of every member */
+} op2amd_ensemble_spec;
+
+typedef struct {
+  int I, J, ncell;
+  double mach, alpha_deg;
+  double qinf[4];
+} op2amd_ensemble_member;
+
+/* Fill members[0..nmesh-1].  Returns 0 on success. */
+int op2amd_ensemble_describe(const op2amd_ensemble_spec *spec,
+                             op2amd_ensemble_member *members);
+
+typedef struct {
+  int niter, warmup;
+  int batched;     /* 1: one launch per loop for all members; 0: per member */
+  int graphed;     /* batched only: time step replayed from a HIP graph */
+  int member_timing; /* per-member step time with HIP events (looped mode) */
+} op2amd_ensemble_opts;
+
+typedef struct {
+  /* caller-supplied arrays (any may be NULL) */
+  double *rms;          /* [niter][nmesh] sqrt(rms/ncell) of the 2nd update */
+  double *maxerr;       /* [niter][nmesh] */
+  int *errloc;          /* [niter][nmesh] cell index within the member */
+  double *q_out;        /* [total cells][4], members in order */
+  double *member_step_ms; /* [nmesh] mean step time per member (looped mode
+                             with member_timing; batched: the whole step) */
+  /* filled in */
+  double seconds;       /* wall time of iterations warmup+1..niter */
+  double setup_seconds;
+  long long total_cells, total_edges;
+} op2amd_ensemble_result;
+
+int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
+                        const op2amd_ensemble_opts *opts,
+                        op2amd_ensemble_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -23,6 +23,7 @@
 #include <cstring>
 
 #include "op2_hip_rt.h"
+#include "airfoil_device.h"
 
 #define KCHECK(x)                                                            \
   do {                                                                       \
@@ -33,23 +34,6 @@
       exit(-1);                                                              \
     }                                                                        \
   } while (0)
-
-// ---------------------------------------------------------------------------
-// application constants (op_decl_const, airfoil.cpp:204-210)
-// ---------------------------------------------------------------------------
-__constant__ double gam_d;
-__constant__ double gm1_d;
-__constant__ double cfl_d;
-__constant__ double eps_d;
-__constant__ double mach_d;
-__constant__ double alpha_d;
-__constant__ double qinf_d[4];
-// FP32 copies for the mixed-precision flux kernels
-__constant__ float gam_f;
-__constant__ float gm1_f;
-__constant__ float cfl_f;
-__constant__ float eps_f;
-__constant__ float qinf_f[4];
 
 extern "C" void op_decl_const_char(int dim, char const *type, int size,
                                    char *dat, char const *name) {
@@ -78,291 +62,9 @@ extern "C" void op_decl_const_char(int dim, char const *type, int size,
   else if (!strcmp(name, "qinf")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(qinf_f), f, 4 * sizeof(float)));
 }
 
-// ---------------------------------------------------------------------------
-// user kernels (apps/c/airfoil/airfoil_plain/dp/*.h)
-// ---------------------------------------------------------------------------
-namespace op2_k {
 
-__device__ __forceinline__ void adt_calc(const double *x1, const double *x2,
-                                         const double *x3, const double *x4,
-                                         const double *q, double *adt) {
-  double dx, dy, ri, u, v, c;
-  ri = 1.0 / q[0];
-  u = ri * q[1];
-  v = ri * q[2];
-  c = sqrt(gam_d * gm1_d * (ri * q[3] - 0.5 * (u * u + v * v)));
-  dx = x2[0] - x1[0];
-  dy = x2[1] - x1[1];
-  *adt = fabs(u * dy - v * dx) + c * sqrt(dx * dx + dy * dy);
-  dx = x3[0] - x2[0];
-  dy = x3[1] - x2[1];
-  *adt += fabs(u * dy - v * dx) + c * sqrt(dx * dx + dy * dy);
-  dx = x4[0] - x3[0];
-  dy = x4[1] - x3[1];
-  *adt += fabs(u * dy - v * dx) + c * sqrt(dx * dx + dy * dy);
-  dx = x1[0] - x4[0];
-  dy = x1[1] - x4[1];
-  *adt += fabs(u * dy - v * dx) + c * sqrt(dx * dx + dy * dy);
-  *adt = (*adt) / cfl_d;
-}
-
-__device__ __forceinline__ void res_calc(const double *x1, const double *x2,
-                                         const double *q1, const double *q2,
-                                         const double *adt1, const double *adt2,
-                                         double *res1, double *res2) {
-  double dx, dy, mu, ri, p1, vol1, p2, vol2, f;
-  dx = x1[0] - x2[0];
-  dy = x1[1] - x2[1];
-  ri = 1.0 / q1[0];
-  p1 = gm1_d * (q1[3] - 0.5 * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
-  vol1 = ri * (q1[1] * dy - q1[2] * dx);
-  ri = 1.0 / q2[0];
-  p2 = gm1_d * (q2[3] - 0.5 * ri * (q2[1] * q2[1] + q2[2] * q2[2]));
-  vol2 = ri * (q2[1] * dy - q2[2] * dx);
-  mu = 0.5 * ((*adt1) + (*adt2)) * eps_d;
-  f = 0.5 * (vol1 * q1[0] + vol2 * q2[0]) + mu * (q1[0] - q2[0]);
-  res1[0] += f;
-  res2[0] -= f;
-  f = 0.5 * (vol1 * q1[1] + p1 * dy + vol2 * q2[1] + p2 * dy) +
-      mu * (q1[1] - q2[1]);
-  res1[1] += f;
-  res2[1] -= f;
-  f = 0.5 * (vol1 * q1[2] - p1 * dx + vol2 * q2[2] - p2 * dx) +
-      mu * (q1[2] - q2[2]);
-  res1[2] += f;
-  res2[2] -= f;
-  f = 0.5 * (vol1 * (q1[3] + p1) + vol2 * (q2[3] + p2)) + mu * (q1[3] - q2[3]);
-  res1[3] += f;
-  res2[3] -= f;
-}
-
-__device__ __forceinline__ void bres_calc(const double *x1, const double *x2,
-                                          const double *q1, const double *adt1,
-                                          double *res1, const int *bound) {
-  double dx, dy, mu, ri, p1, vol1, p2, vol2, f;
-  dx = x1[0] - x2[0];
-  dy = x1[1] - x2[1];
-  ri = 1.0 / q1[0];
-  p1 = gm1_d * (q1[3] - 0.5 * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
-  if (*bound == 1) {
-    res1[1] += +p1 * dy;
-    res1[2] += -p1 * dx;
-  } else {
-    vol1 = ri * (q1[1] * dy - q1[2] * dx);
-    ri = 1.0 / qinf_d[0];
-    p2 = gm1_d * (qinf_d[3] - 0.5 * ri * (qinf_d[1] * qinf_d[1] + qinf_d[2] * qinf_d[2]));
-    vol2 = ri * (qinf_d[1] * dy - qinf_d[2] * dx);
-    mu = (*adt1) * eps_d;
-    f = 0.5 * (vol1 * q1[0] + vol2 * qinf_d[0]) + mu * (q1[0] - qinf_d[0]);
-    res1[0] += f;
-    f = 0.5 * (vol1 * q1[1] + p1 * dy + vol2 * qinf_d[1] + p2 * dy) +
-        mu * (q1[1] - qinf_d[1]);
-    res1[1] += f;
-    f = 0.5 * (vol1 * q1[2] - p1 * dx + vol2 * qinf_d[2] - p2 * dx) +
-        mu * (q1[2] - qinf_d[2]);
-    res1[2] += f;
-    f = 0.5 * (vol1 * (q1[3] + p1) + vol2 * (qinf_d[3] + p2)) +
-        mu * (q1[3] - qinf_d[3]);
-    res1[3] += f;
-  }
-}
-
-__device__ __forceinline__ void update(const double *qold, double *q,
-                                       double *res, const double *adt,
-                                       double *rms, double *maxerr,
-                                       const int *idx, int *errloc) {
-  double del, adti;
-  adti = 1.0 / (*adt);
-  for (int n = 0; n < 4; n++) {
-    del = adti * res[n];
-    q[n] = qold[n] - del;
-    res[n] = 0.0;
-    double sqdel = del * del;
-    *rms += sqdel;
-    if (sqdel > *maxerr) {
-      *maxerr = sqdel;
-      *errloc = *idx;
-    }
-  }
-}
-
-}  // namespace op2_k
-
-// FP32 user kernels of the mixed-precision mode: the reference's
-// single-precision airfoil kernels (apps/c/airfoil/airfoil_plain/sp/
-// adt_calc.h, res_calc.h, bres_calc.h) on FP32 copies of q and x.
-namespace op2_kf {
-
-__device__ __forceinline__ void adt_calc(const float *x1, const float *x2,
-                                         const float *x3, const float *x4,
-                                         const float *q, float *adt) {
-  float dx, dy, ri, u, v, c;
-  ri = 1.0f / q[0];
-  u = ri * q[1];
-  v = ri * q[2];
-  c = sqrtf(gam_f * gm1_f * (ri * q[3] - 0.5f * (u * u + v * v)));
-  dx = x2[0] - x1[0];
-  dy = x2[1] - x1[1];
-  *adt = fabsf(u * dy - v * dx) + c * sqrtf(dx * dx + dy * dy);
-  dx = x3[0] - x2[0];
-  dy = x3[1] - x2[1];
-  *adt += fabsf(u * dy - v * dx) + c * sqrtf(dx * dx + dy * dy);
-  dx = x4[0] - x3[0];
-  dy = x4[1] - x3[1];
-  *adt += fabsf(u * dy - v * dx) + c * sqrtf(dx * dx + dy * dy);
-  dx = x1[0] - x4[0];
-  dy = x1[1] - x4[1];
-  *adt += fabsf(u * dy - v * dx) + c * sqrtf(dx * dx + dy * dy);
-  *adt = (*adt) / cfl_f;
-}
-
-__device__ __forceinline__ void res_calc(const float *x1, const float *x2,
-                                         const float *q1, const float *q2,
-                                         const float *adt1, const float *adt2,
-                                         float *res1, float *res2) {
-  float dx, dy, mu, ri, p1, vol1, p2, vol2, f;
-  dx = x1[0] - x2[0];
-  dy = x1[1] - x2[1];
-  ri = 1.0f / q1[0];
-  p1 = gm1_f * (q1[3] - 0.5f * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
-  vol1 = ri * (q1[1] * dy - q1[2] * dx);
-  ri = 1.0f / q2[0];
-  p2 = gm1_f * (q2[3] - 0.5f * ri * (q2[1] * q2[1] + q2[2] * q2[2]));
-  vol2 = ri * (q2[1] * dy - q2[2] * dx);
-  mu = 0.5f * ((*adt1) + (*adt2)) * eps_f;
-  f = 0.5f * (vol1 * q1[0] + vol2 * q2[0]) + mu * (q1[0] - q2[0]);
-  res1[0] += f;
-  res2[0] -= f;
-  f = 0.5f * (vol1 * q1[1] + p1 * dy + vol2 * q2[1] + p2 * dy) + mu * (q1[1] - q2[1]);
-  res1[1] += f;
-  res2[1] -= f;
-  f = 0.5f * (vol1 * q1[2] - p1 * dx + vol2 * q2[2] - p2 * dx) + mu * (q1[2] - q2[2]);
-  res1[2] += f;
-  res2[2] -= f;
-  f = 0.5f * (vol1 * (q1[3] + p1) + vol2 * (q2[3] + p2)) + mu * (q1[3] - q2[3]);
-  res1[3] += f;
-  res2[3] -= f;
-}
-
-__device__ __forceinline__ void bres_calc(const float *x1, const float *x2,
-                                          const float *q1, const float *adt1,
-                                          float *res1, const int *bound) {
-  float dx, dy, mu, ri, p1, vol1, p2, vol2, f;
-  dx = x1[0] - x2[0];
-  dy = x1[1] - x2[1];
-  ri = 1.0f / q1[0];
-  p1 = gm1_f * (q1[3] - 0.5f * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
-  if (*bound == 1) {
-    res1[1] += +p1 * dy;
-    res1[2] += -p1 * dx;
-  } else {
-    vol1 = ri * (q1[1] * dy - q1[2] * dx);
-    ri = 1.0f / qinf_f[0];
-    p2 = gm1_f * (qinf_f[3] - 0.5f * ri * (qinf_f[1] * qinf_f[1] + qinf_f[2] * qinf_f[2]));
-    vol2 = ri * (qinf_f[1] * dy - qinf_f[2] * dx);
-    mu = (*adt1) * eps_f;
-    f = 0.5f * (vol1 * q1[0] + vol2 * qinf_f[0]) + mu * (q1[0] - qinf_f[0]);
-    res1[0] += f;
-    f = 0.5f * (vol1 * q1[1] + p1 * dy + vol2 * qinf_f[1] + p2 * dy) + mu * (q1[1] - qinf_f[1]);
-    res1[1] += f;
-    f = 0.5f * (vol1 * q1[2] - p1 * dx + vol2 * qinf_f[2] - p2 * dx) + mu * (q1[2] - qinf_f[2]);
-    res1[2] += f;
-    f = 0.5f * (vol1 * (q1[3] + p1) + vol2 * (qinf_f[3] + p2)) + mu * (q1[3] - qinf_f[3]);
-    res1[3] += f;
-  }
-}
-
-}  // namespace op2_kf
-
-// ---------------------------------------------------------------------------
-// dat access: AoS (element-contiguous, 16-byte vector loads) or SoA
-// (component rows `stride` elements apart)
-// ---------------------------------------------------------------------------
 namespace {
-
-template <bool SOA>
-__device__ __forceinline__ void ld2(const double *__restrict p, int n, int s,
-                                    double *v) {
-  if (SOA) {
-    v[0] = p[n];
-    v[1] = p[(size_t)s + n];
-  } else {
-    double2 a = *reinterpret_cast<const double2 *>(p + 2 * (size_t)n);
-    v[0] = a.x;
-    v[1] = a.y;
-  }
-}
-
-template <bool SOA>
-__device__ __forceinline__ void ld4(const double *__restrict p, int n, int s,
-                                    double *v) {
-  if (SOA) {
-    v[0] = p[n];
-    v[1] = p[(size_t)s + n];
-    v[2] = p[2 * (size_t)s + n];
-    v[3] = p[3 * (size_t)s + n];
-  } else {
-    double2 a = *reinterpret_cast<const double2 *>(p + 4 * (size_t)n);
-    double2 b = *reinterpret_cast<const double2 *>(p + 4 * (size_t)n + 2);
-    v[0] = a.x;
-    v[1] = a.y;
-    v[2] = b.x;
-    v[3] = b.y;
-  }
-}
-
-template <bool SOA>
-__device__ __forceinline__ void st4(double *__restrict p, int n, int s,
-                                    const double *v) {
-  if (SOA) {
-    p[n] = v[0];
-    p[(size_t)s + n] = v[1];
-    p[2 * (size_t)s + n] = v[2];
-    p[3 * (size_t)s + n] = v[3];
-  } else {
-    *reinterpret_cast<double2 *>(p + 4 * (size_t)n) = make_double2(v[0], v[1]);
-    *reinterpret_cast<double2 *>(p + 4 * (size_t)n + 2) = make_double2(v[2], v[3]);
-  }
-}
-
-template <bool SOA>
-__device__ __forceinline__ void add4(double *__restrict p, int n, int s,
-                                     const double *v) {
-  double o[4];
-  ld4<SOA>(p, n, s, o);
-  o[0] += v[0];
-  o[1] += v[1];
-  o[2] += v[2];
-  o[3] += v[3];
-  st4<SOA>(p, n, s, o);
-}
-
-template <bool SOA>
-__device__ __forceinline__ void atomic_add4(double *p, int n, int s,
-                                            const double *v) {
-  size_t st = SOA ? (size_t)s : 1, base = SOA ? (size_t)n : 4 * (size_t)n;
-  unsafeAtomicAdd(p + base, v[0]);
-  unsafeAtomicAdd(p + base + st, v[1]);
-  unsafeAtomicAdd(p + base + 2 * st, v[2]);
-  unsafeAtomicAdd(p + base + 3 * st, v[3]);
-}
-
-int grid_for(long long n, int bs) { return n > 0 ? (int)((n + bs - 1) / bs) : 0; }
-
-// Workgroups are dispatched round-robin over the XCDs (blockIdx % nxcd).
-// Logical block ids are assigned so that each XCD runs one contiguous range
-// of them: neighbours in a locally numbered mesh then stay in one XCD's L2.
-// The map is a bijection of [0, gridDim.x).
-__device__ __forceinline__ int xcd_block(int nxcd) {
-  const int b = blockIdx.x, n = gridDim.x;
-  if (nxcd <= 1 || n < nxcd) return b;
-  const int q = n / nxcd, r = n % nxcd;
-  const int x = b % nxcd, i = b / nxcd;
-  const int start = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
-  return start + i;
-}
-
-constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
+using namespace op2_dev;
 
 // ---------------------------------------------------------------------------
 // save_soln: qold = q over the whole dat, 16-byte vectors (direct copy)

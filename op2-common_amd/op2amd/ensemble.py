@@ -1,0 +1,80 @@
+"""Batched airfoil ensembles (include/op2amd_ensemble.h)."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import lib
+from .mesh import ORDERINGS
+
+
+class Spec(C.Structure):
+    _fields_ = [("nmesh", C.c_int), ("min_I", C.c_int), ("min_J", C.c_int),
+                ("max_I", C.c_int), ("max_J", C.c_int), ("seed", C.c_uint64),
+                ("mach_lo", C.c_double), ("mach_hi", C.c_double),
+                ("alpha_lo_deg", C.c_double), ("alpha_hi_deg", C.c_double),
+                ("ordering", C.c_int)]
+
+
+class Member(C.Structure):
+    _fields_ = [("I", C.c_int), ("J", C.c_int), ("ncell", C.c_int),
+                ("mach", C.c_double), ("alpha_deg", C.c_double),
+                ("qinf", C.c_double * 4)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("niter", C.c_int), ("warmup", C.c_int), ("batched", C.c_int),
+                ("graphed", C.c_int), ("member_timing", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("rms", C.POINTER(C.c_double)), ("maxerr", C.POINTER(C.c_double)),
+                ("errloc", C.POINTER(C.c_int)), ("q_out", C.POINTER(C.c_double)),
+                ("member_step_ms", C.POINTER(C.c_double)), ("seconds", C.c_double),
+                ("setup_seconds", C.c_double), ("total_cells", C.c_longlong),
+                ("total_edges", C.c_longlong)]
+
+
+lib.op2amd_ensemble_describe.restype = C.c_int
+lib.op2amd_ensemble_describe.argtypes = [C.POINTER(Spec), C.POINTER(Member)]
+lib.op2amd_ensemble_run.restype = C.c_int
+lib.op2amd_ensemble_run.argtypes = [C.POINTER(Spec), C.POINTER(Opts), C.POINTER(Result)]
+
+
+def spec(nmesh=256, min_IJ=(40, 20), max_IJ=(640, 320), seed=0, mach=(0.3, 0.6),
+         alpha_deg=(0.0, 5.0), ordering="rcm"):
+    return Spec(nmesh, min_IJ[0], min_IJ[1], max_IJ[0], max_IJ[1], seed, mach[0],
+                mach[1], alpha_deg[0], alpha_deg[1], ORDERINGS[ordering])
+
+
+def describe(sp):
+    out = (Member * sp.nmesh)()
+    if lib.op2amd_ensemble_describe(C.byref(sp), out):
+        raise ValueError("bad ensemble spec")
+    return [dict(I=m.I, J=m.J, ncell=m.ncell, mach=m.mach, alpha_deg=m.alpha_deg,
+                 qinf=list(m.qinf)) for m in out]
+
+
+def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
+        want_q=False):
+    members = describe(sp)
+    M = sp.nmesh
+    total = sum(m["ncell"] for m in members)
+    rms = np.zeros((niter, M))
+    maxerr = np.zeros((niter, M))
+    errloc = np.zeros((niter, M), np.int32)
+    step_ms = np.zeros(M)
+    q = np.zeros((total, 4)) if want_q else None
+    r = Result()
+    r.rms = rms.ctypes.data_as(C.POINTER(C.c_double))
+    r.maxerr = maxerr.ctypes.data_as(C.POINTER(C.c_double))
+    r.errloc = errloc.ctypes.data_as(C.POINTER(C.c_int))
+    r.q_out = q.ctypes.data_as(C.POINTER(C.c_double)) if want_q else C.POINTER(C.c_double)()
+    r.member_step_ms = step_ms.ctypes.data_as(C.POINTER(C.c_double))
+    o = Opts(niter, warmup, int(batched), int(graphed), int(member_timing))
+    rc = lib.op2amd_ensemble_run(C.byref(sp), C.byref(o), C.byref(r))
+    if rc:
+        raise RuntimeError(f"ensemble run failed ({rc})")
+    return {"members": members, "rms": rms, "maxerr": maxerr, "errloc": errloc, "q": q,
+            "member_step_ms": step_ms, "seconds": r.seconds,
+            "setup_seconds": r.setup_seconds, "total_cells": r.total_cells,
+            "total_edges": r.total_edges}

@@ -398,6 +398,11 @@ void run_blocks(const orc_plan &p, int nthreads, Body body) {
 
 }  // namespace
 
+static int run_with_consts(const op2amd_mesh *m, int niter, const orc_consts &c,
+                           int exec, int nthreads, int part_size,
+                           double *rms_k0_raw, double *rms_out, double *maxerr_out,
+                           int *errloc_out, double *q_out, double *loop_seconds);
+
 extern "C" int orc_airfoil_run(const op2amd_mesh *m, int niter, int alpha_mode,
                                int exec, int nthreads, int part_size,
                                double *rms_k0_raw, double *rms_out,
@@ -405,6 +410,27 @@ extern "C" int orc_airfoil_run(const op2amd_mesh *m, int niter, int alpha_mode,
                                double *q_out, double *loop_seconds) {
   orc_consts c;
   orc_airfoil_consts(alpha_mode, &c);
+  return run_with_consts(m, niter, c, exec, nthreads, part_size, rms_k0_raw, rms_out,
+                         maxerr_out, errloc_out, q_out, loop_seconds);
+}
+
+extern "C" int orc_airfoil_run_qinf(const op2amd_mesh *m, int niter,
+                                    const double *qinf, int exec, int nthreads,
+                                    int part_size, double *rms_k0_raw,
+                                    double *rms_out, double *maxerr_out,
+                                    int *errloc_out, double *q_out,
+                                    double *loop_seconds) {
+  orc_consts c;
+  orc_airfoil_consts(0, &c);
+  for (int k = 0; k < 4; ++k) c.qinf[k] = qinf[k];
+  return run_with_consts(m, niter, c, exec, nthreads, part_size, rms_k0_raw, rms_out,
+                         maxerr_out, errloc_out, q_out, loop_seconds);
+}
+
+static int run_with_consts(const op2amd_mesh *m, int niter, const orc_consts &c,
+                           int exec, int nthreads, int part_size,
+                           double *rms_k0_raw, double *rms_out, double *maxerr_out,
+                           int *errloc_out, double *q_out, double *loop_seconds) {
   const int nc = m->ncell;
   std::vector<double> q(4 * (size_t)nc), qold(4 * (size_t)nc),
       res(4 * (size_t)nc, 0.0), adt(nc);

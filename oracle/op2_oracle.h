@@ -70,6 +70,13 @@ int orc_airfoil_run(const op2amd_mesh *m, int niter, int alpha_mode, int exec,
                     double *rms, double *maxerr, int *errloc, double *q_out,
                     double *loop_seconds);
 
+/* As orc_airfoil_run with the free stream `qinf` given explicitly (the other
+ * constants as airfoil.cpp:149-152); the initial q is qinf everywhere. */
+int orc_airfoil_run_qinf(const op2amd_mesh *m, int niter, const double *qinf,
+                         int exec, int nthreads, int part_size,
+                         double *rms_k0_raw, double *rms, double *maxerr,
+                         int *errloc, double *q_out, double *loop_seconds);
+
 /* op_plan_core(OP_STAGE_ALL) restatement for one indirect loop on one rank.
  * Argument m (0..nargs-1) uses map table maps[m] (row-major [set_size][mdim[m]],
  * NULL for a direct arg) at index idx[m]; inds[m] >= 0 numbers the distinct

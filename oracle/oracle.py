@@ -69,6 +69,9 @@ def _setup(L):
     L.orc_airfoil_run.restype = C.c_int
     L.orc_airfoil_run.argtypes = [M, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   dp, dp, dp, C.POINTER(C.c_int), dp, dp]
+    L.orc_airfoil_run_qinf.restype = C.c_int
+    L.orc_airfoil_run_qinf.argtypes = [M, C.c_int, dp, C.c_int, C.c_int, C.c_int,
+                                       dp, dp, dp, C.POINTER(C.c_int), dp, dp]
     L.orc_plan_build.restype = C.c_int
     L.orc_plan_build.argtypes = [C.c_int, C.c_int, C.POINTER(C.POINTER(C.c_int)),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -140,6 +143,24 @@ def airfoil_run(mesh, niter, alpha_mode=1, exec_="seq", nthreads=0, part_size=0)
         raise RuntimeError("oracle airfoil run failed")
     return {"rms_k0": rms_k0, "rms": rms, "maxerr": maxerr, "errloc": errloc,
             "q": q, "loop_seconds": secs}
+
+
+def airfoil_run_qinf(mesh, niter, qinf, exec_="seq", nthreads=0):
+    """Airfoil run with an explicit free stream (ensemble members)."""
+    rms_k0 = np.zeros(niter)
+    rms = np.zeros(niter)
+    maxerr = np.zeros(niter)
+    errloc = np.zeros(niter, np.int32)
+    q = np.zeros((mesh.ncell, 4))
+    qi = np.ascontiguousarray(qinf, dtype=np.float64)
+    rc = lib().orc_airfoil_run_qinf(C.byref(mesh.c()), niter, _p(qi),
+                                    {"seq": 0, "openmp": 1}[exec_], nthreads, 0,
+                                    _p(rms_k0), _p(rms), _p(maxerr),
+                                    errloc.ctypes.data_as(C.POINTER(C.c_int)), _p(q),
+                                    None)
+    if rc:
+        raise RuntimeError("oracle airfoil run failed")
+    return {"rms_k0": rms_k0, "rms": rms, "maxerr": maxerr, "errloc": errloc, "q": q}
 
 
 def plan_build(set_size, args, part_size=0):

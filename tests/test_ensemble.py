@@ -1,0 +1,45 @@
+"""Ensemble member definitions (CPU) and the batched ensemble on the GPU."""
+import math
+
+import numpy as np
+import pytest
+
+import op2amd
+from oracle import oracle
+
+SMALL = dict(nmesh=5, min_IJ=(8, 4), max_IJ=(40, 20), seed=3, ordering="rcm")
+
+
+def test_members_deterministic_and_in_range():
+    sp = op2amd.ensemble.spec()
+    a = op2amd.ensemble.describe(sp)
+    b = op2amd.ensemble.describe(op2amd.ensemble.spec())
+    assert a == b and len(a) == 256
+    for m in a:
+        assert 40 <= m["I"] <= 640 and 20 <= m["J"] <= 320 and abs(m["I"] - 2 * m["J"]) <= 1
+        assert 0.3 <= m["mach"] <= 0.6 and 0.0 <= m["alpha_deg"] <= 5.0
+        u = math.hypot(m["qinf"][1], m["qinf"][2])
+        assert abs(u - math.sqrt(1.4 * 1.0) * m["mach"]) < 1e-6
+        assert abs(math.degrees(math.atan2(m["qinf"][2], m["qinf"][1])) - m["alpha_deg"]) < 1e-9
+
+
+@pytest.mark.gpu
+def test_batched_equals_looped_and_each_member_equals_sequential_run():
+    sp = op2amd.ensemble.spec(**SMALL)
+    niter = 60
+    batched = op2amd.ensemble.run(sp, niter, batched=True, graphed=True, want_q=True)
+    eager = op2amd.ensemble.run(sp, niter, batched=True, graphed=False, want_q=True)
+    looped = op2amd.ensemble.run(sp, niter, batched=False, want_q=True)
+    for r in (eager, looped):
+        for key in ("rms", "maxerr", "errloc", "q"):
+            assert np.array_equal(r[key], batched[key]), key
+    off = 0
+    for m, mb in enumerate(batched["members"]):
+        mesh = op2amd.mesh.generate(mb["I"], mb["J"], SMALL["seed"] + 1 + m, "rcm")
+        ref = oracle.airfoil_run_qinf(mesh, niter, mb["qinf"])
+        q = batched["q"][off:off + mesh.ncell]
+        off += mesh.ncell
+        assert np.array_equal(q, ref["q"]), m
+        assert np.array_equal(batched["maxerr"][:, m], ref["maxerr"]), m
+        assert np.array_equal(batched["errloc"][:, m], ref["errloc"]), m
+        assert np.max(np.abs(batched["rms"][:, m] / ref["rms"] - 1)) <= 1e-14, m
