@@ -198,16 +198,110 @@ def workload_ensemble(op2amd, args):
     print(json.dumps(line), flush=True)
 
 
+def workload_costmodel(op2amd, args):
+    """Plan selection over synthetic op_par_loop descriptors: exhaustive sweep
+    of each loop's variants on the GPU, an MLP (2 x 64, FP32) trained on 80 %
+    of the loops with the library's HIP kernels, regret on the held-out 20 %."""
+    cm = op2amd.costmodel
+    rng = np.random.default_rng(args.cm_seed)
+    recs = []
+    t_sweep = time.time()
+    for i in range(args.cm_loops):
+        n = float(10 ** rng.uniform(4, 7))
+        arity = int(rng.integers(1, 5))
+        dim = int(rng.choice([1, 2, 4, 8]))
+        access = str(rng.choice(["read", "inc", "rw"]))
+        dbl = int(rng.integers(0, 2))
+        ordering = str(rng.choice(["random", "rcm", "hilbert"]))
+        k = int(rng.integers(4, 65))
+        d = cm.desc_from(n, arity, dim, access, dbl, ordering, k, 100 + i)
+        var = cm.variants(access)
+        ms, shape = cm.time_variants(d, var, reps=3)
+        recs.append(dict(arity=arity, dim=dim, access=access, is_double=dbl, ordering=ordering,
+                         k=k, variants=var, ms=ms, **shape))
+        best = min(t for t in ms if t > 0)
+        log(f"  loop {i}: n={shape['set_size']} a={arity} d={dim} {access} "
+            f"{'f64' if dbl else 'f32'} {ordering} k={k} best {best * 1e3:.1f}us")
+    t_sweep = time.time() - t_sweep
+    perm = rng.permutation(len(recs))
+    ntest = max(1, len(recs) // 5)
+    test_ids, train_ids = perm[:ntest], perm[ntest:]
+
+    def rows(ids):
+        X, y, owner = [], [], []
+        for i in ids:
+            r = recs[i]
+            for (plan, bs), t in zip(r["variants"], r["ms"]):
+                if t > 0:
+                    X.append(cm.features(r, plan, bs))
+                    y.append(np.log(t))
+                    owner.append(i)
+        return np.array(X, np.float32), np.array(y, np.float32), owner
+    Xtr, ytr, _ = rows(train_ids)
+    mu, sd = Xtr.mean(axis=0), Xtr.std(axis=0) + 1e-6
+    ymu, ysd = ytr.mean(), ytr.std() + 1e-6
+    mlp = cm.MLP(Xtr.shape[1], 64, seed=args.cm_seed)
+    t_train = time.time()
+    loss = mlp.train((Xtr - mu) / sd, (ytr - ymu) / ysd, epochs=args.cm_epochs, lr=3e-3)
+    t_train = time.time() - t_train
+
+    def regret(ids, pick):
+        out = []
+        for i in ids:
+            r = recs[i]
+            valid = [(v, t) for v, t in zip(r["variants"], r["ms"]) if t > 0]
+            best = min(t for _, t in valid)
+            out.append(pick(r, valid) / best - 1.0)
+        return np.array(out)
+
+    def model_pick(r, valid):
+        X = np.array([cm.features(r, p, b) for (p, b), _ in valid], np.float32)
+        pred = mlp.forward((X - mu) / sd)
+        return valid[int(np.argmin(pred))][1]
+
+    def default_pick(r, valid):
+        want = (cm.DIRECT, 256) if r["access"] == "read" else (op2amd.api.PLAN_STAGED, 256)
+        return dict(valid)[want]
+    t_inf = time.time()
+    reg_model = regret(test_ids, model_pick)
+    t_inf = (time.time() - t_inf) / len(test_ids)
+    reg_default = regret(test_ids, default_pick)
+    reg_model_train = regret(train_ids, model_pick)
+    line = {
+        "metric": "cost-model plan selection: mean time regret vs exhaustive sweep, held-out loops",
+        "value": round(float(reg_model.mean()), 4), "unit": "fraction", "n_gpus": 1,
+        "steps": len(recs), "warmup": 0, "ms_per_step": None, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (model), f32/f64 (loops)",
+        "data": "synthetic",
+        "config": {"workload": f"{len(recs)} synthetic op_par_loop descriptors, 80/20 split",
+                   "train_loops": int(len(train_ids)), "test_loops": int(len(test_ids)),
+                   "train_rows": int(Xtr.shape[0]), "final_train_loss": loss,
+                   "regret_model_mean": float(reg_model.mean()),
+                   "regret_model_p90": float(np.percentile(reg_model, 90)),
+                   "regret_model_within_5pct": float(np.mean(reg_model <= 0.05)),
+                   "regret_model_train_mean": float(reg_model_train.mean()),
+                   "regret_default_plan_mean": float(reg_default.mean()),
+                   "regret_default_plan_p90": float(np.percentile(reg_default, 90)),
+                   "sweep_seconds": t_sweep, "train_seconds": t_train,
+                   "model_pick_seconds_per_loop": t_inf},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="airfoil",
-                    choices=["airfoil", "airfoil-large", "ensemble"],
+                    choices=["airfoil", "airfoil-large", "ensemble", "costmodel"],
                     help="airfoil: the headline 720k line (default); the others "
                          "measure the 11.5M-cell FP64/mixed config and the ensemble")
     ap.add_argument("--large-mesh", default="4800x2400")
     ap.add_argument("--large-iters", type=int, default=2000)
     ap.add_argument("--ensemble-size", type=int, default=256)
     ap.add_argument("--ensemble-iters", type=int, default=200)
+    ap.add_argument("--cm-loops", type=int, default=200)
+    ap.add_argument("--cm-epochs", type=int, default=3000)
+    ap.add_argument("--cm-seed", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
@@ -230,6 +324,8 @@ def main():
         return workload_large(op2amd, args)
     if args.workload == "ensemble":
         return workload_ensemble(op2amd, args)
+    if args.workload == "costmodel":
+        return workload_costmodel(op2amd, args)
     from oracle import oracle
     I, J = (int(v) for v in args.mesh.split("x"))
     K, W = args.steps, args.warmup

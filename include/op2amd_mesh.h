@@ -28,6 +28,10 @@
  *   OP2AMD_ORDER_RCM         RANDOM, then reverse Cuthill-McKee on the cell
  *                            graph, propagated to the other sets by the
  *                            reference's rules (op2/src/externlib/op_renumber.cpp:72-151).
+ *   OP2AMD_ORDER_HILBERT     RANDOM, then cells sorted by the Hilbert-curve
+ *                            index of their structured position (i, j) on the
+ *                            smallest 2^k x 2^k grid holding the mesh, the
+ *                            other sets propagated as for RCM.
  */
 #ifndef OP2AMD_MESH_H
 #define OP2AMD_MESH_H
@@ -41,7 +45,8 @@ extern "C" {
 enum {
   OP2AMD_ORDER_STRUCTURED = 0,
   OP2AMD_ORDER_RANDOM = 1,
-  OP2AMD_ORDER_RCM = 2
+  OP2AMD_ORDER_RCM = 2,
+  OP2AMD_ORDER_HILBERT = 3
 };
 
 /* All arrays are row-major AoS, 0-based, as the reference's host arrays

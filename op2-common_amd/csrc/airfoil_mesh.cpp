@@ -96,9 +96,15 @@ void cell_graph(const op2amd_mesh *m, std::vector<int> &off,
 
 }  // namespace
 
+namespace {
+int renumber_cells(op2amd_mesh *m, const std::vector<int> &cnew,
+                   std::vector<int> *enew_out, std::vector<int> *bnew_out,
+                   std::vector<int> *nnew_out);
+}
+
 extern "C" int op2amd_mesh_generate(int I, int J, uint64_t seed, int ordering,
                                     op2amd_mesh *m) {
-  if (!m || I < 2 || J < 2 || ordering < 0 || ordering > 2) return 1;
+  if (!m || I < 2 || J < 2 || ordering < 0 || ordering > 3) return 1;
   std::memset(m, 0, sizeof(*m));
   m->nnode = (I + 1) * (J + 1);
   m->ncell = I * J;
@@ -182,6 +188,37 @@ extern "C" int op2amd_mesh_generate(int I, int J, uint64_t seed, int ordering,
   }
   if (ordering == OP2AMD_ORDER_RCM)
     return op2amd_mesh_renumber_rcm(m, nullptr, nullptr, nullptr, nullptr);
+  if (ordering == OP2AMD_ORDER_HILBERT) {
+    // structured position of every (now randomly numbered) cell from its
+    // lowest-left node, which is node (i, j) = cell[4c]
+    std::vector<long long> key(m->ncell);
+    int order = 1;
+    while ((1 << order) < std::max(I, J)) ++order;
+    const long long side = 1LL << order;
+    for (int c = 0; c < m->ncell; ++c) {
+      int n0 = m->cell[4 * c];
+      long long x = n0 % (I + 1), y = n0 / (I + 1);
+      // Hilbert index of (x, y) on a side x side grid (iterative d2 mapping)
+      long long d = 0;
+      for (long long s = side / 2; s > 0; s /= 2) {
+        long long rx = (x & s) > 0, ry = (y & s) > 0;
+        d += s * s * ((3 * rx) ^ ry);
+        if (ry == 0) {
+          if (rx == 1) {
+            x = side - 1 - x;
+            y = side - 1 - y;
+          }
+          std::swap(x, y);
+        }
+      }
+      key[c] = d;
+    }
+    std::vector<int> order_c(m->ncell);
+    std::iota(order_c.begin(), order_c.end(), 0);
+    std::stable_sort(order_c.begin(), order_c.end(),
+                     [&](int a, int b) { return key[a] < key[b]; });
+    return renumber_cells(m, invert(order_c), nullptr, nullptr, nullptr);
+  }
   return 0;
 }
 
@@ -246,13 +283,13 @@ extern "C" int op2amd_mesh_read(const char *path, op2amd_mesh *m) {
   return rc;
 }
 
-extern "C" int op2amd_mesh_renumber_rcm(op2amd_mesh *m, int *cell_perm,
-                                        int *edge_perm, int *bedge_perm,
-                                        int *node_perm) {
+namespace {
+// Renumber cells by `cnew` (new index of each old cell) and propagate to the
+// other sets with the rules of op_renumber.cpp:72-151 (see op2amd_mesh.h).
+int renumber_cells(op2amd_mesh *m, const std::vector<int> &cnew,
+                   std::vector<int> *enew_out, std::vector<int> *bnew_out,
+                   std::vector<int> *nnew_out) {
   const int nc = m->ncell;
-  std::vector<int> off, adj;
-  cell_graph(m, off, adj);
-  std::vector<int> cnew = op2h::rcm_order(nc, off, adj);
   std::vector<int> cm(nc);  // cells in new order
   for (int c = 0; c < nc; ++c) cm[cnew[c]] = c;
   // edges: stable sort by new index of ecell[e][0]
@@ -294,6 +331,22 @@ extern "C" int op2amd_mesh_renumber_rcm(op2amd_mesh *m, int *cell_perm,
   remap_entries(m->bedge, 2 * (size_t)m->nbedge, nnew);
   remap_entries(m->becell, (size_t)m->nbedge, cnew);
 
+  if (enew_out) *enew_out = enew;
+  if (bnew_out) *bnew_out = bnew;
+  if (nnew_out) *nnew_out = nnew;
+  return 0;
+}
+}  // namespace
+
+extern "C" int op2amd_mesh_renumber_rcm(op2amd_mesh *m, int *cell_perm,
+                                        int *edge_perm, int *bedge_perm,
+                                        int *node_perm) {
+  const int nc = m->ncell;
+  std::vector<int> off, adj;
+  cell_graph(m, off, adj);
+  std::vector<int> cnew = op2h::rcm_order(nc, off, adj);
+  std::vector<int> enew, bnew, nnew;
+  renumber_cells(m, cnew, &enew, &bnew, &nnew);
   if (cell_perm) std::copy(cnew.begin(), cnew.end(), cell_perm);
   if (edge_perm) std::copy(enew.begin(), enew.end(), edge_perm);
   if (bedge_perm) std::copy(bnew.begin(), bnew.end(), bedge_perm);

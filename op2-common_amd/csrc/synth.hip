@@ -1,0 +1,403 @@
+// Synthetic op_par_loops (include/op2amd_synth.h): the runtime's plans timed
+// on parametrised loops, the training data of the plan cost model.
+#include "op2amd_synth.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "op2_hip_rt.h"
+#include "op2amd_mesh.h"
+
+#define SCHECK(x)                                                            \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess) {                                                  \
+      printf("op2hip synth: %s failed at %s:%d: %s\n", #x, __FILE__,         \
+             __LINE__, hipGetErrorString(e_));                               \
+      exit(-1);                                                              \
+    }                                                                        \
+  } while (0)
+
+namespace {
+
+constexpr int kMaxA = 4;
+
+template <class T, int D>
+__device__ __forceinline__ void synth_body(const T *__restrict V, const T *__restrict R,
+                                           const int *__restrict map, int mdim, int A,
+                                           int e, int k, T *v, int *t) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) v[d] = V[(size_t)e * D + d];
+#pragma unroll
+  for (int s = 0; s < kMaxA; ++s) {
+    if (s < A) {
+      t[s] = map[(size_t)e * mdim + s];
+#pragma unroll
+      for (int d = 0; d < D; ++d) v[d] += R[(size_t)t[s] * D + d];
+    }
+  }
+  for (int i = 0; i < k; ++i) {
+    const int d = i % D;
+    v[d] = v[d] * (T)0.9990234375 + (T)0.0009765625;
+  }
+}
+
+template <class T, int D, bool INC>
+__device__ __forceinline__ void apply(T *w, const T *v) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) w[d] = INC ? w[d] + v[d] : w[d] * (T)0.5 + v[d];
+}
+
+template <class T, int D>
+__global__ void k_syn_direct(const T *__restrict V, const T *__restrict R,
+                             const int *__restrict map, int mdim, int A, int n, int k,
+                             T *__restrict out) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  T v[D];
+  int t[kMaxA];
+  synth_body<T, D>(V, R, map, mdim, A, e, k, v, t);
+#pragma unroll
+  for (int d = 0; d < D; ++d) out[(size_t)e * D + d] = v[d];
+}
+
+template <class T, int D>
+__global__ void k_syn_atomic(const T *__restrict V, const T *__restrict R,
+                             const int *__restrict map, int mdim, int A, int n, int k,
+                             T *W) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  T v[D];
+  int t[kMaxA];
+  synth_body<T, D>(V, R, map, mdim, A, e, k, v, t);
+  for (int s = 0; s < A; ++s)
+#pragma unroll
+    for (int d = 0; d < D; ++d) unsafeAtomicAdd(W + (size_t)t[s] * D + d, v[d]);
+}
+
+template <class T, int D, bool INC>
+__global__ void k_syn_color(const T *__restrict V, const T *__restrict R,
+                            const int *__restrict map, int mdim, int A, int k,
+                            const int *__restrict reord, int start, int end, T *W) {
+  int i = start + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  int e = reord[i];
+  T v[D];
+  int t[kMaxA];
+  synth_body<T, D>(V, R, map, mdim, A, e, k, v, t);
+  for (int s = 0; s < A; ++s) apply<T, D, INC>(W + (size_t)t[s] * D, v);
+}
+
+struct SynStaged {
+  const int *blkmap, *offset, *nelems, *nthrcol;
+  const unsigned char *thrcol;
+  const int *ind_offs, *ind_sizes, *ind_map;
+  const unsigned short *loc[kMaxA];
+};
+
+template <class T, int D, bool INC>
+__global__ void __launch_bounds__(1024)
+    k_syn_staged(const T *__restrict V, const T *__restrict R,
+                 const int *__restrict map, int mdim, int A, int k, SynStaged P,
+                 int block_offset, T *__restrict W) {
+  extern __shared__ unsigned char lds_raw[];
+  T *lw = reinterpret_cast<T *>(lds_raw);
+  const int b = P.blkmap[blockIdx.x + block_offset];
+  const int off = P.offset[b], nel = P.nelems[b];
+  const int nt = P.ind_sizes[b], base = P.ind_offs[b];
+  for (int i = threadIdx.x; i < nt * D; i += blockDim.x) {
+    int kk = i / D, d = i % D;
+    lw[i] = INC ? (T)0 : W[(size_t)P.ind_map[base + kk] * D + d];
+  }
+  __syncthreads();
+  T v[D];
+  int t[kMaxA];
+  int col = -1;
+  if ((int)threadIdx.x < nel) {
+    synth_body<T, D>(V, R, map, mdim, A, off + threadIdx.x, k, v, t);
+    col = P.thrcol[off + threadIdx.x];
+  }
+  const int ncol = P.nthrcol[b];
+  for (int c = 0; c < ncol; ++c) {
+    if (col == c)
+      for (int s = 0; s < A; ++s)
+        apply<T, D, INC>(lw + (size_t)P.loc[s][off + threadIdx.x] * D, v);
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < nt * D; i += blockDim.x) {
+    int kk = i / D, d = i % D;
+    T *dst = W + (size_t)P.ind_map[base + kk] * D + d;
+    *dst = INC ? *dst + lw[i] : lw[i];
+  }
+}
+
+template <class T, int D, bool INC>
+__global__ void k_syn_gather(const T *__restrict V, const T *__restrict R,
+                             const int *__restrict map, int mdim, int A, int k,
+                             const int *__restrict offsets,
+                             const int *__restrict entries,
+                             const int *__restrict targets, int ntargets, T *W) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= ntargets) return;
+  int tgt = targets[ti];
+  T acc[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) acc[d] = W[(size_t)tgt * D + d];
+  for (int j = offsets[ti]; j < offsets[ti + 1]; ++j) {
+    int ent = entries[j];
+    int e = ent / mdim, s = ent % mdim;
+    if (s >= A) continue;  // a map column the loop does not use
+    T v[D];
+    int t[kMaxA];
+    synth_body<T, D>(V, R, map, mdim, A, e, k, v, t);
+    apply<T, D, INC>(acc, v);
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) W[(size_t)tgt * D + d] = acc[d];
+}
+
+int grid(long long n, int bs) { return n > 0 ? (int)((n + bs - 1) / bs) : 0; }
+
+struct Loop {
+  op2amd_mesh mesh{};
+  op_set from = nullptr, to = nullptr;
+  op_map map = nullptr;
+  op_dat V = nullptr, R = nullptr, W = nullptr, OUT = nullptr;
+  std::vector<op_arg> args;
+  int A = 1, D = 1, access = OP_READ, mdim = 1, k = 4, is_double = 1;
+  std::vector<unsigned char> w0;  // initial W bytes
+};
+
+template <class T>
+std::vector<T> seeded(size_t n, uint64_t seed) {
+  std::vector<T> v(n);
+  uint64_t s = seed * 0x9E3779B97F4A7C15ull + 1;
+  for (size_t i = 0; i < n; ++i) {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    v[i] = (T)(0.5 + (double)(s >> 11) * 0x1.0p-54);
+  }
+  return v;
+}
+
+bool setup(const op2amd_synth_desc *d, Loop &L) {
+  if (d->arity < 1 || d->arity > kMaxA || !(d->dim == 1 || d->dim == 2 || d->dim == 4 || d->dim == 8) ||
+      !(d->access == OP_READ || d->access == OP_INC || d->access == OP_RW))
+    return false;
+  if (op2amd_mesh_generate(d->I, d->J, d->seed, d->ordering, &L.mesh)) return false;
+  char prog[] = "synth";
+  char *argv[1] = {prog};
+  op_init(1, argv, 0);
+  op2h_set_plan_builder(1);
+  L.A = d->arity;
+  L.D = d->dim;
+  L.access = d->access;
+  L.k = d->k_fma;
+  L.is_double = d->is_double;
+  op_set nodes = op_decl_set(L.mesh.nnode, "nodes");
+  op_set cells = op_decl_set(L.mesh.ncell, "cells");
+  if (d->arity == 2) {
+    op_set edges = op_decl_set(L.mesh.nedge, "edges");
+    L.from = edges;
+    L.to = cells;
+    L.map = op_decl_map(edges, cells, 2, L.mesh.ecell, "pecell");
+    L.mdim = 2;
+  } else {
+    L.from = cells;
+    L.to = nodes;
+    L.map = op_decl_map(cells, nodes, 4, L.mesh.cell, "pcell");
+    L.mdim = 4;
+  }
+  const char *ty = d->is_double ? "double" : "float";
+  const int es = d->is_double ? 8 : 4;
+  size_t nf = (size_t)L.from->size * L.D, nt = (size_t)L.to->size * L.D;
+  std::vector<unsigned char> bv(nf * es), br(nt * es), bw(nt * es);
+  if (d->is_double) {
+    auto a = seeded<double>(nf, 1), b = seeded<double>(nt, 2), c = seeded<double>(nt, 3);
+    std::memcpy(bv.data(), a.data(), bv.size());
+    std::memcpy(br.data(), b.data(), br.size());
+    std::memcpy(bw.data(), c.data(), bw.size());
+  } else {
+    auto a = seeded<float>(nf, 1), b = seeded<float>(nt, 2), c = seeded<float>(nt, 3);
+    std::memcpy(bv.data(), a.data(), bv.size());
+    std::memcpy(br.data(), b.data(), br.size());
+    std::memcpy(bw.data(), c.data(), bw.size());
+  }
+  L.w0 = bw;
+  L.V = op_decl_dat_char(L.from, L.D, ty, es, (char *)bv.data(), "V");
+  L.R = op_decl_dat_char(L.to, L.D, ty, es, (char *)br.data(), "R");
+  L.W = op_decl_dat_char(L.to, L.D, ty, es, (char *)bw.data(), "W");
+  L.OUT = op_decl_dat_char(L.from, L.D, ty, es, nullptr, "OUT");
+  L.args.push_back(op_arg_dat(L.V, -1, OP_ID, L.D, ty, OP_READ));
+  for (int s = 0; s < L.A; ++s) L.args.push_back(op_arg_dat(L.R, s, L.map, L.D, ty, OP_READ));
+  if (L.access == OP_READ)
+    L.args.push_back(op_arg_dat(L.OUT, -1, OP_ID, L.D, ty, OP_WRITE));
+  else
+    for (int s = 0; s < L.A; ++s)
+      L.args.push_back(op_arg_dat(L.W, s, L.map, L.D, ty, L.access));
+  return true;
+}
+
+void teardown(Loop &L) {
+  op_exit();
+  op2amd_mesh_free(&L.mesh);
+}
+
+template <class T, int D>
+bool launch_typed(Loop &L, int plan, int bs, hipStream_t st) {
+  const T *V = (const T *)L.V->data_d, *R = (const T *)L.R->data_d;
+  T *W = (T *)L.W->data_d, *OUT = (T *)L.OUT->data_d;
+  const int *map = L.map->map_d;
+  const int n = L.from->size, A = L.A, k = L.k, md = L.mdim;
+  const bool inc = L.access == OP_INC;
+  if (L.access == OP_READ) {
+    if (plan != OP2AMD_SYNTH_DIRECT) return false;
+    k_syn_direct<T, D><<<grid(n, bs), bs, 0, st>>>(V, R, map, md, A, n, k, OUT);
+    return true;
+  }
+  switch (plan) {
+    case OP2H_PLAN_ATOMIC:
+      if (!inc) return false;
+      k_syn_atomic<T, D><<<grid(n, bs), bs, 0, st>>>(V, R, map, md, A, n, k, W);
+      return true;
+    case OP2H_PLAN_COLOR: {
+      const op2h_color_plan *p =
+          op2h_plan_color_for("synth", L.from, (int)L.args.size(), L.args.data());
+      for (int c = 0; c < p->ncolors; ++c) {
+        int s0 = p->col_offsets_h[c], s1 = p->col_offsets_h[c + 1];
+        if (s1 <= s0) continue;
+        if (inc) k_syn_color<T, D, true><<<grid(s1 - s0, bs), bs, 0, st>>>(V, R, map, md, A, k, p->col_reord, s0, s1, W);
+        else k_syn_color<T, D, false><<<grid(s1 - s0, bs), bs, 0, st>>>(V, R, map, md, A, k, p->col_reord, s0, s1, W);
+      }
+      return true;
+    }
+    case OP2H_PLAN_STAGED: {
+      const op2h_staged_plan *p =
+          op2h_plan_staged_for("synth", L.from, bs, (int)L.args.size(), L.args.data());
+      int warg = (int)L.args.size() - L.A;  // first W arg
+      int slot = p->map_of_arg[warg];
+      SynStaged S;
+      S.blkmap = p->blkmap;
+      S.offset = p->offset;
+      S.nelems = p->nelems;
+      S.nthrcol = p->nthrcol;
+      S.thrcol = p->thrcol;
+      S.ind_offs = p->ind_offs[slot];
+      S.ind_sizes = p->ind_sizes[slot];
+      S.ind_map = p->ind_map[slot];
+      for (int s = 0; s < kMaxA; ++s) S.loc[s] = s < A ? p->loc_map[p->loc_of_arg[warg + s]] : nullptr;
+      size_t lds = sizeof(T) * (size_t)p->max_ind_h[slot] * D;
+      if (lds > 64 * 1024) return false;
+      for (int c = 0; c < p->ncolors; ++c) {
+        int nb = p->ncolblk_h[c];
+        if (!nb) continue;
+        if (inc) k_syn_staged<T, D, true><<<nb, bs, lds, st>>>(V, R, map, md, A, k, S, p->colstart_h[c], W);
+        else k_syn_staged<T, D, false><<<nb, bs, lds, st>>>(V, R, map, md, A, k, S, p->colstart_h[c], W);
+      }
+      return true;
+    }
+    case OP2H_PLAN_GATHER: {
+      const op2h_gather_plan *p = op2h_plan_gather(L.map);
+      int g = grid(p->ntargets, bs);
+      if (inc) k_syn_gather<T, D, true><<<g, bs, 0, st>>>(V, R, map, md, A, k, p->offsets, p->entries, p->targets, p->ntargets, W);
+      else k_syn_gather<T, D, false><<<g, bs, 0, st>>>(V, R, map, md, A, k, p->offsets, p->entries, p->targets, p->ntargets, W);
+      return true;
+    }
+  }
+  return false;
+}
+
+template <class T>
+bool launch_dim(Loop &L, int plan, int bs, hipStream_t st) {
+  switch (L.D) {
+    case 1: return launch_typed<T, 1>(L, plan, bs, st);
+    case 2: return launch_typed<T, 2>(L, plan, bs, st);
+    case 4: return launch_typed<T, 4>(L, plan, bs, st);
+    case 8: return launch_typed<T, 8>(L, plan, bs, st);
+  }
+  return false;
+}
+
+bool launch(Loop &L, int plan, int bs) {
+  hipStream_t st = (hipStream_t)op2h_stream();
+  bool ok = L.is_double ? launch_dim<double>(L, plan, bs, st) : launch_dim<float>(L, plan, bs, st);
+  if (ok) SCHECK(hipGetLastError());
+  return ok;
+}
+
+}  // namespace
+
+extern "C" int op2amd_synth_time(const op2amd_synth_desc *d, int nvar, const int *plan,
+                                 const int *block, int reps, double *ms,
+                                 op2amd_synth_shape *shape) {
+  Loop L;
+  if (!setup(d, L)) return 1;
+  if (shape) {
+    shape->set_size = L.from->size;
+    shape->target_size = L.to->size;
+    double sum = 0.0;
+    for (int e = 0; e < L.from->size; ++e) {
+      int mx = 0;
+      for (int a = 0; a < L.A; ++a)
+        for (int b = 0; b < L.A; ++b)
+          mx = std::max(mx, std::abs(L.map->map[(size_t)e * L.mdim + a] -
+                                     L.map->map[(size_t)e * L.mdim + b]));
+      sum += mx;
+    }
+    shape->mean_bandwidth = L.from->size ? sum / L.from->size : 0.0;
+  }
+  hipStream_t st = (hipStream_t)op2h_stream();
+  hipEvent_t a, b;
+  SCHECK(hipEventCreate(&a));
+  SCHECK(hipEventCreate(&b));
+  for (int v = 0; v < nvar; ++v) {
+    ms[v] = -1.0;
+    if (!launch(L, plan[v], block[v])) continue;  // untimed: builds the plan
+    SCHECK(hipStreamSynchronize(st));
+    double best = 1e300;
+    for (int r = 0; r < reps; ++r) {
+      SCHECK(hipEventRecord(a, st));
+      launch(L, plan[v], block[v]);
+      SCHECK(hipEventRecord(b, st));
+      SCHECK(hipEventSynchronize(b));
+      float t = 0.f;
+      SCHECK(hipEventElapsedTime(&t, a, b));
+      best = std::min(best, (double)t);
+    }
+    ms[v] = best;
+  }
+  SCHECK(hipEventDestroy(a));
+  SCHECK(hipEventDestroy(b));
+  teardown(L);
+  return 0;
+}
+
+extern "C" int op2amd_synth_run_once(const op2amd_synth_desc *d, int plan, int block,
+                                     double *out, int max_out) {
+  Loop L;
+  if (!setup(d, L)) return 1;
+  if (!launch(L, plan, block)) {
+    teardown(L);
+    return 2;
+  }
+  op2h_device_sync();
+  op_dat res = L.access == OP_READ ? L.OUT : L.W;
+  size_t n = (size_t)res->set->size * L.D;
+  if ((size_t)max_out < n) {
+    teardown(L);
+    return 3;
+  }
+  // written on the device: fetch through the API (downloads device-newer data)
+  res->dirty_hd = 2;
+  std::vector<unsigned char> buf(n * (L.is_double ? 8 : 4));
+  op_fetch_data_char(res, (char *)buf.data());
+  for (size_t i = 0; i < n; ++i)
+    out[i] = L.is_double ? ((double *)buf.data())[i] : (double)((float *)buf.data())[i];
+  teardown(L);
+  return 0;
+}

@@ -6,4 +6,4 @@ include/op2amd_airfoil.h) to tests and benchmarks.  There is no CPU fallback:
 importing the bindings without the built library raises.
 """
 from ._lib import lib, LIB_PATH, REPO_ROOT  # noqa: F401
-from . import api, mesh, airfoil, ensemble  # noqa: F401
+from . import api, mesh, airfoil, ensemble, costmodel  # noqa: F401

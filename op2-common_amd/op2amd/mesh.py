@@ -6,8 +6,8 @@ import numpy as np
 
 from ._lib import lib
 
-STRUCTURED, RANDOM, RCM = 0, 1, 2
-ORDERINGS = {"structured": STRUCTURED, "random": RANDOM, "rcm": RCM}
+STRUCTURED, RANDOM, RCM, HILBERT = 0, 1, 2, 3
+ORDERINGS = {"structured": STRUCTURED, "random": RANDOM, "rcm": RCM, "hilbert": HILBERT}
 
 
 class op2amd_mesh(C.Structure):

@@ -1,0 +1,82 @@
+"""The MLP cost model's kernels against the float64 numpy oracle, and the
+synthetic loops behind its training data."""
+import numpy as np
+import pytest
+
+import op2amd
+from oracle import mlp_ref
+
+NIN, H = 19, 64
+
+
+def batch(n=300, seed=0):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, NIN)).astype(np.float32)
+    t = (x[:, 0] - 0.5 * x[:, 3] + np.sin(x[:, 5])).astype(np.float32)
+    return x, t
+
+
+def test_oracle_backward_matches_finite_differences():
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal(NIN * H + H + H * H + H + H + 1) * 0.2
+    x, t = batch(40)
+    _, g = mlp_ref.loss_grad(p, x, t, NIN, H)
+    for i in rng.choice(len(p), 60, replace=False):
+        e = np.zeros_like(p)
+        e[i] = 1e-6
+        fd = (mlp_ref.loss_grad(p + e, x, t, NIN, H)[0] -
+              mlp_ref.loss_grad(p - e, x, t, NIN, H)[0]) / 2e-6
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(g[i]), i
+
+
+@pytest.mark.gpu
+def test_hip_mlp_forward_and_gradients_match_oracle():
+    m = op2amd.costmodel.MLP(NIN, H, seed=7)
+    p = m.params()
+    x, t = batch()
+    y = m.forward(x)
+    y_ref, _ = mlp_ref.forward(p, x, NIN, H)
+    assert np.max(np.abs(y - y_ref)) <= 1e-4 * max(1.0, np.max(np.abs(y_ref)))
+    loss, g = m.loss_grad(x, t)
+    loss_ref, g_ref = mlp_ref.loss_grad(p, x, t, NIN, H)
+    assert abs(loss - loss_ref) <= 1e-5 * loss_ref
+    assert np.max(np.abs(g - g_ref)) <= 1e-4 * np.max(np.abs(g_ref))
+
+
+@pytest.mark.gpu
+def test_hip_mlp_training_fits():
+    m = op2amd.costmodel.MLP(NIN, H, seed=3)
+    x, t = batch(2000)
+    l0, _ = m.loss_grad(x, t)
+    l1 = m.train(x, t, epochs=400, lr=3e-3)
+    assert l1 < 0.05 * l0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("is_double,tol", [(1, 1e-12), (0, 2e-5)])
+@pytest.mark.parametrize("arity", [2, 4])
+def test_synthetic_inc_loop_agrees_across_plans(is_double, tol, arity):
+    cm = op2amd.costmodel
+    d = cm.desc_from(3000, arity, 4, "inc", is_double, "rcm", 12, 5)
+    size = 0
+    outs = {}
+    for plan, block in [(cm.PLAN_ATOMIC, 128), (cm.PLAN_COLOR, 256), (cm.PLAN_STAGED, 128),
+                        (cm.PLAN_GATHER, 64)]:
+        if not size:
+            _, shape = cm.time_variants(d, [(plan, block)], reps=1)
+            size = shape["target_size"] * 4
+        outs[plan] = cm.run_once(d, plan, block, size)
+    ref = outs[cm.PLAN_GATHER]
+    for plan, o in outs.items():
+        assert np.max(np.abs(o - ref)) <= tol * np.max(np.abs(ref)), plan
+
+
+@pytest.mark.gpu
+def test_synthetic_variants_time_and_refuse_invalid():
+    cm = op2amd.costmodel
+    d = cm.desc_from(5000, 2, 2, "rw", 0, "random", 8, 1)
+    ms, shape = cm.time_variants(d, [(cm.PLAN_COLOR, 128), (cm.PLAN_ATOMIC, 128),
+                                     (cm.PLAN_STAGED, 256), (cm.PLAN_GATHER, 128)], reps=2)
+    assert ms[0] > 0 and ms[2] > 0 and ms[3] > 0
+    assert ms[1] == -1.0  # atomics cannot implement OP_RW
+    assert 4000 <= shape["set_size"] <= 6000

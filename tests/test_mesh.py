@@ -22,7 +22,7 @@ def test_sizes(I, J):
     assert m.nbedge == 2 * I + 2 * J
 
 
-@pytest.mark.parametrize("ordering", ["structured", "random", "rcm"])
+@pytest.mark.parametrize("ordering", ["structured", "random", "rcm", "hilbert"])
 def test_orientation_conventions(ordering):
     m = op2amd.mesh.generate(30, 15, 4, ordering)
     x, cen = m.x, centroids(m)
@@ -56,7 +56,7 @@ def test_orientation_conventions(ordering):
 
 def test_orderings_are_relabelings():
     a = op2amd.mesh.generate(20, 10, 9, "structured")
-    for ordering in ("random", "rcm"):
+    for ordering in ("random", "rcm", "hilbert"):
         b = op2amd.mesh.generate(20, 10, 9, ordering)
         ka = np.sort(np.round(centroids(a), 12).view(np.complex128).ravel())
         kb = np.sort(np.round(centroids(b), 12).view(np.complex128).ravel())
