@@ -33,7 +33,28 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "airfoil steps/s, 720k cells: tuned/graphed vs plain HIP at rms rel err ≤1e-10"
-PLANS = ("atomic", "color", "staged", "gather")
+PLANS = ("atomic", "color", "staged", "gather", "owner")
+
+
+def profiled_traffic(loop_plan, workload):
+    """HBM bytes per launch of a loop's kernel, from the committed rocprofv3
+    counter summary for that (loop/plan/block, mesh/ordering), if there is one:
+    a line 'traffic: <loop/plan/block> <mesh/ordering> <bytes>' in a text file
+    under profiles/.  The summary records the FETCH_SIZE/WRITE_SIZE runs and
+    the copy-kernel scale factors the bytes were derived with."""
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    if not os.path.isdir(prof):
+        return None
+    for name in sorted(os.listdir(prof)):
+        if not name.endswith((".txt", ".md")):
+            continue
+        with open(os.path.join(prof, name)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic:" and parts[1] == loop_plan \
+                        and parts[2] == workload:
+                    return float(parts[3])
+    return None
 
 
 def log(msg):
@@ -198,75 +219,71 @@ def workload_ensemble(op2amd, args):
     print(json.dumps(line), flush=True)
 
 
+def workload_copy(op2amd, args):
+    """Counter calibration: save_soln (qold = q, a plain copy) on a set of
+    16.8M elements, 512 MiB read and 512 MiB written per call, far beyond the
+    L2 and the Infinity Cache.  Under rocprofv3 --pmc its FETCH_SIZE and
+    WRITE_SIZE per dispatch, divided by 512 MiB, give the counters' scale."""
+    api = op2amd.api
+    n = 1 << 24
+    api.init()
+    cells = api.op_decl_set(n, b"cells")
+    q = api.decl_dat(cells, 4, "double", np.ones((n, 4)), "q")
+    qold = api.decl_dat(cells, 4, "double", np.zeros((n, 4)), "qold")
+    a_q = api.arg_dat(q, -1, None, 4, "double", 0)
+    a_qold = api.arg_dat(qold, -1, None, 4, "double", 1)
+    loop = api.loops["op_par_loop_airfoil_1_save_soln"]
+    for _ in range(args.warmup):
+        loop(b"save_soln", cells, a_q, a_qold)
+    api.op2h_device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loop(b"save_soln", cells, a_q, a_qold)
+    api.op2h_device_sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    bytes_per_call = 2.0 * n * 32
+    out = api.fetch(qold, (n, 4), np.float64)
+    line = {"metric": "copy kernel bandwidth (counter calibration)",
+            "value": round(bytes_per_call / dt / 1e9, 1), "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"save_soln copy of {n} x 4 doubles (512 MiB in, 512 MiB out)",
+                       "read_bytes_per_call": n * 32, "write_bytes_per_call": n * 32,
+                       "copy_correct": bool(np.all(out == 1.0))},
+            "roofline": None, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+
+
 def workload_costmodel(op2amd, args):
     """Plan selection over synthetic op_par_loop descriptors: exhaustive sweep
-    of each loop's variants on the GPU, an MLP (2 x 64, FP32) trained on 80 %
-    of the loops with the library's HIP kernels, regret on the held-out 20 %."""
+    of each loop's variants on the GPU, an ensemble of MLPs (2 x 64, FP32)
+    trained on 80 % of the loops with the library's HIP kernels (epoch count
+    chosen on a validation part of the training loops), regret on the held-out
+    20 % against the sweep's best, the runtime's static default and the best
+    single static rule fitted on the training loops."""
     cm = op2amd.costmodel
-    rng = np.random.default_rng(args.cm_seed)
-    recs = []
-    t_sweep = time.time()
-    for i in range(args.cm_loops):
-        n = float(10 ** rng.uniform(4, 7))
-        arity = int(rng.integers(1, 5))
-        dim = int(rng.choice([1, 2, 4, 8]))
-        access = str(rng.choice(["read", "inc", "rw"]))
-        dbl = int(rng.integers(0, 2))
-        ordering = str(rng.choice(["random", "rcm", "hilbert"]))
-        k = int(rng.integers(4, 65))
-        d = cm.desc_from(n, arity, dim, access, dbl, ordering, k, 100 + i)
-        var = cm.variants(access)
-        ms, shape = cm.time_variants(d, var, reps=3)
-        recs.append(dict(arity=arity, dim=dim, access=access, is_double=dbl, ordering=ordering,
-                         k=k, variants=var, ms=ms, **shape))
-        best = min(t for t in ms if t > 0)
-        log(f"  loop {i}: n={shape['set_size']} a={arity} d={dim} {access} "
-            f"{'f64' if dbl else 'f32'} {ordering} k={k} best {best * 1e3:.1f}us")
-    t_sweep = time.time() - t_sweep
-    perm = rng.permutation(len(recs))
+    descs = cm.random_descs(args.cm_loops, args.cm_seed)
+    t0 = time.time()
+    recs = cm.sweep(descs, reps=5, log=log)
+    t_sweep = time.time() - t0
+    if args.cm_dump:
+        with open(args.cm_dump, "w") as f:
+            json.dump(recs, f)
+    perm = np.random.default_rng(args.cm_seed).permutation(len(recs))
     ntest = max(1, len(recs) // 5)
-    test_ids, train_ids = perm[:ntest], perm[ntest:]
-
-    def rows(ids):
-        X, y, owner = [], [], []
-        for i in ids:
-            r = recs[i]
-            for (plan, bs), t in zip(r["variants"], r["ms"]):
-                if t > 0:
-                    X.append(cm.features(r, plan, bs))
-                    y.append(np.log(t))
-                    owner.append(i)
-        return np.array(X, np.float32), np.array(y, np.float32), owner
-    Xtr, ytr, _ = rows(train_ids)
-    mu, sd = Xtr.mean(axis=0), Xtr.std(axis=0) + 1e-6
-    ymu, ysd = ytr.mean(), ytr.std() + 1e-6
-    mlp = cm.MLP(Xtr.shape[1], 64, seed=args.cm_seed)
-    t_train = time.time()
-    loss = mlp.train((Xtr - mu) / sd, (ytr - ymu) / ysd, epochs=args.cm_epochs, lr=3e-3)
-    t_train = time.time() - t_train
-
-    def regret(ids, pick):
-        out = []
-        for i in ids:
-            r = recs[i]
-            valid = [(v, t) for v, t in zip(r["variants"], r["ms"]) if t > 0]
-            best = min(t for _, t in valid)
-            out.append(pick(r, valid) / best - 1.0)
-        return np.array(out)
-
-    def model_pick(r, valid):
-        X = np.array([cm.features(r, p, b) for (p, b), _ in valid], np.float32)
-        pred = mlp.forward((X - mu) / sd)
-        return valid[int(np.argmin(pred))][1]
-
-    def default_pick(r, valid):
-        want = (cm.DIRECT, 256) if r["access"] == "read" else (op2amd.api.PLAN_STAGED, 256)
-        return dict(valid)[want]
-    t_inf = time.time()
-    reg_model = regret(test_ids, model_pick)
-    t_inf = (time.time() - t_inf) / len(test_ids)
-    reg_default = regret(test_ids, default_pick)
-    reg_model_train = regret(train_ids, model_pick)
+    test_ids, train_ids = list(perm[:ntest]), list(perm[ntest:])
+    t0 = time.time()
+    model = cm.CostModel(seed=args.cm_seed).fit_validated(recs, train_ids,
+                                                           max_epochs=args.cm_epochs)
+    t_train = time.time() - t0
+    log(f"  validation curve (epochs, mean regret): {model.validation_curve}")
+    t0 = time.time()
+    reg_model = cm.regret(recs, test_ids, model.choose)
+    t_pick = (time.time() - t0) / len(test_ids)
+    reg_default = cm.regret(recs, test_ids, cm.default_choice)
+    reg_fixed = cm.regret(recs, test_ids, cm.best_fixed_choice(recs, train_ids))
+    reg_train = cm.regret(recs, train_ids, model.choose)
     line = {
         "metric": "cost-model plan selection: mean time regret vs exhaustive sweep, held-out loops",
         "value": round(float(reg_model.mean()), 4), "unit": "fraction", "n_gpus": 1,
@@ -274,16 +291,19 @@ def workload_costmodel(op2amd, args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 (model), f32/f64 (loops)",
         "data": "synthetic",
         "config": {"workload": f"{len(recs)} synthetic op_par_loop descriptors, 80/20 split",
-                   "train_loops": int(len(train_ids)), "test_loops": int(len(test_ids)),
-                   "train_rows": int(Xtr.shape[0]), "final_train_loss": loss,
+                   "train_loops": len(train_ids), "test_loops": len(test_ids),
+                   "ensemble": model.members, "epochs": model.epochs,
+                   "weight_decay": model.wd,
                    "regret_model_mean": float(reg_model.mean()),
+                   "regret_model_median": float(np.median(reg_model)),
                    "regret_model_p90": float(np.percentile(reg_model, 90)),
-                   "regret_model_within_5pct": float(np.mean(reg_model <= 0.05)),
-                   "regret_model_train_mean": float(reg_model_train.mean()),
+                   "loops_within_5pct_model": float(np.mean(reg_model <= 0.05)),
+                   "regret_model_train_mean": float(reg_train.mean()),
                    "regret_default_plan_mean": float(reg_default.mean()),
-                   "regret_default_plan_p90": float(np.percentile(reg_default, 90)),
+                   "regret_best_fixed_rule_mean": float(reg_fixed.mean()),
+                   "loops_within_5pct_best_fixed_rule": float(np.mean(reg_fixed <= 0.05)),
                    "sweep_seconds": t_sweep, "train_seconds": t_train,
-                   "model_pick_seconds_per_loop": t_inf},
+                   "model_pick_seconds_per_loop": t_pick},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -292,7 +312,7 @@ def workload_costmodel(op2amd, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="airfoil",
-                    choices=["airfoil", "airfoil-large", "ensemble", "costmodel"],
+                    choices=["airfoil", "airfoil-large", "ensemble", "costmodel", "copy"],
                     help="airfoil: the headline 720k line (default); the others "
                          "measure the 11.5M-cell FP64/mixed config and the ensemble")
     ap.add_argument("--large-mesh", default="4800x2400")
@@ -300,7 +320,8 @@ def main():
     ap.add_argument("--ensemble-size", type=int, default=256)
     ap.add_argument("--ensemble-iters", type=int, default=200)
     ap.add_argument("--cm-loops", type=int, default=200)
-    ap.add_argument("--cm-epochs", type=int, default=3000)
+    ap.add_argument("--cm-epochs", type=int, default=4000)
+    ap.add_argument("--cm-dump", default="", help="write the sweep records (JSON) here")
     ap.add_argument("--cm-seed", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
@@ -326,6 +347,8 @@ def main():
         return workload_ensemble(op2amd, args)
     if args.workload == "costmodel":
         return workload_costmodel(op2amd, args)
+    if args.workload == "copy":
+        return workload_copy(op2amd, args)
     from oracle import oracle
     I, J = (int(v) for v in args.mesh.split("x"))
     K, W = args.steps, args.warmup
@@ -411,7 +434,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom["achieved_GBs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(f"res_calc/{kw['plan_res']}/{kw['block_res']}",
+                                        f"{args.mesh}/{args.ordering}"),
         },
         "cpu_baseline": cpu,
     }

@@ -34,6 +34,14 @@
 #define OP_INC 3
 #define OP_MIN 4
 #define OP_MAX 5
+
+/* Helper macros user kernels use (op_lib_core.h:245-250). */
+#ifndef MIN
+#define MIN(a, b) ((a < b) ? (a) : (b))
+#endif
+#ifndef MAX
+#define MAX(a, b) ((a > b) ? (a) : (b))
+#endif
 #define OP_WORK 6
 
 #define OP_ARG_GBL 0
@@ -248,7 +256,13 @@ void op_mpi_barrier(void);
 #define OP2H_PLAN_GATHER 4 /* owner-compute: one thread per target element,
                               increments summed in element order (deterministic,
                               same order as the sequential executor) */
-#define OP2H_PLAN_COUNT 5
+#define OP2H_PLAN_OWNER 5  /* owner-compute partition: the target set is cut
+                              into compact blocks; a workgroup stages its
+                              block's indirect data in LDS, evaluates each
+                              element touching the block once, then sums every
+                              owned target's increments in element order
+                              (deterministic, sequential-executor order) */
+#define OP2H_PLAN_COUNT 6
 
 /* Synchronisation modes. */
 #define OP2H_SYNC_EVERY_LOOP 0 /* reference behaviour: device sync and gbl

@@ -78,6 +78,24 @@ void *op2h_gbl_slot(op2h_loop_ctx *ctx, int argi, int info_argi);
 void *op2h_reduct_scratch(size_t bytes);
 unsigned int *op2h_reduct_counter(void);
 
+/* ---- constants ----
+ * op_decl_const_char offers each constant to every registered hook; a hook
+ * copies the constants it knows (by name) to its device copies and returns 1,
+ * or returns 0.  Generated kernel files register one hook each at
+ * load time.  A name no hook knows is fatal, as in the reference
+ * (hip/master_kernel.cpp.jinja:21-25). */
+typedef int (*op2h_const_hook)(int dim, const char *type, int size,
+                               const char *dat, const char *name);
+void op2h_register_const_hook(op2h_const_hook hook);
+
+/* Begin a loop whose stub implements only the plans in `plan_mask` (bit
+ * 1 << OP2H_PLAN_*): the autotuner tries those; a request for another plan
+ * made for this loop by name is fatal, one made for all loops is ignored.  op2h_loop_begin(ctx, ...) is op2h_loop_begin_mask
+ * with every plan when def_plan is OP2H_PLAN_STAGED, else only def_plan. */
+void op2h_loop_begin_mask(op2h_loop_ctx *ctx, int kernel_idx, const char *name,
+                          op_set set, int nargs, op_arg *args, int def_plan,
+                          int def_block, unsigned plan_mask);
+
 /* ---- execution plans ---- */
 
 /* Block partition + two-level colouring (OP_STAGE_ALL semantics), with
@@ -125,9 +143,64 @@ typedef struct {
   double build_ms;
 } op2h_gather_plan;
 
+/* Owner-compute partition of a loop whose increments all go through one map
+ * into one dat (the "owner" set is that map's target set).  Targets are cut
+ * into blocks of at most `block_cells` by breadth-first growth from the
+ * lowest-numbered unassigned target (so blocks are compact patches whatever
+ * the numbering); every element with an increment target in a block is an
+ * "instance" of that block (elements on a block boundary belong to two).
+ * Per block: the instances in ascending element order, LDS staging lists per
+ * map (ascending unique targets), local (ushort) ids per (map, map index), the
+ * owned targets in ascending order, and per owned target its increments as
+ * ushort entries `local instance * ninc + k` (k = position of the increment
+ * among the loop's increment args) in ascending order, i.e. the order of the
+ * sequential executor. */
+typedef struct {
+  int set_size, to_size, block_cells, nblocks;
+  int ninc;                    /* increment args through the owner map */
+  int total;                   /* instances over all blocks */
+  int max_elems, max_owned;    /* per-block maxima */
+  int nmaps;
+  int max_ind_h[OP2H_MAX_ARGS];  /* per staged map: max unique targets */
+  int *owned_offs;             /* [nblocks+1] */
+  int *owned;                  /* owned targets, block by block */
+  int *elem_offs;              /* [nblocks+1] */
+  int *elems;                  /* [total] element of each instance */
+  int *csr_offs;               /* [owned count + 1], in `owned` order */
+  unsigned short *csr;         /* increment entries */
+  int *ind_offs[OP2H_MAX_ARGS];  /* per staged map: [nblocks] */
+  int *ind_sizes[OP2H_MAX_ARGS]; /* per staged map: [nblocks] */
+  int *ind_map[OP2H_MAX_ARGS];   /* per staged map: unique targets */
+  unsigned short *loc_map[OP2H_MAX_ARGS]; /* per (map, index): [total] */
+  int map_of_arg[OP2H_MAX_ARGS]; /* arg -> staged map slot, -1 if direct */
+  int loc_of_arg[OP2H_MAX_ARGS]; /* arg -> loc_map slot, -1 if direct */
+  int inc_of_arg[OP2H_MAX_ARGS]; /* arg -> k among increments, -1 if none */
+  double redundancy;           /* total / set_size */
+  double build_ms;
+} op2h_owner_plan;
+
 const op2h_staged_plan *op2h_plan_staged(op2h_loop_ctx *ctx);
 const op2h_color_plan *op2h_plan_color(op2h_loop_ctx *ctx);
 const op2h_gather_plan *op2h_plan_gather(op_map map);
+/* Owner plan with blocks of ctx->part_size targets (NULL and a message if the
+ * loop's increments do not all go through one map into one dat, or a block
+ * outgrows the 16-bit local ids). */
+const op2h_owner_plan *op2h_plan_owner(op2h_loop_ctx *ctx);
+const op2h_owner_plan *op2h_plan_owner_for(const char *name, op_set set,
+                                           int block_cells, int nargs,
+                                           op_arg *args);
+/* Copy an owner plan's block structure back to the host (any pointer may be
+ * NULL): owned_offs [nblocks+1], owned, elem_offs [nblocks+1], elems,
+ * csr_offs, csr.  Returns 0. */
+int op2h_owner_plan_download(const op2h_owner_plan *p, int *owned_offs,
+                             int *owned, int *elem_offs, int *elems,
+                             int *csr_offs, unsigned short *csr);
+/* The partition alone, on the host: targets [0,nto) of a map (nel x dim,
+ * row-major) joined by the map indices `inc_idx[0..ninc)` of each element;
+ * block_of[t] gets the block of each target (-1 for targets no element
+ * references).  Returns the number of blocks, or -1 on bad arguments. */
+int op2h_owner_partition(int nto, int nel, int dim, const int *map, int ninc,
+                         const int *inc_idx, int block_cells, int *block_of);
 
 /* Copy a plan's colouring back to the host for checking: thread colours,
  * block colours (per block), and block offsets/sizes.  Returns 0. */

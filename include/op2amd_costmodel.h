@@ -7,7 +7,7 @@
  * Adam (beta1 0.9, beta2 0.999, eps 1e-8).  Parameters are stored flat in the
  * order W1[nin][H], b1[H], W2[H][H], b2[H], W3[H], b3.  All reductions over
  * samples are sequential per output element, so a gradient is the same from
- * run to run.
+ * run to run.  Optional decoupled weight decay (AdamW).
  */
 #ifndef OP2AMD_COSTMODEL_H
 #define OP2AMD_COSTMODEL_H
@@ -35,7 +35,12 @@ int op2amd_mlp_forward(op2amd_mlp *m, const float *x, int n, float *y);
 int op2amd_mlp_loss_grad(op2amd_mlp *m, const float *x, const float *t, int n,
                          float *loss, float *grad);
 
-/* `epochs` full-batch Adam steps with learning rate `lr`; final loss out. */
+/* Decoupled weight decay for the following training steps (AdamW:
+ * p -= lr * wd * p after each Adam update; default 0).  Returns 0. */
+int op2amd_mlp_set_weight_decay(op2amd_mlp *m, float wd);
+
+/* `epochs` full-batch Adam steps with learning rate `lr`; final loss out.
+ * Training continues from the current parameters and optimiser state. */
 int op2amd_mlp_train(op2amd_mlp *m, const float *x, const float *t, int n,
                      int epochs, float lr, float *final_loss);
 

@@ -84,6 +84,36 @@ __device__ __forceinline__ void res_calc(const double *x1, const double *x2,
   res2[3] -= f;
 }
 
+// res_calc split into its per-cell and per-edge parts, for plans that
+// evaluate the per-cell part once per staged cell.  The operations and their
+// order are those of res_calc above, so the flux is bit-identical:
+//   cell terms: ri = 1/q[0], p = gm1*(q[3] - 0.5*ri*(q[1]^2 + q[2]^2));
+//   c[0..3] = q, c[4] = adt, c[5] = ri, c[6] = p;
+//   f = the increment of res1 (res2 receives -f).
+__device__ __forceinline__ void res_cell_terms(const double *q, double *ri,
+                                               double *p) {
+  *ri = 1.0 / q[0];
+  *p = gm1_d * (q[3] - 0.5 * (*ri) * (q[1] * q[1] + q[2] * q[2]));
+}
+
+__device__ __forceinline__ void res_edge_flux(const double *x1, const double *x2,
+                                              const double *c1, const double *c2,
+                                              double *f) {
+  double dx, dy, mu, vol1, vol2;
+  const double p1 = c1[6], p2 = c2[6];
+  dx = x1[0] - x2[0];
+  dy = x1[1] - x2[1];
+  vol1 = c1[5] * (c1[1] * dy - c1[2] * dx);
+  vol2 = c2[5] * (c2[1] * dy - c2[2] * dx);
+  mu = 0.5 * (c1[4] + c2[4]) * eps_d;
+  f[0] = 0.5 * (vol1 * c1[0] + vol2 * c2[0]) + mu * (c1[0] - c2[0]);
+  f[1] = 0.5 * (vol1 * c1[1] + p1 * dy + vol2 * c2[1] + p2 * dy) +
+         mu * (c1[1] - c2[1]);
+  f[2] = 0.5 * (vol1 * c1[2] - p1 * dx + vol2 * c2[2] - p2 * dx) +
+         mu * (c1[2] - c2[2]);
+  f[3] = 0.5 * (vol1 * (c1[3] + p1) + vol2 * (c2[3] + p2)) + mu * (c1[3] - c2[3]);
+}
+
 __device__ __forceinline__ void bres_calc(const double *x1, const double *x2,
                                           const double *q1, const double *adt1,
                                           double *res1, const int *bound) {
@@ -218,6 +248,29 @@ __device__ __forceinline__ void bres_calc(const float *x1, const float *x2,
     f = 0.5f * (vol1 * (q1[3] + p1) + vol2 * (qinf_f[3] + p2)) + mu * (q1[3] - qinf_f[3]);
     res1[3] += f;
   }
+}
+
+// res_calc (SP) split like op2_k::res_cell_terms / res_edge_flux:
+// c[0..3] = q, c[4] = adt, c[5] = 1/q[0], c[6] = pressure.
+__device__ __forceinline__ void res_cell_terms(const float *q, float *ri, float *p) {
+  *ri = 1.0f / q[0];
+  *p = gm1_f * (q[3] - 0.5f * (*ri) * (q[1] * q[1] + q[2] * q[2]));
+}
+
+__device__ __forceinline__ void res_edge_flux(const float *x1, const float *x2,
+                                              const float *c1, const float *c2,
+                                              float *f) {
+  float dx, dy, mu, vol1, vol2;
+  const float p1 = c1[6], p2 = c2[6];
+  dx = x1[0] - x2[0];
+  dy = x1[1] - x2[1];
+  vol1 = c1[5] * (c1[1] * dy - c1[2] * dx);
+  vol2 = c2[5] * (c2[1] * dy - c2[2] * dx);
+  mu = 0.5f * (c1[4] + c2[4]) * eps_f;
+  f[0] = 0.5f * (vol1 * c1[0] + vol2 * c2[0]) + mu * (c1[0] - c2[0]);
+  f[1] = 0.5f * (vol1 * c1[1] + p1 * dy + vol2 * c2[1] + p2 * dy) + mu * (c1[1] - c2[1]);
+  f[2] = 0.5f * (vol1 * c1[2] - p1 * dx + vol2 * c2[2] - p2 * dx) + mu * (c1[2] - c2[2]);
+  f[3] = 0.5f * (vol1 * (c1[3] + p1) + vol2 * (c2[3] + p2)) + mu * (c1[3] - c2[3]);
 }
 
 }  // namespace op2_kf

@@ -35,10 +35,10 @@
     }                                                                        \
   } while (0)
 
-extern "C" void op_decl_const_char(int dim, char const *type, int size,
-                                   char *dat, char const *name) {
+// The airfoil constants (airfoil.cpp:149-164), offered by op_decl_const_char.
+static int airfoil_consts(int dim, char const *type, int size, const char *dat,
+                          char const *name) {
   (void)type;
-  op2h_device_sync();
   size_t bytes = (size_t)dim * size;
   if (!strcmp(name, "gam")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(gam_d), dat, bytes));
   else if (!strcmp(name, "gm1")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(gm1_d), dat, bytes));
@@ -47,10 +47,7 @@ extern "C" void op_decl_const_char(int dim, char const *type, int size,
   else if (!strcmp(name, "mach")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(mach_d), dat, bytes));
   else if (!strcmp(name, "alpha")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(alpha_d), dat, bytes));
   else if (!strcmp(name, "qinf")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(qinf_d), dat, bytes));
-  else {
-    printf("op_decl_const error -- unknown constant %s\n", name);
-    exit(-1);
-  }
+  else return 0;
   // FP32 copies of the flux constants, rounded once on the host
   float f[4];
   for (int i = 0; i < dim && i < 4; ++i)
@@ -60,7 +57,12 @@ extern "C" void op_decl_const_char(int dim, char const *type, int size,
   else if (!strcmp(name, "cfl")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(cfl_f), f, sizeof(float)));
   else if (!strcmp(name, "eps")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(eps_f), f, sizeof(float)));
   else if (!strcmp(name, "qinf")) KCHECK(hipMemcpyToSymbol(HIP_SYMBOL(qinf_f), f, 4 * sizeof(float)));
+  return 1;
 }
+
+static const struct AirfoilConstHook {
+  AirfoilConstHook() { op2h_register_const_hook(airfoil_consts); }
+} airfoil_const_hook;
 
 
 namespace {
@@ -263,15 +265,19 @@ __global__ void __launch_bounds__(1024)
     ld2<SOA>(x, rr.y, sx, x2);
     ld4<SOA>(q, cell, sq, qa);
     ld4<SOA>(q, rr.z, sq, qb);
-    double aa = adt[cell], ab = adt[rr.z];
-    double r1[4] = {0.0, 0.0, 0.0, 0.0}, r2[4] = {0.0, 0.0, 0.0, 0.0};
-    if (rr.w == 0) {
-      op2_k::res_calc(x1, x2, qa, qb, &aa, &ab, r1, r2);
-      for (int d = 0; d < 4; ++d) r[d] = r1[d];
-    } else {
-      op2_k::res_calc(x1, x2, qb, qa, &ab, &aa, r1, r2);
-      for (int d = 0; d < 4; ++d) r[d] = r2[d];
+    // the edge's ecell[0] side is q1 whichever side this lane's cell is:
+    // operands are selected, not branched on, so the wave evaluates one flux
+    const bool second = rr.w != 0;
+    double q1[4], q2[4];
+    for (int d = 0; d < 4; ++d) {
+      q1[d] = second ? qb[d] : qa[d];
+      q2[d] = second ? qa[d] : qb[d];
     }
+    const double aa = adt[cell], ab = adt[rr.z];
+    double a1 = second ? ab : aa, a2 = second ? aa : ab;
+    double r1[4] = {0.0, 0.0, 0.0, 0.0}, r2[4] = {0.0, 0.0, 0.0, 0.0};
+    op2_k::res_calc(x1, x2, q1, q2, &a1, &a2, r1, r2);
+    for (int d = 0; d < 4; ++d) r[d] = second ? r2[d] : r1[d];
   }
   const int lead = (threadIdx.x & 63) & ~(K - 1);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -356,6 +362,123 @@ __global__ void __launch_bounds__(1024)
   }
   for (int k = threadIdx.x; k < nc; k += blockDim.x)
     add4<SOA>(res, P.ind_map_c[cbase + k], sq, lr + 4 * k);
+}
+
+struct OwnerArgs {
+  const int *owned_offs, *owned, *elem_offs, *csr_offs;
+  const unsigned short *csr;
+  const int *ind_offs_n, *ind_sizes_n, *ind_map_n;  // node map slot
+  const int *ind_offs_c, *ind_sizes_c, *ind_map_c;  // cell map slot
+  const unsigned short *loc_n0, *loc_n1, *loc_c0, *loc_c1;
+  int max_n, max_c, max_e;
+};
+
+// Owner plan pointers for res_calc's args (x via pedge 0/1, q/adt via
+// pecell 0/1, res incremented via pecell 0 (+f) and 1 (-f)); NULL plan or an
+// unexpected shape is fatal, as the reference's plan errors are.
+static OwnerArgs owner_args_res(const op2h_owner_plan *p) {
+  if (!p) exit(-1);
+  if (p->inc_of_arg[6] != 0 || p->inc_of_arg[7] != 1 || p->ninc != 2) {
+    printf("op2hip: res_calc owner plan expects res via ecell[0], ecell[1]\n");
+    exit(-1);
+  }
+  int sn = p->map_of_arg[0], sc = p->map_of_arg[2];
+  OwnerArgs A;
+  A.owned_offs = p->owned_offs;
+  A.owned = p->owned;
+  A.elem_offs = p->elem_offs;
+  A.csr_offs = p->csr_offs;
+  A.csr = p->csr;
+  A.ind_offs_n = p->ind_offs[sn];
+  A.ind_sizes_n = p->ind_sizes[sn];
+  A.ind_map_n = p->ind_map[sn];
+  A.ind_offs_c = p->ind_offs[sc];
+  A.ind_sizes_c = p->ind_sizes[sc];
+  A.ind_map_c = p->ind_map[sc];
+  A.loc_n0 = p->loc_map[p->loc_of_arg[0]];
+  A.loc_n1 = p->loc_map[p->loc_of_arg[1]];
+  A.loc_c0 = p->loc_map[p->loc_of_arg[2]];
+  A.loc_c1 = p->loc_map[p->loc_of_arg[3]];
+  A.max_n = p->max_ind_h[sn];
+  A.max_c = p->max_ind_h[sc];
+  A.max_e = p->max_elems;
+  return A;
+}
+
+// LDS bytes of the owner kernels: x [max_n][2], cells [max_c][8],
+// fluxes [max_e][4] of `word` bytes each; fatal beyond the 160 KiB of a CU.
+static size_t owner_lds(const OwnerArgs &A, size_t word) {
+  size_t lds = word * (2 * (size_t)A.max_n + 8 * (size_t)A.max_c + 4 * (size_t)A.max_e);
+  if (lds > 160 * 1024) {
+    printf("op2hip: res_calc owner plan needs %zu bytes of LDS; use a smaller block\n", lds);
+    exit(-1);
+  }
+  return lds;
+}
+
+// Owner-compute res_calc: one workgroup per block of owned cells.
+//  1. stage x of the block's nodes, and per staged cell q, adt, 1/q[0] and
+//     the pressure (the per-cell half of res_calc) in LDS;
+//  2. one flux per edge instance into LDS;
+//  3. each owned cell adds its edges' fluxes to res in ascending edge order
+//     (+f where it is ecell[0], -f where it is ecell[1]), as the sequential
+//     executor does, so res is bit-identical to it.
+// LDS: x [max_n][2], cells [max_c][8], fluxes [max_e][4] doubles.
+template <bool SOA>
+__global__ void __launch_bounds__(1024)
+    k_res_owner(const double *__restrict x, const double *__restrict q,
+                const double *__restrict adt, double *__restrict res,
+                OwnerArgs P, int sx, int sq) {
+  extern __shared__ double lds[];
+  const int b = xcd_block(kXcds);
+  double *lx = lds;
+  double *lc = lx + 2 * P.max_n;
+  double *lf = lc + 8 * P.max_c;
+  const int nn = P.ind_sizes_n[b], nbase = P.ind_offs_n[b];
+  for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+    double v[2];
+    ld2<SOA>(x, P.ind_map_n[nbase + k], sx, v);
+    lx[2 * k] = v[0];
+    lx[2 * k + 1] = v[1];
+  }
+  const int nc = P.ind_sizes_c[b], cbase = P.ind_offs_c[b];
+  for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+    const int cell = P.ind_map_c[cbase + k];
+    double v[4];
+    ld4<SOA>(q, cell, sq, v);
+    double *c = lc + 8 * k;
+    c[0] = v[0];
+    c[1] = v[1];
+    c[2] = v[2];
+    c[3] = v[3];
+    c[4] = adt[cell];
+    op2_k::res_cell_terms(v, c + 5, c + 6);
+  }
+  __syncthreads();
+  const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
+  for (int j = threadIdx.x; j < ne; j += blockDim.x) {
+    const int a0 = P.loc_n0[e0 + j], a1 = P.loc_n1[e0 + j];
+    const int l0 = P.loc_c0[e0 + j], l1 = P.loc_c1[e0 + j];
+    op2_k::res_edge_flux(lx + 2 * a0, lx + 2 * a1, lc + 8 * l0, lc + 8 * l1,
+                         lf + 4 * j);
+  }
+  __syncthreads();
+  const int o0 = P.owned_offs[b], no = P.owned_offs[b + 1] - o0;
+  for (int i = threadIdx.x; i < no; i += blockDim.x) {
+    const int cell = P.owned[o0 + i];
+    double acc[4];
+    ld4<SOA>(res, cell, sq, acc);
+    for (int k = P.csr_offs[o0 + i]; k < P.csr_offs[o0 + i + 1]; ++k) {
+      const int ent = P.csr[k];
+      const double *f = lf + 4 * (ent >> 1);
+      if (ent & 1) {
+        for (int d = 0; d < 4; ++d) acc[d] -= f[d];
+      } else {
+        for (int d = 0; d < 4; ++d) acc[d] += f[d];
+      }
+    }
+    st4<SOA>(res, cell, sq, acc);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -717,15 +840,17 @@ __global__ void __launch_bounds__(1024)
     ld2f<SOA>(x, rr.y, sx, x2);
     ld4f<SOA>(q, cell, sq, qa);
     ld4f<SOA>(q, rr.z, sq, qb);
-    float aa = adt[cell], ab = adt[rr.z];
-    float r1[4] = {0.f, 0.f, 0.f, 0.f}, r2[4] = {0.f, 0.f, 0.f, 0.f};
-    if (rr.w == 0) {
-      op2_kf::res_calc(x1, x2, qa, qb, &aa, &ab, r1, r2);
-      for (int d = 0; d < 4; ++d) r[d] = r1[d];
-    } else {
-      op2_kf::res_calc(x1, x2, qb, qa, &ab, &aa, r1, r2);
-      for (int d = 0; d < 4; ++d) r[d] = r2[d];
+    const bool second = rr.w != 0;
+    float q1[4], q2[4];
+    for (int d = 0; d < 4; ++d) {
+      q1[d] = second ? qb[d] : qa[d];
+      q2[d] = second ? qa[d] : qb[d];
     }
+    const float aa = adt[cell], ab = adt[rr.z];
+    float a1 = second ? ab : aa, a2 = second ? aa : ab;
+    float r1[4] = {0.f, 0.f, 0.f, 0.f}, r2[4] = {0.f, 0.f, 0.f, 0.f};
+    op2_kf::res_calc(x1, x2, q1, q2, &a1, &a2, r1, r2);
+    for (int d = 0; d < 4; ++d) r[d] = second ? r2[d] : r1[d];
   }
   const int lead = (threadIdx.x & 63) & ~(K - 1);
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
@@ -738,6 +863,56 @@ __global__ void __launch_bounds__(1024)
       for (int d = 0; d < 4; ++d) acc[d] += v[d];
   }
   if (k == 0 && valid) st4s<SOA>(res, cell, sq, acc);
+}
+
+// Mixed precision (SP fluxes, FP32 adt and res; x and q read as FP64 and
+// rounded), same structure as k_res_owner.
+template <bool SOA>
+__global__ void __launch_bounds__(1024)
+    k_res_owner_mixed(const double *__restrict x, const double *__restrict q,
+                      const float *__restrict adt, float *__restrict res,
+                      OwnerArgs P, int sx, int sq) {
+  extern __shared__ float ldsf[];
+  const int b = xcd_block(kXcds);
+  float *lx = ldsf;
+  float *lc = lx + 2 * P.max_n;
+  float *lf = lc + 8 * P.max_c;
+  const int nn = P.ind_sizes_n[b], nbase = P.ind_offs_n[b];
+  for (int k = threadIdx.x; k < nn; k += blockDim.x)
+    ld2f<SOA>(x, P.ind_map_n[nbase + k], sx, lx + 2 * k);
+  const int nc = P.ind_sizes_c[b], cbase = P.ind_offs_c[b];
+  for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+    const int cell = P.ind_map_c[cbase + k];
+    float *c = lc + 8 * k;
+    ld4f<SOA>(q, cell, sq, c);
+    c[4] = adt[cell];
+    op2_kf::res_cell_terms(c, c + 5, c + 6);
+  }
+  __syncthreads();
+  const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
+  for (int j = threadIdx.x; j < ne; j += blockDim.x) {
+    const int a0 = P.loc_n0[e0 + j], a1 = P.loc_n1[e0 + j];
+    const int l0 = P.loc_c0[e0 + j], l1 = P.loc_c1[e0 + j];
+    op2_kf::res_edge_flux(lx + 2 * a0, lx + 2 * a1, lc + 8 * l0, lc + 8 * l1,
+                          lf + 4 * j);
+  }
+  __syncthreads();
+  const int o0 = P.owned_offs[b], no = P.owned_offs[b + 1] - o0;
+  for (int i = threadIdx.x; i < no; i += blockDim.x) {
+    const int cell = P.owned[o0 + i];
+    float acc[4];
+    ld4s<SOA>(res, cell, sq, acc);
+    for (int k = P.csr_offs[o0 + i]; k < P.csr_offs[o0 + i + 1]; ++k) {
+      const int ent = P.csr[k];
+      const float *f = lf + 4 * (ent >> 1);
+      if (ent & 1) {
+        for (int d = 0; d < 4; ++d) acc[d] -= f[d];
+      } else {
+        for (int d = 0; d < 4; ++d) acc[d] += f[d];
+      }
+    }
+    st4s<SOA>(res, cell, sq, acc);
+  }
 }
 
 template <bool SOA>
@@ -963,11 +1138,25 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       // mixed precision: atomic plan, else the owner-compute plan
       const float *adtf = (const float *)args[4].data_d;
       float *resf = (float *)args[6].data_d;
+      const op2h_owner_plan *owner_p = nullptr;
+      if (ctx.plan == OP2H_PLAN_OWNER) {  // built before the timed region
+        ctx.part_size = bs;
+        owner_p = op2h_plan_owner(&ctx);
+        if (!owner_p) exit(-1);
+      }
       op2h_kernel_timer_start(&ctx);
       if (ctx.plan == OP2H_PLAN_ATOMIC) {
         int g = grid_for(set->size, bs);
         if (soa) k_res_atomic_mixed<true><<<g, bs, 0, st>>>(x, q, adtf, resf, pedge, pecell, set->size, sx, sq);
         else k_res_atomic_mixed<false><<<g, bs, 0, st>>>(x, q, adtf, resf, pedge, pecell, set->size, sx, sq);
+      } else if (ctx.plan == OP2H_PLAN_OWNER) {
+        const int nblocks = owner_p->nblocks;
+        OwnerArgs A = owner_args_res(owner_p);
+        size_t lds = owner_lds(A, sizeof(float));
+        if (nblocks) {
+          if (soa) k_res_owner_mixed<true><<<nblocks, bs, lds, st>>>(x, q, adtf, resf, A, sx, sq);
+          else k_res_owner_mixed<false><<<nblocks, bs, lds, st>>>(x, q, adtf, resf, A, sx, sq);
+        }
       } else {
         const op2h_gather_plan *p = op2h_plan_gather(args[6].map);
         if (p->max_count > 4) {
@@ -1038,6 +1227,19 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
           int g = grid_for(p->ntargets, bs);
           if (soa) k_res_gather<true><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
           else k_res_gather<false><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
+        }
+        op2h_kernel_timer_stop(&ctx);
+        break;
+      }
+      case OP2H_PLAN_OWNER: {
+        ctx.part_size = bs;
+        const op2h_owner_plan *p = op2h_plan_owner(&ctx);
+        OwnerArgs A = owner_args_res(p);
+        size_t lds = owner_lds(A, sizeof(double));
+        op2h_kernel_timer_start(&ctx);
+        if (p->nblocks) {
+          if (soa) k_res_owner<true><<<p->nblocks, bs, lds, st>>>(x, q, adt, res, A, sx, sq);
+          else k_res_owner<false><<<p->nblocks, bs, lds, st>>>(x, q, adt, res, A, sx, sq);
         }
         op2h_kernel_timer_stop(&ctx);
         break;
@@ -1148,6 +1350,9 @@ extern "C" void op_par_loop_airfoil_4_bres_calc(char const *name, op_set set,
         op2h_kernel_timer_stop(&ctx);
         break;
       }
+      // One increment per boundary edge: an owner partition would evaluate
+      // each edge once in its cell's block, which is what the gather plan does.
+      case OP2H_PLAN_OWNER:
       case OP2H_PLAN_GATHER: {
         const op2h_gather_plan *p = op2h_plan_gather(args[4].map);
         op2h_kernel_timer_start(&ctx);

@@ -293,18 +293,16 @@ int renumber_cells(op2amd_mesh *m, const std::vector<int> &cnew,
   std::vector<int> cm(nc);  // cells in new order
   for (int c = 0; c < nc; ++c) cm[cnew[c]] = c;
   // edges: stable sort by new index of ecell[e][0]
-  std::vector<int> eorder(m->nedge);
-  std::iota(eorder.begin(), eorder.end(), 0);
-  std::stable_sort(eorder.begin(), eorder.end(), [&](int a, int b) {
-    return cnew[m->ecell[2 * a]] < cnew[m->ecell[2 * b]];
-  });
-  std::vector<int> enew = invert(eorder);
-  std::vector<int> border(m->nbedge);
-  std::iota(border.begin(), border.end(), 0);
-  std::stable_sort(border.begin(), border.end(), [&](int a, int b) {
-    return cnew[m->becell[a]] < cnew[m->becell[b]];
-  });
-  std::vector<int> bnew = invert(border);
+  // (a counting sort on the key: stable and linear)
+  auto stable_by_cell = [&](const int *col, int stride, int n) {
+    std::vector<int> start(nc + 1, 0), out(n);
+    for (int e = 0; e < n; ++e) ++start[cnew[col[(size_t)stride * e]] + 1];
+    for (int c = 0; c < nc; ++c) start[c + 1] += start[c];
+    for (int e = 0; e < n; ++e) out[e] = start[cnew[col[(size_t)stride * e]]]++;
+    return out;  // new index of each old element
+  };
+  std::vector<int> enew = stable_by_cell(m->ecell, 2, m->nedge);
+  std::vector<int> bnew = stable_by_cell(m->becell, 1, m->nbedge);
   // nodes: first visit through cell->node in the new cell order
   std::vector<int> nnew(m->nnode, -1);
   int counter = 0;

@@ -30,6 +30,7 @@ struct op2amd_mlp {
         *A2 = nullptr, *Y = nullptr, *DY = nullptr, *D2 = nullptr, *D1 = nullptr,
         *loss = nullptr;
   int adam_t = 0;
+  float wd = 0.f;  // decoupled weight decay
   hipStream_t st = nullptr;
   // parameter offsets
   int oW1, ob1, oW2, ob2, oW3, ob3;
@@ -106,9 +107,10 @@ __global__ void k_back(const float *__restrict D, const float *__restrict W,
   Din[i] = Z[i] > 0.f ? s : 0.f;
 }
 
+// Adam with decoupled weight decay `wd` (AdamW; wd = 0 is plain Adam).
 __global__ void k_adam(float *__restrict P, const float *__restrict G,
                        float *__restrict M, float *__restrict V, int np, float lr,
-                       float c1, float c2) {
+                       float c1, float c2, float wd) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   float g = G[i];
@@ -116,7 +118,8 @@ __global__ void k_adam(float *__restrict P, const float *__restrict G,
   float v = 0.999f * V[i] + 0.001f * g * g;
   M[i] = m;
   V[i] = v;
-  P[i] -= lr * (m / c1) / (sqrtf(v / c2) + 1e-8f);
+  float p = P[i];
+  P[i] = p - lr * (m / c1) / (sqrtf(v / c2) + 1e-8f) - lr * wd * p;
 }
 
 void ensure(op2amd_mlp *m, int n) {
@@ -187,6 +190,7 @@ extern "C" op2amd_mlp *op2amd_mlp_create(int nin, int hidden, uint64_t seed) {
   MCHECK(hipMalloc(&m->loss, sizeof(float)));
   MCHECK(hipMemset(m->M, 0, sizeof(float) * m->np));
   MCHECK(hipMemset(m->V, 0, sizeof(float) * m->np));
+  MCHECK(hipDeviceSynchronize());  // null-stream work done before m->st uses it
   // He initialisation: normal(0, sqrt(2/fan_in)) by Box-Muller on SplitMix64
   std::vector<float> p(m->np, 0.f);
   uint64_t s = seed;
@@ -251,6 +255,12 @@ extern "C" int op2amd_mlp_loss_grad(op2amd_mlp *m, const float *x, const float *
   return 0;
 }
 
+extern "C" int op2amd_mlp_set_weight_decay(op2amd_mlp *m, float wd) {
+  if (!m || wd < 0.f) return 1;
+  m->wd = wd;
+  return 0;
+}
+
 extern "C" int op2amd_mlp_train(op2amd_mlp *m, const float *x, const float *t, int n,
                                 int epochs, float lr, float *final_loss) {
   if (n < 1) return 1;
@@ -261,7 +271,8 @@ extern "C" int op2amd_mlp_train(op2amd_mlp *m, const float *x, const float *t, i
     ++m->adam_t;
     float c1 = 1.f - std::pow(0.9f, (float)m->adam_t);
     float c2 = 1.f - std::pow(0.999f, (float)m->adam_t);
-    k_adam<<<blocks(m->np, 256), 256, 0, m->st>>>(m->P, m->G, m->M, m->V, m->np, lr, c1, c2);
+    k_adam<<<blocks(m->np, 256), 256, 0, m->st>>>(m->P, m->G, m->M, m->V, m->np, lr, c1, c2,
+                                                  m->wd);
   }
   forward(m, n);
   k_loss<<<1, 256, 0, m->st>>>(m->Y, m->T, m->loss, n);

@@ -313,6 +313,7 @@ template <class T> T *dev_alloc(size_t n, std::vector<void *> &keep) {
   void *p = nullptr;
   ECHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
   ECHECK(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  ECHECK(hipDeviceSynchronize());  // done before the non-blocking stream uses it
   keep.push_back(p);
   return (T *)p;
 }

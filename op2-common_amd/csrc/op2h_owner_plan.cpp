@@ -1,0 +1,313 @@
+// Owner-compute partition plan (OP2H_PLAN_OWNER).
+//
+// The reference's plans execute increments either through a two-level
+// colouring of contiguous element blocks (op_plan_core with OP_STAGE_ALL,
+// op2/src/core/op_rt_support.cpp:426-450 block partition, :609-682 staging
+// lists, :684-884 colouring) or with atomics.  This plan partitions the
+// *target* set instead: each block owns a compact patch of targets, evaluates
+// every element touching the patch once (boundary elements are evaluated by
+// both neighbouring blocks, as the reference's MPI backend does for its exec
+// halo, docs/developer_guide.rst:254-333), and each owned target sums its own
+// increments in ascending element order.  No colours, no atomics, and the sum
+// order is the sequential executor's (op_seq.h:918-1073), so results are
+// bit-identical to it.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "op2h_plan_build.h"
+
+#define OCHECK(x)                                                            \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess) {                                                  \
+      printf("op2hip: %s failed at %s:%d: %s\n", #x, __FILE__, __LINE__,     \
+             hipGetErrorString(e_));                                         \
+      exit(-1);                                                              \
+    }                                                                        \
+  } while (0)
+
+namespace op2h {
+
+// Breadth-first growth of blocks over the target graph (targets joined when
+// one element increments both).  Seeds are taken in ascending target order;
+// when a block's search runs dry before block_cells targets (a fragment cut
+// off by earlier blocks, or no adjacency at all) it continues from the next
+// unassigned seed, so every block but the last holds exactly block_cells
+// targets.  Returns the block count.
+static int grow_blocks(int nto, int nel, int dim, const int *map, int ninc,
+                       const int *inc_idx, int block_cells,
+                       std::vector<int> &block_of) {
+  std::vector<int> deg(nto + 1, 0);
+  std::vector<char> used(nto, 0);
+  for (int e = 0; e < nel; ++e)
+    for (int k = 0; k < ninc; ++k) {
+      int t = map[(size_t)e * dim + inc_idx[k]];
+      used[t] = 1;
+      for (int l = 0; l < ninc; ++l) {
+        int u = map[(size_t)e * dim + inc_idx[l]];
+        if (l != k && u != t) deg[t + 1]++;
+      }
+    }
+  for (int t = 0; t < nto; ++t) deg[t + 1] += deg[t];
+  std::vector<int> adj(deg[nto]), fill(deg.begin(), deg.end() - 1);
+  for (int e = 0; e < nel; ++e)
+    for (int k = 0; k < ninc; ++k) {
+      int t = map[(size_t)e * dim + inc_idx[k]];
+      for (int l = 0; l < ninc; ++l) {
+        int u = map[(size_t)e * dim + inc_idx[l]];
+        if (l != k && u != t) adj[fill[t]++] = u;
+      }
+    }
+  block_of.assign(nto, -1);
+  std::vector<int> mark(nto, -1), queue;
+  queue.reserve(4 * (size_t)block_cells + 16);
+  int nb = 0, seed = 0;
+  auto next_seed = [&]() {
+    // (called only with an empty or fully drained queue, so no unassigned
+    // target is waiting in it)
+    while (seed < nto && (block_of[seed] >= 0 || !used[seed])) ++seed;
+    return seed < nto;
+  };
+  while (next_seed()) {
+    queue.clear();
+    size_t head = 0;
+    int count = 0;
+    while (count < block_cells) {
+      if (head == queue.size()) {
+        if (!next_seed()) break;
+        mark[seed] = nb;
+        queue.push_back(seed);
+      }
+      int c = queue[head++];
+      block_of[c] = nb;
+      ++count;
+      for (int j = deg[c]; j < deg[c + 1]; ++j) {
+        int u = adj[j];
+        if (block_of[u] < 0 && mark[u] != nb) {
+          mark[u] = nb;
+          queue.push_back(u);
+        }
+      }
+    }
+    ++nb;
+  }
+  return nb;
+}
+
+std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
+                                               int nargs, const op_arg *args,
+                                               hipStream_t stream,
+                                               std::string &err) {
+  LoopShape sh = loop_shape(set, nargs, args);
+  auto rec = std::make_unique<OwnerPlanRec>();
+  op2h_owner_plan &P = rec->plan;
+  // the increment args: all through one map into one dat
+  op_map own = nullptr;
+  op_dat owndat = nullptr;
+  std::vector<int> inc_idx;
+  for (int m = 0; m < nargs; ++m) P.inc_of_arg[m] = -1;
+  for (int m = 0; m < nargs; ++m) {
+    const op_arg &a = args[m];
+    if (a.argtype != OP_ARG_DAT || !a.opt || !a.map || a.idx < 0) continue;
+    if (a.acc == OP_RW || a.acc == OP_WRITE) {
+      err = "owner plan: indirect OP_RW/OP_WRITE args are not supported";
+      return nullptr;
+    }
+    if (a.acc != OP_INC) continue;
+    if ((own && a.map != own) || (owndat && a.dat != owndat)) {
+      err = "owner plan: increments must all go through one map into one dat";
+      return nullptr;
+    }
+    own = a.map;
+    owndat = a.dat;
+    P.inc_of_arg[m] = (int)inc_idx.size();
+    inc_idx.push_back(a.idx);
+  }
+  if (!own) {
+    err = "owner plan: the loop has no indirect increments";
+    return nullptr;
+  }
+  if (block_cells < 1 || block_cells > 4096) {
+    err = "owner plan: block size must be in [1, 4096]";
+    return nullptr;
+  }
+  const int nel = set->size, nto = own->to->size, dim = own->dim;
+  const int ninc = (int)inc_idx.size();
+  const int *map = own->map;
+  std::vector<int> block_of;
+  const int nb = grow_blocks(nto, nel, dim, map, ninc, inc_idx.data(),
+                             block_cells, block_of);
+  P.set_size = nel;
+  P.to_size = nto;
+  P.block_cells = block_cells;
+  P.nblocks = nb;
+  P.ninc = ninc;
+
+  // owned targets, ascending within each block
+  std::vector<int> owned_offs(nb + 1, 0);
+  for (int t = 0; t < nto; ++t)
+    if (block_of[t] >= 0) owned_offs[block_of[t] + 1]++;
+  for (int b = 0; b < nb; ++b) owned_offs[b + 1] += owned_offs[b];
+  std::vector<int> owned(owned_offs[nb]);
+  {
+    std::vector<int> f(owned_offs.begin(), owned_offs.end() - 1);
+    for (int t = 0; t < nto; ++t)
+      if (block_of[t] >= 0) owned[f[block_of[t]]++] = t;
+  }
+  // instances: each element once per distinct block among its targets
+  auto blocks_of_elem = [&](int e, int *out) {
+    int n = 0;
+    for (int k = 0; k < ninc; ++k) {
+      int b = block_of[map[(size_t)e * dim + inc_idx[k]]];
+      bool seen = false;
+      for (int j = 0; j < n; ++j) seen |= out[j] == b;
+      if (!seen) out[n++] = b;
+    }
+    return n;
+  };
+  std::vector<int> elem_offs(nb + 1, 0), tmp(ninc);
+  for (int e = 0; e < nel; ++e) {
+    int n = blocks_of_elem(e, tmp.data());
+    for (int j = 0; j < n; ++j) elem_offs[tmp[j] + 1]++;
+  }
+  for (int b = 0; b < nb; ++b) elem_offs[b + 1] += elem_offs[b];
+  const int total = elem_offs[nb];
+  std::vector<int> elems(total);
+  {
+    std::vector<int> f(elem_offs.begin(), elem_offs.end() - 1);
+    for (int e = 0; e < nel; ++e) {
+      int n = blocks_of_elem(e, tmp.data());
+      for (int j = 0; j < n; ++j) elems[f[tmp[j]]++] = e;
+    }
+  }
+  P.total = total;
+  P.redundancy = nel ? (double)total / nel : 1.0;
+  P.max_elems = P.max_owned = 0;
+  for (int b = 0; b < nb; ++b) {
+    P.max_elems = std::max(P.max_elems, elem_offs[b + 1] - elem_offs[b]);
+    P.max_owned = std::max(P.max_owned, owned_offs[b + 1] - owned_offs[b]);
+  }
+  if ((long long)P.max_elems * ninc > 65535) {
+    err = "owner plan: a block has too many elements for 16-bit local ids";
+    return nullptr;
+  }
+
+  // increments of each owned target, in (instance, k) order
+  std::vector<int> ownpos(nto, -1);
+  for (int i = 0; i < (int)owned.size(); ++i) ownpos[owned[i]] = i;
+  std::vector<int> csr_offs(owned.size() + 1, 0);
+  for (int b = 0; b < nb; ++b)
+    for (int j = elem_offs[b]; j < elem_offs[b + 1]; ++j)
+      for (int k = 0; k < ninc; ++k) {
+        int t = map[(size_t)elems[j] * dim + inc_idx[k]];
+        if (block_of[t] == b) csr_offs[ownpos[t] + 1]++;
+      }
+  for (size_t i = 0; i < owned.size(); ++i) csr_offs[i + 1] += csr_offs[i];
+  std::vector<unsigned short> csr(csr_offs.back());
+  {
+    std::vector<int> f(csr_offs.begin(), csr_offs.end() - 1);
+    for (int b = 0; b < nb; ++b)
+      for (int j = elem_offs[b]; j < elem_offs[b + 1]; ++j)
+        for (int k = 0; k < ninc; ++k) {
+          int t = map[(size_t)elems[j] * dim + inc_idx[k]];
+          if (block_of[t] == b)
+            csr[f[ownpos[t]]++] =
+                (unsigned short)((j - elem_offs[b]) * ninc + k);
+        }
+  }
+
+  // staging lists per map and local ids per (map, index)
+  P.nmaps = (int)sh.maps.size();
+  for (int m = 0; m < nargs; ++m) {
+    P.map_of_arg[m] = sh.map_slot[m];
+    P.loc_of_arg[m] = sh.loc_slot[m];
+  }
+  std::vector<std::vector<unsigned short>> loc(sh.locs.size(),
+                                               std::vector<unsigned short>(total));
+  for (int s = 0; s < P.nmaps; ++s) {
+    op_map M = sh.maps[s];
+    std::vector<int> idxs;  // map indices of this map the loop uses
+    std::vector<int> lslots;
+    for (size_t l = 0; l < sh.locs.size(); ++l)
+      if (sh.locs[l].first == M) {
+        idxs.push_back(sh.locs[l].second);
+        lslots.push_back((int)l);
+      }
+    std::vector<int> offs(nb), sizes(nb), list, pos(M->to->size, -1), work;
+    int mx = 0;
+    for (int b = 0; b < nb; ++b) {
+      work.clear();
+      for (int j = elem_offs[b]; j < elem_offs[b + 1]; ++j)
+        for (int i : idxs) work.push_back(M->map[(size_t)elems[j] * M->dim + i]);
+      std::sort(work.begin(), work.end());
+      work.erase(std::unique(work.begin(), work.end()), work.end());
+      if (work.size() > 65535) {
+        err = "owner plan: a block stages too many targets for 16-bit local ids";
+        return nullptr;
+      }
+      offs[b] = (int)list.size();
+      sizes[b] = (int)work.size();
+      mx = std::max(mx, sizes[b]);
+      for (size_t i = 0; i < work.size(); ++i) pos[work[i]] = (int)i;
+      list.insert(list.end(), work.begin(), work.end());
+      for (int j = elem_offs[b]; j < elem_offs[b + 1]; ++j)
+        for (size_t q = 0; q < idxs.size(); ++q)
+          loc[lslots[q]][j] =
+              (unsigned short)pos[M->map[(size_t)elems[j] * M->dim + idxs[q]]];
+      for (int t : work) pos[t] = -1;
+    }
+    P.max_ind_h[s] = mx;
+    P.ind_offs[s] = rec->mem.upload(offs, stream);
+    P.ind_sizes[s] = rec->mem.upload(sizes, stream);
+    P.ind_map[s] = rec->mem.upload(list, stream);
+  }
+  for (size_t l = 0; l < loc.size(); ++l) P.loc_map[l] = rec->mem.upload(loc[l], stream);
+  P.owned_offs = rec->mem.upload(owned_offs, stream);
+  P.owned = rec->mem.upload(owned, stream);
+  P.elem_offs = rec->mem.upload(elem_offs, stream);
+  P.elems = rec->mem.upload(elems, stream);
+  P.csr_offs = rec->mem.upload(csr_offs, stream);
+  P.csr = rec->mem.upload(csr, stream);
+  rec->n_owned = (int)owned.size();
+  rec->n_csr = (int)csr.size();
+  OCHECK(hipStreamSynchronize(stream));
+  return rec;
+}
+
+int download_owner(const OwnerPlanRec &r, int *owned_offs, int *owned,
+                   int *elem_offs, int *elems, int *csr_offs,
+                   unsigned short *csr) {
+  const op2h_owner_plan &p = r.plan;
+  OCHECK(hipDeviceSynchronize());
+  auto get = [](void *dst, const void *src, size_t bytes) {
+    if (dst && bytes) OCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  };
+  get(owned_offs, p.owned_offs, sizeof(int) * (p.nblocks + 1));
+  get(owned, p.owned, sizeof(int) * r.n_owned);
+  get(elem_offs, p.elem_offs, sizeof(int) * (p.nblocks + 1));
+  get(elems, p.elems, sizeof(int) * p.total);
+  get(csr_offs, p.csr_offs, sizeof(int) * (r.n_owned + 1));
+  get(csr, p.csr, sizeof(unsigned short) * r.n_csr);
+  return 0;
+}
+
+}  // namespace op2h
+
+extern "C" int op2h_owner_partition(int nto, int nel, int dim, const int *map,
+                                    int ninc, const int *inc_idx,
+                                    int block_cells, int *block_of) {
+  if (nto < 0 || nel < 0 || dim < 1 || ninc < 1 || !map || !inc_idx ||
+      block_cells < 1 || !block_of)
+    return -1;
+  for (int k = 0; k < ninc; ++k)
+    if (inc_idx[k] < 0 || inc_idx[k] >= dim) return -1;
+  for (size_t i = 0; i < (size_t)nel * dim; ++i)
+    if (map[i] < 0 || map[i] >= nto) return -1;
+  std::vector<int> b;
+  int nb = op2h::grow_blocks(nto, nel, dim, map, ninc, inc_idx, block_cells, b);
+  std::copy(b.begin(), b.end(), block_of);
+  return nb;
+}

@@ -37,6 +37,13 @@ struct GatherPlanRec {
   DeviceAllocs mem;
 };
 
+struct OwnerPlanRec {
+  std::string signature;
+  op2h_owner_plan plan{};
+  int n_owned = 0, n_csr = 0;
+  DeviceAllocs mem;
+};
+
 // Host-side description of the indirect structure of a loop.
 struct LoopShape {
   int set_size = 0;
@@ -72,6 +79,15 @@ std::unique_ptr<ColorPlanRec> build_color_plan(op_set set, int nargs,
                                                int on_device,
                                                hipStream_t stream);
 std::unique_ptr<GatherPlanRec> build_gather_plan(op_map map, hipStream_t stream);
+// Owner-compute partition plan (op2h_owner_plan); nullptr and `err` set when
+// the loop does not have the required shape.
+std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
+                                               int nargs, const op_arg *args,
+                                               hipStream_t stream,
+                                               std::string &err);
+int download_owner(const OwnerPlanRec &r, int *owned_offs, int *owned,
+                   int *elem_offs, int *elems, int *csr_offs,
+                   unsigned short *csr);
 
 // GPU colouring (op2h_plan_gpu.hip).  gpu_colour_elements colours elements
 // [0,n) so that no two elements sharing an increment target get the same

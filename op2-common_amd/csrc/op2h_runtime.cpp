@@ -113,6 +113,7 @@ struct Runtime {
   // plans
   std::vector<std::unique_ptr<op2h::StagedPlanRec>> staged;
   std::vector<std::unique_ptr<op2h::ColorPlanRec>> color;
+  std::vector<std::unique_ptr<op2h::OwnerPlanRec>> owner;
   std::map<op_map, std::unique_ptr<op2h::GatherPlanRec>> gather;
   // reductions
   void *red_scratch = nullptr;
@@ -163,7 +164,8 @@ void ensure_init() {
   HIPCHECK(hipSetDevice(0));
   HIPCHECK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
   HIPCHECK(hipMalloc(&R.red_counter, 64 * sizeof(unsigned int)));
-  HIPCHECK(hipMemset(R.red_counter, 0, 64 * sizeof(unsigned int)));
+  HIPCHECK(hipMemsetAsync(R.red_counter, 0, 64 * sizeof(unsigned int), R.stream));
+  HIPCHECK(hipStreamSynchronize(R.stream));
   OP_hybrid_gpu = 1;
   R.initialised = true;
 }
@@ -202,7 +204,9 @@ void upload_dat(op_dat d) {
   size_t bytes = (size_t)d->size * (size_t)std::max(s->stride, 1);
   if (!d->data_d) {
     HIPCHECK(hipMalloc(&d->data_d, std::max(bytes, (size_t)64)));
-    HIPCHECK(hipMemset(d->data_d, 0, std::max(bytes, (size_t)64)));
+    // on the library's stream: it does not synchronise with the null stream,
+    // so a null-stream memset could land after the copy below
+    HIPCHECK(hipMemsetAsync(d->data_d, 0, std::max(bytes, (size_t)64), R.stream));
   }
   if (!dat_is_soa(d)) {
     HIPCHECK(hipMemcpyAsync(d->data_d, d->data, (size_t)d->size * n,
@@ -339,7 +343,8 @@ extern "C" void op_init(int argc, char **argv, int diags) {
       int p = plan == "atomic" ? OP2H_PLAN_ATOMIC
               : plan == "color" ? OP2H_PLAN_COLOR
               : plan == "staged" ? OP2H_PLAN_STAGED
-              : plan == "gather" ? OP2H_PLAN_GATHER : OP2H_PLAN_AUTO;
+              : plan == "gather" ? OP2H_PLAN_GATHER
+              : plan == "owner" ? OP2H_PLAN_OWNER : OP2H_PLAN_AUTO;
       op2h_set_loop_plan(loop == "*" ? nullptr : loop.c_str(), p, bs);
     }
   }
@@ -359,6 +364,7 @@ extern "C" void op_exit(void) {
   op2h_graph_destroy();
   R.staged.clear();
   R.color.clear();
+  R.owner.clear();
   R.gather.clear();
   for (auto &kv : R.user_plans) (void)hipFree(kv.second.first);
   R.user_plans.clear();
@@ -1030,6 +1036,7 @@ extern "C" void op_renumber(op_map base) {
     if (d) upload_dat(d);
   R.staged.clear();
   R.color.clear();
+  R.owner.clear();
   R.gather.clear();
   for (auto &kv : R.user_plans) (void)hipFree(kv.second.first);
   R.user_plans.clear();
@@ -1151,9 +1158,42 @@ extern "C" const char *op2h_version(void) { return "op2hip 0.1 gfx950"; }
 // ===========================================================================
 // loop lifecycle
 // ===========================================================================
+namespace {
+std::vector<op2h_const_hook> &const_hooks() {
+  static std::vector<op2h_const_hook> hooks;  // filled by static initialisers
+  return hooks;
+}
+}  // namespace
+
+extern "C" void op2h_register_const_hook(op2h_const_hook hook) {
+  if (hook) const_hooks().push_back(hook);
+}
+
+extern "C" void op_decl_const_char(int dim, char const *type, int size,
+                                   char *dat, char const *name) {
+  op2h_device_sync();
+  int known = 0;  // every kernel file holding a copy of the constant gets it
+  for (op2h_const_hook h : const_hooks()) known |= h(dim, type, size, dat, name);
+  if (!known) {
+    printf("op_decl_const error -- unknown constant %s\n", name);
+    exit(-1);
+  }
+}
+
 extern "C" void op2h_loop_begin(op2h_loop_ctx *ctx, int kernel_idx,
                                 const char *name, op_set set, int nargs,
                                 op_arg *args, int def_plan, int def_block) {
+  unsigned all = 0;
+  for (int p = 1; p < OP2H_PLAN_COUNT; ++p) all |= 1u << p;
+  op2h_loop_begin_mask(ctx, kernel_idx, name, set, nargs, args, def_plan,
+                       def_block,
+                       def_plan == OP2H_PLAN_STAGED ? all : 1u << def_plan);
+}
+
+extern "C" void op2h_loop_begin_mask(op2h_loop_ctx *ctx, int kernel_idx,
+                                     const char *name, op_set set, int nargs,
+                                     op_arg *args, int def_plan, int def_block,
+                                     unsigned plan_mask) {
   Runtime &R = rt();
   ensure_init();
   std::memset(ctx, 0, sizeof(*ctx));
@@ -1191,9 +1231,12 @@ extern "C" void op2h_loop_begin(op2h_loop_ctx *ctx, int kernel_idx,
   if (R.autotune && c.plan == OP2H_PLAN_AUTO && c.block_size == 0 && set->size > 0) {
     TuneState &ts = R.tune[name];
     if (ts.cands.empty()) {
-      if (def_plan == OP2H_PLAN_STAGED) {
-        for (int p : {OP2H_PLAN_ATOMIC, OP2H_PLAN_COLOR, OP2H_PLAN_STAGED, OP2H_PLAN_GATHER})
-          for (int b : {128, 256, 512}) ts.cands.push_back({p, b});
+      int nplans = 0;
+      for (int p = 1; p < OP2H_PLAN_COUNT; ++p) nplans += (plan_mask >> p) & 1;
+      if (nplans > 1) {
+        for (int p = 1; p < OP2H_PLAN_COUNT; ++p)
+          if ((plan_mask >> p) & 1)
+            for (int b : {128, 256, 512}) ts.cands.push_back({p, b});
       } else {
         for (int b : {128, 256, 512, 1024}) ts.cands.push_back({def_plan, b});
       }
@@ -1221,6 +1264,13 @@ extern "C" void op2h_loop_begin(op2h_loop_ctx *ctx, int kernel_idx,
     return;
   }
   ctx->plan = c.plan == OP2H_PLAN_AUTO ? def_plan : c.plan;
+  if (!((plan_mask >> ctx->plan) & 1)) {
+    if (it != R.loop_cfg.end()) {
+      printf("op2hip: loop %s does not implement plan %d\n", name, ctx->plan);
+      exit(-1);
+    }
+    ctx->plan = def_plan;  // a setting for all loops: this loop keeps its own
+  }
   ctx->block_size = c.block_size ? c.block_size
                                  : (OP_block_size > 0 && OP_block_size % 64 == 0 &&
                                             OP_block_size != 256
@@ -1531,6 +1581,47 @@ extern "C" const op2h_gather_plan *op2h_plan_gather(op_map map) {
   auto *p = &rec->plan;
   R.gather[map] = std::move(rec);
   return p;
+}
+
+extern "C" const op2h_owner_plan *op2h_plan_owner_for(const char *name,
+                                                      op_set set,
+                                                      int block_cells,
+                                                      int nargs, op_arg *args) {
+  Runtime &R = rt();
+  std::string sig = arg_signature(name, set, nargs, args, block_cells);
+  for (auto &p : R.owner)
+    if (p->signature == sig) return &p->plan;
+  if (R.capturing) {
+    printf("op2hip: plan for %s must be built before graph capture\n", name);
+    exit(-1);
+  }
+  double t0 = wall_now();
+  std::string err;
+  auto rec = op2h::build_owner_plan(set, block_cells, nargs, args, R.stream, err);
+  if (!rec) {
+    printf("op2hip: %s: %s\n", name, err.c_str());
+    return nullptr;
+  }
+  rec->signature = sig;
+  rec->plan.build_ms = (wall_now() - t0) * 1e3;
+  R.owner.push_back(std::move(rec));
+  return &R.owner.back()->plan;
+}
+
+extern "C" const op2h_owner_plan *op2h_plan_owner(op2h_loop_ctx *ctx) {
+  return op2h_plan_owner_for(ctx->name, ctx->set, ctx->part_size, ctx->nargs,
+                             ctx->args);
+}
+
+extern "C" int op2h_owner_plan_download(const op2h_owner_plan *p,
+                                        int *owned_offs, int *owned,
+                                        int *elem_offs, int *elems,
+                                        int *csr_offs, unsigned short *csr) {
+  for (auto &r : rt().owner)
+    if (&r->plan == p)
+      return op2h::download_owner(*r, owned_offs, owned, elem_offs, elems,
+                                  csr_offs, csr);
+  return 1;
 }
 
 extern "C" void *op2h_plan_user(const char *key, size_t bytes, int *fresh) {

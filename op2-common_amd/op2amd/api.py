@@ -11,8 +11,8 @@ from ._lib import lib
 OP_READ, OP_WRITE, OP_RW, OP_INC, OP_MIN, OP_MAX = 0, 1, 2, 3, 4, 5
 OP_ARG_GBL, OP_ARG_DAT, OP_ARG_IDX, OP_ARG_INFO = 0, 1, 2, 3
 
-PLAN_AUTO, PLAN_ATOMIC, PLAN_COLOR, PLAN_STAGED, PLAN_GATHER = 0, 1, 2, 3, 4
-PLAN_NAMES = {"auto": 0, "atomic": 1, "color": 2, "staged": 3, "gather": 4}
+PLAN_AUTO, PLAN_ATOMIC, PLAN_COLOR, PLAN_STAGED, PLAN_GATHER, PLAN_OWNER = 0, 1, 2, 3, 4, 5
+PLAN_NAMES = {"auto": 0, "atomic": 1, "color": 2, "staged": 3, "gather": 4, "owner": 5}
 SYNC_EVERY_LOOP, SYNC_GBL_ONLY = 0, 1
 
 
@@ -128,6 +128,46 @@ class op_color_plan(C.Structure):
                 ("col_offsets_h", C.POINTER(C.c_int)),
                 ("col_reord", C.c_void_p), ("built_on_device", C.c_int),
                 ("build_ms", C.c_double)]
+
+
+class op_owner_plan(C.Structure):
+    _fields_ = [("set_size", C.c_int), ("to_size", C.c_int), ("block_cells", C.c_int),
+                ("nblocks", C.c_int), ("ninc", C.c_int), ("total", C.c_int),
+                ("max_elems", C.c_int), ("max_owned", C.c_int), ("nmaps", C.c_int),
+                ("max_ind_h", C.c_int * 32),
+                ("owned_offs", C.c_void_p), ("owned", C.c_void_p),
+                ("elem_offs", C.c_void_p), ("elems", C.c_void_p),
+                ("csr_offs", C.c_void_p), ("csr", C.c_void_p),
+                ("ind_offs", C.c_void_p * 32), ("ind_sizes", C.c_void_p * 32),
+                ("ind_map", C.c_void_p * 32), ("loc_map", C.c_void_p * 32),
+                ("map_of_arg", C.c_int * 32), ("loc_of_arg", C.c_int * 32),
+                ("inc_of_arg", C.c_int * 32),
+                ("redundancy", C.c_double), ("build_ms", C.c_double)]
+
+
+op2h_plan_owner_for = _fn("op2h_plan_owner_for", C.POINTER(op_owner_plan),
+                          _cstr, op_set, C.c_int, C.c_int, C.POINTER(op_arg))
+op2h_owner_plan_download = _fn("op2h_owner_plan_download", C.c_int,
+                               C.POINTER(op_owner_plan), C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
+op2h_owner_partition = _fn("op2h_owner_partition", C.c_int, C.c_int, C.c_int, C.c_int,
+                           C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.c_int,
+                           C.POINTER(C.c_int))
+
+
+def owner_partition(nto, table, inc_idx, block_cells):
+    """Host-only owner partition of targets [0, nto) of a map table; returns
+    (number of blocks, block of each target, -1 where unreferenced)."""
+    import numpy as np
+    t = np.ascontiguousarray(table, dtype=np.int32)
+    idx = (C.c_int * len(inc_idx))(*inc_idx)
+    out = np.empty(nto, np.int32)
+    nb = op2h_owner_partition(nto, t.shape[0], t.shape[1],
+                              t.ctypes.data_as(C.POINTER(C.c_int)), len(inc_idx), idx,
+                              block_cells, out.ctypes.data_as(C.POINTER(C.c_int)))
+    if nb < 0:
+        raise ValueError("bad owner partition arguments")
+    return nb, out
 
 
 op2h_plan_staged_for = _fn("op2h_plan_staged_for", C.POINTER(op_staged_plan),

@@ -46,6 +46,8 @@ lib.op2amd_mlp_forward.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int,
 lib.op2amd_mlp_loss_grad.restype = C.c_int
 lib.op2amd_mlp_loss_grad.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                      C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+lib.op2amd_mlp_set_weight_decay.restype = C.c_int
+lib.op2amd_mlp_set_weight_decay.argtypes = [C.c_void_p, C.c_float]
 lib.op2amd_mlp_train.restype = C.c_int
 lib.op2amd_mlp_train.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                  C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]
@@ -129,6 +131,10 @@ class MLP:
         lib.op2amd_mlp_loss_grad(self.h, _fp(x), _fp(t), x.shape[0], C.byref(loss), _fp(g))
         return loss.value, g
 
+    def set_weight_decay(self, wd):
+        if lib.op2amd_mlp_set_weight_decay(self.h, wd):
+            raise ValueError("weight decay must be >= 0")
+
     def train(self, x, t, epochs, lr):
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(t, np.float32)
@@ -158,3 +164,146 @@ def features(rec, plan, block):
     f += [float(plan == p) for p in PLAN_IDS]
     f += [math.log2(block) / 10.0]
     return f
+
+
+# ---------------------------------------------------------------------------
+# Plan selection with the cost model
+# ---------------------------------------------------------------------------
+
+def sweep(descs, reps=5, log=None):
+    """Exhaustive timing of every variant of every loop description
+    (dicts with keys n, arity, dim, access, is_double, ordering, k, seed)."""
+    recs = []
+    for i, p in enumerate(descs):
+        d = desc_from(p["n"], p["arity"], p["dim"], p["access"], p["is_double"],
+                      p["ordering"], p["k"], p["seed"])
+        var = variants(p["access"])
+        ms, shape = time_variants(d, var, reps=reps)
+        rec = dict(p)
+        rec.update(variants=[list(v) for v in var], ms=ms, **shape)
+        recs.append(rec)
+        if log:
+            best = min(t for t in ms if t > 0)
+            log(f"  loop {i}: n={shape['set_size']} a={p['arity']} d={p['dim']} {p['access']} "
+                f"{'f64' if p['is_double'] else 'f32'} {p['ordering']} k={p['k']} "
+                f"best {best * 1e3:.1f}us")
+    return recs
+
+
+def random_descs(count, seed, n_lo=1e4, n_hi=4e6):
+    """Seeded loop descriptions: set sizes log-uniform in [n_lo, n_hi]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        out.append(dict(n=float(10 ** rng.uniform(math.log10(n_lo), math.log10(n_hi))),
+                        arity=int(rng.integers(1, 5)), dim=int(rng.choice([1, 2, 4, 8])),
+                        access=str(rng.choice(["read", "inc", "rw"])),
+                        is_double=int(rng.integers(0, 2)),
+                        ordering=str(rng.choice(list(ORDER_NAMES))),
+                        k=int(rng.integers(4, 65)), seed=100 + i + 1000 * seed))
+    return out
+
+
+def valid_variants(rec):
+    return [(tuple(v), t) for v, t in zip(rec["variants"], rec["ms"]) if t > 0]
+
+
+def regret(recs, ids, choose):
+    """time(chosen) / time(fastest) - 1 per loop."""
+    out = []
+    for i in ids:
+        valid = valid_variants(recs[i])
+        best = min(t for _, t in valid)
+        out.append(dict(valid)[choose(recs[i])] / best - 1.0)
+    return np.array(out)
+
+
+def default_choice(rec):
+    """The runtime's static choice without a model: the reference's
+    OP_STAGE_ALL plan (staged, block 256) for increments and read-write
+    loops, block 256 for conflict-free loops."""
+    return (DIRECT, 256) if rec["access"] == "read" else (PLAN_STAGED, 256)
+
+
+def best_fixed_choice(recs, ids):
+    """The single (plan, block) per access mode with the lowest mean regret
+    over the loops `ids`: the best a model-free static rule can do."""
+    table = {}
+    for acc in ("read", "inc", "rw"):
+        mine = [i for i in ids if recs[i]["access"] == acc]
+        if not mine:
+            continue
+        cands = [tuple(v) for v in recs[mine[0]]["variants"]]
+        score = {c: np.mean(regret(recs, mine, lambda r, c=c: c)) for c in cands
+                 if all(dict(valid_variants(recs[i])).get(c) for i in mine)}
+        table[acc] = min(score, key=score.get)
+    return lambda rec: table.get(rec["access"], default_choice(rec))
+
+
+class CostModel:
+    """An ensemble of the library's HIP MLPs predicting each variant's log
+    time relative to the mean over the loop's variants (only the ranking
+    within a loop matters for the choice)."""
+
+    def __init__(self, hidden=64, members=5, seed=0, weight_decay=1e-4, lr=3e-3):
+        self.hidden, self.members, self.seed = hidden, members, seed
+        self.wd, self.lr = weight_decay, lr
+        self.nets = []
+
+    def _rows(self, recs, ids):
+        X, y = [], []
+        for i in ids:
+            r = recs[i]
+            valid = valid_variants(r)
+            lt = np.log([t for _, t in valid])
+            for (v, _), t in zip(valid, lt - lt.mean()):
+                X.append(features(r, v[0], v[1]))
+                y.append(t)
+        return np.array(X, np.float32), np.array(y, np.float32)
+
+    def _start(self, X):
+        self.mu, self.sd = X.mean(axis=0), X.std(axis=0) + 1e-6
+        self.nets = []
+        for m in range(self.members):
+            net = MLP(X.shape[1], self.hidden, seed=self.seed * 1000 + m)
+            net.set_weight_decay(self.wd)
+            self.nets.append(net)
+
+    def _train(self, Xn, y, epochs):
+        return [net.train(Xn, y, epochs, self.lr) for net in self.nets]
+
+    def fit(self, recs, ids, epochs):
+        X, y = self._rows(recs, ids)
+        self._start(X)
+        self.loss = self._train((X - self.mu) / self.sd, y, epochs)
+        self.epochs = epochs
+        return self
+
+    def fit_validated(self, recs, ids, max_epochs=4000, chunk=250, holdout=0.2):
+        """Choose the epoch count on a held-out part of `ids`, then fit on all
+        of `ids` with it."""
+        ids = list(ids)
+        rng = np.random.default_rng(self.seed + 7)
+        perm = rng.permutation(len(ids))
+        nval = max(1, int(len(ids) * holdout))
+        val = [ids[j] for j in perm[:nval]]
+        tr = [ids[j] for j in perm[nval:]]
+        X, y = self._rows(recs, tr)
+        self._start(X)
+        Xn = (X - self.mu) / self.sd
+        curve = []
+        for e in range(chunk, max_epochs + 1, chunk):
+            self._train(Xn, y, chunk)
+            curve.append((e, float(np.mean(regret(recs, val, self.choose)))))
+        best_e = min(curve, key=lambda c: (c[1], c[0]))[0]
+        self.validation_curve = curve
+        return self.fit(recs, ids, best_e)
+
+    def predict(self, X):
+        Xn = ((np.asarray(X, np.float32) - self.mu) / self.sd).astype(np.float32)
+        return np.mean([net.forward(Xn) for net in self.nets], axis=0)
+
+    def choose(self, rec):
+        valid = valid_variants(rec)
+        pred = self.predict([features(rec, v[0], v[1]) for v, _ in valid])
+        return valid[int(np.argmin(pred))][0]

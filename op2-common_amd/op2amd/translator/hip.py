@@ -1,0 +1,444 @@
+"""HIP (gfx950) code generation for translated OP2 loops: one kernel file per
+program holding, per loop, a device copy of the user kernel, a HIP kernel and
+the op_par_loop stub that drives it through the runtime ABI
+(include/op2_hip_rt.h); plus the program's constants and their hook.
+
+What a stub does, per loop (the reference's HIP target does the same job with
+its own runtime, translator-v2/resources/templates/hip/loop_host.hpp.jinja):
+- plans: loops with indirect OP_INC args run with atomics (default, as the
+  reference's `atomics` option) or a flat colouring, one launch per colour;
+  loops with indirect OP_RW/OP_WRITE args run coloured;
+- dats in SoA layout are copied into registers around the user kernel call
+  (the user kernel always sees AoS pointers, no index rewriting);
+- gbl OP_READ values travel by value in the kernel arguments; OP_INC/OP_MIN/
+  OP_MAX reduce per workgroup into partials and a one-workgroup finish kernel
+  writes the loop's result to the runtime's result slot; a linked
+  op_arg_info reports the payload of the first element (lowest index)
+  reaching the extremum, as the sequential executor does.
+"""
+import math
+
+from .op import ArgDat, ArgGbl, ArgIdx, ArgInfo
+
+PLAN_ATOMIC, PLAN_COLOR = 1, 2
+
+
+def _ctype(t):
+    return t
+
+
+def _size(t):
+    return {"double": 8, "float": 4, "int": 4, "unsigned int": 4, "long long": 8,
+            "bool": 1, "char": 1}[t]
+
+
+def _pad8(n):
+    return (n + 7) // 8 * 8
+
+
+def _device_kernel(text):
+    return "__device__ " + text.lstrip()
+
+
+class LoopGen:
+    def __init__(self, loop, ns):
+        self.L, self.ns = loop, ns
+        self.info_of = {}  # gbl arg index -> info arg index
+        for j, a in enumerate(loop.args):
+            if isinstance(a, ArgInfo):
+                if a.ref in self.info_of:
+                    raise ValueError(f"{loop.kernel}: two op_arg_info args for arg {a.ref}")
+                self.info_of[a.ref] = j
+        self.reds = [j for j, a in enumerate(loop.args)
+                     if isinstance(a, ArgGbl) and a.acc in ("OP_INC", "OP_MIN", "OP_MAX")]
+        for j in self.reds:
+            if j in self.info_of and loop.args[j].dim != 1:
+                raise ValueError(f"{loop.kernel}: op_arg_info needs a gbl of dim 1")
+        for j, a in enumerate(loop.args):
+            if isinstance(a, ArgGbl) and a.acc in ("OP_WRITE", "OP_RW"):
+                raise ValueError(f"{loop.kernel}: gbl OP_WRITE/OP_RW is not supported")
+        # partial record layout
+        self.rec_off, off = {}, 0
+        for j in self.reds:
+            a = loop.args[j]
+            self.rec_off[j] = off
+            off += _pad8(a.dim * _size(a.typ))
+            if j in self.info_of:
+                ia = loop.args[self.info_of[j]]
+                off += _pad8(ia.dim * _size(ia.typ)) + 8
+        self.rec = max(off, 8)
+        if loop.indirect_writes():
+            self.def_plan, self.mask = PLAN_COLOR, 1 << PLAN_COLOR
+        elif loop.indirect_incs():
+            self.def_plan, self.mask = PLAN_ATOMIC, (1 << PLAN_ATOMIC) | (1 << PLAN_COLOR)
+        else:
+            self.def_plan, self.mask = PLAN_ATOMIC, 1 << PLAN_ATOMIC
+
+    @property
+    def name(self):
+        return f"{self.ns}_{self.L.index}"
+
+    # ---- kernel argument struct ----
+    def args_struct(self):
+        f = []
+        for j, a in enumerate(self.L.args):
+            if isinstance(a, ArgDat):
+                f.append(f"  {a.typ} *d{j}; int s{j}; int opt{j};")
+                if not a.direct:
+                    f.append(f"  const int *m{j}; int md{j};")
+            elif isinstance(a, ArgGbl) and a.acc == "OP_READ":
+                f.append(f"  {a.typ} g{j}[{a.dim}]; int opt{j};")
+            elif isinstance(a, ArgIdx) and a.map:
+                f.append(f"  const int *m{j}; int md{j};")
+        f.append("  int _unused;")
+        return f"struct {self.name}_args {{\n" + "\n".join(f) + "\n};\n"
+
+    # ---- device kernel ----
+    def kernel(self):
+        L = self.L
+        pre, call, post = [], [], []
+        for j, a in enumerate(L.args):
+            if isinstance(a, ArgDat):
+                T, D = a.typ, a.dim
+                tgt = "n" if a.direct else f"A.m{j}[(size_t)n * A.md{j} + {a.idx}]"
+                pre.append(f"    {T} *p{j} = nullptr;")
+                pre.append(f"    {T} b{j}[{D}];")
+                pre.append(f"    const int t{j} = A.opt{j} ? {tgt} : 0;")
+                inc_ind = (not a.direct) and a.acc == "OP_INC"
+                if inc_ind:
+                    pre.append(f"    for (int k = 0; k < {D}; ++k) b{j}[k] = ({T})0;")
+                    pre.append(f"    if (A.opt{j}) p{j} = b{j};")
+                    post.append(f"    if (A.opt{j}) {{")
+                    post.append(f"      if (atomic) {{")
+                    post.append(f"        for (int k = 0; k < {D}; ++k) op2gen_atomic_add(A.d{j} + "
+                                f"(A.s{j} ? t{j} + (size_t)k * A.s{j} : (size_t)t{j} * {D} + k), b{j}[k]);")
+                    post.append(f"      }} else {{")
+                    post.append(f"        for (int k = 0; k < {D}; ++k) A.d{j}[A.s{j} ? t{j} + (size_t)k * A.s{j} "
+                                f": (size_t)t{j} * {D} + k] += b{j}[k];")
+                    post.append(f"      }}")
+                    post.append(f"    }}")
+                else:
+                    pre.append(f"    if (A.opt{j}) {{")
+                    pre.append(f"      if (A.s{j}) {{")
+                    pre.append(f"        for (int k = 0; k < {D}; ++k) b{j}[k] = A.d{j}[t{j} + (size_t)k * A.s{j}];")
+                    pre.append(f"        p{j} = b{j};")
+                    pre.append(f"      }} else {{")
+                    pre.append(f"        p{j} = A.d{j} + (size_t)t{j} * {D};")
+                    pre.append(f"      }}")
+                    pre.append(f"    }}")
+                    if a.acc != "OP_READ":
+                        post.append(f"    if (A.opt{j} && A.s{j})")
+                        post.append(f"      for (int k = 0; k < {D}; ++k) A.d{j}[t{j} + (size_t)k * A.s{j}] = b{j}[k];")
+                call.append(f"p{j}")
+            elif isinstance(a, ArgGbl):
+                if a.acc == "OP_READ":
+                    pre.append(f"    {a.typ} g{j}[{a.dim}];")
+                    pre.append(f"    for (int k = 0; k < {a.dim}; ++k) g{j}[k] = A.g{j}[k];")
+                    call.append(f"A.opt{j} ? g{j} : nullptr")
+                else:
+                    call.append(f"r{j}")
+            elif isinstance(a, ArgIdx):
+                if a.map:
+                    pre.append(f"    int x{j} = A.m{j}[(size_t)n * A.md{j} + {a.idx}];")
+                else:
+                    pre.append(f"    int x{j} = n;")
+                call.append(f"&x{j}")
+            elif isinstance(a, ArgInfo):
+                call.append(f"v{j}")
+        red_decl = []
+        for j in self.reds:
+            a = L.args[j]
+            ident = {"OP_INC": f"({a.typ})0", "OP_MIN": f"op2gen_lim<{a.typ}>::max()",
+                     "OP_MAX": f"op2gen_lim<{a.typ}>::lowest()"}[a.acc]
+            red_decl.append(f"  {a.typ} r{j}[{a.dim}];")
+            red_decl.append(f"  for (int k = 0; k < {a.dim}; ++k) r{j}[k] = {ident};")
+            if j in self.info_of:
+                ia = L.args[self.info_of[j]]
+                red_decl.append(f"  {ia.typ} v{self.info_of[j]}[{ia.dim}] = {{}};")
+                red_decl.append(f"  int at{j} = 0x7fffffff;")
+        track = []
+        for j in self.reds:
+            if j in self.info_of:
+                track.append(f"    const {L.args[j].typ} before{j} = r{j}[0];")
+        track_after = [f"    if (r{j}[0] != before{j}) at{j} = n;" for j in self.reds if j in self.info_of]
+        body = []
+        body.append(f"__global__ void __launch_bounds__(1024)")
+        body.append(f"    {self.name}_kernel({self.name}_args A, const int *__restrict elems, int start,")
+        body.append(f"                 int end, int atomic, unsigned char *partials, int part_base) {{")
+        body.append(f"  const int tid = start + (int)(blockIdx.x * blockDim.x + threadIdx.x);")
+        body.append(f"  const bool live = tid < end;")
+        body.append(f"  const int n = live ? (elems ? elems[tid] : tid) : 0;")
+        body += red_decl
+        body.append(f"  if (live) {{")
+        body += pre
+        body += track
+        body.append(f"    {self.ns}::{L.kernel}({', '.join(call)});")
+        body += track_after
+        body += post
+        body.append(f"  }}")
+        if self.reds:
+            body.append(f"  unsigned char *rec = partials + (size_t)(part_base + blockIdx.x) * {self.rec};")
+            for j in self.reds:
+                a = L.args[j]
+                op = {"OP_INC": "0", "OP_MIN": "1", "OP_MAX": "2"}[a.acc]
+                o = self.rec_off[j]
+                if j in self.info_of:
+                    ij = self.info_of[j]
+                    ia = L.args[ij]
+                    io = o + _pad8(_size(a.typ))
+                    ao = io + _pad8(ia.dim * _size(ia.typ))
+                    body.append(f"  {{")
+                    body.append(f"    {a.typ} bv = r{j}[0]; int ba = at{j};")
+                    body.append(f"    const int win = op2gen_block_argbest<{a.typ}, {op}>(bv, ba);")
+                    body.append(f"    if (threadIdx.x == (unsigned)win) {{")
+                    body.append(f"      *({a.typ} *)(rec + {o}) = r{j}[0];")
+                    body.append(f"      for (int k = 0; k < {ia.dim}; ++k) (({ia.typ} *)(rec + {io}))[k] = v{ij}[k];")
+                    body.append(f"      *(int *)(rec + {ao}) = at{j};")
+                    body.append(f"    }}")
+                    body.append(f"  }}")
+                else:
+                    body.append(f"  for (int k = 0; k < {a.dim}; ++k) {{")
+                    body.append(f"    const {a.typ} v = op2gen_block_reduce<{a.typ}, {op}>(r{j}[k]);")
+                    body.append(f"    if (threadIdx.x == 0) (({a.typ} *)(rec + {o}))[k] = v;")
+                    body.append(f"  }}")
+        body.append("}")
+        return "\n".join(body) + "\n"
+
+    def finish(self):
+        if not self.reds:
+            return ""
+        L = self.L
+        params = ", ".join(f"unsigned char *slot{j}" for j in self.reds)
+        b = [f"__global__ void __launch_bounds__(256)",
+             f"    {self.name}_finish(const unsigned char *__restrict partials, int nparts, {params}) {{"]
+        for j in self.reds:
+            a = L.args[j]
+            op = {"OP_INC": "0", "OP_MIN": "1", "OP_MAX": "2"}[a.acc]
+            o = self.rec_off[j]
+            ident = {"OP_INC": f"({a.typ})0", "OP_MIN": f"op2gen_lim<{a.typ}>::max()",
+                     "OP_MAX": f"op2gen_lim<{a.typ}>::lowest()"}[a.acc]
+            if j in self.info_of:
+                ia = L.args[self.info_of[j]]
+                io = o + _pad8(_size(a.typ))
+                ao = io + _pad8(ia.dim * _size(ia.typ))
+                b.append(f"  {{")
+                b.append(f"    {a.typ} bv = {ident}; int ba = 0x7fffffff, bp = -1;")
+                b.append(f"    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {{")
+                b.append(f"      const unsigned char *rec = partials + (size_t)p * {self.rec};")
+                b.append(f"      const {a.typ} v = *(const {a.typ} *)(rec + {o});")
+                b.append(f"      const int at = *(const int *)(rec + {ao});")
+                b.append(f"      if (op2gen_better<{a.typ}, {op}>(v, at, bv, ba)) {{ bv = v; ba = at; bp = p; }}")
+                b.append(f"    }}")
+                b.append(f"    const int win = op2gen_block_argbest<{a.typ}, {op}>(bv, ba);")
+                b.append(f"    if (threadIdx.x == (unsigned)win) {{")
+                b.append(f"      *({a.typ} *)slot{j} = bv;")
+                b.append(f"      if (bp >= 0) for (int k = 0; k < {ia.dim}; ++k)")
+                b.append(f"        (({ia.typ} *)(slot{j} + {_size(a.typ)}))[k] = "
+                         f"((const {ia.typ} *)(partials + (size_t)bp * {self.rec} + {io}))[k];")
+                b.append(f"    }}")
+                b.append(f"  }}")
+            else:
+                b.append(f"  for (int k = 0; k < {a.dim}; ++k) {{")
+                b.append(f"    {a.typ} acc = {ident};")
+                b.append(f"    for (int p = threadIdx.x; p < nparts; p += blockDim.x)")
+                b.append(f"      acc = op2gen_combine<{a.typ}, {op}>(acc, ((const {a.typ} *)(partials + "
+                         f"(size_t)p * {self.rec} + {o}))[k]);")
+                b.append(f"    const {a.typ} v = op2gen_block_reduce<{a.typ}, {op}>(acc);")
+                b.append(f"    if (threadIdx.x == 0) (({a.typ} *)slot{j})[k] = v;")
+                b.append(f"  }}")
+        b.append("}")
+        return "\n".join(b) + "\n"
+
+    def stub(self):
+        L = self.L
+        N = len(L.args)
+        sig = ", ".join(["char const *name", "op_set set"] + [f"op_arg arg{j}" for j in range(N)])
+        s = [f'extern "C" void {L.stub}({sig}) {{']
+        s.append(f"  op_arg args[{max(N, 1)}] = {{{', '.join(f'arg{j}' for j in range(N)) or '{}'}}};")
+        s.append(f"  op2h_loop_ctx ctx;")
+        s.append(f"  op2h_loop_begin_mask(&ctx, {L.index - 1}, name, set, {N}, args, {self.def_plan}, 256, "
+                 f"{self.mask}u);")
+        s.append(f"  if (set->size > 0) {{")
+        s.append(f"    hipStream_t st = (hipStream_t)ctx.stream;")
+        s.append(f"    {self.name}_args A;")
+        s.append(f"    memset(&A, 0, sizeof(A));")
+        for j, a in enumerate(L.args):
+            if isinstance(a, ArgDat):
+                s.append(f"    A.opt{j} = args[{j}].opt;")
+                s.append(f"    A.d{j} = ({a.typ} *)args[{j}].data_d;")
+                s.append(f"    A.s{j} = args[{j}].opt ? op2h_dat_soa_stride(args[{j}].dat) : 0;")
+                if not a.direct:
+                    s.append(f"    A.m{j} = args[{j}].opt ? args[{j}].map_data_d : nullptr;")
+                    s.append(f"    A.md{j} = args[{j}].opt ? args[{j}].map->dim : 0;")
+            elif isinstance(a, ArgGbl) and a.acc == "OP_READ":
+                s.append(f"    A.opt{j} = args[{j}].opt;")
+                s.append(f"    if (args[{j}].opt)")
+                s.append(f"      for (int k = 0; k < {a.dim}; ++k) A.g{j}[k] = (({a.typ} *)args[{j}].data)[k];")
+            elif isinstance(a, ArgIdx) and a.map:
+                s.append(f"    A.m{j} = args[{j}].map_data_d;")
+                s.append(f"    A.md{j} = args[{j}].map->dim;")
+        s.append(f"    const int bs = ctx.block_size;")
+        s.append(f"    const op2h_color_plan *cp = ctx.plan == {PLAN_COLOR} ? op2h_plan_color(&ctx) : nullptr;")
+        s.append(f"    int nparts = 0;")
+        s.append(f"    if (cp) {{")
+        s.append(f"      for (int c = 0; c < cp->ncolors; ++c)")
+        s.append(f"        nparts += (cp->col_offsets_h[c + 1] - cp->col_offsets_h[c] + bs - 1) / bs;")
+        s.append(f"    }} else {{")
+        s.append(f"      nparts = (set->size + bs - 1) / bs;")
+        s.append(f"    }}")
+        if self.reds:
+            s.append(f"    unsigned char *partials = (unsigned char *)op2h_reduct_scratch((size_t)nparts * {self.rec});")
+        else:
+            s.append(f"    unsigned char *partials = nullptr;")
+        s.append(f"    op2h_kernel_timer_start(&ctx);")
+        s.append(f"    if (cp) {{")
+        s.append(f"      int base = 0;")
+        s.append(f"      for (int c = 0; c < cp->ncolors; ++c) {{")
+        s.append(f"        const int s0 = cp->col_offsets_h[c], s1 = cp->col_offsets_h[c + 1];")
+        s.append(f"        const int g = (s1 - s0 + bs - 1) / bs;")
+        s.append(f"        if (g) {self.name}_kernel<<<g, bs, 0, st>>>(A, cp->col_reord, s0, s1, 0, partials, base);")
+        s.append(f"        base += g;")
+        s.append(f"      }}")
+        s.append(f"    }} else {{")
+        s.append(f"      {self.name}_kernel<<<nparts, bs, 0, st>>>(A, nullptr, 0, set->size, 1, partials, 0);")
+        s.append(f"    }}")
+        if self.reds:
+            slots = ", ".join(f"(unsigned char *)op2h_gbl_slot(&ctx, {j}, {self.info_of.get(j, -1)})"
+                              for j in self.reds)
+            s.append(f"    {self.name}_finish<<<1, 256, 0, st>>>(partials, nparts, {slots});")
+        s.append(f"    op2h_kernel_timer_stop(&ctx);")
+        s.append(f"    OP2GEN_CHECK(hipGetLastError());")
+        s.append(f"  }}")
+        s.append(f"  OP_kernels[{L.index - 1}].transfer += (float)op2h_algorithmic_bytes(set, {N}, args);")
+        s.append(f"  op2h_loop_end(&ctx);")
+        s.append(f"}}")
+        return "\n".join(s) + "\n"
+
+
+PRELUDE = r'''#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+
+#include "op2_hip.h"
+#include "op2_hip_rt.h"
+
+#define OP2GEN_CHECK(x)                                                      \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess) {                                                  \
+      printf("op2hip: %s failed at %s:%d: %s\n", #x, __FILE__, __LINE__,     \
+             hipGetErrorString(e_));                                         \
+      exit(-1);                                                              \
+    }                                                                        \
+  } while (0)
+
+namespace {
+
+template <class T> struct op2gen_lim {
+  __device__ static T max() { return std::numeric_limits<T>::max(); }
+  __device__ static T lowest() { return std::numeric_limits<T>::lowest(); }
+};
+
+__device__ __forceinline__ void op2gen_atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void op2gen_atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void op2gen_atomic_add(int *p, int v) { atomicAdd(p, v); }
+
+// OP: 0 sum, 1 min, 2 max
+template <class T, int OP> __device__ __forceinline__ T op2gen_combine(T a, T b) {
+  return OP == 0 ? a + b : OP == 1 ? (b < a ? b : a) : (b > a ? b : a);
+}
+
+// (value, element) pairs: the better value wins, ties go to the lower element
+template <class T, int OP>
+__device__ __forceinline__ bool op2gen_better(T v, int at, T bv, int ba) {
+  return OP == 1 ? (v < bv || (v == bv && at < ba)) : (v > bv || (v == bv && at < ba));
+}
+
+// Workgroup reduction (fixed tree: deterministic for a launch shape); the
+// result is returned to thread 0.
+template <class T, int OP> __device__ T op2gen_block_reduce(T v) {
+  __shared__ T buf[1024];
+  const int t = threadIdx.x, n = blockDim.x;
+  buf[t] = v;
+  __syncthreads();
+  int p = 1;
+  while (p < n) p <<= 1;
+  for (int s = p >> 1; s > 0; s >>= 1) {
+    if (t < s && t + s < n) buf[t] = op2gen_combine<T, OP>(buf[t], buf[t + s]);
+    __syncthreads();
+  }
+  T r = buf[0];
+  __syncthreads();
+  return r;
+}
+
+// Workgroup arg-best over (value, element) pairs: returns the winning thread.
+template <class T, int OP> __device__ int op2gen_block_argbest(T v, int at) {
+  __shared__ T bv[1024];
+  __shared__ int ba[1024], bt[1024];
+  const int t = threadIdx.x, n = blockDim.x;
+  bv[t] = v;
+  ba[t] = at;
+  bt[t] = t;
+  __syncthreads();
+  int p = 1;
+  while (p < n) p <<= 1;
+  for (int s = p >> 1; s > 0; s >>= 1) {
+    if (t < s && t + s < n && op2gen_better<T, OP>(bv[t + s], ba[t + s], bv[t], ba[t])) {
+      bv[t] = bv[t + s];
+      ba[t] = ba[t + s];
+      bt[t] = bt[t + s];
+    }
+    __syncthreads();
+  }
+  int w = bt[0];
+  __syncthreads();
+  return w;
+}
+
+}  // namespace
+'''
+
+
+def generate(prog):
+    """(kernel file text, header text) for a parsed program."""
+    ns = f"op2gen_{prog.stem}"
+    out = [f"// Generated by op2amd.translator from {prog.stem}: {len(prog.loops)} loops.\n",
+           PRELUDE, f"namespace {ns} {{\n"]
+    for c in prog.consts:
+        out.append(f"__constant__ {c.typ} {c.ptr}" + (f"[{c.dim}]" if c.dim > 1 else "") + ";\n")
+    out.append("\n")
+    for name, text in prog.kernel_text.items():
+        out.append(_device_kernel(text) + "\n\n")
+    out.append(f"}}  // namespace {ns}\n\n")
+    # constants hook
+    out.append(f"static int {ns}_consts(int dim, const char *type, int size, const char *dat,\n"
+               f"                        const char *name) {{\n  (void)type;\n")
+    for c in prog.consts:
+        out.append(f'  if (!strcmp(name, "{c.ptr}")) {{\n'
+                   f"    OP2GEN_CHECK(hipMemcpyToSymbol(HIP_SYMBOL({ns}::{c.ptr}), dat, (size_t)dim * size));\n"
+                   f"    return 1;\n  }}\n")
+    out.append("  return 0;\n}\n\n")
+    out.append(f"static const struct {ns}_hook {{\n  {ns}_hook() {{ op2h_register_const_hook({ns}_consts); }}\n"
+               f"}} {ns}_hook_v;\n\n")
+    for loop in prog.loops:
+        g = LoopGen(loop, ns)
+        out.append(f"// ---- loop {loop.index}: {loop.kernel} ----\n")
+        out.append("namespace {\n")
+        out.append(g.args_struct())
+        out.append(g.kernel())
+        out.append(g.finish())
+        out.append("}  // namespace\n\n")
+        out.append(g.stub())
+        out.append("\n")
+    guard = f"{ns.upper()}_H"
+    hdr = [f"// Generated by op2amd.translator: stubs of {prog.stem}.\n",
+           f"#ifndef {guard}\n#define {guard}\n#include \"op2_hip.h\"\n",
+           '#ifdef __cplusplus\nextern "C" {\n#endif\n']
+    for loop in prog.loops:
+        hdr.append(f"void {loop.stub}(char const *, op_set" + ", op_arg" * len(loop.args) + ");\n")
+    hdr.append("#ifdef __cplusplus\n}\n#endif\n#endif\n")
+    return "".join(out), "".join(hdr)

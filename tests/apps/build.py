@@ -1,0 +1,51 @@
+"""Translate the OP2 programs under tests/apps with op2amd.translator and
+compile them for gfx950 against the in-tree libop2hip.so.  Outputs go to
+tests/apps/_build/<program>/ (the executable has the program's name).
+
+    python tests/apps/build.py [program ...]
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "op2-common_amd"))
+
+PROGRAMS = {
+    "gbl": "functional/gbl.cpp",
+    "reductions": "functional/reductions.cpp",
+    "idx": "functional/idx.cpp",
+    "strides": "functional/strides.cpp",
+    "consts": "functional/consts.cpp",
+    "airfoil_tr": "airfoil/airfoil_tr.cpp",
+}
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["-O2", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def build(name):
+    from op2amd.translator import translate
+    src = os.path.join(HERE, PROGRAMS[name])
+    out = os.path.join(HERE, "_build", name)
+    paths = translate(src, out)
+    inc = ["-I" + os.path.join(REPO, "include"), "-I" + os.path.dirname(src), "-I" + out]
+    lib = os.path.join(REPO, "op2-common_amd", "lib")
+    objs = []
+    for key, lang in (("kernels", "hip"), ("app", "c++")):
+        obj = paths[key] + ".o"
+        cmd = [HIPCC, *FLAGS, *inc, "-x", lang, "-c", paths[key], "-o", obj, "-w"]
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    exe = os.path.join(out, name)
+    subprocess.run([HIPCC, "--offload-arch=gfx950", *objs, "-L" + lib, "-lop2hip",
+                    "-Wl,-rpath," + lib, "-Wl,-rpath,$ORIGIN/../../../../op2-common_amd/lib",
+                    "-o", exe], check=True)
+    return exe
+
+
+if __name__ == "__main__":
+    for n in sys.argv[1:] or PROGRAMS:
+        print(build(n), flush=True)

@@ -8,7 +8,8 @@ import pytest
 
 from conftest import REPO
 
-HEADERS = ["op2_hip.h", "op2_hip_rt.h", "op2amd_mesh.h", "op2amd_airfoil.h"]
+HEADERS = ["op2_hip.h", "op2_hip_rt.h", "op2amd_mesh.h", "op2amd_airfoil.h",
+           "op2amd_ensemble.h", "op2amd_synth.h", "op2amd_costmodel.h"]
 
 
 def declared_functions(header):
@@ -19,6 +20,7 @@ def declared_functions(header):
     c_part = text.split('extern "C" {', 1)[1] if 'extern "C" {' in text else text
     c_part = c_part.split("#ifdef __cplusplus\n}", 1)[0]
     c_part = c_part.split("} /* extern", 1)[0]
+    c_part = re.sub(r"typedef[^;]*;", "", c_part)  # function-pointer typedefs
     names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", c_part)
     return sorted(set(n for n in names if n not in ("if", "return", "sizeof")))
 

@@ -27,6 +27,8 @@ CASES = [
     dict(mode="tuned", plan_res="staged", plan_bres="staged", plan_on_device=True),
     dict(mode="tuned", plan_res="color", plan_bres="color", plan_on_device=True),
     dict(mode="graphed", plan_res="staged", plan_bres="atomic", renumber=True),
+    dict(mode="tuned", plan_res="owner", plan_bres="owner", block_res=128),
+    dict(mode="graphed", plan_res="owner", plan_bres="atomic", soa=True),
 ]
 
 
@@ -49,6 +51,17 @@ def test_airfoil_rcm_graphed():
     assert rel_err(r["q"], g["q"]) <= TOL
 
 
+@pytest.mark.parametrize("block", [64, 256, 1024])
+def test_owner_plan_reproduces_sequential_order(block):
+    """The owner partition plan evaluates each edge once per block that owns
+    one of its cells and sums increments in edge order: bit for bit."""
+    g = golden("airfoil_24x12_random_a1")
+    r = op2amd.airfoil.run(mesh_of(g), int(g["niter"]), mode="tuned",
+                           plan_res="owner", block_res=block, plan_bres="gather")
+    assert np.array_equal(r["q"], g["q"])
+    assert np.array_equal(r["maxerr"], g["maxerr"])
+
+
 def test_gather_plan_reproduces_sequential_order():
     """The owner-compute plan sums increments in the sequential executor's
     order: q, maxerr and errloc come out bit for bit; rms differs only by the
@@ -62,7 +75,7 @@ def test_gather_plan_reproduces_sequential_order():
     assert pointwise_rel_err(r["rms"], g["rms"]) <= 1e-14
 
 
-@pytest.mark.parametrize("plans", [("gather", "gather"), ("atomic", "atomic")])
+@pytest.mark.parametrize("plans", [("gather", "gather"), ("atomic", "atomic"), ("owner", "gather")])
 def test_mixed_precision_error_bounds(plans):
     """FP32 fluxes/adt/res with FP64 q and update: error against the FP64
     reference trajectory after 1000 iterations.  Measured on MI355X:

@@ -98,7 +98,7 @@ def inputs(mesh, seed):
 
 
 @pytest.mark.parametrize("soa", [False, True])
-@pytest.mark.parametrize("plan", ["atomic", "color", "staged", "gather"])
+@pytest.mark.parametrize("plan", ["atomic", "color", "staged", "gather", "owner"])
 @pytest.mark.parametrize("ordering", ["random", "rcm"])
 def test_indirect_loops_match_oracle(plan, soa, ordering):
     mesh = op2amd.mesh.generate(60, 30, 11, ordering)
@@ -119,7 +119,7 @@ def test_indirect_loops_match_oracle(plan, soa, ordering):
     assert np.array_equal(adt, adt_o)
     res1_o = oracle.res_calc(mesh, q, adt_o, res0)
     res2_o = oracle.bres_calc(mesh, q, adt_o, res1_o)
-    if plan == "gather":
+    if plan in ("gather", "owner"):
         assert np.array_equal(res1, res1_o) and np.array_equal(res2, res2_o)
     else:
         assert rel_err(res1, res1_o) < 1e-14

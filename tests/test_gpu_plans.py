@@ -96,6 +96,33 @@ def test_staged_and_flat_colourings_conflict_free(I, J, builder):
     assert ncol_flat <= 8
 
 
+@pytest.mark.parametrize("builder", [0, 1])
+def test_flat_colouring_of_four_increments_per_element(builder):
+    """Cells incrementing their four nodes (a map of dim 4, four OP_INC args):
+    no two cells of one colour share a node."""
+    mesh = op2amd.mesh.generate(90, 45, 2, "rcm")
+    q, res = inputs(mesh, 0)
+    app = App(mesh, q, res)
+    try:
+        api.op2h_set_plan_builder(builder)
+        w = api.decl_dat(app.nodes, 2, "double", np.zeros((mesh.nnode, 2)), "w")
+        args = (api.op_arg * 4)(*[app.d(w, k, app.pcell, 2, "double", api.OP_INC)
+                                  for k in range(4)])
+        c = api.op2h_plan_color_for(b"cell_to_node", app.cells, 4, args)
+        col = np.zeros(mesh.ncell, np.int32)
+        import ctypes as C
+        api.op2h_color_plan_download(c, col.ctypes.data_as(C.c_void_p))
+        ncol = c.contents.ncolors
+    finally:
+        app.close()
+    key = np.concatenate([col.astype(np.int64) * (1 << 32) + mesh.cell[:, k] for k in range(4)])
+    cell = np.concatenate([np.arange(mesh.ncell)] * 4)
+    order = np.lexsort((cell, key))
+    k, e = key[order], cell[order]
+    assert int(((k[1:] == k[:-1]) & (e[1:] != e[:-1])).sum()) == 0
+    assert 4 <= ncol <= 16
+
+
 @pytest.mark.parametrize("part", [64, 256])
 def test_device_thread_colours_equal_reference_greedy(part):
     """The device builder colours threads with the reference's sequential
@@ -117,3 +144,67 @@ def test_device_thread_colours_equal_reference_greedy(part):
     block_of_edge = np.repeat(np.arange(len(nel)), nel)
     assert block_conflicts(mesh.ecell, blk[block_of_edge], block_of_edge) == 0
     assert ncolors <= 2 * ref["ncolors"] + 2
+
+
+@pytest.mark.parametrize("ordering,block", [("rcm", 256), ("random", 64), ("hilbert", 512)])
+def test_owner_plan_invariants(ordering, block):
+    """Every (edge, side) increment appears once, in its cell's own block, in
+    ascending edge order; blocks own disjoint cells; boundary edges are the
+    only ones evaluated twice."""
+    mesh = op2amd.mesh.generate(80, 40, 3, ordering)
+    q, res = inputs(mesh, 0)
+    app = App(mesh, q, res)
+    try:
+        p = api.op2h_plan_owner_for(b"res_calc", app.edges, block, 8, res_args(app))
+        P = p.contents
+        nb, tot = P.nblocks, P.total
+        owned_offs = np.zeros(nb + 1, np.int32)
+        owned = np.zeros(mesh.ncell, np.int32)
+        elem_offs = np.zeros(nb + 1, np.int32)
+        elems = np.zeros(tot, np.int32)
+        csr_offs = np.zeros(mesh.ncell + 1, np.int32)
+        csr = np.zeros(2 * mesh.nedge, np.uint16)
+        import ctypes as C
+        api.op2h_owner_plan_download(p, *[a.ctypes.data_as(C.c_void_p) for a in
+                                          (owned_offs, owned, elem_offs, elems, csr_offs, csr)])
+    finally:
+        app.close()
+    ec = mesh.ecell
+    assert owned_offs[-1] == mesh.ncell and np.array_equal(np.sort(owned), np.arange(mesh.ncell))
+    block_of = np.empty(mesh.ncell, np.int64)
+    for b in range(nb):
+        cells = owned[owned_offs[b]:owned_offs[b + 1]]
+        assert np.all(np.diff(cells) > 0) and 0 < len(cells) <= block
+        block_of[cells] = b
+    seen = np.zeros((mesh.nedge, 2), np.int32)
+    for b in range(nb):
+        el = elems[elem_offs[b]:elem_offs[b + 1]]
+        assert np.all(np.diff(el) > 0)
+        # an edge is an instance of b iff one of its cells is owned by b
+        touch = (block_of[ec[:, 0]] == b) | (block_of[ec[:, 1]] == b)
+        assert np.array_equal(el, np.nonzero(touch)[0])
+        for i in range(owned_offs[b], owned_offs[b + 1]):
+            ent = csr[csr_offs[i]:csr_offs[i + 1]].astype(np.int64)
+            j, side = ent >> 1, ent & 1
+            e = el[j]
+            assert np.all(np.diff(e * 2 + side) > 0)
+            assert np.all(ec[e, side] == owned[i])
+            seen[e, side] += 1
+    assert np.all(seen == 1)
+    assert abs(P.redundancy - tot / mesh.nedge) < 1e-12
+    assert P.redundancy < 1.35
+
+
+def test_owner_plan_refuses_unsupported_loops():
+    """Loops whose increments use two maps cannot be partitioned by owner."""
+    mesh = op2amd.mesh.generate(20, 10, 0, "structured")
+    q, res = inputs(mesh, 0)
+    app = App(mesh, q, res)
+    try:
+        R, I = api.OP_READ, api.OP_INC
+        d = app.d
+        args = (api.op_arg * 2)(d(app.res, 0, app.pecell, 4, "double", I),
+                                d(app.x, 0, app.pedge, 2, "double", I))
+        assert not api.op2h_plan_owner_for(b"two_maps", app.edges, 64, 2, args)
+    finally:
+        app.close()

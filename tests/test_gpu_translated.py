@@ -1,0 +1,56 @@
+"""Programs translated by op2amd.translator and built by tests/apps/build.py
+(run by __graft_entry__.build()), on the GPU:
+- the functional programs (global reductions, dat increments, op_arg_idx,
+  SoA strides, constants) with the default plans, with every dat in SoA
+  layout (OP_AUTO_SOA) and with coloured increments instead of atomics;
+- the translated airfoil program against the reference-pinned trajectory."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import op2amd
+from conftest import REPO, golden, pointwise_rel_err
+
+pytestmark = pytest.mark.gpu
+BUILD = REPO / "tests" / "apps" / "_build"
+
+
+def run(name, *args, timeout=120):
+    exe = BUILD / name / name
+    assert exe.exists(), f"{exe} missing: run __graft_entry__.build()"
+    print("running", name, *args, flush=True)
+    r = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=timeout)
+    return r
+
+
+@pytest.mark.parametrize("variant", [[], ["OP_AUTO_SOA"], ["OP_HIP_PLAN=*:color"]],
+                         ids=["default", "soa", "color"])
+@pytest.mark.parametrize("name", ["gbl", "reductions", "idx", "strides", "consts"])
+def test_functional_program(name, variant):
+    r = run(name, *variant)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert f"{name}: all passed" in r.stdout
+
+
+@pytest.mark.parametrize("variant", [[], ["OP_HIP_PLAN=*:color"], ["OP_AUTO_SOA"]],
+                         ids=["atomic", "color", "soa"])
+def test_translated_airfoil_matches_reference_trajectory(variant, tmp_path):
+    g = golden("airfoil_120x60_random_a1")
+    mesh = op2amd.mesh.generate(int(g["I"]), int(g["J"]), 0, str(g["ordering"]))
+    grid = tmp_path / "new_grid.dat"
+    mesh.write(str(grid))
+    r = run("airfoil_tr", str(grid), str(int(g["niter"])), *variant)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = [l.split() for l in r.stdout.splitlines() if l and l[0].isdigit()]
+    rms = np.array([float(x[1]) for x in rows])
+    maxerr = np.array([float(x[2]) for x in rows])
+    errloc = np.array([int(x[3]) for x in rows])
+    assert len(rms) == int(g["niter"])
+    assert pointwise_rel_err(rms, g["rms"]) <= 1e-10
+    assert pointwise_rel_err(maxerr, g["maxerr"]) <= 1e-10
+    assert np.mean(errloc == g["errloc"]) >= 0.99
+    qline = [l.split() for l in r.stdout.splitlines() if l.startswith("q ")][0]
+    q = np.asarray(g["q"], dtype=np.float64)
+    assert abs(float(qline[1]) - q.sum()) <= 1e-10 * abs(q.sum())
+    assert abs(float(qline[2]) - (q * q).sum()) <= 1e-10 * (q * q).sum()

@@ -1,0 +1,77 @@
+"""op2amd.translator on the OP2 programs under tests/apps: the program model
+it reads (loops, argument kinds, constants), the rewritten application and the
+generated HIP code, which must compile for gfx950."""
+import os
+import subprocess
+
+import pytest
+
+from conftest import REPO
+from op2amd.translator import op, source, translate
+from op2amd.translator.op import ArgDat, ArgGbl, ArgIdx, ArgInfo
+
+APPS = REPO / "tests" / "apps"
+
+
+def test_source_helpers():
+    text = 'f(a, g(b, "x,y"), {1, 2}) // f(z)\n/* f(q) */ f(c)'
+    clean = source.strip_comments(text)
+    assert len(clean) == len(text)
+    calls = source.find_calls(clean, "f")
+    assert [c[2] for c in calls] == [["a", 'g(b, "x,y")', "{1, 2}"], ["c"]]
+    assert source.int_constants("#define NN 12\nconst int M = 3;\n") == {"NN": 12, "M": 3}
+    fn = "static inline void k(const double *a,\n    double *b) {\n  b[0] = a[0];\n}\n"
+    s, e = source.find_function("int x;\n" + fn + "k(p, q);", "k")
+    assert ("int x;\n" + fn)[s:e].startswith("static inline void k(")
+
+
+def test_airfoil_program_model():
+    prog = op.parse_program(str(APPS / "airfoil" / "airfoil_tr.cpp"))
+    assert [l.kernel for l in prog.loops] == ["save_soln", "adt_calc", "res_calc",
+                                              "bres_calc", "update"]
+    assert [l.stub for l in prog.loops][2] == "op_par_loop_airfoil_tr_3_res_calc"
+    res = prog.loops[2]
+    assert [a.acc for a in res.args] == ["OP_READ"] * 6 + ["OP_INC"] * 2
+    assert all(isinstance(a, ArgDat) and a.map == "pecell" for a in res.args[2:])
+    upd = prog.loops[4]
+    assert isinstance(upd.args[4], ArgGbl) and upd.args[4].acc == "OP_INC"
+    assert isinstance(upd.args[6], ArgIdx) and upd.args[6].map is None
+    assert isinstance(upd.args[7], ArgInfo) and upd.args[7].ref == 5
+    assert [(c.ptr, c.dim) for c in prog.consts] == [
+        ("gam", 1), ("gm1", 1), ("cfl", 1), ("eps", 1), ("mach", 1), ("alpha", 1), ("qinf", 4)]
+    assert set(prog.kernel_text) == {"save_soln", "adt_calc", "res_calc", "bres_calc", "update"}
+
+
+def test_rewritten_application(tmp_path):
+    paths = translate(str(APPS / "functional" / "consts.cpp"), str(tmp_path))
+    app = open(paths["app"]).read()
+    assert '#include "op2_hip.h"' in app and '#include "consts_op2gen.h"' in app
+    assert "op_par_loop(" not in source.strip_comments(app)
+    assert app.count("op_par_loop_consts_1_use_scalar(") == 1
+    assert app.count("op_par_loop_consts_3_use_scalar(") == 1
+    assert app.count('op_decl_const2("scale", ') == 2
+    hdr = open(paths["header"]).read()
+    assert "void op_par_loop_consts_2_use_array(char const *, op_set, op_arg);" in hdr
+
+
+def test_unsupported_forms_are_refused(tmp_path):
+    bad = tmp_path / "bad.cpp"
+    bad.write_text('#include "op_seq.h"\nvoid k(double *a) {}\nint main() {\n'
+                   '  op_par_loop(k, "k", s, op_arg_dat(d, -1, OP_ID, n, "double", OP_RW));\n}\n')
+    with pytest.raises(op.OpError, match="dim must be an integer constant"):
+        op.parse_program(str(bad))
+    bad.write_text('#include "op_seq.h"\nint main() {\n'
+                   '  op_par_loop(k, "k", s, op_arg_dat(d, -1, OP_ID, 1, "double", OP_RW));\n}\n')
+    with pytest.raises(op.OpError, match="user kernel k not found"):
+        op.parse_program(str(bad))
+
+
+@pytest.mark.parametrize("name", ["gbl", "airfoil_tr"])
+def test_generated_kernels_compile_for_gfx950(name, tmp_path):
+    src = {"gbl": APPS / "functional" / "gbl.cpp", "airfoil_tr": APPS / "airfoil" / "airfoil_tr.cpp"}[name]
+    paths = translate(str(src), str(tmp_path))
+    cmd = ["/opt/rocm/bin/hipcc", "-O1", "-std=c++17", "--offload-arch=gfx950", "-w",
+           "-I" + str(REPO / "include"), "-I" + str(src.parent), "-I" + str(tmp_path),
+           "-x", "hip", "-c", paths["kernels"], "-o", str(tmp_path / "k.o")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
