@@ -284,6 +284,7 @@ def workload_costmodel(op2amd, args):
     reg_default = cm.regret(recs, test_ids, cm.default_choice)
     reg_fixed = cm.regret(recs, test_ids, cm.best_fixed_choice(recs, train_ids))
     reg_train = cm.regret(recs, train_ids, model.choose)
+    reg_top2 = cm.regret(recs, test_ids, lambda r: model.choose_timed(r, 2))
     line = {
         "metric": "cost-model plan selection: mean time regret vs exhaustive sweep, held-out loops",
         "value": round(float(reg_model.mean()), 4), "unit": "fraction", "n_gpus": 1,
@@ -299,6 +300,7 @@ def workload_costmodel(op2amd, args):
                    "regret_model_p90": float(np.percentile(reg_model, 90)),
                    "loops_within_5pct_model": float(np.mean(reg_model <= 0.05)),
                    "regret_model_train_mean": float(reg_train.mean()),
+                   "regret_model_top2_timed_mean": float(reg_top2.mean()),
                    "regret_default_plan_mean": float(reg_default.mean()),
                    "regret_best_fixed_rule_mean": float(reg_fixed.mean()),
                    "loops_within_5pct_best_fixed_rule": float(np.mean(reg_fixed <= 0.05)),
@@ -402,6 +404,15 @@ def main():
                          f"({secs:.1f} s): the oracle's OpenMP executor (block colours "
                          "in sequence, blocks in parallel), g++ -O3 -fopenmp"}
         log(f"cpu baseline: {cpu['value']:.2f} steps/s on {nthreads} threads")
+        # single-thread sequential executor beside it (a ~5 s sample)
+        t = time.time()
+        oracle.airfoil_run(mesh, 1, 1, "seq")
+        one = max(time.time() - t, 1e-3)
+        iters_seq = int(max(1, min(100, 5.0 / one)))
+        t = time.time()
+        oracle.airfoil_run(mesh, iters_seq, 1, "seq")
+        others["cpu_seq_1thread_steps_per_s"] = round(iters_seq / (time.time() - t), 3)
+        log(f"cpu seq, 1 thread: {others['cpu_seq_1thread_steps_per_s']:.2f} steps/s")
 
     dom = head["per_loop"]["res_calc"]
     kw = head["tuned_plans"]

@@ -43,8 +43,8 @@ typedef struct {
 } op2amd_synth_shape;
 
 /* Time `nvar` variants (plan[i] in OP2H_PLAN_* or OP2AMD_SYNTH_DIRECT, block[i]
- * threads) of the loop: one untimed call (plan build), then `reps` timed
- * calls; ms[i] is the fastest.  A variant that does not apply to the access
+ * threads) of the loop: one untimed call (plan build), then `reps` samples of
+ * 8 back-to-back calls each; ms[i] is the fastest sample's mean per call.  A variant that does not apply to the access
  * mode (atomic for RW, anything but direct for READ) gets ms[i] = -1.
  * Returns 0 on success. */
 int op2amd_synth_time(const op2amd_synth_desc *d, int nvar, const int *plan,

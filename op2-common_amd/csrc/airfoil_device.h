@@ -84,34 +84,25 @@ __device__ __forceinline__ void res_calc(const double *x1, const double *x2,
   res2[3] -= f;
 }
 
-// res_calc split into its per-cell and per-edge parts, for plans that
-// evaluate the per-cell part once per staged cell.  The operations and their
-// order are those of res_calc above, so the flux is bit-identical:
-//   cell terms: ri = 1/q[0], p = gm1*(q[3] - 0.5*ri*(q[1]^2 + q[2]^2));
-//   c[0..3] = q, c[4] = adt, c[5] = ri, c[6] = p;
-//   f = the increment of res1 (res2 receives -f).
-__device__ __forceinline__ void res_cell_terms(const double *q, double *ri,
-                                               double *p) {
-  *ri = 1.0 / q[0];
-  *p = gm1_d * (q[3] - 0.5 * (*ri) * (q[1] * q[1] + q[2] * q[2]));
-}
-
-__device__ __forceinline__ void res_edge_flux(const double *x1, const double *x2,
-                                              const double *c1, const double *c2,
-                                              double *f) {
-  double dx, dy, mu, vol1, vol2;
-  const double p1 = c1[6], p2 = c2[6];
+// The increment f of res_calc (res1 += f, res2 -= f), with res_calc's
+// operations in res_calc's order: f is bit-identical to what res_calc adds.
+__device__ __forceinline__ void res_flux(const double *x1, const double *x2,
+                                         const double *q1, const double *q2,
+                                         double adt1, double adt2, double *f) {
+  double dx, dy, mu, ri, p1, vol1, p2, vol2;
   dx = x1[0] - x2[0];
   dy = x1[1] - x2[1];
-  vol1 = c1[5] * (c1[1] * dy - c1[2] * dx);
-  vol2 = c2[5] * (c2[1] * dy - c2[2] * dx);
-  mu = 0.5 * (c1[4] + c2[4]) * eps_d;
-  f[0] = 0.5 * (vol1 * c1[0] + vol2 * c2[0]) + mu * (c1[0] - c2[0]);
-  f[1] = 0.5 * (vol1 * c1[1] + p1 * dy + vol2 * c2[1] + p2 * dy) +
-         mu * (c1[1] - c2[1]);
-  f[2] = 0.5 * (vol1 * c1[2] - p1 * dx + vol2 * c2[2] - p2 * dx) +
-         mu * (c1[2] - c2[2]);
-  f[3] = 0.5 * (vol1 * (c1[3] + p1) + vol2 * (c2[3] + p2)) + mu * (c1[3] - c2[3]);
+  ri = 1.0 / q1[0];
+  p1 = gm1_d * (q1[3] - 0.5 * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
+  vol1 = ri * (q1[1] * dy - q1[2] * dx);
+  ri = 1.0 / q2[0];
+  p2 = gm1_d * (q2[3] - 0.5 * ri * (q2[1] * q2[1] + q2[2] * q2[2]));
+  vol2 = ri * (q2[1] * dy - q2[2] * dx);
+  mu = 0.5 * (adt1 + adt2) * eps_d;
+  f[0] = 0.5 * (vol1 * q1[0] + vol2 * q2[0]) + mu * (q1[0] - q2[0]);
+  f[1] = 0.5 * (vol1 * q1[1] + p1 * dy + vol2 * q2[1] + p2 * dy) + mu * (q1[1] - q2[1]);
+  f[2] = 0.5 * (vol1 * q1[2] - p1 * dx + vol2 * q2[2] - p2 * dx) + mu * (q1[2] - q2[2]);
+  f[3] = 0.5 * (vol1 * (q1[3] + p1) + vol2 * (q2[3] + p2)) + mu * (q1[3] - q2[3]);
 }
 
 __device__ __forceinline__ void bres_calc(const double *x1, const double *x2,
@@ -250,27 +241,24 @@ __device__ __forceinline__ void bres_calc(const float *x1, const float *x2,
   }
 }
 
-// res_calc (SP) split like op2_k::res_cell_terms / res_edge_flux:
-// c[0..3] = q, c[4] = adt, c[5] = 1/q[0], c[6] = pressure.
-__device__ __forceinline__ void res_cell_terms(const float *q, float *ri, float *p) {
-  *ri = 1.0f / q[0];
-  *p = gm1_f * (q[3] - 0.5f * (*ri) * (q[1] * q[1] + q[2] * q[2]));
-}
-
-__device__ __forceinline__ void res_edge_flux(const float *x1, const float *x2,
-                                              const float *c1, const float *c2,
-                                              float *f) {
-  float dx, dy, mu, vol1, vol2;
-  const float p1 = c1[6], p2 = c2[6];
+// The increment f of res_calc (SP), like op2_k::res_flux.
+__device__ __forceinline__ void res_flux(const float *x1, const float *x2,
+                                         const float *q1, const float *q2,
+                                         float adt1, float adt2, float *f) {
+  float dx, dy, mu, ri, p1, vol1, p2, vol2;
   dx = x1[0] - x2[0];
   dy = x1[1] - x2[1];
-  vol1 = c1[5] * (c1[1] * dy - c1[2] * dx);
-  vol2 = c2[5] * (c2[1] * dy - c2[2] * dx);
-  mu = 0.5f * (c1[4] + c2[4]) * eps_f;
-  f[0] = 0.5f * (vol1 * c1[0] + vol2 * c2[0]) + mu * (c1[0] - c2[0]);
-  f[1] = 0.5f * (vol1 * c1[1] + p1 * dy + vol2 * c2[1] + p2 * dy) + mu * (c1[1] - c2[1]);
-  f[2] = 0.5f * (vol1 * c1[2] - p1 * dx + vol2 * c2[2] - p2 * dx) + mu * (c1[2] - c2[2]);
-  f[3] = 0.5f * (vol1 * (c1[3] + p1) + vol2 * (c2[3] + p2)) + mu * (c1[3] - c2[3]);
+  ri = 1.0f / q1[0];
+  p1 = gm1_f * (q1[3] - 0.5f * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
+  vol1 = ri * (q1[1] * dy - q1[2] * dx);
+  ri = 1.0f / q2[0];
+  p2 = gm1_f * (q2[3] - 0.5f * ri * (q2[1] * q2[1] + q2[2] * q2[2]));
+  vol2 = ri * (q2[1] * dy - q2[2] * dx);
+  mu = 0.5f * (adt1 + adt2) * eps_f;
+  f[0] = 0.5f * (vol1 * q1[0] + vol2 * q2[0]) + mu * (q1[0] - q2[0]);
+  f[1] = 0.5f * (vol1 * q1[1] + p1 * dy + vol2 * q2[1] + p2 * dy) + mu * (q1[1] - q2[1]);
+  f[2] = 0.5f * (vol1 * q1[2] - p1 * dx + vol2 * q2[2] - p2 * dx) + mu * (q1[2] - q2[2]);
+  f[3] = 0.5f * (vol1 * (q1[3] + p1) + vol2 * (q2[3] + p2)) + mu * (q1[3] - q2[3]);
 }
 
 }  // namespace op2_kf
@@ -364,5 +352,25 @@ __device__ __forceinline__ int xcd_block(int nxcd) {
 }
 
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
+
+// The value held by lane J of this lane's quad (lanes 4q..4q+3 of the wave),
+// by a DPP quad_perm(J, J, J, J) move: a VALU operation, no LDS traffic.
+// Every lane of the quad must be active.
+template <int J>
+__device__ __forceinline__ double quad_bcast(double v) {
+  constexpr int ctrl = J * 0x55;
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), ctrl, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int J>
+__device__ __forceinline__ float quad_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), J * 0x55, 0xF, 0xF, false));
+}
+template <int J>
+__device__ __forceinline__ int quad_bcast(int v) {
+  return __builtin_amdgcn_mov_dpp(v, J * 0x55, 0xF, 0xF, false);
+}
 
 }  // namespace op2_dev

@@ -279,16 +279,25 @@ __global__ void __launch_bounds__(1024)
     op2_k::res_calc(x1, x2, q1, q2, &a1, &a2, r1, r2);
     for (int d = 0; d < 4; ++d) r[d] = second ? r2[d] : r1[d];
   }
-  const int lead = (threadIdx.x & 63) & ~(K - 1);
+  static_assert(K == 4, "quad broadcasts assume four lanes per cell");
+  // the quad's increments in edge order (lane 0 first), added to res
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (k == 0 && valid) ld4<SOA>(res, cell, sq, acc);
-  for (int j = 0; j < K; ++j) {
-    double v[4];
-    for (int d = 0; d < 4; ++d) v[d] = __shfl(r[d], lead + j, 64);
-    int hv = __shfl(has, lead + j, 64);
-    if (hv)
-      for (int d = 0; d < 4; ++d) acc[d] += v[d];
+  double v[4][4];
+  int hv[4];
+  for (int d = 0; d < 4; ++d) {
+    v[0][d] = quad_bcast<0>(r[d]);
+    v[1][d] = quad_bcast<1>(r[d]);
+    v[2][d] = quad_bcast<2>(r[d]);
+    v[3][d] = quad_bcast<3>(r[d]);
   }
+  hv[0] = quad_bcast<0>(has);
+  hv[1] = quad_bcast<1>(has);
+  hv[2] = quad_bcast<2>(has);
+  hv[3] = quad_bcast<3>(has);
+  for (int j = 0; j < 4; ++j)
+    if (hv[j])
+      for (int d = 0; d < 4; ++d) acc[d] += v[j][d];
   if (k == 0 && valid) st4<SOA>(res, cell, sq, acc);
 }
 
@@ -365,102 +374,77 @@ __global__ void __launch_bounds__(1024)
 }
 
 struct OwnerArgs {
-  const int *owned_offs, *owned, *elem_offs, *csr_offs;
+  const int4 *inst;  // per instance: edge nodes (x, y), edge cells (z, w)
+  const int *elem_offs, *owned_offs, *owned, *csr_offs;
   const unsigned short *csr;
-  const int *ind_offs_n, *ind_sizes_n, *ind_map_n;  // node map slot
-  const int *ind_offs_c, *ind_sizes_c, *ind_map_c;  // cell map slot
-  const unsigned short *loc_n0, *loc_n1, *loc_c0, *loc_c1;
-  int max_n, max_c, max_e;
+  int max_e;
 };
 
-// Owner plan pointers for res_calc's args (x via pedge 0/1, q/adt via
-// pecell 0/1, res incremented via pecell 0 (+f) and 1 (-f)); NULL plan or an
+// Per-instance records of res_calc's owner plan: {pedge[e], pecell[e]}.
+__global__ void k_pack_owner_res(const int2 *__restrict pedge,
+                                 const int2 *__restrict pecell,
+                                 const int *__restrict elems, int total,
+                                 int4 *__restrict inst) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= total) return;
+  int e = elems[j];
+  int2 n = pedge[e], c = pecell[e];
+  inst[j] = make_int4(n.x, n.y, c.x, c.y);
+}
+
+// Owner plan for res_calc's args (res incremented via pecell 0 (+f) and 1
+// (-f)), with its instance records packed once per plan; a NULL plan or an
 // unexpected shape is fatal, as the reference's plan errors are.
-static OwnerArgs owner_args_res(const op2h_owner_plan *p) {
+static OwnerArgs owner_args_res(const op2h_owner_plan *p, const op_arg *args,
+                                int bs, hipStream_t st) {
   if (!p) exit(-1);
   if (p->inc_of_arg[6] != 0 || p->inc_of_arg[7] != 1 || p->ninc != 2) {
     printf("op2hip: res_calc owner plan expects res via ecell[0], ecell[1]\n");
     exit(-1);
   }
-  int sn = p->map_of_arg[0], sc = p->map_of_arg[2];
+  char key[96];
+  snprintf(key, sizeof key, "res_calc.owner.%d.%d.%d", bs, args[0].map->index,
+           args[2].map->index);
+  int fresh = 0;
+  int4 *inst = (int4 *)op2h_plan_user(key, sizeof(int4) * (size_t)p->total, &fresh);
+  if (fresh && p->total)
+    k_pack_owner_res<<<grid_for(p->total, 256), 256, 0, st>>>(
+        (const int2 *)args[0].map->map_d, (const int2 *)args[2].map->map_d, p->elems,
+        p->total, inst);
   OwnerArgs A;
+  A.inst = inst;
+  A.elem_offs = p->elem_offs;
   A.owned_offs = p->owned_offs;
   A.owned = p->owned;
-  A.elem_offs = p->elem_offs;
   A.csr_offs = p->csr_offs;
   A.csr = p->csr;
-  A.ind_offs_n = p->ind_offs[sn];
-  A.ind_sizes_n = p->ind_sizes[sn];
-  A.ind_map_n = p->ind_map[sn];
-  A.ind_offs_c = p->ind_offs[sc];
-  A.ind_sizes_c = p->ind_sizes[sc];
-  A.ind_map_c = p->ind_map[sc];
-  A.loc_n0 = p->loc_map[p->loc_of_arg[0]];
-  A.loc_n1 = p->loc_map[p->loc_of_arg[1]];
-  A.loc_c0 = p->loc_map[p->loc_of_arg[2]];
-  A.loc_c1 = p->loc_map[p->loc_of_arg[3]];
-  A.max_n = p->max_ind_h[sn];
-  A.max_c = p->max_ind_h[sc];
   A.max_e = p->max_elems;
   return A;
 }
 
-// LDS bytes of the owner kernels: x [max_n][2], cells [max_c][8],
-// fluxes [max_e][4] of `word` bytes each; fatal beyond the 160 KiB of a CU.
-static size_t owner_lds(const OwnerArgs &A, size_t word) {
-  size_t lds = word * (2 * (size_t)A.max_n + 8 * (size_t)A.max_c + 4 * (size_t)A.max_e);
-  if (lds > 160 * 1024) {
-    printf("op2hip: res_calc owner plan needs %zu bytes of LDS; use a smaller block\n", lds);
-    exit(-1);
-  }
-  return lds;
-}
-
 // Owner-compute res_calc: one workgroup per block of owned cells.
-//  1. stage x of the block's nodes, and per staged cell q, adt, 1/q[0] and
-//     the pressure (the per-cell half of res_calc) in LDS;
-//  2. one flux per edge instance into LDS;
-//  3. each owned cell adds its edges' fluxes to res in ascending edge order
+//  1. one flux per edge touching the block (edges on the block boundary are
+//     evaluated by both neighbouring blocks), from global memory, into LDS;
+//  2. each owned cell adds its edges' fluxes to res in ascending edge order
 //     (+f where it is ecell[0], -f where it is ecell[1]), as the sequential
 //     executor does, so res is bit-identical to it.
-// LDS: x [max_n][2], cells [max_c][8], fluxes [max_e][4] doubles.
+// LDS: fluxes [max_e][4].
 template <bool SOA>
 __global__ void __launch_bounds__(1024)
     k_res_owner(const double *__restrict x, const double *__restrict q,
                 const double *__restrict adt, double *__restrict res,
                 OwnerArgs P, int sx, int sq) {
-  extern __shared__ double lds[];
+  extern __shared__ double lf[];
   const int b = xcd_block(kXcds);
-  double *lx = lds;
-  double *lc = lx + 2 * P.max_n;
-  double *lf = lc + 8 * P.max_c;
-  const int nn = P.ind_sizes_n[b], nbase = P.ind_offs_n[b];
-  for (int k = threadIdx.x; k < nn; k += blockDim.x) {
-    double v[2];
-    ld2<SOA>(x, P.ind_map_n[nbase + k], sx, v);
-    lx[2 * k] = v[0];
-    lx[2 * k + 1] = v[1];
-  }
-  const int nc = P.ind_sizes_c[b], cbase = P.ind_offs_c[b];
-  for (int k = threadIdx.x; k < nc; k += blockDim.x) {
-    const int cell = P.ind_map_c[cbase + k];
-    double v[4];
-    ld4<SOA>(q, cell, sq, v);
-    double *c = lc + 8 * k;
-    c[0] = v[0];
-    c[1] = v[1];
-    c[2] = v[2];
-    c[3] = v[3];
-    c[4] = adt[cell];
-    op2_k::res_cell_terms(v, c + 5, c + 6);
-  }
-  __syncthreads();
   const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
   for (int j = threadIdx.x; j < ne; j += blockDim.x) {
-    const int a0 = P.loc_n0[e0 + j], a1 = P.loc_n1[e0 + j];
-    const int l0 = P.loc_c0[e0 + j], l1 = P.loc_c1[e0 + j];
-    op2_k::res_edge_flux(lx + 2 * a0, lx + 2 * a1, lc + 8 * l0, lc + 8 * l1,
-                         lf + 4 * j);
+    const int4 r = P.inst[e0 + j];
+    double x1[2], x2[2], q1[4], q2[4];
+    ld2<SOA>(x, r.x, sx, x1);
+    ld2<SOA>(x, r.y, sx, x2);
+    ld4<SOA>(q, r.z, sq, q1);
+    ld4<SOA>(q, r.w, sq, q2);
+    op2_k::res_flux(x1, x2, q1, q2, adt[r.z], adt[r.w], lf + 4 * j);
   }
   __syncthreads();
   const int o0 = P.owned_offs[b], no = P.owned_offs[b + 1] - o0;
@@ -852,16 +836,24 @@ __global__ void __launch_bounds__(1024)
     op2_kf::res_calc(x1, x2, q1, q2, &a1, &a2, r1, r2);
     for (int d = 0; d < 4; ++d) r[d] = second ? r2[d] : r1[d];
   }
-  const int lead = (threadIdx.x & 63) & ~(K - 1);
+  static_assert(K == 4, "quad broadcasts assume four lanes per cell");
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
   if (k == 0 && valid) ld4s<SOA>(res, cell, sq, acc);
-  for (int j = 0; j < K; ++j) {
-    float v[4];
-    for (int d = 0; d < 4; ++d) v[d] = __shfl(r[d], lead + j, 64);
-    int hv = __shfl(has, lead + j, 64);
-    if (hv)
-      for (int d = 0; d < 4; ++d) acc[d] += v[d];
+  float v[4][4];
+  int hv[4];
+  for (int d = 0; d < 4; ++d) {
+    v[0][d] = quad_bcast<0>(r[d]);
+    v[1][d] = quad_bcast<1>(r[d]);
+    v[2][d] = quad_bcast<2>(r[d]);
+    v[3][d] = quad_bcast<3>(r[d]);
   }
+  hv[0] = quad_bcast<0>(has);
+  hv[1] = quad_bcast<1>(has);
+  hv[2] = quad_bcast<2>(has);
+  hv[3] = quad_bcast<3>(has);
+  for (int j = 0; j < 4; ++j)
+    if (hv[j])
+      for (int d = 0; d < 4; ++d) acc[d] += v[j][d];
   if (k == 0 && valid) st4s<SOA>(res, cell, sq, acc);
 }
 
@@ -872,29 +864,17 @@ __global__ void __launch_bounds__(1024)
     k_res_owner_mixed(const double *__restrict x, const double *__restrict q,
                       const float *__restrict adt, float *__restrict res,
                       OwnerArgs P, int sx, int sq) {
-  extern __shared__ float ldsf[];
+  extern __shared__ float lff[];
   const int b = xcd_block(kXcds);
-  float *lx = ldsf;
-  float *lc = lx + 2 * P.max_n;
-  float *lf = lc + 8 * P.max_c;
-  const int nn = P.ind_sizes_n[b], nbase = P.ind_offs_n[b];
-  for (int k = threadIdx.x; k < nn; k += blockDim.x)
-    ld2f<SOA>(x, P.ind_map_n[nbase + k], sx, lx + 2 * k);
-  const int nc = P.ind_sizes_c[b], cbase = P.ind_offs_c[b];
-  for (int k = threadIdx.x; k < nc; k += blockDim.x) {
-    const int cell = P.ind_map_c[cbase + k];
-    float *c = lc + 8 * k;
-    ld4f<SOA>(q, cell, sq, c);
-    c[4] = adt[cell];
-    op2_kf::res_cell_terms(c, c + 5, c + 6);
-  }
-  __syncthreads();
   const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
   for (int j = threadIdx.x; j < ne; j += blockDim.x) {
-    const int a0 = P.loc_n0[e0 + j], a1 = P.loc_n1[e0 + j];
-    const int l0 = P.loc_c0[e0 + j], l1 = P.loc_c1[e0 + j];
-    op2_kf::res_edge_flux(lx + 2 * a0, lx + 2 * a1, lc + 8 * l0, lc + 8 * l1,
-                          lf + 4 * j);
+    const int4 r = P.inst[e0 + j];
+    float x1[2], x2[2], q1[4], q2[4];
+    ld2f<SOA>(x, r.x, sx, x1);
+    ld2f<SOA>(x, r.y, sx, x2);
+    ld4f<SOA>(q, r.z, sq, q1);
+    ld4f<SOA>(q, r.w, sq, q2);
+    op2_kf::res_flux(x1, x2, q1, q2, adt[r.z], adt[r.w], lff + 4 * j);
   }
   __syncthreads();
   const int o0 = P.owned_offs[b], no = P.owned_offs[b + 1] - o0;
@@ -904,7 +884,7 @@ __global__ void __launch_bounds__(1024)
     ld4s<SOA>(res, cell, sq, acc);
     for (int k = P.csr_offs[o0 + i]; k < P.csr_offs[o0 + i + 1]; ++k) {
       const int ent = P.csr[k];
-      const float *f = lf + 4 * (ent >> 1);
+      const float *f = lff + 4 * (ent >> 1);
       if (ent & 1) {
         for (int d = 0; d < 4; ++d) acc[d] -= f[d];
       } else {
@@ -1139,10 +1119,11 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       const float *adtf = (const float *)args[4].data_d;
       float *resf = (float *)args[6].data_d;
       const op2h_owner_plan *owner_p = nullptr;
+      OwnerArgs OA{};
       if (ctx.plan == OP2H_PLAN_OWNER) {  // built before the timed region
         ctx.part_size = bs;
         owner_p = op2h_plan_owner(&ctx);
-        if (!owner_p) exit(-1);
+        OA = owner_args_res(owner_p, args, bs, st);
       }
       op2h_kernel_timer_start(&ctx);
       if (ctx.plan == OP2H_PLAN_ATOMIC) {
@@ -1151,11 +1132,10 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
         else k_res_atomic_mixed<false><<<g, bs, 0, st>>>(x, q, adtf, resf, pedge, pecell, set->size, sx, sq);
       } else if (ctx.plan == OP2H_PLAN_OWNER) {
         const int nblocks = owner_p->nblocks;
-        OwnerArgs A = owner_args_res(owner_p);
-        size_t lds = owner_lds(A, sizeof(float));
+        const size_t lds = sizeof(float) * 4 * (size_t)OA.max_e;
         if (nblocks) {
-          if (soa) k_res_owner_mixed<true><<<nblocks, bs, lds, st>>>(x, q, adtf, resf, A, sx, sq);
-          else k_res_owner_mixed<false><<<nblocks, bs, lds, st>>>(x, q, adtf, resf, A, sx, sq);
+          if (soa) k_res_owner_mixed<true><<<nblocks, bs, lds, st>>>(x, q, adtf, resf, OA, sx, sq);
+          else k_res_owner_mixed<false><<<nblocks, bs, lds, st>>>(x, q, adtf, resf, OA, sx, sq);
         }
       } else {
         const op2h_gather_plan *p = op2h_plan_gather(args[6].map);
@@ -1234,8 +1214,12 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       case OP2H_PLAN_OWNER: {
         ctx.part_size = bs;
         const op2h_owner_plan *p = op2h_plan_owner(&ctx);
-        OwnerArgs A = owner_args_res(p);
-        size_t lds = owner_lds(A, sizeof(double));
+        OwnerArgs A = owner_args_res(p, args, bs, st);
+        const size_t lds = sizeof(double) * 4 * (size_t)A.max_e;
+        if (lds > 160 * 1024) {
+          printf("op2hip: res_calc owner plan needs %zu bytes of LDS; use a smaller block\n", lds);
+          exit(-1);
+        }
         op2h_kernel_timer_start(&ctx);
         if (p->nblocks) {
           if (soa) k_res_owner<true><<<p->nblocks, bs, lds, st>>>(x, q, adt, res, A, sx, sq);

@@ -22,6 +22,8 @@
 #include <vector>
 
 #include "op2_hip_rt.h"
+
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include "op2h_plan_build.h"
 #include "op2h_rcm.h"
 
@@ -838,9 +840,12 @@ extern "C" void op_rank(int *rank) { *rank = 0; }
 
 // ---- profiling C API: a flat per-name wall-clock tree (op_profile.h:197-212) ----
 extern "C" void op_profile_start(const char *name) { op_profile_enter(name); }
+// Profile regions and op_par_loops are also roctx ranges, so a rocprofv3
+// --marker-trace run attributes kernels to them.
 extern "C" void op_profile_enter(const char *name) {
   op2h_device_sync();
   rt().prof_stack.push_back({name, wall_now()});
+  roctxRangePushA(name);
 }
 extern "C" void op_profile_enter_kernel(const char *name, const char *,
                                         const char *) {
@@ -856,6 +861,7 @@ extern "C" void op_profile_exit(void) {
   op2h_device_sync();
   auto top = R.prof_stack.back();
   R.prof_stack.pop_back();
+  roctxRangePop();
   std::string path;
   for (auto &p : R.prof_stack) path += p.first + "/";
   R.prof_totals[path + top.first] += wall_now() - top.second;
@@ -1197,6 +1203,7 @@ extern "C" void op2h_loop_begin_mask(op2h_loop_ctx *ctx, int kernel_idx,
   Runtime &R = rt();
   ensure_init();
   std::memset(ctx, 0, sizeof(*ctx));
+  roctxRangePushA(name);
   ctx->kernel_idx = kernel_idx;
   ctx->name = name;
   ctx->set = set;
@@ -1347,8 +1354,10 @@ extern "C" void op2h_loop_end(op2h_loop_ctx *ctx) {
   for (int g = 0; g < ctx->ngbl; ++g) {
     op_arg &a = ctx->args[ctx->gbl_arg[g]];
     int info = -1;
-    for (int m = 0; m < ctx->nargs; ++m)
-      if (ctx->args[m].argtype == OP_ARG_INFO && ctx->args[m].opt) info = m;
+    for (int m = 0; m < ctx->nargs; ++m)  // the info arg names its gbl in acc
+      if (ctx->args[m].argtype == OP_ARG_INFO && ctx->args[m].opt &&
+          ctx->args[m].acc == ctx->gbl_arg[g])
+        info = m;
     bool with_info = (a.acc == OP_MIN || a.acc == OP_MAX) && info >= 0;
     size_t bytes = (size_t)a.size + (with_info ? ctx->args[info].size : 0);
     // inside a captured graph the slots reach the host through the history
@@ -1382,7 +1391,9 @@ extern "C" void op2h_loop_end(op2h_loop_ctx *ctx) {
       op_arg &a = ctx->args[ctx->gbl_arg[g]];
       int info = -1;
       for (int m = 0; m < ctx->nargs; ++m)
-        if (ctx->args[m].argtype == OP_ARG_INFO && ctx->args[m].opt) info = m;
+        if (ctx->args[m].argtype == OP_ARG_INFO && ctx->args[m].opt &&
+            ctx->args[m].acc == ctx->gbl_arg[g])
+          info = m;
       bool with_info = (a.acc == OP_MIN || a.acc == OP_MAX) && info >= 0;
       const char *res = (const char *)ctx->gbl_slot_h[g];
       combine_gbl(a.acc, type_code(a.type), a.dim, a.data, res,

@@ -66,9 +66,12 @@ def desc_from(n_target, arity, dim, access, is_double, ordering, k, seed):
                 k, seed)
 
 
+BLOCKS = (64, 128, 256, 512, 1024)
+
+
 def variants(access):
     """Candidate (plan, block) pairs of the exhaustive sweep for an access mode."""
-    blocks = (64, 128, 256, 512, 1024)
+    blocks = BLOCKS
     if access == "read":
         return [(DIRECT, b) for b in blocks]
     plans = [PLAN_COLOR, PLAN_GATHER] + ([PLAN_ATOMIC] if access == "inc" else [])
@@ -162,7 +165,9 @@ def features(rec, plan, block):
          float(rec["access"] == "rw")]
     f += [float(rec["ordering"] == o) for o in ORDER_NAMES]
     f += [float(plan == p) for p in PLAN_IDS]
+    # block size both as a magnitude and one-hot: its effect is not monotonic
     f += [math.log2(block) / 10.0]
+    f += [float(block == b) for b in BLOCKS]
     return f
 
 
@@ -303,7 +308,17 @@ class CostModel:
         Xn = ((np.asarray(X, np.float32) - self.mu) / self.sd).astype(np.float32)
         return np.mean([net.forward(Xn) for net in self.nets], axis=0)
 
-    def choose(self, rec):
+    def ranked(self, rec):
+        """The loop's valid variants, best predicted first."""
         valid = valid_variants(rec)
         pred = self.predict([features(rec, v[0], v[1]) for v, _ in valid])
-        return valid[int(np.argmin(pred))][0]
+        return [valid[i][0] for i in np.argsort(pred, kind="stable")]
+
+    def choose(self, rec):
+        return self.ranked(rec)[0]
+
+    def choose_timed(self, rec, k):
+        """Model-pruned tuning: time only the k best predicted variants (here
+        read from the sweep) and keep the fastest."""
+        times = dict(valid_variants(rec))
+        return min(self.ranked(rec)[:k], key=lambda v: times[v])
