@@ -63,6 +63,14 @@ void op2h_loop_end(op2h_loop_ctx *ctx);
 void op2h_kernel_timer_start(op2h_loop_ctx *ctx);
 void op2h_kernel_timer_stop(op2h_loop_ctx *ctx);
 
+/* Kernel-attached timing events: when kernel timing is on (and the call is
+ * neither a tuning trial nor captured), returns 1 with two events in *start
+ * and *stop for hipExtLaunchKernel, which stamps them at the first launch's
+ * start and the last launch's end, so the time is the kernels' own (no event
+ * packets between them); the stub then skips op2h_kernel_timer_start/stop.
+ * Otherwise returns 0 with both NULL. */
+int op2h_kernel_timer_events(op2h_loop_ctx *ctx, void **start, void **stop);
+
 /* Device layout of a dat: 0 AoS, else the SoA stride in elements. */
 int op2h_dat_soa_stride(op_dat dat);
 

@@ -16,6 +16,10 @@
 //           and res accumulated in registers — the sequential executor's
 //           summation order, so results match it bit for bit.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
+
+#include <tuple>
+#include <utility>
 
 #include <algorithm>
 #include <cmath>
@@ -67,6 +71,43 @@ static const struct AirfoilConstHook {
 
 namespace {
 using namespace op2_dev;
+
+// Launch with the kernel's declared argument types; with timing events
+// (op2h_kernel_timer_events) through hipExtLaunchKernel, which stamps `e0` at
+// the kernel's start and `e1` at its end (either may be NULL).
+template <class Tuple, std::size_t... I>
+void arg_ptrs(Tuple &t, void **p, std::index_sequence<I...>) {
+  ((p[I] = (void *)&std::get<I>(t)), ...);
+}
+
+template <typename... KArgs, typename... Args>
+void launch_k(bool ext, void *e0, void *e1, void (*kernel)(KArgs...), dim3 grid,
+              dim3 block, size_t lds, hipStream_t st, Args... args) {
+  static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+  std::tuple<KArgs...> t{static_cast<KArgs>(args)...};
+  void *p[sizeof...(KArgs) > 0 ? sizeof...(KArgs) : 1];
+  arg_ptrs(t, p, std::index_sequence_for<KArgs...>{});
+  if (ext)
+    KCHECK(hipExtLaunchKernel((const void *)kernel, grid, block, p, lds, st,
+                              (hipEvent_t)e0, (hipEvent_t)e1, 0));
+  else
+    KCHECK(hipLaunchKernel((const void *)kernel, grid, block, p, lds, st));
+}
+
+// Timing for a loop launched with launch_k: kernel-attached events when
+// timing is on, the event-pair timer (or the autotuner's) otherwise.
+struct LoopTimer {
+  op2h_loop_ctx *ctx;
+  void *e0 = nullptr, *e1 = nullptr;
+  bool ext = false;
+  explicit LoopTimer(op2h_loop_ctx *c) : ctx(c) {
+    ext = op2h_kernel_timer_events(c, &e0, &e1) != 0;
+    if (!ext) op2h_kernel_timer_start(c);
+  }
+  void done() {
+    if (!ext) op2h_kernel_timer_stop(ctx);
+  }
+};
 
 // ---------------------------------------------------------------------------
 // save_soln: qold = q over the whole dat, 16-byte vectors (direct copy)
@@ -1046,10 +1087,10 @@ extern "C" void op_par_loop_airfoil_1_save_soln(char const *name, op_set set,
     // identical layouts: the copy covers the whole device allocation
     long long ndouble = soa ? 4LL * set->stride : 4LL * set->size;
     int n2 = (int)(ndouble / 2);
-    op2h_kernel_timer_start(&ctx);
-    k_save_soln<<<grid_for(n2, ctx.block_size), ctx.block_size, 0, st>>>(
-        (const double2 *)args[0].data_d, (double2 *)args[1].data_d, n2);
-    op2h_kernel_timer_stop(&ctx);
+    LoopTimer tm(&ctx);
+    launch_k(tm.ext, tm.e0, tm.e1, k_save_soln, grid_for(n2, ctx.block_size), ctx.block_size,
+             0, st, (const double2 *)args[0].data_d, (double2 *)args[1].data_d, n2);
+    tm.done();
     KCHECK(hipGetLastError());
   }
   account(0, set, 2, args);
@@ -1073,16 +1114,16 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
     double *adt = (double *)args[5].data_d;
     const int *pcell = args[0].map->map_d;
     int g = grid_for(set->size, ctx.block_size);
-    op2h_kernel_timer_start(&ctx);
+    LoopTimer tm(&ctx);
     if (is_float(args[5])) {
       float *adtf = (float *)args[5].data_d;
-      if (soa) k_adt_calc_mixed<true><<<g, ctx.block_size, 0, st>>>(x, q, adtf, pcell, set->size, sx, sq);
-      else k_adt_calc_mixed<false><<<g, ctx.block_size, 0, st>>>(x, q, adtf, pcell, set->size, sx, sq);
-    } else if (soa)
-      k_adt_calc<true><<<g, ctx.block_size, 0, st>>>(x, q, adt, pcell, set->size, sx, sq);
-    else
-      k_adt_calc<false><<<g, ctx.block_size, 0, st>>>(x, q, adt, pcell, set->size, sx, sq);
-    op2h_kernel_timer_stop(&ctx);
+      launch_k(tm.ext, tm.e0, tm.e1, soa ? k_adt_calc_mixed<true> : k_adt_calc_mixed<false>, g,
+               ctx.block_size, 0, st, x, q, adtf, pcell, set->size, sx, sq);
+    } else {
+      launch_k(tm.ext, tm.e0, tm.e1, soa ? k_adt_calc<true> : k_adt_calc<false>, g,
+               ctx.block_size, 0, st, x, q, adt, pcell, set->size, sx, sq);
+    }
+    tm.done();
     KCHECK(hipGetLastError());
   }
   account(1, set, 6, args);
@@ -1185,7 +1226,7 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       }
       case OP2H_PLAN_GATHER: {
         const op2h_gather_plan *p = op2h_plan_gather(args[6].map);
-        op2h_kernel_timer_start(&ctx);
+        LoopTimer tm(&ctx);
         if (p->max_count <= 4) {
           // packed per-(cell, edge) records, built once per plan
           char key[96];
@@ -1197,18 +1238,22 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
             k_pack_res_gather<4><<<grid_for(4 * (long long)p->ntargets, 256), 256, 0, st>>>(
                 pedge, pecell, p->offsets, p->entries, p->ntargets, rec);
           int g = grid_for(4 * (long long)p->ntargets, bs);
-          if (soa) k_res_gather_packed<true, 4><<<g, bs, 0, st>>>(x, q, adt, res, rec, p->targets, p->ntargets, sx, sq);
-          else k_res_gather_packed<false, 4><<<g, bs, 0, st>>>(x, q, adt, res, rec, p->targets, p->ntargets, sx, sq);
+          launch_k(tm.ext, tm.e0, tm.e1,
+                   soa ? k_res_gather_packed<true, 4> : k_res_gather_packed<false, 4>, g, bs, 0,
+                   st, x, q, adt, res, rec, p->targets, p->ntargets, sx, sq);
         } else if (p->max_count <= 8) {
+          if (tm.ext) KCHECK(hipEventRecord((hipEvent_t)tm.e0, st));
           int g = grid_for(8 * (long long)p->ntargets, bs);
           if (soa) k_res_gather_lanes<true, 8><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
           else k_res_gather_lanes<false, 8><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
         } else {
+          if (tm.ext) KCHECK(hipEventRecord((hipEvent_t)tm.e0, st));
           int g = grid_for(p->ntargets, bs);
           if (soa) k_res_gather<true><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
           else k_res_gather<false><<<g, bs, 0, st>>>(x, q, adt, res, pedge, pecell, p->offsets, p->entries, p->targets, p->ntargets, sx, sq);
         }
-        op2h_kernel_timer_stop(&ctx);
+        if (tm.ext && p->max_count > 4) KCHECK(hipEventRecord((hipEvent_t)tm.e1, st));
+        tm.done();
         break;
       }
       case OP2H_PLAN_OWNER: {
@@ -1220,12 +1265,11 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
           printf("op2hip: res_calc owner plan needs %zu bytes of LDS; use a smaller block\n", lds);
           exit(-1);
         }
-        op2h_kernel_timer_start(&ctx);
-        if (p->nblocks) {
-          if (soa) k_res_owner<true><<<p->nblocks, bs, lds, st>>>(x, q, adt, res, A, sx, sq);
-          else k_res_owner<false><<<p->nblocks, bs, lds, st>>>(x, q, adt, res, A, sx, sq);
-        }
-        op2h_kernel_timer_stop(&ctx);
+        LoopTimer tm(&ctx);
+        if (p->nblocks)
+          launch_k(tm.ext, tm.e0, tm.e1, soa ? k_res_owner<true> : k_res_owner<false>,
+                   p->nblocks, bs, lds, st, x, q, adt, res, A, sx, sq);
+        tm.done();
         break;
       }
       default: {  // STAGED
@@ -1314,11 +1358,11 @@ extern "C" void op_par_loop_airfoil_4_bres_calc(char const *name, op_set set,
     }
     switch (ctx.plan) {
       case OP2H_PLAN_ATOMIC: {
-        op2h_kernel_timer_start(&ctx);
+        LoopTimer tm(&ctx);
         int g = grid_for(set->size, bs);
-        if (soa) k_bres_atomic<true><<<g, bs, 0, st>>>(x, q, adt, res, pbedge, pbecell, bound, set->size, sx, sq);
-        else k_bres_atomic<false><<<g, bs, 0, st>>>(x, q, adt, res, pbedge, pbecell, bound, set->size, sx, sq);
-        op2h_kernel_timer_stop(&ctx);
+        launch_k(tm.ext, tm.e0, tm.e1, soa ? k_bres_atomic<true> : k_bres_atomic<false>, g, bs,
+                 0, st, x, q, adt, res, pbedge, pbecell, bound, set->size, sx, sq);
+        tm.done();
         break;
       }
       case OP2H_PLAN_COLOR: {
@@ -1412,30 +1456,25 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
   double *part_sum = (double *)scratch;
   double *part_max = part_sum + g;
   int *part_loc = (int *)(part_max + g);
-  op2h_kernel_timer_start(&ctx);
+  LoopTimer tm(&ctx);
   if (is_float(args[2]) != is_float(args[3])) {
     printf("op2hip: update needs res and adt both FP32 (mixed mode) or both FP64\n");
     exit(-1);
   }
   if (is_float(args[2])) {
-    if (soa)
-      k_update_mixed<true><<<g, bs, 0, st>>>((const double *)args[0].data_d, (double *)args[1].data_d,
-                                             (float *)args[2].data_d, (const float *)args[3].data_d,
-                                             set->size, s, part_sum, part_max, part_loc);
-    else
-      k_update_mixed<false><<<g, bs, 0, st>>>((const double *)args[0].data_d, (double *)args[1].data_d,
-                                              (float *)args[2].data_d, (const float *)args[3].data_d,
-                                              set->size, s, part_sum, part_max, part_loc);
-  } else if (soa)
-    k_update<true><<<g, bs, 0, st>>>((const double *)args[0].data_d, (double *)args[1].data_d,
-                                     (double *)args[2].data_d, (const double *)args[3].data_d,
-                                     set->size, s, part_sum, part_max, part_loc);
-  else
-    k_update<false><<<g, bs, 0, st>>>((const double *)args[0].data_d, (double *)args[1].data_d,
-                                      (double *)args[2].data_d, (const double *)args[3].data_d,
-                                      set->size, s, part_sum, part_max, part_loc);
-  k_update_finish<<<1, 256, 0, st>>>(part_sum, part_max, part_loc, g, slot_rms, slot_max);
-  op2h_kernel_timer_stop(&ctx);
+    launch_k(tm.ext, tm.e0, nullptr, soa ? k_update_mixed<true> : k_update_mixed<false>, g, bs,
+             0, st, (const double *)args[0].data_d, (double *)args[1].data_d,
+             (float *)args[2].data_d, (const float *)args[3].data_d, set->size, s, part_sum,
+             part_max, part_loc);
+  } else {
+    launch_k(tm.ext, tm.e0, nullptr, soa ? k_update<true> : k_update<false>, g, bs, 0, st,
+             (const double *)args[0].data_d, (double *)args[1].data_d,
+             (double *)args[2].data_d, (const double *)args[3].data_d, set->size, s,
+             part_sum, part_max, part_loc);
+  }
+  launch_k(tm.ext, nullptr, tm.e1, k_update_finish, 1, 256, 0, st, part_sum, part_max,
+           part_loc, g, slot_rms, slot_max);
+  tm.done();
   KCHECK(hipGetLastError());
   account(4, set, 8, args);
   op2h_loop_end(&ctx);

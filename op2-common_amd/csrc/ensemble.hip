@@ -94,26 +94,35 @@ __global__ void __launch_bounds__(1024)
     ld2<false>(x, rr.y, 0, x2);
     ld4<false>(q, cell, 0, qa);
     ld4<false>(q, rr.z, 0, qb);
-    double aa = adt[cell], ab = adt[rr.z];
-    double r1[4] = {0.0, 0.0, 0.0, 0.0}, r2[4] = {0.0, 0.0, 0.0, 0.0};
-    if (rr.w == 0) {
-      op2_k::res_calc(x1, x2, qa, qb, &aa, &ab, r1, r2);
-      for (int d = 0; d < 4; ++d) r[d] = r1[d];
-    } else {
-      op2_k::res_calc(x1, x2, qb, qa, &ab, &aa, r1, r2);
-      for (int d = 0; d < 4; ++d) r[d] = r2[d];
+    // operands selected so that the wave evaluates one flux (the edge's
+    // ecell[0] side is q1 whichever side this lane's cell is)
+    const bool second = rr.w != 0;
+    double q1[4], q2[4];
+    for (int d = 0; d < 4; ++d) {
+      q1[d] = second ? qb[d] : qa[d];
+      q2[d] = second ? qa[d] : qb[d];
     }
+    const double aa = adt[cell], ab = adt[rr.z];
+    double a1 = second ? ab : aa, a2 = second ? aa : ab;
+    double r1[4] = {0.0, 0.0, 0.0, 0.0}, r2[4] = {0.0, 0.0, 0.0, 0.0};
+    op2_k::res_calc(x1, x2, q1, q2, &a1, &a2, r1, r2);
+    for (int d = 0; d < 4; ++d) r[d] = second ? r2[d] : r1[d];
   }
-  const int lead = (threadIdx.x & 63) & ~3;
+  // the quad's increments in edge order (lane 0 first), read by DPP
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (k == 0 && valid) ld4<false>(res, cell, 0, acc);
-  for (int j = 0; j < 4; ++j) {
-    double v[4];
-    for (int d = 0; d < 4; ++d) v[d] = __shfl(r[d], lead + j, 64);
-    int hv = __shfl(has, lead + j, 64);
-    if (hv)
-      for (int d = 0; d < 4; ++d) acc[d] += v[d];
+  double v[4][4];
+  for (int d = 0; d < 4; ++d) {
+    v[0][d] = quad_bcast<0>(r[d]);
+    v[1][d] = quad_bcast<1>(r[d]);
+    v[2][d] = quad_bcast<2>(r[d]);
+    v[3][d] = quad_bcast<3>(r[d]);
   }
+  const int hv[4] = {quad_bcast<0>(has), quad_bcast<1>(has), quad_bcast<2>(has),
+                     quad_bcast<3>(has)};
+  for (int j = 0; j < 4; ++j)
+    if (hv[j])
+      for (int d = 0; d < 4; ++d) acc[d] += v[j][d];
   if (k == 0 && valid) st4<false>(res, cell, 0, acc);
 }
 

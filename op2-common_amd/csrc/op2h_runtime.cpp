@@ -1302,6 +1302,18 @@ extern "C" void op2h_kernel_timer_start(op2h_loop_ctx *ctx) {
   R.pending.push_back(t);
 }
 
+extern "C" int op2h_kernel_timer_events(op2h_loop_ctx *ctx, void **start,
+                                        void **stop) {
+  Runtime &R = rt();
+  *start = *stop = nullptr;
+  if (ctx->tuning || !R.kernel_timing || R.capturing) return 0;
+  PendingTimer t{ctx->kernel_idx, take_event(), take_event()};
+  R.pending.push_back(t);
+  *start = t.a;
+  *stop = t.b;
+  return 1;
+}
+
 extern "C" void op2h_kernel_timer_stop(op2h_loop_ctx *ctx) {
   Runtime &R = rt();
   if (ctx->tuning) {
