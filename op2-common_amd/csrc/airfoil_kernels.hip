@@ -723,20 +723,29 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-// One workgroup of 256: lane l combines partials l, l+256, ... in order, then
-// the wave trees and the four wave results in wave order.
-__global__ void __launch_bounds__(256)
+// One workgroup of 1024: thread t combines partials t and t + 1024 (both
+// loads issued before either is used; the update grid is capped at 2048
+// workgroups), then any further ones in order; then the wave trees and the
+// 16 wave results in wave order.  Deterministic for a launch shape.
+constexpr int kFinishThreads = 1024;
+
+__global__ void __launch_bounds__(kFinishThreads)
     k_update_finish(const double *__restrict part_sum,
                     const double *__restrict part_max,
                     const int *__restrict part_loc, int nparts,
                     double *__restrict slot_rms, double *__restrict slot_max) {
-  __shared__ double s_sum[4];
-  __shared__ double s_max[4];
-  __shared__ int s_loc[4];
+  __shared__ double s_sum[kFinishThreads / 64];
+  __shared__ double s_max[kFinishThreads / 64];
+  __shared__ int s_loc[kFinishThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  double tot = 0.0;
-  MaxLoc tm{-INFINITY, 0x7fffffff};
-  for (int b = tid; b < nparts; b += 256) {
+  const int b0 = tid, b1 = tid + kFinishThreads;
+  const bool h0 = b0 < nparts, h1 = b1 < nparts;
+  const double s0 = h0 ? part_sum[b0] : 0.0, s1 = h1 ? part_sum[b1] : 0.0;
+  const double m0 = h0 ? part_max[b0] : -INFINITY, m1 = h1 ? part_max[b1] : -INFINITY;
+  const int l0 = h0 ? part_loc[b0] : 0x7fffffff, l1 = h1 ? part_loc[b1] : 0x7fffffff;
+  double tot = s0 + s1;
+  MaxLoc tm = maxloc_combine(MaxLoc{m0, l0}, MaxLoc{m1, l1});
+  for (int b = tid + 2 * kFinishThreads; b < nparts; b += kFinishThreads) {
     tot += part_sum[b];
     tm = maxloc_combine(tm, MaxLoc{part_max[b], part_loc[b]});
   }
@@ -751,7 +760,7 @@ __global__ void __launch_bounds__(256)
   if (tid == 0) {
     double t = 0.0;
     MaxLoc m{-INFINITY, 0x7fffffff};
-    for (int w = 0; w < 4; ++w) {
+    for (int w = 0; w < kFinishThreads / 64; ++w) {
       t += s_sum[w];
       m = maxloc_combine(m, MaxLoc{s_max[w], s_loc[w]});
     }
@@ -1472,8 +1481,8 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
              (double *)args[2].data_d, (const double *)args[3].data_d, set->size, s,
              part_sum, part_max, part_loc);
   }
-  launch_k(tm.ext, nullptr, tm.e1, k_update_finish, 1, 256, 0, st, part_sum, part_max,
-           part_loc, g, slot_rms, slot_max);
+  launch_k(tm.ext, nullptr, tm.e1, k_update_finish, 1, kFinishThreads, 0, st, part_sum,
+           part_max, part_loc, g, slot_rms, slot_max);
   tm.done();
   KCHECK(hipGetLastError());
   account(4, set, 8, args);
