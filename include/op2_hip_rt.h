@@ -80,6 +80,20 @@ int op2h_dat_soa_stride(op_dat dat);
  * into the slot; op2h_loop_end combines it with the host value. */
 void *op2h_gbl_slot(op2h_loop_ctx *ctx, int argi, int info_argi);
 
+/* Deferred gbl reductions inside a captured graph.  A stub that reduces a
+ * dim-1 double gbl arg through per-workgroup partials may, while a graph is
+ * being captured, hand the partials to the runtime instead of finishing
+ * them: op2h_gbl_partials returns 1 with device buffers for `nparts`
+ * partials, vals[nparts] and, for OP_MIN/OP_MAX with a linked dim-1 int info
+ * arg, locs[nparts] (the info payload, which must be the element index: equal
+ * values keep the lowest).  The graph's history kernel then combines them in
+ * partial order (thread t takes partials t and t + 1024, then wave trees, then
+ * the 16 waves in order) when it records the launch, so one kernel finishes
+ * every deferred reduction of a step.  Outside capture it returns 0 with
+ * NULLs, and the stub finishes the reduction into op2h_gbl_slot itself. */
+int op2h_gbl_partials(op2h_loop_ctx *ctx, int argi, int info_argi, int nparts,
+                      double **vals, int **locs);
+
 /* Device scratch for per-workgroup partials and the zero-initialised
  * completion counter of the single-pass (last-workgroup-reduces) reduction.
  * Kernels must leave the counter at zero when they finish. */
@@ -157,34 +171,29 @@ typedef struct {
  * lowest-numbered unassigned target (so blocks are compact patches whatever
  * the numbering); every element with an increment target in a block is an
  * "instance" of that block (elements on a block boundary belong to two).
- * Per block: the instances in ascending element order, LDS staging lists per
- * map (ascending unique targets), local (ushort) ids per (map, map index), the
- * owned targets in ascending order, and per owned target its increments as
- * ushort entries `local instance * ninc + k` (k = position of the increment
- * among the loop's increment args) in ascending order, i.e. the order of the
- * sequential executor. */
+ * Per block: the instances in ascending element order, the owned targets in
+ * ascending order, and per owned target its increments as ushort entries
+ * `local instance * ninc + k` (k = position of the increment among the loop's
+ * increment args) in ascending order, i.e. the order of the sequential
+ * executor.  A stub evaluates each instance once (reading its indirect data
+ * through the element's map rows) and sums each owned target's increments in
+ * list order. */
 typedef struct {
   int set_size, to_size, block_cells, nblocks;
   int ninc;                    /* increment args through the owner map */
   int total;                   /* instances over all blocks */
   int max_elems, max_owned;    /* per-block maxima */
-  int nmaps;
-  int max_ind_h[OP2H_MAX_ARGS];  /* per staged map: max unique targets */
   int *owned_offs;             /* [nblocks+1] */
   int *owned;                  /* owned targets, block by block */
   int *elem_offs;              /* [nblocks+1] */
   int *elems;                  /* [total] element of each instance */
-  int *csr_offs;               /* [owned count + 1], in `owned` order */
+  int *csr_offs;               /* [n_owned + 1], in `owned` order */
   unsigned short *csr;         /* increment entries */
-  int *ind_offs[OP2H_MAX_ARGS];  /* per staged map: [nblocks] */
-  int *ind_sizes[OP2H_MAX_ARGS]; /* per staged map: [nblocks] */
-  int *ind_map[OP2H_MAX_ARGS];   /* per staged map: unique targets */
-  unsigned short *loc_map[OP2H_MAX_ARGS]; /* per (map, index): [total] */
-  int map_of_arg[OP2H_MAX_ARGS]; /* arg -> staged map slot, -1 if direct */
-  int loc_of_arg[OP2H_MAX_ARGS]; /* arg -> loc_map slot, -1 if direct */
   int inc_of_arg[OP2H_MAX_ARGS]; /* arg -> k among increments, -1 if none */
   double redundancy;           /* total / set_size */
   double build_ms;
+  int n_owned;                 /* owned targets over all blocks */
+  int max_incs;                /* most increments of one owned target */
 } op2h_owner_plan;
 
 const op2h_staged_plan *op2h_plan_staged(op2h_loop_ctx *ctx);

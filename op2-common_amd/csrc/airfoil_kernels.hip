@@ -418,8 +418,23 @@ struct OwnerArgs {
   const int4 *inst;  // per instance: edge nodes (x, y), edge cells (z, w)
   const int *elem_offs, *owned_offs, *owned, *csr_offs;
   const unsigned short *csr;
+  const ushort4 *csr4;  // per owned cell: up to four increments, 0xFFFF = none
   int max_e;
 };
+
+// Fixed-width increment lists of an owner plan whose targets have at most
+// four increments each (quadrilateral cells): one 8-byte load per cell in
+// place of an offset pair and a list.
+__global__ void k_pack_owner_csr4(const int *__restrict csr_offs,
+                                  const unsigned short *__restrict csr, int n,
+                                  ushort4 *__restrict out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k0 = csr_offs[i], cnt = csr_offs[i + 1] - k0;
+  unsigned short v[4];
+  for (int j = 0; j < 4; ++j) v[j] = j < cnt ? csr[k0 + j] : (unsigned short)0xFFFF;
+  out[i] = make_ushort4(v[0], v[1], v[2], v[3]);
+}
 
 // Per-instance records of res_calc's owner plan: {pedge[e], pecell[e]}.
 __global__ void k_pack_owner_res(const int2 *__restrict pedge,
@@ -459,8 +474,30 @@ static OwnerArgs owner_args_res(const op2h_owner_plan *p, const op_arg *args,
   A.owned = p->owned;
   A.csr_offs = p->csr_offs;
   A.csr = p->csr;
+  A.csr4 = nullptr;
   A.max_e = p->max_elems;
+  if (p->max_incs <= 4) {
+    snprintf(key, sizeof key, "res_calc.owner4.%d.%d.%d", bs, args[0].map->index,
+             args[2].map->index);
+    ushort4 *c4 = (ushort4 *)op2h_plan_user(key, sizeof(ushort4) * (size_t)p->n_owned, &fresh);
+    if (fresh && p->n_owned)
+      k_pack_owner_csr4<<<grid_for(p->n_owned, 256), 256, 0, st>>>(p->csr_offs, p->csr,
+                                                                   p->n_owned, c4);
+    A.csr4 = c4;
+  }
   return A;
+}
+
+// Adds one owned cell's increments to acc in list order: +f for ecell[0]
+// (even entries), -f for ecell[1] (odd).
+template <class T>
+__device__ __forceinline__ void owner_apply(T *acc, const T *lf, int ent) {
+  const T *f = lf + 4 * (ent >> 1);
+  if (ent & 1) {
+    for (int d = 0; d < 4; ++d) acc[d] -= f[d];
+  } else {
+    for (int d = 0; d < 4; ++d) acc[d] += f[d];
+  }
 }
 
 // Owner-compute res_calc: one workgroup per block of owned cells.
@@ -470,7 +507,7 @@ static OwnerArgs owner_args_res(const op2h_owner_plan *p, const op_arg *args,
 //     (+f where it is ecell[0], -f where it is ecell[1]), as the sequential
 //     executor does, so res is bit-identical to it.
 // LDS: fluxes [max_e][4].
-template <bool SOA>
+template <bool SOA, bool C4>
 __global__ void __launch_bounds__(1024)
     k_res_owner(const double *__restrict x, const double *__restrict q,
                 const double *__restrict adt, double *__restrict res,
@@ -493,14 +530,15 @@ __global__ void __launch_bounds__(1024)
     const int cell = P.owned[o0 + i];
     double acc[4];
     ld4<SOA>(res, cell, sq, acc);
-    for (int k = P.csr_offs[o0 + i]; k < P.csr_offs[o0 + i + 1]; ++k) {
-      const int ent = P.csr[k];
-      const double *f = lf + 4 * (ent >> 1);
-      if (ent & 1) {
-        for (int d = 0; d < 4; ++d) acc[d] -= f[d];
-      } else {
-        for (int d = 0; d < 4; ++d) acc[d] += f[d];
-      }
+    if (C4) {
+      const ushort4 e = P.csr4[o0 + i];
+      if (e.x != 0xFFFF) owner_apply(acc, lf, e.x);
+      if (e.y != 0xFFFF) owner_apply(acc, lf, e.y);
+      if (e.z != 0xFFFF) owner_apply(acc, lf, e.z);
+      if (e.w != 0xFFFF) owner_apply(acc, lf, e.w);
+    } else {
+      for (int k = P.csr_offs[o0 + i]; k < P.csr_offs[o0 + i + 1]; ++k)
+        owner_apply(acc, lf, (int)P.csr[k]);
     }
     st4<SOA>(res, cell, sq, acc);
   }
@@ -909,7 +947,7 @@ __global__ void __launch_bounds__(1024)
 
 // Mixed precision (SP fluxes, FP32 adt and res; x and q read as FP64 and
 // rounded), same structure as k_res_owner.
-template <bool SOA>
+template <bool SOA, bool C4>
 __global__ void __launch_bounds__(1024)
     k_res_owner_mixed(const double *__restrict x, const double *__restrict q,
                       const float *__restrict adt, float *__restrict res,
@@ -932,14 +970,15 @@ __global__ void __launch_bounds__(1024)
     const int cell = P.owned[o0 + i];
     float acc[4];
     ld4s<SOA>(res, cell, sq, acc);
-    for (int k = P.csr_offs[o0 + i]; k < P.csr_offs[o0 + i + 1]; ++k) {
-      const int ent = P.csr[k];
-      const float *f = lff + 4 * (ent >> 1);
-      if (ent & 1) {
-        for (int d = 0; d < 4; ++d) acc[d] -= f[d];
-      } else {
-        for (int d = 0; d < 4; ++d) acc[d] += f[d];
-      }
+    if (C4) {
+      const ushort4 e = P.csr4[o0 + i];
+      if (e.x != 0xFFFF) owner_apply(acc, lff, e.x);
+      if (e.y != 0xFFFF) owner_apply(acc, lff, e.y);
+      if (e.z != 0xFFFF) owner_apply(acc, lff, e.z);
+      if (e.w != 0xFFFF) owner_apply(acc, lff, e.w);
+    } else {
+      for (int k = P.csr_offs[o0 + i]; k < P.csr_offs[o0 + i + 1]; ++k)
+        owner_apply(acc, lff, (int)P.csr[k]);
     }
     st4s<SOA>(res, cell, sq, acc);
   }
@@ -1184,8 +1223,10 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
         const int nblocks = owner_p->nblocks;
         const size_t lds = sizeof(float) * 4 * (size_t)OA.max_e;
         if (nblocks) {
-          if (soa) k_res_owner_mixed<true><<<nblocks, bs, lds, st>>>(x, q, adtf, resf, OA, sx, sq);
-          else k_res_owner_mixed<false><<<nblocks, bs, lds, st>>>(x, q, adtf, resf, OA, sx, sq);
+          auto kern = OA.csr4 ? (soa ? k_res_owner_mixed<true, true> : k_res_owner_mixed<false, true>)
+                              : (soa ? k_res_owner_mixed<true, false> : k_res_owner_mixed<false, false>);
+          launch_k(false, nullptr, nullptr, kern, nblocks, bs, lds, st, x, q, adtf, resf, OA,
+                   sx, sq);
         }
       } else {
         const op2h_gather_plan *p = op2h_plan_gather(args[6].map);
@@ -1276,7 +1317,9 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
         }
         LoopTimer tm(&ctx);
         if (p->nblocks)
-          launch_k(tm.ext, tm.e0, tm.e1, soa ? k_res_owner<true> : k_res_owner<false>,
+          launch_k(tm.ext, tm.e0, tm.e1,
+                   A.csr4 ? (soa ? k_res_owner<true, true> : k_res_owner<false, true>)
+                          : (soa ? k_res_owner<true, false> : k_res_owner<false, false>),
                    p->nblocks, bs, lds, st, x, q, adt, res, A, sx, sq);
         tm.done();
         break;
@@ -1451,8 +1494,6 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
   op2h_loop_ctx ctx;
   op2h_loop_begin(&ctx, 4, name, set, 8, args, OP2H_PLAN_ATOMIC, 256);
   hipStream_t st = (hipStream_t)ctx.stream;
-  double *slot_rms = (double *)op2h_gbl_slot(&ctx, 4, -1);
-  double *slot_max = (double *)op2h_gbl_slot(&ctx, 5, 7);
   int dummy;
   bool soa = loop_soa(args, 8, &dummy);
   int s = set_stride(args[0]);
@@ -1461,10 +1502,24 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
   // strides over the set when it is larger
   int g = grid_for(set->size, bs);
   g = std::max(1, std::min(g, 2048));
-  char *scratch = (char *)op2h_reduct_scratch((size_t)g * 24);
-  double *part_sum = (double *)scratch;
-  double *part_max = part_sum + g;
-  int *part_loc = (int *)(part_max + g);
+  // inside a captured graph the partials go to the runtime, whose history
+  // kernel finishes both reductions of every update of the step in one pass
+  // (same tree as k_update_finish); otherwise finish them here
+  double *part_sum = nullptr, *part_max = nullptr, *slot_rms = nullptr, *slot_max = nullptr;
+  int *part_loc = nullptr, *unused = nullptr;
+  const bool deferred = op2h_gbl_partials(&ctx, 4, -1, g, &part_sum, &unused) != 0;
+  if (deferred && !op2h_gbl_partials(&ctx, 5, 7, g, &part_max, &part_loc)) {
+    printf("op2hip: update: maxerr/errloc cannot be deferred with rms\n");
+    exit(-1);
+  }
+  if (!deferred) {
+    slot_rms = (double *)op2h_gbl_slot(&ctx, 4, -1);
+    slot_max = (double *)op2h_gbl_slot(&ctx, 5, 7);
+    char *scratch = (char *)op2h_reduct_scratch((size_t)g * 24);
+    part_sum = (double *)scratch;
+    part_max = part_sum + g;
+    part_loc = (int *)(part_max + g);
+  }
   LoopTimer tm(&ctx);
   if (is_float(args[2]) != is_float(args[3])) {
     printf("op2hip: update needs res and adt both FP32 (mixed mode) or both FP64\n");
@@ -1481,8 +1536,9 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
              (double *)args[2].data_d, (const double *)args[3].data_d, set->size, s,
              part_sum, part_max, part_loc);
   }
-  launch_k(tm.ext, nullptr, tm.e1, k_update_finish, 1, kFinishThreads, 0, st, part_sum,
-           part_max, part_loc, g, slot_rms, slot_max);
+  if (!deferred)
+    launch_k(tm.ext, nullptr, tm.e1, k_update_finish, 1, kFinishThreads, 0, st, part_sum,
+             part_max, part_loc, g, slot_rms, slot_max);
   tm.done();
   KCHECK(hipGetLastError());
   account(4, set, 8, args);

@@ -101,7 +101,6 @@ std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
                                                int nargs, const op_arg *args,
                                                hipStream_t stream,
                                                std::string &err) {
-  LoopShape sh = loop_shape(set, nargs, args);
   auto rec = std::make_unique<OwnerPlanRec>();
   op2h_owner_plan &P = rec->plan;
   // the increment args: all through one map into one dat
@@ -219,52 +218,6 @@ std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
         }
   }
 
-  // staging lists per map and local ids per (map, index)
-  P.nmaps = (int)sh.maps.size();
-  for (int m = 0; m < nargs; ++m) {
-    P.map_of_arg[m] = sh.map_slot[m];
-    P.loc_of_arg[m] = sh.loc_slot[m];
-  }
-  std::vector<std::vector<unsigned short>> loc(sh.locs.size(),
-                                               std::vector<unsigned short>(total));
-  for (int s = 0; s < P.nmaps; ++s) {
-    op_map M = sh.maps[s];
-    std::vector<int> idxs;  // map indices of this map the loop uses
-    std::vector<int> lslots;
-    for (size_t l = 0; l < sh.locs.size(); ++l)
-      if (sh.locs[l].first == M) {
-        idxs.push_back(sh.locs[l].second);
-        lslots.push_back((int)l);
-      }
-    std::vector<int> offs(nb), sizes(nb), list, pos(M->to->size, -1), work;
-    int mx = 0;
-    for (int b = 0; b < nb; ++b) {
-      work.clear();
-      for (int j = elem_offs[b]; j < elem_offs[b + 1]; ++j)
-        for (int i : idxs) work.push_back(M->map[(size_t)elems[j] * M->dim + i]);
-      std::sort(work.begin(), work.end());
-      work.erase(std::unique(work.begin(), work.end()), work.end());
-      if (work.size() > 65535) {
-        err = "owner plan: a block stages too many targets for 16-bit local ids";
-        return nullptr;
-      }
-      offs[b] = (int)list.size();
-      sizes[b] = (int)work.size();
-      mx = std::max(mx, sizes[b]);
-      for (size_t i = 0; i < work.size(); ++i) pos[work[i]] = (int)i;
-      list.insert(list.end(), work.begin(), work.end());
-      for (int j = elem_offs[b]; j < elem_offs[b + 1]; ++j)
-        for (size_t q = 0; q < idxs.size(); ++q)
-          loc[lslots[q]][j] =
-              (unsigned short)pos[M->map[(size_t)elems[j] * M->dim + idxs[q]]];
-      for (int t : work) pos[t] = -1;
-    }
-    P.max_ind_h[s] = mx;
-    P.ind_offs[s] = rec->mem.upload(offs, stream);
-    P.ind_sizes[s] = rec->mem.upload(sizes, stream);
-    P.ind_map[s] = rec->mem.upload(list, stream);
-  }
-  for (size_t l = 0; l < loc.size(); ++l) P.loc_map[l] = rec->mem.upload(loc[l], stream);
   P.owned_offs = rec->mem.upload(owned_offs, stream);
   P.owned = rec->mem.upload(owned, stream);
   P.elem_offs = rec->mem.upload(elem_offs, stream);
@@ -273,6 +226,10 @@ std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
   P.csr = rec->mem.upload(csr, stream);
   rec->n_owned = (int)owned.size();
   rec->n_csr = (int)csr.size();
+  P.n_owned = rec->n_owned;
+  P.max_incs = 0;
+  for (size_t i = 0; i < owned.size(); ++i)
+    P.max_incs = std::max(P.max_incs, csr_offs[i + 1] - csr_offs[i]);
   OCHECK(hipStreamSynchronize(stream));
   return rec;
 }

@@ -131,6 +131,15 @@ struct Runtime {
   hipGraphExec_t graph_exec = nullptr;
   std::vector<DeferredGbl> deferred;
   std::vector<std::pair<void *, size_t>> history_sources;  // device slots
+  // deferred reductions: slot -> (kind 1 sum / 2 max / 3 min, nparts, vals, locs)
+  struct Partials {
+    int kind, nparts;
+    const double *vals;
+    const int *locs;
+  };
+  std::map<void *, Partials> partial_of_slot;
+  char *partial_pool = nullptr;
+  size_t partial_pool_bytes = 0, partial_pool_next = 0;
   double *history = nullptr;
   unsigned long long *history_count = nullptr;
   int history_words = 0;   // words per launch
@@ -1461,6 +1470,41 @@ extern "C" void *op2h_gbl_slot(op2h_loop_ctx *ctx, int argi, int info_argi) {
   return slot.first;
 }
 
+extern "C" int op2h_gbl_partials(op2h_loop_ctx *ctx, int argi, int info_argi,
+                                 int nparts, double **vals, int **locs) {
+  Runtime &R = rt();
+  *vals = nullptr;
+  *locs = nullptr;
+  const op_arg &a = ctx->args[argi];
+  if (!R.capturing || !R.partial_pool || nparts < 1 || a.dim != 1 ||
+      type_code(a.type) != 0)
+    return 0;
+  const bool extremum = a.acc == OP_MIN || a.acc == OP_MAX;
+  if (a.acc != OP_INC && !extremum) return 0;
+  if (extremum && info_argi >= 0 &&
+      (ctx->args[info_argi].dim != 1 || type_code(ctx->args[info_argi].type) != 2))
+    return 0;
+  const size_t need = (size_t)nparts * (sizeof(double) + (extremum ? sizeof(int) : 0));
+  const size_t at = (R.partial_pool_next + 255) / 256 * 256;
+  if (at + need > R.partial_pool_bytes) {
+    printf("op2hip: deferred reductions of one graph exceed %zu bytes\n",
+           R.partial_pool_bytes);
+    exit(-1);
+  }
+  R.partial_pool_next = at + need;
+  void *slot = op2h_gbl_slot(ctx, argi, info_argi);
+  Runtime::Partials P;
+  P.kind = a.acc == OP_INC ? 1 : a.acc == OP_MAX ? 2 : 3;
+  P.nparts = nparts;
+  P.vals = (const double *)(R.partial_pool + at);
+  P.locs = extremum ? (const int *)(R.partial_pool + at + (size_t)nparts * sizeof(double))
+                    : nullptr;
+  R.partial_of_slot[slot] = P;
+  *vals = (double *)P.vals;
+  *locs = (int *)P.locs;
+  return 1;
+}
+
 extern "C" void *op2h_reduct_scratch(size_t bytes) {
   Runtime &R = rt();
   if (bytes > R.red_scratch_bytes) {
@@ -1688,29 +1732,96 @@ constexpr int kSlotPool = 64;
 constexpr int kHistSources = 16;
 constexpr size_t kHistoryBytes = 64u << 20;
 
+constexpr int kHistThreads = 1024;
+
 struct HistSources {
-  const unsigned long long *p[kHistSources];
+  const unsigned long long *p[kHistSources];  // slot words, or partial values
+  const int *locs[kHistSources];              // partial payloads (max/min)
   int words[kHistSources];
+  int kind[kHistSources];    // 0 copy the slot, 1 sum, 2 max, 3 min of partials
+  int nparts[kHistSources];
   int n;
 };
 
-// Appends the gbl result slots recorded in a graph to the device history:
-// record r of the history holds all slot words of launch r, in record order.
-// One workgroup; the launch counter is read by every lane before the barrier
-// and advanced by lane 0 after it.
-__global__ void history_append(HistSources src, double *hist,
-                               unsigned long long *count, int capacity,
-                               int total_words) {
-  unsigned long long c = *count;
-  size_t base = (size_t)(c % (unsigned long long)capacity) * total_words;
+struct ValLoc {
+  double v;
+  int loc;
+};
+
+// kind 2: larger wins; kind 3: smaller wins; equal values keep the lower loc
+__device__ __forceinline__ ValLoc vl_combine(ValLoc a, ValLoc b, int kind) {
+  const bool better = kind == 2 ? b.v > a.v : b.v < a.v;
+  if (better || (b.v == a.v && b.loc < a.loc)) return b;
+  return a;
+}
+
+// Appends the gbl results recorded in a graph to the device history: record
+// r holds all result words of launch r, in record order.  Slot sources are
+// copied; deferred reductions are finished here from their partials (one
+// pass per source over the workgroup).  The launch counter is read by every
+// lane before the first barrier and advanced by lane 0 after the last.
+__global__ void __launch_bounds__(kHistThreads)
+    history_append(HistSources src, double *hist, unsigned long long *count,
+                   int capacity, int total_words) {
+  __shared__ double s_v[kHistThreads / 64];
+  __shared__ int s_l[kHistThreads / 64];
+  const unsigned long long c = *count;
+  const size_t base = (size_t)(c % (unsigned long long)capacity) * total_words;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int w = 0;
   for (int s = 0; s < src.n; ++s) {
-    for (int k = threadIdx.x; k < src.words[s]; k += blockDim.x)
-      hist[base + w + k] = __longlong_as_double((long long)src.p[s][k]);
+    const int kind = src.kind[s];
+    if (kind == 0) {
+      for (int k = tid; k < src.words[s]; k += blockDim.x)
+        hist[base + w + k] = __longlong_as_double((long long)src.p[s][k]);
+    } else {
+      const double *v = reinterpret_cast<const double *>(src.p[s]);
+      const int *l = src.locs[s];
+      const int n = src.nparts[s];
+      const double ident = kind == 1 ? 0.0 : kind == 2 ? -INFINITY : INFINITY;
+      const int b0 = tid, b1 = tid + kHistThreads;
+      ValLoc a{b0 < n ? v[b0] : ident, (kind != 1 && b0 < n) ? l[b0] : 0x7fffffff};
+      ValLoc b{b1 < n ? v[b1] : ident, (kind != 1 && b1 < n) ? l[b1] : 0x7fffffff};
+      ValLoc m;
+      if (kind == 1) {
+        m.v = a.v + b.v;
+        m.loc = 0;
+        for (int p = tid + 2 * kHistThreads; p < n; p += kHistThreads) m.v += v[p];
+        for (int o = 32; o > 0; o >>= 1) m.v += __shfl_xor(m.v, o, 64);
+      } else {
+        m = vl_combine(a, b, kind);
+        for (int p = tid + 2 * kHistThreads; p < n; p += kHistThreads)
+          m = vl_combine(m, ValLoc{v[p], l[p]}, kind);
+        for (int o = 32; o > 0; o >>= 1) {
+          ValLoc t{__shfl_xor(m.v, o, 64), __shfl_xor(m.loc, o, 64)};
+          m = vl_combine(m, t, kind);
+        }
+      }
+      if (lane == 0) {
+        s_v[wave] = m.v;
+        s_l[wave] = m.loc;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        ValLoc r{kind == 1 ? 0.0 : ident, 0x7fffffff};
+        for (int k = 0; k < kHistThreads / 64; ++k) {
+          if (kind == 1)
+            r.v += s_v[k];
+          else
+            r = vl_combine(r, ValLoc{s_v[k], s_l[k]}, kind);
+        }
+        hist[base + w] = r.v;
+        if (src.words[s] > 1) {
+          long long word = (long long)(unsigned int)r.loc;  // int in the low half
+          hist[base + w + 1] = __longlong_as_double(word);
+        }
+      }
+      __syncthreads();
+    }
     w += src.words[s];
   }
   __syncthreads();
-  if (threadIdx.x == 0) *count = c + 1;
+  if (tid == 0) *count = c + 1;
 }
 }  // namespace
 
@@ -1731,6 +1842,10 @@ extern "C" int op2h_graph_begin(void) {
     R.capture_slots.push_back(slot);
   }
   R.capture_slot_next = 0;
+  R.partial_pool_bytes = 16u << 20;
+  HIPCHECK(hipMalloc(&R.partial_pool, R.partial_pool_bytes));
+  R.partial_pool_next = 0;
+  R.partial_of_slot.clear();
   HIPCHECK(hipMalloc(&R.history, kHistoryBytes));
   HIPCHECK(hipMalloc(&R.history_count, sizeof(unsigned long long)));
   HIPCHECK(hipMemset(R.history_count, 0, sizeof(unsigned long long)));
@@ -1755,6 +1870,14 @@ extern "C" int op2h_graph_end(void) {
   for (auto &s : R.history_sources) {
     src.p[src.n] = (const unsigned long long *)s.first;
     src.words[src.n] = (int)((s.second + 7) / 8);
+    src.kind[src.n] = 0;
+    auto pi = R.partial_of_slot.find(s.first);
+    if (pi != R.partial_of_slot.end()) {
+      src.p[src.n] = (const unsigned long long *)pi->second.vals;
+      src.locs[src.n] = pi->second.locs;
+      src.kind[src.n] = pi->second.kind;
+      src.nparts[src.n] = pi->second.nparts;
+    }
     words += src.words[src.n];
     src.n++;
   }
@@ -1762,8 +1885,8 @@ extern "C" int op2h_graph_end(void) {
   R.history_capacity =
       words ? std::max(1, (int)(kHistoryBytes / (words * sizeof(double)))) : 0;
   if (words)
-    history_append<<<1, 64, 0, R.stream>>>(src, R.history, R.history_count,
-                                           R.history_capacity, (int)words);
+    history_append<<<1, kHistThreads, 0, R.stream>>>(src, R.history, R.history_count,
+                                                     R.history_capacity, (int)words);
   hipGraph_t g;
   HIPCHECK(hipStreamEndCapture(R.stream, &g));
   R.capturing = false;
@@ -1842,6 +1965,10 @@ extern "C" void op2h_graph_destroy(void) {
   R.capture_slot_next = 0;
   R.deferred.clear();
   R.history_sources.clear();
+  R.partial_of_slot.clear();
+  if (R.partial_pool) (void)hipFree(R.partial_pool);
+  R.partial_pool = nullptr;
+  R.partial_pool_bytes = R.partial_pool_next = 0;
   if (R.history) (void)hipFree(R.history);
   if (R.history_count) (void)hipFree(R.history_count);
   R.history = nullptr;

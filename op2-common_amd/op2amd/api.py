@@ -133,16 +133,13 @@ class op_color_plan(C.Structure):
 class op_owner_plan(C.Structure):
     _fields_ = [("set_size", C.c_int), ("to_size", C.c_int), ("block_cells", C.c_int),
                 ("nblocks", C.c_int), ("ninc", C.c_int), ("total", C.c_int),
-                ("max_elems", C.c_int), ("max_owned", C.c_int), ("nmaps", C.c_int),
-                ("max_ind_h", C.c_int * 32),
+                ("max_elems", C.c_int), ("max_owned", C.c_int),
                 ("owned_offs", C.c_void_p), ("owned", C.c_void_p),
                 ("elem_offs", C.c_void_p), ("elems", C.c_void_p),
                 ("csr_offs", C.c_void_p), ("csr", C.c_void_p),
-                ("ind_offs", C.c_void_p * 32), ("ind_sizes", C.c_void_p * 32),
-                ("ind_map", C.c_void_p * 32), ("loc_map", C.c_void_p * 32),
-                ("map_of_arg", C.c_int * 32), ("loc_of_arg", C.c_int * 32),
                 ("inc_of_arg", C.c_int * 32),
-                ("redundancy", C.c_double), ("build_ms", C.c_double)]
+                ("redundancy", C.c_double), ("build_ms", C.c_double),
+                ("n_owned", C.c_int), ("max_incs", C.c_int)]
 
 
 op2h_plan_owner_for = _fn("op2h_plan_owner_for", C.POINTER(op_owner_plan),

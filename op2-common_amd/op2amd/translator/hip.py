@@ -5,9 +5,13 @@ the op_par_loop stub that drives it through the runtime ABI
 
 What a stub does, per loop (the reference's HIP target does the same job with
 its own runtime, translator-v2/resources/templates/hip/loop_host.hpp.jinja):
-- plans: loops with indirect OP_INC args run with atomics (default, as the
-  reference's `atomics` option) or a flat colouring, one launch per colour;
-  loops with indirect OP_RW/OP_WRITE args run coloured;
+- plans: loops whose increments all go through one map into one dat, and
+  whose other side effects are idempotent, run the runtime's owner partition
+  by default (each block evaluates the elements touching its targets once,
+  increments go to LDS, each owned target sums them in element order: no
+  atomics, deterministic, the sequential executor's order); atomics (the
+  reference's `atomics` option) and a flat colouring, one launch per colour,
+  are selectable; loops with indirect OP_RW/OP_WRITE args run coloured;
 - dats in SoA layout are copied into registers around the user kernel call
   (the user kernel always sees AoS pointers, no index rewriting);
 - gbl OP_READ values travel by value in the kernel arguments; OP_INC/OP_MIN/
@@ -20,7 +24,7 @@ import math
 
 from .op import ArgDat, ArgGbl, ArgIdx, ArgInfo
 
-PLAN_ATOMIC, PLAN_COLOR = 1, 2
+PLAN_ATOMIC, PLAN_COLOR, PLAN_OWNER = 1, 2, 5
 
 
 def _ctype(t):
@@ -67,12 +71,30 @@ class LoopGen:
                 ia = loop.args[self.info_of[j]]
                 off += _pad8(ia.dim * _size(ia.typ)) + 8
         self.rec = max(off, 8)
+        self.owner = self._owner_eligible()
         if loop.indirect_writes():
             self.def_plan, self.mask = PLAN_COLOR, 1 << PLAN_COLOR
         elif loop.indirect_incs():
             self.def_plan, self.mask = PLAN_ATOMIC, (1 << PLAN_ATOMIC) | (1 << PLAN_COLOR)
+            if self.owner:
+                # owner-compute: no atomics, deterministic sums in element order
+                self.def_plan = PLAN_OWNER
+                self.mask |= 1 << PLAN_OWNER
         else:
             self.def_plan, self.mask = PLAN_ATOMIC, 1 << PLAN_ATOMIC
+
+    def _owner_eligible(self):
+        """The owner partition evaluates elements on a block boundary twice, so
+        every side effect besides the increments must be idempotent: only
+        OP_READ/OP_WRITE direct dats, indirect reads, gbl reads and indices,
+        and increments all through one map into one dat."""
+        incs = self.L.indirect_incs()
+        if not incs or self.L.indirect_writes() or self.reds:
+            return False
+        if len({(a.dat, a.map) for a in incs}) != 1:
+            return False
+        return all(a.acc in ("OP_READ", "OP_WRITE") for a in self.L.args
+                   if isinstance(a, ArgDat) and a.direct)
 
     @property
     def name(self):
@@ -204,6 +226,82 @@ class LoopGen:
         body.append("}")
         return "\n".join(body) + "\n"
 
+    def owner_kernel(self):
+        """Owner-partition variant: phase 1 evaluates each instance of the
+        block once with its increments in registers and stores them in LDS;
+        phase 2 adds each owned target's increments in list (element) order."""
+        if not self.owner:
+            return ""
+        L = self.L
+        incs = [j for j, a in enumerate(L.args) if isinstance(a, ArgDat) and not a.direct
+                and a.acc == "OP_INC"]
+        T, D, ninc = L.args[incs[0]].typ, L.args[incs[0]].dim, len(incs)
+        pre, call, post = [], [], []
+        for j, a in enumerate(L.args):
+            if isinstance(a, ArgDat):
+                tgt = "n" if a.direct else f"A.m{j}[(size_t)n * A.md{j} + {a.idx}]"
+                pre.append(f"    {a.typ} *p{j} = nullptr;")
+                pre.append(f"    {a.typ} b{j}[{a.dim}];")
+                pre.append(f"    const int t{j} = A.opt{j} ? {tgt} : 0;")
+                if j in incs:
+                    k = incs.index(j)
+                    pre.append(f"    for (int k = 0; k < {a.dim}; ++k) b{j}[k] = ({a.typ})0;")
+                    pre.append(f"    p{j} = b{j};")
+                    post.append(f"    for (int k = 0; k < {D}; ++k) lds[((size_t)j * {ninc} + {k}) * {D} + k] = b{j}[k];")
+                else:
+                    pre.append(f"    if (A.opt{j}) {{")
+                    pre.append(f"      if (A.s{j}) {{")
+                    pre.append(f"        for (int k = 0; k < {a.dim}; ++k) b{j}[k] = A.d{j}[t{j} + (size_t)k * A.s{j}];")
+                    pre.append(f"        p{j} = b{j};")
+                    pre.append(f"      }} else {{")
+                    pre.append(f"        p{j} = A.d{j} + (size_t)t{j} * {a.dim};")
+                    pre.append(f"      }}")
+                    pre.append(f"    }}")
+                    if a.acc != "OP_READ":
+                        post.append(f"    if (A.opt{j} && A.s{j})")
+                        post.append(f"      for (int k = 0; k < {a.dim}; ++k) A.d{j}[t{j} + (size_t)k * A.s{j}] = b{j}[k];")
+                call.append(f"p{j}")
+            elif isinstance(a, ArgGbl):
+                pre.append(f"    {a.typ} g{j}[{a.dim}];")
+                pre.append(f"    for (int k = 0; k < {a.dim}; ++k) g{j}[k] = A.g{j}[k];")
+                call.append(f"A.opt{j} ? g{j} : nullptr")
+            elif isinstance(a, ArgIdx):
+                if a.map:
+                    pre.append(f"    int x{j} = A.m{j}[(size_t)n * A.md{j} + {a.idx}];")
+                else:
+                    pre.append(f"    int x{j} = n;")
+                call.append(f"&x{j}")
+        w = incs[0]
+        b = [f"__global__ void __launch_bounds__(1024)",
+             f"    {self.name}_owner({self.name}_args A, const int *__restrict elem_offs,",
+             f"                 const int *__restrict elems, const int *__restrict owned_offs,",
+             f"                 const int *__restrict owned, const int *__restrict csr_offs,",
+             f"                 const unsigned short *__restrict csr) {{",
+             f"  extern __shared__ unsigned char lds_raw[];",
+             f"  {T} *lds = reinterpret_cast<{T} *>(lds_raw);",
+             f"  const int blk = blockIdx.x;",
+             f"  const int e0 = elem_offs[blk], ne = elem_offs[blk + 1] - e0;",
+             f"  for (int j = threadIdx.x; j < ne; j += blockDim.x) {{",
+             f"    const int n = elems[e0 + j];"]
+        b += pre
+        b.append(f"    {self.ns}::{L.kernel}({', '.join(call)});")
+        b += post
+        b += [f"  }}",
+              f"  __syncthreads();",
+              f"  const int o0 = owned_offs[blk], no = owned_offs[blk + 1] - o0;",
+              f"  for (int i = threadIdx.x; i < no; i += blockDim.x) {{",
+              f"    const int t = owned[o0 + i];",
+              f"    {T} acc[{D}];",
+              f"    for (int k = 0; k < {D}; ++k) acc[k] = A.d{w}[A.s{w} ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k];",
+              f"    for (int c = csr_offs[o0 + i]; c < csr_offs[o0 + i + 1]; ++c) {{",
+              f"      const {T} *inc = lds + (size_t)csr[c] * {D};",
+              f"      for (int k = 0; k < {D}; ++k) acc[k] += inc[k];",
+              f"    }}",
+              f"    for (int k = 0; k < {D}; ++k) A.d{w}[A.s{w} ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k] = acc[k];",
+              f"  }}",
+              f"}}"]
+        return "\n".join(b) + "\n"
+
     def finish(self):
         if not self.reds:
             return ""
@@ -278,6 +376,30 @@ class LoopGen:
                 s.append(f"    A.m{j} = args[{j}].map_data_d;")
                 s.append(f"    A.md{j} = args[{j}].map->dim;")
         s.append(f"    const int bs = ctx.block_size;")
+        if self.owner:
+            L2 = self.L
+            incs = [j for j, a in enumerate(L2.args) if isinstance(a, ArgDat) and not a.direct
+                    and a.acc == "OP_INC"]
+            T, D = L2.args[incs[0]].typ, L2.args[incs[0]].dim
+            s.append(f"    if (ctx.plan == {PLAN_OWNER}) {{")
+            s.append(f"      ctx.part_size = bs;")
+            s.append(f"      const op2h_owner_plan *op = op2h_plan_owner(&ctx);")
+            s.append(f"      if (!op) exit(-1);")
+            s.append(f"      const size_t lds = (size_t)op->max_elems * {len(incs)} * {D} * sizeof({T});")
+            s.append(f"      if (lds > 160 * 1024) {{")
+            s.append(f'        printf("op2hip: %s: owner plan needs %zu bytes of LDS; use a smaller block\\n", name, lds);')
+            s.append(f"        exit(-1);")
+            s.append(f"      }}")
+            s.append(f"      op2h_kernel_timer_start(&ctx);")
+            s.append(f"      if (op->nblocks)")
+            s.append(f"        {self.name}_owner<<<op->nblocks, bs, lds, st>>>(A, op->elem_offs, op->elems,")
+            s.append(f"            op->owned_offs, op->owned, op->csr_offs, op->csr);")
+            s.append(f"      op2h_kernel_timer_stop(&ctx);")
+            s.append(f"      OP2GEN_CHECK(hipGetLastError());")
+            s.append(f"      OP_kernels[{L2.index - 1}].transfer += (float)op2h_algorithmic_bytes(set, {len(L2.args)}, args);")
+            s.append(f"      op2h_loop_end(&ctx);")
+            s.append(f"      return;")
+            s.append(f"    }}")
         s.append(f"    const op2h_color_plan *cp = ctx.plan == {PLAN_COLOR} ? op2h_plan_color(&ctx) : nullptr;")
         s.append(f"    int nparts = 0;")
         s.append(f"    if (cp) {{")
@@ -430,6 +552,7 @@ def generate(prog):
         out.append("namespace {\n")
         out.append(g.args_struct())
         out.append(g.kernel())
+        out.append(g.owner_kernel())
         out.append(g.finish())
         out.append("}  // namespace\n\n")
         out.append(g.stub())

@@ -96,3 +96,19 @@ def test_autotuned_run_matches_reference(mode):
         assert plan in (1, 2, 3, 4) and block in (128, 256, 512, 1024), loop
     if mode == "graphed":
         assert r["eager_iterations"] >= 2
+
+
+def test_graphed_reductions_equal_eager_ones():
+    """Inside a captured step the update's rms/maxerr partials are finished by
+    the graph's history kernel with the same reduction tree as the eager
+    finish kernel: every iteration's rms, maxerr and errloc agree bit for bit
+    between the tuned (eager) and graphed runs of the same plans."""
+    g = golden("airfoil_120x60_rcm_a1")
+    kw = dict(plan_res="owner", block_res=128, plan_bres="gather")
+    eager = op2amd.airfoil.run(mesh_of(g), 200, mode="tuned", **kw)
+    graphed = op2amd.airfoil.run(mesh_of(g), 200, mode="graphed", **kw)
+    assert np.array_equal(eager["rms"], graphed["rms"])
+    assert np.array_equal(eager["rms_k0"], graphed["rms_k0"])
+    assert np.array_equal(eager["maxerr"], graphed["maxerr"])
+    assert np.array_equal(eager["errloc"], graphed["errloc"])
+    assert np.array_equal(eager["q"], graphed["q"])

@@ -192,6 +192,8 @@ def test_owner_plan_invariants(ordering, block):
             seen[e, side] += 1
     assert np.all(seen == 1)
     assert abs(P.redundancy - tot / mesh.nedge) < 1e-12
+    assert P.n_owned == mesh.ncell
+    assert P.max_incs == int(np.max(np.diff(csr_offs))) == 4  # quads: four edges
     assert P.redundancy < 1.35
 
 

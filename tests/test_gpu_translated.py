@@ -1,8 +1,9 @@
 """Programs translated by op2amd.translator and built by tests/apps/build.py
 (run by __graft_entry__.build()), on the GPU:
 - the functional programs (global reductions, dat increments, op_arg_idx,
-  SoA strides, constants) with the default plans, with every dat in SoA
-  layout (OP_AUTO_SOA) and with coloured increments instead of atomics;
+  SoA strides, constants) with the default plans (owner partition for
+  increments), with every dat in SoA layout (OP_AUTO_SOA), and with
+  coloured or atomic increments;
 - the translated airfoil program against the reference-pinned trajectory."""
 import subprocess
 
@@ -24,8 +25,9 @@ def run(name, *args, timeout=120):
     return r
 
 
-@pytest.mark.parametrize("variant", [[], ["OP_AUTO_SOA"], ["OP_HIP_PLAN=*:color"]],
-                         ids=["default", "soa", "color"])
+@pytest.mark.parametrize("variant", [[], ["OP_AUTO_SOA"], ["OP_HIP_PLAN=*:color"],
+                                     ["OP_HIP_PLAN=*:atomic"]],
+                         ids=["default", "soa", "color", "atomic"])
 @pytest.mark.parametrize("name", ["gbl", "reductions", "idx", "strides", "consts"])
 def test_functional_program(name, variant):
     r = run(name, *variant)
@@ -33,8 +35,9 @@ def test_functional_program(name, variant):
     assert f"{name}: all passed" in r.stdout
 
 
-@pytest.mark.parametrize("variant", [[], ["OP_HIP_PLAN=*:color"], ["OP_AUTO_SOA"]],
-                         ids=["atomic", "color", "soa"])
+@pytest.mark.parametrize("variant", [[], ["OP_HIP_PLAN=*:atomic"], ["OP_HIP_PLAN=*:color"],
+                                     ["OP_AUTO_SOA"]],
+                         ids=["owner", "atomic", "color", "soa"])
 def test_translated_airfoil_matches_reference_trajectory(variant, tmp_path):
     g = golden("airfoil_120x60_random_a1")
     mesh = op2amd.mesh.generate(int(g["I"]), int(g["J"]), 0, str(g["ordering"]))
