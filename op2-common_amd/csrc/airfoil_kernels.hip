@@ -28,6 +28,7 @@
 
 #include "op2_hip_rt.h"
 #include "airfoil_device.h"
+#include "op2h_reduce.h"
 
 #define KCHECK(x)                                                            \
   do {                                                                       \
@@ -761,50 +762,31 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-// One workgroup of 1024: thread t combines partials t and t + 1024 (both
-// loads issued before either is used; the update grid is capped at 2048
-// workgroups), then any further ones in order; then the wave trees and the
-// 16 wave results in wave order.  Deterministic for a launch shape.
-constexpr int kFinishThreads = 1024;
+// Two groups of 256 threads: group 0 finishes rms (sum), group 1 maxerr and
+// errloc (max, lowest element on ties), both with op2h_red::group_reduce, the
+// reduction the graph's history kernel applies to deferred partials.
+constexpr int kFinishThreads = 2 * op2h_red::kRedThreads;
 
 __global__ void __launch_bounds__(kFinishThreads)
     k_update_finish(const double *__restrict part_sum,
                     const double *__restrict part_max,
                     const int *__restrict part_loc, int nparts,
                     double *__restrict slot_rms, double *__restrict slot_max) {
-  __shared__ double s_sum[kFinishThreads / 64];
-  __shared__ double s_max[kFinishThreads / 64];
-  __shared__ int s_loc[kFinishThreads / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int b0 = tid, b1 = tid + kFinishThreads;
-  const bool h0 = b0 < nparts, h1 = b1 < nparts;
-  const double s0 = h0 ? part_sum[b0] : 0.0, s1 = h1 ? part_sum[b1] : 0.0;
-  const double m0 = h0 ? part_max[b0] : -INFINITY, m1 = h1 ? part_max[b1] : -INFINITY;
-  const int l0 = h0 ? part_loc[b0] : 0x7fffffff, l1 = h1 ? part_loc[b1] : 0x7fffffff;
-  double tot = s0 + s1;
-  MaxLoc tm = maxloc_combine(MaxLoc{m0, l0}, MaxLoc{m1, l1});
-  for (int b = tid + 2 * kFinishThreads; b < nparts; b += kFinishThreads) {
-    tot += part_sum[b];
-    tm = maxloc_combine(tm, MaxLoc{part_max[b], part_loc[b]});
-  }
-  tot = wave_sum(tot);
-  tm = wave_maxloc(tm);
-  if (lane == 0) {
-    s_sum[wave] = tot;
-    s_max[wave] = tm.v;
-    s_loc[wave] = tm.loc;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double t = 0.0;
-    MaxLoc m{-INFINITY, 0x7fffffff};
-    for (int w = 0; w < kFinishThreads / 64; ++w) {
-      t += s_sum[w];
-      m = maxloc_combine(m, MaxLoc{s_max[w], s_loc[w]});
+  __shared__ double s_v[2][op2h_red::kRedThreads / 64];
+  __shared__ int s_l[2][op2h_red::kRedThreads / 64];
+  const int grp = threadIdx.x / op2h_red::kRedThreads;
+  const int t = threadIdx.x % op2h_red::kRedThreads;
+  // one call site for both groups (the reduction holds a workgroup barrier)
+  const op2h_red::ValLoc r =
+      op2h_red::group_reduce(grp == 0 ? part_sum : part_max, grp == 0 ? nullptr : part_loc,
+                             nparts, grp == 0 ? 1 : 2, t, s_v[grp], s_l[grp]);
+  if (t == 0) {
+    if (grp == 0) {
+      slot_rms[0] = r.v;
+    } else {
+      slot_max[0] = r.v;
+      reinterpret_cast<int *>(slot_max + 1)[0] = r.loc;
     }
-    slot_rms[0] = t;
-    slot_max[0] = m.v;
-    reinterpret_cast<int *>(slot_max + 1)[0] = m.loc;
   }
 }
 

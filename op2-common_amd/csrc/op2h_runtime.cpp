@@ -25,6 +25,7 @@
 
 #include <rocprofiler-sdk-roctx/roctx.h>
 #include "op2h_plan_build.h"
+#include "op2h_reduce.h"
 #include "op2h_rcm.h"
 
 // ---------------------------------------------------------------------------
@@ -1476,7 +1477,8 @@ extern "C" int op2h_gbl_partials(op2h_loop_ctx *ctx, int argi, int info_argi,
   *vals = nullptr;
   *locs = nullptr;
   const op_arg &a = ctx->args[argi];
-  if (!R.capturing || !R.partial_pool || nparts < 1 || a.dim != 1 ||
+  if (!R.capturing || !R.partial_pool || nparts < 1 ||
+      nparts > op2h_red::kRedThreads * op2h_red::kRedMaxLoads || a.dim != 1 ||
       type_code(a.type) != 0)
     return 0;
   const bool extremum = a.acc == OP_MIN || a.acc == OP_MAX;
@@ -1732,93 +1734,53 @@ constexpr int kSlotPool = 64;
 constexpr int kHistSources = 16;
 constexpr size_t kHistoryBytes = 64u << 20;
 
-constexpr int kHistThreads = 1024;
+constexpr int kHistGroups = 4;
+constexpr int kHistThreads = kHistGroups * op2h_red::kRedThreads;
 
 struct HistSources {
   const unsigned long long *p[kHistSources];  // slot words, or partial values
   const int *locs[kHistSources];              // partial payloads (max/min)
   int words[kHistSources];
+  int off[kHistSources];     // first word of the source in a record
   int kind[kHistSources];    // 0 copy the slot, 1 sum, 2 max, 3 min of partials
   int nparts[kHistSources];
   int n;
+  int red[kHistSources];     // indices of the deferred reductions
+  int nred;
 };
-
-struct ValLoc {
-  double v;
-  int loc;
-};
-
-// kind 2: larger wins; kind 3: smaller wins; equal values keep the lower loc
-__device__ __forceinline__ ValLoc vl_combine(ValLoc a, ValLoc b, int kind) {
-  const bool better = kind == 2 ? b.v > a.v : b.v < a.v;
-  if (better || (b.v == a.v && b.loc < a.loc)) return b;
-  return a;
-}
 
 // Appends the gbl results recorded in a graph to the device history: record
 // r holds all result words of launch r, in record order.  Slot sources are
-// copied; deferred reductions are finished here from their partials (one
-// pass per source over the workgroup).  The launch counter is read by every
-// lane before the first barrier and advanced by lane 0 after the last.
+// copied; deferred reductions are finished here from their partials, four at
+// a time (one group of 256 threads each, op2h_red::group_reduce: the eager
+// finish kernels' reduction, so results are the same bits).  The launch
+// counter is read by every lane first and advanced by lane 0 at the end.
 __global__ void __launch_bounds__(kHistThreads)
     history_append(HistSources src, double *hist, unsigned long long *count,
                    int capacity, int total_words) {
-  __shared__ double s_v[kHistThreads / 64];
-  __shared__ int s_l[kHistThreads / 64];
+  __shared__ double s_v[kHistGroups][op2h_red::kRedThreads / 64];
+  __shared__ int s_l[kHistGroups][op2h_red::kRedThreads / 64];
   const unsigned long long c = *count;
   const size_t base = (size_t)(c % (unsigned long long)capacity) * total_words;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  int w = 0;
-  for (int s = 0; s < src.n; ++s) {
-    const int kind = src.kind[s];
-    if (kind == 0) {
+  const int tid = threadIdx.x;
+  for (int s = 0; s < src.n; ++s)
+    if (src.kind[s] == 0)
       for (int k = tid; k < src.words[s]; k += blockDim.x)
-        hist[base + w + k] = __longlong_as_double((long long)src.p[s][k]);
-    } else {
-      const double *v = reinterpret_cast<const double *>(src.p[s]);
-      const int *l = src.locs[s];
-      const int n = src.nparts[s];
-      const double ident = kind == 1 ? 0.0 : kind == 2 ? -INFINITY : INFINITY;
-      const int b0 = tid, b1 = tid + kHistThreads;
-      ValLoc a{b0 < n ? v[b0] : ident, (kind != 1 && b0 < n) ? l[b0] : 0x7fffffff};
-      ValLoc b{b1 < n ? v[b1] : ident, (kind != 1 && b1 < n) ? l[b1] : 0x7fffffff};
-      ValLoc m;
-      if (kind == 1) {
-        m.v = a.v + b.v;
-        m.loc = 0;
-        for (int p = tid + 2 * kHistThreads; p < n; p += kHistThreads) m.v += v[p];
-        for (int o = 32; o > 0; o >>= 1) m.v += __shfl_xor(m.v, o, 64);
-      } else {
-        m = vl_combine(a, b, kind);
-        for (int p = tid + 2 * kHistThreads; p < n; p += kHistThreads)
-          m = vl_combine(m, ValLoc{v[p], l[p]}, kind);
-        for (int o = 32; o > 0; o >>= 1) {
-          ValLoc t{__shfl_xor(m.v, o, 64), __shfl_xor(m.loc, o, 64)};
-          m = vl_combine(m, t, kind);
-        }
-      }
-      if (lane == 0) {
-        s_v[wave] = m.v;
-        s_l[wave] = m.loc;
-      }
-      __syncthreads();
-      if (tid == 0) {
-        ValLoc r{kind == 1 ? 0.0 : ident, 0x7fffffff};
-        for (int k = 0; k < kHistThreads / 64; ++k) {
-          if (kind == 1)
-            r.v += s_v[k];
-          else
-            r = vl_combine(r, ValLoc{s_v[k], s_l[k]}, kind);
-        }
-        hist[base + w] = r.v;
-        if (src.words[s] > 1) {
-          long long word = (long long)(unsigned int)r.loc;  // int in the low half
-          hist[base + w + 1] = __longlong_as_double(word);
-        }
-      }
-      __syncthreads();
+        hist[base + src.off[s] + k] = __longlong_as_double((long long)src.p[s][k]);
+  const int grp = tid / op2h_red::kRedThreads, t = tid % op2h_red::kRedThreads;
+  for (int r0 = 0; r0 < src.nred; r0 += kHistGroups) {
+    const int r = r0 + grp;
+    const int s = r < src.nred ? src.red[r] : -1;
+    const op2h_red::ValLoc v = op2h_red::group_reduce(
+        s >= 0 ? reinterpret_cast<const double *>(src.p[s]) : nullptr,
+        s >= 0 ? src.locs[s] : nullptr, s >= 0 ? src.nparts[s] : 0,
+        s >= 0 ? src.kind[s] : 1, t, s_v[grp], s_l[grp]);
+    if (t == 0 && s >= 0) {
+      hist[base + src.off[s]] = v.v;
+      if (src.words[s] > 1)  // the info int in the low half of the next word
+        hist[base + src.off[s] + 1] = __longlong_as_double((long long)(unsigned int)v.loc);
     }
-    w += src.words[s];
+    __syncthreads();
   }
   __syncthreads();
   if (tid == 0) *count = c + 1;
@@ -1870,6 +1832,7 @@ extern "C" int op2h_graph_end(void) {
   for (auto &s : R.history_sources) {
     src.p[src.n] = (const unsigned long long *)s.first;
     src.words[src.n] = (int)((s.second + 7) / 8);
+    src.off[src.n] = (int)words;
     src.kind[src.n] = 0;
     auto pi = R.partial_of_slot.find(s.first);
     if (pi != R.partial_of_slot.end()) {
@@ -1877,6 +1840,7 @@ extern "C" int op2h_graph_end(void) {
       src.locs[src.n] = pi->second.locs;
       src.kind[src.n] = pi->second.kind;
       src.nparts[src.n] = pi->second.nparts;
+      src.red[src.nred++] = src.n;
     }
     words += src.words[src.n];
     src.n++;

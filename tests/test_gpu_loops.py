@@ -173,3 +173,64 @@ def test_op_renumber_matches_generator_rcm():
     assert np.array_equal(x, rcm.x)
     assert np.array_equal(ecell, rcm.ecell) and np.array_equal(edge, rcm.edge)
     assert np.array_equal(cell, rcm.cell) and np.array_equal(becell, rcm.becell)
+
+
+@pytest.mark.parametrize("plan,block", [("owner", 128), ("gather", 256)])
+def test_full_size_loops_bit_exact(plan, block):
+    """At the benchmark's size (1200x600, 720k cells, RCM numbering): adt_calc,
+    and res_calc then bres_calc with an owner-compute plan, equal the oracle's
+    sequential loops bit for bit; the interior fluxes conserve (sum of res after
+    res_calc from zero is zero up to rounding)."""
+    mesh = op2amd.mesh.generate(1200, 600, 0, "rcm")
+    q, _ = inputs(mesh, 4)
+    res0 = np.zeros((mesh.ncell, 4))
+    app = App(mesh, q, res0)
+    try:
+        api.op2h_set_loop_plan(b"res_calc", api.PLAN_NAMES[plan], block)
+        api.op2h_set_loop_plan(b"bres_calc", api.PLAN_NAMES["gather"], 128)
+        app.adt_calc()
+        adt = app.fetch(app.adt, (mesh.ncell,))
+        app.res_calc()
+        res1 = app.fetch(app.res, (mesh.ncell, 4))
+        app.bres_calc()
+        res2 = app.fetch(app.res, (mesh.ncell, 4))
+    finally:
+        app.close()
+    adt_o = oracle.adt_calc(mesh, q)
+    assert np.array_equal(adt, adt_o)
+    res1_o = oracle.res_calc(mesh, q, adt_o, res0)
+    assert np.array_equal(res1, res1_o)
+    assert np.array_equal(res2, oracle.bres_calc(mesh, q, adt_o, res1_o))
+    scale = np.abs(res1).sum(axis=0)
+    assert np.all(np.abs(res1.sum(axis=0)) <= 1e-12 * scale)
+
+
+def test_empty_sets():
+    """Loops over empty sets leave dats and gbl arguments as given (a
+    reduction over nothing is the identity, combined with the caller's
+    value)."""
+    api.init()
+    try:
+        cells = api.op_decl_set(0, b"cells")
+        empty = np.zeros((0, 4))
+        q = api.decl_dat(cells, 4, "double", empty, "q")
+        qold = api.decl_dat(cells, 4, "double", empty, "qold")
+        res = api.decl_dat(cells, 4, "double", empty, "res")
+        adt = api.decl_dat(cells, 1, "double", np.zeros(0), "adt")
+        d = api.arg_dat
+        api.loops["op_par_loop_airfoil_1_save_soln"](
+            b"save_soln", cells, d(q, -1, None, 4, "double", api.OP_READ),
+            d(qold, -1, None, 4, "double", api.OP_WRITE))
+        rms, maxerr, errloc = C.c_double(0.5), C.c_double(2.0), C.c_int(-3)
+        api.loops["op_par_loop_airfoil_5_update"](
+            b"update", cells, d(qold, -1, None, 4, "double", api.OP_READ),
+            d(q, -1, None, 4, "double", api.OP_WRITE), d(res, -1, None, 4, "double", api.OP_RW),
+            d(adt, -1, None, 1, "double", api.OP_READ),
+            api.op_arg_gbl_char(C.cast(C.pointer(rms), C.c_void_p), 1, b"double", 8, api.OP_INC),
+            api.op_arg_gbl_char(C.cast(C.pointer(maxerr), C.c_void_p), 1, b"double", 8, api.OP_MAX),
+            api.op_arg_idx(-1, api.OP_ID),
+            api.op_arg_info_char(C.cast(C.pointer(errloc), C.c_void_p), 1, b"int", 4, 5))
+        api.op2h_device_sync()
+        assert rms.value == 0.5 and maxerr.value == 2.0 and errloc.value == -3
+    finally:
+        api.op_exit()
