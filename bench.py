@@ -347,7 +347,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the single-mesh path has no multi-GPU sharding in this build)")
+              "(the library runs one mesh on one GPU; it has no multi-GPU sharding)")
         sys.exit(2)
 
     import op2amd

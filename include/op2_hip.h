@@ -317,7 +317,7 @@ void op2h_reset_stats(void);
 void *op2h_stream(void);
 void op2h_device_sync(void);
 
-/* Version string of this build. */
+/* Version string of the library. */
 const char *op2h_version(void);
 
 /* ---- generated loop stubs for the airfoil application ----

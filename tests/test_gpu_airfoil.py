@@ -1,5 +1,5 @@
 """End-to-end airfoil on the GPU through the C ABI vs the reference-pinned
-golden trajectories (FP64: rms and q within 1e-10 relative)."""
+golden trajectories (FP64; bar: rms and q within 1e-10 relative)."""
 import numpy as np
 import pytest
 
