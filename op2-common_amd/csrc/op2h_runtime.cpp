@@ -143,6 +143,7 @@ struct Runtime {
   size_t partial_pool_bytes = 0, partial_pool_next = 0;
   double *history = nullptr;
   unsigned long long *history_count = nullptr;
+  unsigned int *history_done = nullptr;  // history kernel's workgroup ticket
   int history_words = 0;   // words per launch
   int history_capacity = 0;  // launches
   std::map<op_map, long long> distinct_targets;  // per map, for byte counts
@@ -1734,9 +1735,6 @@ constexpr int kSlotPool = 64;
 constexpr int kHistSources = 16;
 constexpr size_t kHistoryBytes = 64u << 20;
 
-constexpr int kHistGroups = 4;
-constexpr int kHistThreads = kHistGroups * op2h_red::kRedThreads;
-
 struct HistSources {
   const unsigned long long *p[kHistSources];  // slot words, or partial values
   const int *locs[kHistSources];              // partial payloads (max/min)
@@ -1750,40 +1748,46 @@ struct HistSources {
 };
 
 // Appends the gbl results recorded in a graph to the device history: record
-// r holds all result words of launch r, in record order.  Slot sources are
-// copied; deferred reductions are finished here from their partials, four at
-// a time (one group of 256 threads each, op2h_red::group_reduce: the eager
-// finish kernels' reduction, so results are the same bits).  The launch
-// counter is read by every lane first and advanced by lane 0 at the end.
-__global__ void __launch_bounds__(kHistThreads)
+// r holds all result words of launch r, in record order.  One workgroup of
+// op2h_red::kRedThreads per deferred reduction (at least one): workgroup b
+// finishes reduction b from its partials with op2h_red::group_reduce (the
+// eager finish kernels' reduction, so results are the same bits), and
+// workgroup 0 also copies the slot sources.  Every workgroup reads the launch
+// counter first; the last one to finish (a ticket on `done`, after a release
+// fence that orders its reads and writes before it) advances the counter and
+// resets the ticket for the next launch.
+__global__ void __launch_bounds__(op2h_red::kRedThreads)
     history_append(HistSources src, double *hist, unsigned long long *count,
-                   int capacity, int total_words) {
-  __shared__ double s_v[kHistGroups][op2h_red::kRedThreads / 64];
-  __shared__ int s_l[kHistGroups][op2h_red::kRedThreads / 64];
+                   unsigned int *done, int capacity, int total_words) {
+  __shared__ double s_v[op2h_red::kRedThreads / 64];
+  __shared__ int s_l[op2h_red::kRedThreads / 64];
   const unsigned long long c = *count;
   const size_t base = (size_t)(c % (unsigned long long)capacity) * total_words;
-  const int tid = threadIdx.x;
-  for (int s = 0; s < src.n; ++s)
-    if (src.kind[s] == 0)
-      for (int k = tid; k < src.words[s]; k += blockDim.x)
-        hist[base + src.off[s] + k] = __longlong_as_double((long long)src.p[s][k]);
-  const int grp = tid / op2h_red::kRedThreads, t = tid % op2h_red::kRedThreads;
-  for (int r0 = 0; r0 < src.nred; r0 += kHistGroups) {
-    const int r = r0 + grp;
-    const int s = r < src.nred ? src.red[r] : -1;
+  const int t = threadIdx.x, b = blockIdx.x;
+  if (b == 0)
+    for (int s = 0; s < src.n; ++s)
+      if (src.kind[s] == 0)
+        for (int k = t; k < src.words[s]; k += blockDim.x)
+          hist[base + src.off[s] + k] = __longlong_as_double((long long)src.p[s][k]);
+  if (b < src.nred) {
+    const int s = src.red[b];
     const op2h_red::ValLoc v = op2h_red::group_reduce(
-        s >= 0 ? reinterpret_cast<const double *>(src.p[s]) : nullptr,
-        s >= 0 ? src.locs[s] : nullptr, s >= 0 ? src.nparts[s] : 0,
-        s >= 0 ? src.kind[s] : 1, t, s_v[grp], s_l[grp]);
-    if (t == 0 && s >= 0) {
+        reinterpret_cast<const double *>(src.p[s]), src.locs[s], src.nparts[s], src.kind[s], t,
+        s_v, s_l);
+    if (t == 0) {
       hist[base + src.off[s]] = v.v;
       if (src.words[s] > 1)  // the info int in the low half of the next word
         hist[base + src.off[s] + 1] = __longlong_as_double((long long)(unsigned int)v.loc);
     }
-    __syncthreads();
   }
   __syncthreads();
-  if (tid == 0) *count = c + 1;
+  if (t == 0) {
+    __threadfence();
+    if (atomicAdd(done, 1u) == gridDim.x - 1) {
+      *done = 0u;
+      *count = c + 1;
+    }
+  }
 }
 }  // namespace
 
@@ -1811,6 +1815,8 @@ extern "C" int op2h_graph_begin(void) {
   HIPCHECK(hipMalloc(&R.history, kHistoryBytes));
   HIPCHECK(hipMalloc(&R.history_count, sizeof(unsigned long long)));
   HIPCHECK(hipMemset(R.history_count, 0, sizeof(unsigned long long)));
+  HIPCHECK(hipMalloc(&R.history_done, sizeof(unsigned int)));
+  HIPCHECK(hipMemset(R.history_done, 0, sizeof(unsigned int)));
   HIPCHECK(hipStreamSynchronize(R.stream));
   HIPCHECK(hipDeviceSynchronize());
   HIPCHECK(hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal));
@@ -1849,8 +1855,8 @@ extern "C" int op2h_graph_end(void) {
   R.history_capacity =
       words ? std::max(1, (int)(kHistoryBytes / (words * sizeof(double)))) : 0;
   if (words)
-    history_append<<<1, kHistThreads, 0, R.stream>>>(src, R.history, R.history_count,
-                                                     R.history_capacity, (int)words);
+    history_append<<<std::max(1, src.nred), op2h_red::kRedThreads, 0, R.stream>>>(
+        src, R.history, R.history_count, R.history_done, R.history_capacity, (int)words);
   hipGraph_t g;
   HIPCHECK(hipStreamEndCapture(R.stream, &g));
   R.capturing = false;
@@ -1935,6 +1941,8 @@ extern "C" void op2h_graph_destroy(void) {
   R.partial_pool_bytes = R.partial_pool_next = 0;
   if (R.history) (void)hipFree(R.history);
   if (R.history_count) (void)hipFree(R.history_count);
+  if (R.history_done) (void)hipFree(R.history_done);
+  R.history_done = nullptr;
   R.history = nullptr;
   R.history_count = nullptr;
   R.history_words = 0;
