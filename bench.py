@@ -408,8 +408,16 @@ def main():
         one = max((time.time() - t) / 5, 1e-4)
         iters = int(max(5, min(4000, 25.0 / one)))
         t = time.time()
-        oracle.airfoil_run(mesh, iters, 1, "openmp", nthreads=nthreads)
+        cpu_run = oracle.airfoil_run(mesh, iters, 1, "openmp", nthreads=nthreads)
         secs = time.time() - t
+        # per-loop CPU time and GB/s on the GPU line's byte model
+        calls = {"save_soln": 1, "adt_calc": 2, "res_calc": 2, "bres_calc": 2, "update": 2}
+        cpu_loops = {}
+        for i, lp in enumerate(op2amd.airfoil.LOOPS):
+            us = 1e6 * cpu_run["loop_seconds"][i] / (iters * calls[lp])
+            b = head["per_loop"][lp]["bytes_per_call"]
+            cpu_loops[lp] = {"avg_us": round(us, 2), "achieved_GBs": round(b / (us * 1e-6) / 1e9, 1)}
+        others["cpu_openmp_per_loop"] = cpu_loops
         cpu = {"value": round(iters / secs, 3), "unit": "steps/s", "cores": nthreads,
                "kind": "port",
                "sample": f"{iters} airfoil steps on the same {I}x{J} {args.ordering} mesh "

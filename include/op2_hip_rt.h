@@ -219,6 +219,21 @@ int op2h_owner_plan_download(const op2h_owner_plan *p, int *owned_offs,
 int op2h_owner_partition(int nto, int nel, int dim, const int *map, int ninc,
                          const int *inc_idx, int block_cells, int *block_of);
 
+/* Reverse Cuthill-McKee order of an undirected graph in CSR form (off[n+1],
+ * adj[off[n]], each neighbour list strictly ascending, no entry outside
+ * [0,n)): new_of_old[v] is the new number of vertex v.  The rule is the one
+ * op_renumber applies to its base set (op_renumber.cpp:224-505 orders that set
+ * with Scotch's Gibbs-Poole-Stockmeyer instead): roots in (degree, index)
+ * order, each improved by up to 16 George-Liu pseudo-peripheral sweeps from
+ * the lowest-(degree, index) vertex of the last level; neighbours queued by
+ * (degree, index); components one after another; the order reversed.
+ * on_gpu = 0 builds it on the host, 1 on the current device, level by level,
+ * and 2 on the device with the two-pass level sort that is otherwise used only
+ * when a packed 64-bit (parent, degree, index) key is too narrow; all give the
+ * same permutation.  Returns 0, or -1 on bad arguments. */
+int op2h_rcm_order(int n, const int *off, const int *adj, int on_gpu,
+                   int *new_of_old);
+
 /* Copy a plan's colouring back to the host for checking: thread colours,
  * block colours (per block), and block offsets/sizes.  Returns 0. */
 int op2h_staged_plan_download(const op2h_staged_plan *p, int *thrcol,

@@ -53,6 +53,13 @@ typedef struct {
                                   graphed mode iterations run eagerly until
                                   every loop has chosen, then the step is
                                   captured */
+  const char *dump;            /* non-NULL: the final q is written as the
+                                  reference app writes it (airfoil.cpp:302-303)
+                                  to <dump>.dat (op_print_dat_to_txtfile) and
+                                  <dump>.bin (op_print_dat_to_binfile), in the
+                                  run's numbering; when renumbered, the cells'
+                                  new_of_old goes to <dump>.perm (count, then
+                                  one index per line) */
 } op2amd_airfoil_opts;
 
 typedef struct {

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -290,6 +291,21 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
     for (int c = 0; c < m->ncell; ++c)
       std::memcpy(r->q_out + 4 * (size_t)c, &qn[4 * (size_t)cell_new_of_old[c]],
                   4 * sizeof(double));
+  }
+  if (o->dump) {
+    const std::string base(o->dump);
+    op_print_dat_to_txtfile(A.p_q, (base + ".dat").c_str());
+    op_print_dat_to_binfile(A.p_q, (base + ".bin").c_str());
+    if (o->renumber) {
+      FILE *f = std::fopen((base + ".perm").c_str(), "w");
+      if (!f) {
+        printf("can't open file %s.perm\n", o->dump);
+        exit(2);
+      }
+      std::fprintf(f, "%d\n", m->ncell);
+      for (int c = 0; c < m->ncell; ++c) std::fprintf(f, "%d\n", cell_new_of_old[c]);
+      std::fclose(f);
+    }
   }
   if (r->errloc && o->renumber)
     for (int it = 0; it < o->niter; ++it)

@@ -27,6 +27,7 @@
 #include "op2h_plan_build.h"
 #include "op2h_reduce.h"
 #include "op2h_rcm.h"
+#include "op2h_rcm_gpu.h"
 
 // ---------------------------------------------------------------------------
 // global state (op_lib_core.cpp:55-86)
@@ -1012,7 +1013,16 @@ extern "C" void op_renumber(op_map base) {
   Renumber rn;
   rn.perm.assign(R.sets.size(), {});
   rn.iperm.assign(R.sets.size(), {});
-  rn.perm[S->index] = op2h::rcm_order(n, off, adj);
+  // the base ordering is built on the device unless OP_HIP_RCM=host; both
+  // builds give the same permutation
+  const char *where = getenv("OP_HIP_RCM");
+  const bool rcm_host = where && !strcmp(where, "host");
+  if (where && !rcm_host && strcmp(where, "gpu")) {
+    printf("op_renumber error -- OP_HIP_RCM must be gpu or host, got %s\n", where);
+    exit(-1);
+  }
+  rn.perm[S->index] = rcm_host ? op2h::rcm_order(n, off, adj)
+                               : op2h::rcm_order_gpu(n, off, adj, R.stream);
   rn.propagate(S, S);
 
   // reorder_set (op_renumber.cpp:153-198): rows of maps from the set and of
@@ -1060,7 +1070,8 @@ extern "C" void op_renumber(op_map base) {
   R.distinct_targets.clear();
   int max1 = bandwidth(&avg1);
   if (OP_diags > 0) {
-    op_printf("Renumbering using base map %s (reverse Cuthill-McKee)\n", base->name);
+    op_printf("Renumbering using base map %s (reverse Cuthill-McKee, built on the %s)\n",
+              base->name, rcm_host ? "host" : "device");
     op_printf("Before renumbering: maximum bandwidth = %d average bandwidth = %.1f\n", max0, avg0);
     op_printf("After renumbering: maximum bandwidth = %d average bandwidth = %.1f\n", max1, avg1);
   }

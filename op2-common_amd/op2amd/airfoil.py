@@ -19,7 +19,7 @@ class Opts(C.Structure):
                 ("block_bres", C.c_int), ("block_direct", C.c_int),
                 ("block_adt", C.c_int), ("kernel_timing", C.c_int),
                 ("plan_on_device", C.c_int), ("renumber", C.c_int),
-                ("mixed", C.c_int), ("autotune", C.c_int)]
+                ("mixed", C.c_int), ("autotune", C.c_int), ("dump", C.c_char_p)]
 
 
 class Result(C.Structure):
@@ -40,12 +40,15 @@ lib.op2amd_airfoil_run.argtypes = [C.POINTER(op2amd_mesh), C.POINTER(Opts),
 def run(mesh, niter, mode="plain", warmup=0, alpha_mode=1, soa=False,
         plan_res="auto", block_res=0, plan_bres="auto", block_bres=0,
         block_direct=0, block_adt=0, kernel_timing=False, plan_on_device=False,
-        renumber=False, mixed=False, autotune=0):
-    """Run the airfoil app; returns a dict of numpy arrays and timings."""
+        renumber=False, mixed=False, autotune=0, dump=None):
+    """Run the airfoil app; returns a dict of numpy arrays and timings.  With
+    `dump` (a path prefix) the final q is also written to <dump>.dat and
+    <dump>.bin, and a renumbered run's cell permutation to <dump>.perm."""
     o = Opts(niter, warmup, alpha_mode, MODES[mode], int(soa),
              PLAN_NAMES[plan_res], block_res, PLAN_NAMES[plan_bres], block_bres,
              block_direct, block_adt, int(kernel_timing), int(plan_on_device),
-             int(renumber), int(mixed), int(autotune))
+             int(renumber), int(mixed), int(autotune),
+             None if dump is None else str(dump).encode())
     rms = np.zeros(niter)
     rms_k0 = np.zeros(niter)
     maxerr = np.zeros(niter)

@@ -167,6 +167,27 @@ def owner_partition(nto, table, inc_idx, block_cells):
     return nb, out
 
 
+op2h_rcm_order = _fn("op2h_rcm_order", C.c_int, C.c_int, C.POINTER(C.c_int),
+                     C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int))
+
+
+def rcm_order(off, adj, on_gpu):
+    """Reverse Cuthill-McKee order (new number of each vertex) of a CSR graph
+    with strictly ascending neighbour lists, built on the host (on_gpu=0) or
+    the device (1; 2 forces the device's two-pass level sort)."""
+    import numpy as np
+    o = np.ascontiguousarray(off, dtype=np.int32)
+    a = np.ascontiguousarray(adj, dtype=np.int32)
+    n = len(o) - 1
+    out = np.empty(max(n, 0), np.int32)
+    ip = C.POINTER(C.c_int)
+    rc = op2h_rcm_order(n, o.ctypes.data_as(ip), a.ctypes.data_as(ip) if len(a) else None,
+                        int(on_gpu), out.ctypes.data_as(ip))
+    if rc != 0:
+        raise ValueError("bad RCM arguments")
+    return out
+
+
 op2h_plan_staged_for = _fn("op2h_plan_staged_for", C.POINTER(op_staged_plan),
                            _cstr, op_set, C.c_int, C.c_int, C.POINTER(op_arg))
 op2h_plan_color_for = _fn("op2h_plan_color_for", C.POINTER(op_color_plan),

@@ -512,20 +512,22 @@ static int run_with_consts(const op2amd_mesh *m, int niter, const orc_consts &c,
         update_range(qold.data(), q.data(), res.data(), adt.data(), &rms,
                      &maxerr, &errloc, 0, nc);
       } else {
-        // openmp/loop_host.hpp.jinja:183-202,221-245,300-324
-        std::vector<double> rms_l(nthreads, 0.0), max_l(nthreads, maxerr);
-        std::vector<int> loc_l(nthreads, 0);
+        // openmp/loop_host.hpp.jinja:183-202,221-245,300-324: per-thread
+        // partials 64 elements apart, so no two threads write one cache line
+        const int pad = 64;
+        std::vector<double> rms_l((size_t)nthreads * pad, 0.0), max_l((size_t)nthreads * pad, maxerr);
+        std::vector<int> loc_l((size_t)nthreads * pad, 0);
 #pragma omp parallel for num_threads(nthreads)
         for (int t = 0; t < nthreads; ++t) {
           int lo = (int)((long long)nc * t / nthreads), hi = (int)((long long)nc * (t + 1) / nthreads);
-          update_range(qold.data(), q.data(), res.data(), adt.data(), &rms_l[t],
-                       &max_l[t], &loc_l[t], lo, hi);
+          update_range(qold.data(), q.data(), res.data(), adt.data(), &rms_l[(size_t)t * pad],
+                       &max_l[(size_t)t * pad], &loc_l[(size_t)t * pad], lo, hi);
         }
-        for (int t = 0; t < nthreads; ++t) rms += rms_l[t];
+        for (int t = 0; t < nthreads; ++t) rms += rms_l[(size_t)t * pad];
         for (int t = 0; t < nthreads; ++t)
-          if (max_l[t] > maxerr) {
-            maxerr = max_l[t];
-            errloc = loc_l[t];
+          if (max_l[(size_t)t * pad] > maxerr) {
+            maxerr = max_l[(size_t)t * pad];
+            errloc = loc_l[(size_t)t * pad];
           }
       }
       tl[4] += now() - t3;

@@ -63,7 +63,7 @@ def test_owner_plan_reproduces_sequential_order(block):
 
 
 def test_gather_plan_reproduces_sequential_order():
-    """The owner-compute plan sums increments in the sequential executor's
+    """The gather plan sums increments in the sequential executor's
     order: q, maxerr and errloc come out bit for bit; rms differs only by the
     order of the reduction over cells."""
     g = golden("airfoil_24x12_random_a1")
@@ -73,6 +73,36 @@ def test_gather_plan_reproduces_sequential_order():
     assert np.array_equal(r["maxerr"], g["maxerr"])
     assert np.array_equal(r["errloc"], g["errloc"])
     assert pointwise_rel_err(r["rms"], g["rms"]) <= 1e-14
+
+
+def test_result_dumps_and_permutation_file(tmp_path):
+    """The app's final-q dumps (op_print_dat_to_binfile / txtfile, written as
+    airfoil.cpp:302-303 writes them) against the reference harness's q: the
+    binary one bit for bit, the text one within compare.cpp's 1e-6 and to its
+    16 printed digits; a renumbered run's binary dump, in its own numbering,
+    matches through the permutation file it wrote (within 1e-10) and not
+    without it."""
+    from op2amd import compare
+    g = golden("airfoil_24x12_random_a1")
+    ref = tmp_path / "ref.bin"
+    compare.write_bin(ref, g["q"])
+    niter = int(g["niter"])
+    op2amd.airfoil.run(mesh_of(g), niter, mode="graphed", plan_res="owner", block_res=64,
+                       plan_bres="gather", dump=tmp_path / "q")
+    assert not (tmp_path / "q.perm").exists()
+    say = []
+    assert compare.compare_files(str(ref), str(tmp_path / "q.bin"), exact=True, out=say.append)
+    assert say[-1] == "Files Identical"
+    _, _, txt = compare.read_txt(tmp_path / "q.dat")
+    assert compare.compare_arrays(txt, g["q"], tol=compare.TXT_TOL) == []
+    assert rel_err(txt, g["q"]) <= 1e-15
+    op2amd.airfoil.run(mesh_of(g), niter, mode="graphed", plan_res="owner", block_res=64,
+                       plan_bres="gather", renumber=True, dump=tmp_path / "r")
+    perm = str(tmp_path / "r.perm")
+    assert compare.compare_files(str(ref), str(tmp_path / "r.bin"), tol=1e-10, perm=perm,
+                                 out=say.append)
+    assert not compare.compare_files(str(ref), str(tmp_path / "r.bin"), tol=1e-10,
+                                     out=say.append)
 
 
 @pytest.mark.parametrize("plans", [("gather", "gather"), ("atomic", "atomic"), ("owner", "gather")])
