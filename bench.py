@@ -119,8 +119,10 @@ def measure(op2amd, mesh, K, W, args, renumber=False, plans=None, plain_too=True
         log(f"  plain: {out['plain_steps_per_s']:.1f} steps/s")
     kw = plans or sweep(op2amd, mesh, args.sweep_iters, renumber)
     out["tuned_plans"] = kw
-    graphed = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, renumber=renumber, **kw)
+    graphed = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, renumber=renumber,
+                                 graph_steps=args.graph_steps, **kw)
     out["graphed_steps_per_s"] = K / graphed["seconds"]
+    out["steps_per_graph_launch"] = graphed["graph_steps"]
     # host-to-device work outside the timed region: declarations (copies and
     # uploads of every dat and map), plan builds, graph capture
     out["graphed_setup_seconds"] = graphed.get("setup_seconds")
@@ -156,6 +158,7 @@ def workload_large(op2amd, args):
     out = {}
     common = dict(plan_res="owner", block_res=128, plan_bres="atomic", block_bres=128,
                   block_direct=1024, block_adt=1024)
+    common["graph_steps"] = args.graph_steps
     ref = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, **common)
     out["fp64_steps_per_s"] = (n - W) / ref["seconds"]
     mixed = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, mixed=True, **common)
@@ -344,6 +347,8 @@ def main():
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-iters", type=int, default=6)
+    ap.add_argument("--graph-steps", type=int, default=8,
+                    help="time steps captured per graph (lowered to a divisor of --steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -402,14 +407,17 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        oracle.airfoil_run(mesh, 2, 1, "openmp", nthreads=nthreads)  # warm the pool
-        t = time.time()
-        oracle.airfoil_run(mesh, 5, 1, "openmp", nthreads=nthreads)
-        one = max((time.time() - t) / 5, 1e-4)
-        iters = int(max(5, min(4000, 25.0 / one)))
-        t = time.time()
+        def cpu_call(n):
+            t = time.time()
+            oracle.airfoil_run(mesh, n, 1, "openmp", nthreads=nthreads)
+            return time.time() - t
+        # the per-step cost from two run lengths, so the plan builds and
+        # allocations of each call cancel; then a ~20 s sample
+        t_short = cpu_call(2)
+        one = max((cpu_call(12) - t_short) / 10, 1e-4)
+        iters = int(max(10, min(20000, 20.0 / one)))
         cpu_run = oracle.airfoil_run(mesh, iters, 1, "openmp", nthreads=nthreads)
-        secs = time.time() - t
+        secs = float(sum(cpu_run["loop_seconds"]))  # the time-step loops only
         # per-loop CPU time and GB/s on the GPU line's byte model
         calls = {"save_soln": 1, "adt_calc": 2, "res_calc": 2, "bres_calc": 2, "update": 2}
         cpu_loops = {}
@@ -421,8 +429,9 @@ def main():
         cpu = {"value": round(iters / secs, 3), "unit": "steps/s", "cores": nthreads,
                "kind": "port",
                "sample": f"{iters} airfoil steps on the same {I}x{J} {args.ordering} mesh "
-                         f"({secs:.1f} s): the oracle's OpenMP executor (block colours "
-                         "in sequence, blocks in parallel), g++ -O3 -fopenmp"}
+                         f"({secs:.1f} s in the five loops, plan builds excluded): the "
+                         "oracle's OpenMP executor (block colours in sequence, blocks in "
+                         "parallel), g++ -O3 -fopenmp"}
         log(f"cpu baseline: {cpu['value']:.2f} steps/s on {nthreads} threads")
         # single-thread sequential executor beside it (a ~5 s sample)
         t = time.time()

@@ -53,6 +53,9 @@ typedef struct {
                                   graphed mode iterations run eagerly until
                                   every loop has chosen, then the step is
                                   captured */
+  int graph_steps;             /* GRAPHED: time steps per captured graph
+                                  (at most 16; 0 or 1: one), lowered to a
+                                  divisor of the timed iteration count */
   const char *dump;            /* non-NULL: the final q is written as the
                                   reference app writes it (airfoil.cpp:302-303)
                                   to <dump>.dat (op_print_dat_to_txtfile) and
@@ -77,6 +80,7 @@ typedef struct {
   double loop_bytes[5];      /* compulsory bytes of one call of each loop */
   int chosen_plan[5], chosen_block[5];  /* autotuner's choices (-1 if none) */
   int eager_iterations;      /* iterations run before graph capture */
+  int graph_steps;           /* time steps per graph launch used */
 } op2amd_airfoil_result;
 
 /* Run the airfoil time-marching loop on mesh `m`.  Returns 0 on success. */

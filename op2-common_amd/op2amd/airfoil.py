@@ -19,7 +19,8 @@ class Opts(C.Structure):
                 ("block_bres", C.c_int), ("block_direct", C.c_int),
                 ("block_adt", C.c_int), ("kernel_timing", C.c_int),
                 ("plan_on_device", C.c_int), ("renumber", C.c_int),
-                ("mixed", C.c_int), ("autotune", C.c_int), ("dump", C.c_char_p)]
+                ("mixed", C.c_int), ("autotune", C.c_int), ("graph_steps", C.c_int),
+                ("dump", C.c_char_p)]
 
 
 class Result(C.Structure):
@@ -29,7 +30,7 @@ class Result(C.Structure):
                 ("setup_seconds", C.c_double), ("loop_kernel_ms", C.c_double * 5),
                 ("loop_calls", C.c_int * 5), ("loop_bytes", C.c_double * 5),
                 ("chosen_plan", C.c_int * 5), ("chosen_block", C.c_int * 5),
-                ("eager_iterations", C.c_int)]
+                ("eager_iterations", C.c_int), ("graph_steps", C.c_int)]
 
 
 lib.op2amd_airfoil_run.restype = C.c_int
@@ -40,14 +41,14 @@ lib.op2amd_airfoil_run.argtypes = [C.POINTER(op2amd_mesh), C.POINTER(Opts),
 def run(mesh, niter, mode="plain", warmup=0, alpha_mode=1, soa=False,
         plan_res="auto", block_res=0, plan_bres="auto", block_bres=0,
         block_direct=0, block_adt=0, kernel_timing=False, plan_on_device=False,
-        renumber=False, mixed=False, autotune=0, dump=None):
+        renumber=False, mixed=False, autotune=0, graph_steps=1, dump=None):
     """Run the airfoil app; returns a dict of numpy arrays and timings.  With
     `dump` (a path prefix) the final q is also written to <dump>.dat and
     <dump>.bin, and a renumbered run's cell permutation to <dump>.perm."""
     o = Opts(niter, warmup, alpha_mode, MODES[mode], int(soa),
              PLAN_NAMES[plan_res], block_res, PLAN_NAMES[plan_bres], block_bres,
              block_direct, block_adt, int(kernel_timing), int(plan_on_device),
-             int(renumber), int(mixed), int(autotune),
+             int(renumber), int(mixed), int(autotune), int(graph_steps),
              None if dump is None else str(dump).encode())
     rms = np.zeros(niter)
     rms_k0 = np.zeros(niter)
@@ -70,5 +71,5 @@ def run(mesh, niter, mode="plain", warmup=0, alpha_mode=1, soa=False,
         "loop_calls": dict(zip(LOOPS, list(r.loop_calls))),
         "loop_bytes": dict(zip(LOOPS, list(r.loop_bytes))),
         "chosen": {lp: (r.chosen_plan[i], r.chosen_block[i]) for i, lp in enumerate(LOOPS)},
-        "eager_iterations": r.eager_iterations,
+        "eager_iterations": r.eager_iterations, "graph_steps": r.graph_steps,
     }

@@ -75,6 +75,22 @@ def test_gather_plan_reproduces_sequential_order():
     assert pointwise_rel_err(r["rms"], g["rms"]) <= 1e-14
 
 
+@pytest.mark.parametrize("graph_steps,warmup,expect", [(8, 0, 3), (8, 10, 6), (16, 0, 9)])
+def test_multi_step_graphs_equal_single_step(graph_steps, warmup, expect):
+    """Several time steps per captured graph: the step count is lowered to a
+    divisor of the timed iterations (999 after the eager first step, or 990
+    after 10 warm-up steps), and every per-iteration result and the final q
+    are the bits of the one-step graph."""
+    g = golden("airfoil_120x60_random_a1")
+    kw = dict(mode="graphed", plan_res="owner", block_res=128, plan_bres="gather")
+    one = op2amd.airfoil.run(mesh_of(g), 1000, warmup=warmup, **kw)
+    many = op2amd.airfoil.run(mesh_of(g), 1000, warmup=warmup, graph_steps=graph_steps, **kw)
+    assert one["graph_steps"] == 1 and many["graph_steps"] == expect
+    for k in ("rms", "rms_k0", "maxerr", "errloc", "q"):
+        assert np.array_equal(many[k], one[k]), k
+    assert pointwise_rel_err(many["rms"], g["rms"]) <= TOL
+
+
 def test_result_dumps_and_permutation_file(tmp_path):
     """The app's final-q dumps (op_print_dat_to_binfile / txtfile, written as
     airfoil.cpp:302-303 writes them) against the reference harness's q: the
