@@ -156,7 +156,7 @@ def workload_large(op2amd, args):
     mesh = op2amd.mesh.generate(I, J, 0, args.ordering)
     n, W = args.large_iters, 20
     out = {}
-    common = dict(plan_res="owner", block_res=128, plan_bres="atomic", block_bres=128,
+    common = dict(plan_res="owner", block_res=256, plan_bres="atomic", block_bres=128,
                   block_direct=1024, block_adt=1024)
     common["graph_steps"] = args.graph_steps
     ref = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, **common)
