@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 
@@ -417,6 +418,11 @@ __global__ void __launch_bounds__(1024)
 
 struct OwnerArgs {
   const int4 *inst;  // per instance: edge nodes (x, y), edge cells (z, w)
+  // compact form: per block {cell base, node base} (node base -1: the block's
+  // spans do not fit 16 bits and it reads `inst`), per instance the offsets
+  // {n0 | n1 << 16, c0 | c1 << 16} from those bases
+  const int2 *base;
+  const uint2 *inst8;
   const int *elem_offs, *owned_offs, *owned, *csr_offs;
   const unsigned short *csr;
   const ushort4 *csr4;  // per owned cell: up to four increments, 0xFFFF = none
@@ -449,6 +455,52 @@ __global__ void k_pack_owner_res(const int2 *__restrict pedge,
   inst[j] = make_int4(n.x, n.y, c.x, c.y);
 }
 
+// Compact records, one workgroup per block: the lowest cell and node its
+// instances touch, and when every cell and node lies within 65535 of those,
+// 8-byte records of 16-bit offsets (else the block keeps the 16-byte ones).
+__global__ void __launch_bounds__(256)
+    k_pack_owner_res8(const int4 *__restrict inst, const int *__restrict elem_offs,
+                      int2 *__restrict base, uint2 *__restrict inst8) {
+  __shared__ int s_min[2][4], s_max[2][4];
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int e0 = elem_offs[b], e1 = elem_offs[b + 1];
+  int cmin = INT_MAX, nmin = INT_MAX, cmax = -1, nmax = -1;
+  for (int j = e0 + t; j < e1; j += blockDim.x) {
+    const int4 r = inst[j];
+    nmin = min(nmin, min(r.x, r.y));
+    nmax = max(nmax, max(r.x, r.y));
+    cmin = min(cmin, min(r.z, r.w));
+    cmax = max(cmax, max(r.z, r.w));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    cmin = min(cmin, __shfl_xor(cmin, o));
+    nmin = min(nmin, __shfl_xor(nmin, o));
+    cmax = max(cmax, __shfl_xor(cmax, o));
+    nmax = max(nmax, __shfl_xor(nmax, o));
+  }
+  if (lane == 0) {
+    s_min[0][w] = cmin;
+    s_min[1][w] = nmin;
+    s_max[0][w] = cmax;
+    s_max[1][w] = nmax;
+  }
+  __syncthreads();
+  for (int k = 0; k < 4; ++k) {
+    cmin = min(cmin, s_min[0][k]);
+    nmin = min(nmin, s_min[1][k]);
+    cmax = max(cmax, s_max[0][k]);
+    nmax = max(nmax, s_max[1][k]);
+  }
+  const bool fits = e1 > e0 && cmax - cmin <= 0xFFFF && nmax - nmin <= 0xFFFF;
+  if (t == 0) base[b] = fits ? make_int2(cmin, nmin) : make_int2(0, -1);
+  if (!fits) return;
+  for (int j = e0 + t; j < e1; j += blockDim.x) {
+    const int4 r = inst[j];
+    inst8[j] = make_uint2((unsigned)(r.x - nmin) | ((unsigned)(r.y - nmin) << 16),
+                          (unsigned)(r.z - cmin) | ((unsigned)(r.w - cmin) << 16));
+  }
+}
+
 // Owner plan for res_calc's args (res incremented via pecell 0 (+f) and 1
 // (-f)), with its instance records packed once per plan; a NULL plan or an
 // unexpected shape is fatal, as the reference's plan errors are.
@@ -468,8 +520,19 @@ static OwnerArgs owner_args_res(const op2h_owner_plan *p, const op_arg *args,
     k_pack_owner_res<<<grid_for(p->total, 256), 256, 0, st>>>(
         (const int2 *)args[0].map->map_d, (const int2 *)args[2].map->map_d, p->elems,
         p->total, inst);
+  // compact records: blocks per launch are at most a few hundred thousand, so
+  // one workgroup per block
+  snprintf(key, sizeof key, "res_calc.owner8.%d.%d.%d", bs, args[0].map->index,
+           args[2].map->index);
+  const size_t base_bytes = (sizeof(int2) * (size_t)p->nblocks + 255) / 256 * 256;
+  char *c8 = (char *)op2h_plan_user(key, base_bytes + sizeof(uint2) * (size_t)p->total, &fresh);
+  if (fresh && p->nblocks)
+    k_pack_owner_res8<<<p->nblocks, 256, 0, st>>>(inst, p->elem_offs, (int2 *)c8,
+                                                  (uint2 *)(c8 + base_bytes));
   OwnerArgs A;
   A.inst = inst;
+  A.base = (const int2 *)c8;
+  A.inst8 = (const uint2 *)(c8 + base_bytes);
   A.elem_offs = p->elem_offs;
   A.owned_offs = p->owned_offs;
   A.owned = p->owned;
@@ -487,6 +550,18 @@ static OwnerArgs owner_args_res(const op2h_owner_plan *p, const op_arg *args,
     A.csr4 = c4;
   }
   return A;
+}
+
+// Instance j of a block with bases bb: from the 8-byte record when the
+// block has one (node base >= 0, uniform over the workgroup), else the
+// 16-byte record.
+__device__ __forceinline__ int4 owner_record(const OwnerArgs &P, int2 bb, int j) {
+  if (bb.y >= 0) {
+    const uint2 r = P.inst8[j];
+    return make_int4(bb.y + (int)(r.x & 0xFFFFu), bb.y + (int)(r.x >> 16),
+                     bb.x + (int)(r.y & 0xFFFFu), bb.x + (int)(r.y >> 16));
+  }
+  return P.inst[j];
 }
 
 // Adds one owned cell's increments to acc in list order: +f for ecell[0]
@@ -516,8 +591,9 @@ __global__ void __launch_bounds__(1024)
   extern __shared__ double lf[];
   const int b = xcd_block(kXcds);
   const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
+  const int2 bb = P.base[b];
   for (int j = threadIdx.x; j < ne; j += blockDim.x) {
-    const int4 r = P.inst[e0 + j];
+    const int4 r = owner_record(P, bb, e0 + j);
     double x1[2], x2[2], q1[4], q2[4];
     ld2<SOA>(x, r.x, sx, x1);
     ld2<SOA>(x, r.y, sx, x2);
@@ -937,8 +1013,9 @@ __global__ void __launch_bounds__(1024)
   extern __shared__ float lff[];
   const int b = xcd_block(kXcds);
   const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
+  const int2 bb = P.base[b];
   for (int j = threadIdx.x; j < ne; j += blockDim.x) {
-    const int4 r = P.inst[e0 + j];
+    const int4 r = owner_record(P, bb, e0 + j);
     float x1[2], x2[2], q1[4], q2[4];
     ld2f<SOA>(x, r.x, sx, x1);
     ld2f<SOA>(x, r.y, sx, x2);

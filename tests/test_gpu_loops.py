@@ -175,13 +175,18 @@ def test_op_renumber_matches_generator_rcm():
     assert np.array_equal(cell, rcm.cell) and np.array_equal(becell, rcm.becell)
 
 
-@pytest.mark.parametrize("plan,block", [("owner", 128), ("gather", 256)])
-def test_full_size_loops_bit_exact(plan, block):
-    """At the benchmark's size (1200x600, 720k cells, RCM numbering): adt_calc,
-    and res_calc then bres_calc with an owner-compute plan, equal the oracle's
-    sequential loops bit for bit; the interior fluxes conserve (sum of res after
-    res_calc from zero is zero up to rounding)."""
-    mesh = op2amd.mesh.generate(1200, 600, 0, "rcm")
+@pytest.mark.parametrize("plan,block,ordering", [
+    ("owner", 128, "rcm"), ("gather", 256, "rcm"), ("owner", 256, "rcm"),
+    ("owner", 128, "hilbert"), ("owner", 128, "random")])
+def test_full_size_loops_bit_exact(plan, block, ordering):
+    """At the benchmark's size (1200x600, 720k cells): adt_calc, and res_calc
+    then bres_calc with an owner-compute plan, equal the oracle's sequential
+    loops bit for bit; the interior fluxes conserve (sum of res after res_calc
+    from zero is zero up to rounding).  The owner plan's 8-byte records (16-bit
+    offsets from a per-patch base) serve every patch of the RCM numbering,
+    some of the Hilbert one's (the others span more than 65535 indices and
+    keep 16-byte records) and none of the random one's."""
+    mesh = op2amd.mesh.generate(1200, 600, 0, ordering)
     q, _ = inputs(mesh, 4)
     res0 = np.zeros((mesh.ncell, 4))
     app = App(mesh, q, res0)
