@@ -197,11 +197,12 @@ def workload_large(op2amd, args):
 
 def workload_ensemble(op2amd, args):
     """256 uneven meshes (40x20 .. 640x320, per-member Mach and alpha):
-    one launch per loop over all members, graph-captured, vs one launch per
+    one launch per loop over all members, graph-captured, block sizes picked
+    by the ensemble's autotuner over the 5 warm-up steps, vs one launch per
     loop per member."""
     sp = op2amd.ensemble.spec(nmesh=args.ensemble_size)
     n, W = args.ensemble_iters, 5
-    batched = op2amd.ensemble.run(sp, n, warmup=W, batched=True, graphed=True)
+    batched = op2amd.ensemble.run(sp, n, warmup=W, batched=True, graphed=True, autotune=True)
     looped = op2amd.ensemble.run(sp, min(n, 30), warmup=W, batched=False)
     timing = op2amd.ensemble.run(sp, 12, warmup=2, batched=False, member_timing=True)
     nc = batched["total_cells"]
@@ -224,7 +225,8 @@ def workload_ensemble(op2amd, args):
                    "looped_member_step_ms_p50": float(np.percentile(per, 50)),
                    "looped_member_step_ms_p99": float(np.percentile(per, 99)),
                    "looped_member_step_ms_sum": float(per.sum()),
-                   "rms_identical_batched_vs_looped": same},
+                   "rms_identical_batched_vs_looped": same,
+                   "batched_block_sizes": batched["blocks"]},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

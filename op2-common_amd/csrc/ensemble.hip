@@ -507,23 +507,38 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
 
   int nsm = 0;
   ECHECK(hipDeviceGetAttribute(&nsm, hipDeviceAttributeMultiprocessorCount, 0));
-  const int bs = 512;
-
-  auto step_range = [&](int mlo, int mhi) {
+  // block sizes of save_soln, adt_calc, res_calc, bres_calc (the defaults, or
+  // the autotuner's picks); every kernel's result is independent of them
+  int blk[4] = {256, 1024, 512, 128};
+  // with `ev`, each launch is bracketed by events ev[2j], ev[2j+1] in launch
+  // order: copy, then adt, res, bres of each half step
+  auto step_range = [&](int mlo, int mhi, hipEvent_t *ev = nullptr) {
     // save_soln, then twice adt_calc, res_calc, bres_calc, update
     const int c0 = coff[mlo], c1 = coff[mhi];
     long long lo2 = 2LL * c0, hi2 = 2LL * c1;
-    k_ens_copy<<<grid_for(hi2 - lo2, 256), 256, 0, st>>>((const double2 *)d_q,
-                                                        (double2 *)d_qold, lo2, hi2);
+    int e = 0;
+    auto mark = [&]() {
+      if (ev) ECHECK(hipEventRecord(ev[e++], st));
+    };
+    mark();
+    k_ens_copy<<<grid_for(hi2 - lo2, blk[0]), blk[0], 0, st>>>((const double2 *)d_q,
+                                                              (double2 *)d_qold, lo2, hi2);
+    mark();
     for (int k = 0; k < 2; ++k) {
-      k_ens_adt<<<grid_for(c1 - c0, 1024), 1024, 0, st>>>(d_x, d_q, d_adt, d_cell, c0, c1);
-      k_ens_res<<<grid_for(4LL * (c1 - c0), bs), bs, 0, st>>>(d_x, d_q, d_adt, d_res,
-                                                              d_rec, c0, c1);
+      mark();
+      k_ens_adt<<<grid_for(c1 - c0, blk[1]), blk[1], 0, st>>>(d_x, d_q, d_adt, d_cell, c0, c1);
+      mark();
+      mark();
+      k_ens_res<<<grid_for(4LL * (c1 - c0), blk[2]), blk[2], 0, st>>>(d_x, d_q, d_adt, d_res,
+                                                                      d_rec, c0, c1);
+      mark();
       const int t0 = mesh_btarget_off[mlo], t1 = mesh_btarget_off[mhi];
+      mark();
       if (t1 > t0)
-        k_ens_bres<<<grid_for(t1 - t0, 128), 128, 0, st>>>(
+        k_ens_bres<<<grid_for(t1 - t0, blk[3]), blk[3], 0, st>>>(
             d_x, d_q, d_adt, d_res, d_bedge, d_bound, d_bmesh, d_qinf, d_boffsets,
             d_bentries, d_btargets, t0, t1);
+      mark();
       const int tl0 = mesh_tile_off[mlo], tl1 = mesh_tile_off[mhi];
       const int grid = std::max(1, std::min(tl1 - tl0, 4 * nsm));
       k_ens_update<<<grid, kTile, 0, st>>>(d_qold, d_q, d_res, d_adt, d_tstart, d_tlen,
@@ -549,6 +564,14 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       ECHECK(hipEventCreate(&ev_b[m]));
     }
   }
+  // autotuning: step i (i < 5) runs every tuned kernel with candidate i; the
+  // fastest total per kernel over its launches wins
+  const int cand[5] = {64, 128, 256, 512, 1024};
+  const int ntune = (o->batched && o->autotune) ? std::min(5, o->niter) : 0;
+  double tune_ms[4][5] = {};
+  hipEvent_t tev[14];
+  if (ntune)
+    for (auto &e : tev) ECHECK(hipEventCreate(&e));
   double t0 = 0.0;
   for (int it = 1; it <= o->niter; ++it) {
     if (it == o->warmup + 1) {
@@ -556,7 +579,26 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       res->setup_seconds = now() - t_setup;
       t0 = now();
     }
-    if (o->batched) {
+    if (it <= ntune) {
+      for (int k = 0; k < 4; ++k) blk[k] = cand[it - 1];
+      step_range(0, M, tev);
+      history();
+      ECHECK(hipStreamSynchronize(st));
+      // event pairs: copy; then (adt, res, bres) twice
+      const int kind[7] = {0, 1, 2, 3, 1, 2, 3};
+      for (int j = 0; j < 7; ++j) {
+        float ms = 0.f;
+        ECHECK(hipEventElapsedTime(&ms, tev[2 * j], tev[2 * j + 1]));
+        tune_ms[kind[j]][it - 1] += ms;
+      }
+      if (it == ntune)
+        for (int k = 0; k < 4; ++k) {
+          int best = 0;
+          for (int c = 1; c < ntune; ++c)
+            if (tune_ms[k][c] < tune_ms[k][best]) best = c;
+          blk[k] = cand[best];
+        }
+    } else if (o->batched) {
       if (o->graphed) {
         if (!gexec) {
           ECHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -618,6 +660,9 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
           o->batched ? 1e3 * res->seconds / std::max(1, o->niter - o->warmup)
                      : (timed_steps ? member_ms[m] / timed_steps : 0.0);
   }
+  for (int k = 0; k < 4; ++k) res->blocks[k] = blk[k];
+  if (ntune)
+    for (auto e : tev) (void)hipEventDestroy(e);
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
   for (auto e : ev_a) (void)hipEventDestroy(e);

@@ -23,7 +23,7 @@ class Member(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("niter", C.c_int), ("warmup", C.c_int), ("batched", C.c_int),
-                ("graphed", C.c_int), ("member_timing", C.c_int)]
+                ("graphed", C.c_int), ("member_timing", C.c_int), ("autotune", C.c_int)]
 
 
 class Result(C.Structure):
@@ -31,7 +31,7 @@ class Result(C.Structure):
                 ("errloc", C.POINTER(C.c_int)), ("q_out", C.POINTER(C.c_double)),
                 ("member_step_ms", C.POINTER(C.c_double)), ("seconds", C.c_double),
                 ("setup_seconds", C.c_double), ("total_cells", C.c_longlong),
-                ("total_edges", C.c_longlong)]
+                ("total_edges", C.c_longlong), ("blocks", C.c_int * 4)]
 
 
 lib.op2amd_ensemble_describe.restype = C.c_int
@@ -55,7 +55,7 @@ def describe(sp):
 
 
 def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
-        want_q=False):
+        want_q=False, autotune=False):
     members = describe(sp)
     M = sp.nmesh
     total = sum(m["ncell"] for m in members)
@@ -70,11 +70,13 @@ def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
     r.errloc = errloc.ctypes.data_as(C.POINTER(C.c_int))
     r.q_out = q.ctypes.data_as(C.POINTER(C.c_double)) if want_q else C.POINTER(C.c_double)()
     r.member_step_ms = step_ms.ctypes.data_as(C.POINTER(C.c_double))
-    o = Opts(niter, warmup, int(batched), int(graphed), int(member_timing))
+    o = Opts(niter, warmup, int(batched), int(graphed), int(member_timing), int(autotune))
     rc = lib.op2amd_ensemble_run(C.byref(sp), C.byref(o), C.byref(r))
     if rc:
         raise RuntimeError(f"ensemble run failed ({rc})")
     return {"members": members, "rms": rms, "maxerr": maxerr, "errloc": errloc, "q": q,
             "member_step_ms": step_ms, "seconds": r.seconds,
             "setup_seconds": r.setup_seconds, "total_cells": r.total_cells,
-            "total_edges": r.total_edges}
+            "total_edges": r.total_edges,
+            "blocks": dict(zip(("save_soln", "adt_calc", "res_calc", "bres_calc"),
+                               list(r.blocks)))}

@@ -43,3 +43,19 @@ def test_batched_equals_looped_and_each_member_equals_sequential_run():
         assert np.array_equal(batched["maxerr"][:, m], ref["maxerr"]), m
         assert np.array_equal(batched["errloc"][:, m], ref["errloc"]), m
         assert np.max(np.abs(batched["rms"][:, m] / ref["rms"] - 1)) <= 1e-14, m
+
+
+@pytest.mark.gpu
+def test_autotuned_block_sizes_change_no_bits():
+    """The batched run with its block sizes picked by timing the first five
+    steps (one candidate per step) gives the bits of the default launch
+    shapes; the picks are candidates."""
+    sp = op2amd.ensemble.spec(**SMALL)
+    plain = op2amd.ensemble.run(sp, 30, batched=True, graphed=True, want_q=True)
+    tuned = op2amd.ensemble.run(sp, 30, batched=True, graphed=True, want_q=True,
+                                autotune=True)
+    for key in ("rms", "maxerr", "errloc", "q"):
+        assert np.array_equal(tuned[key], plain[key]), key
+    assert plain["blocks"] == {"save_soln": 256, "adt_calc": 1024, "res_calc": 512,
+                               "bres_calc": 128}
+    assert set(tuned["blocks"].values()) <= {64, 128, 256, 512, 1024}
