@@ -232,6 +232,70 @@ def workload_ensemble(op2amd, args):
     print(json.dumps(line), flush=True)
 
 
+def workload_translated(op2amd, args):
+    """The airfoil program as op2amd.translator emits it for this backend
+    (tests/apps/airfoil/airfoil_tr.cpp, built by __graft_entry__.build()):
+    per-loop kernel times and GB/s of the generated kernels, from the
+    program's op_timing_output with the OP_HIP_KERNEL_TIMING token, next to the
+    hand-written kernels' on the same mesh with the same plans.  The program
+    reads rms back after every update (it prints it each iteration), so its
+    step rate is an eager one."""
+    import subprocess
+    import tempfile
+    exe = ROOT / "tests" / "apps" / "_build" / "airfoil_tr" / "airfoil_tr"
+    if not exe.exists():
+        raise SystemExit(f"{exe} is missing: run __graft_entry__.build() first")
+    I, J = (int(v) for v in args.mesh.split("x"))
+    mesh = op2amd.mesh.generate(I, J, 0, args.ordering)
+    iters = args.translated_iters
+    plans = {"res_calc": ("owner", 128), "bres_calc": ("atomic", 128)}
+    tokens = ["OP_HIP_KERNEL_TIMING"] + [f"OP_HIP_PLAN={lp}:{p}:{b}" for lp, (p, b) in plans.items()]
+    with tempfile.TemporaryDirectory() as d:
+        grid = os.path.join(d, "new_grid.dat")
+        mesh.write(grid)
+        log(f"translated airfoil: {I}x{J}, {iters} iterations")
+        r = subprocess.run([str(exe), grid, str(iters)] + tokens, capture_output=True, text=True,
+                           timeout=600)
+    if r.returncode != 0:
+        raise SystemExit(f"airfoil_tr failed ({r.returncode}): {r.stdout[-2000:]}{r.stderr[-2000:]}")
+    secs = None
+    gen = {}
+    for line in r.stdout.splitlines():
+        if line.startswith("time "):
+            secs = float(line.split()[1])
+        f = [t.strip() for t in line.split(";")]
+        if len(f) == 6 and f[0].isdigit():
+            count, kms, gbs, name = int(f[0]), float(f[3]), float(f[4]), f[5]
+            gen[name] = {"calls": count, "avg_us": round(1e3 * kms / max(count, 1), 3),
+                         "achieved_GBs": round(gbs, 1)}
+    hand = op2amd.airfoil.run(mesh, iters + 2, mode="tuned", warmup=2, kernel_timing=True,
+                              plan_res="owner", block_res=128, plan_bres="atomic",
+                              block_bres=128)
+    hw = {}
+    for lp in op2amd.airfoil.LOOPS:
+        us = 1e3 * hand["loop_kernel_ms"][lp] / max(hand["loop_calls"][lp], 1)
+        hw[lp] = {"avg_us": round(us, 3),
+                  "achieved_GBs": round(hand["loop_bytes"][lp] / (us * 1e-6) / 1e9, 1)}
+    res_gen = gen.get("res_calc", {}).get("achieved_GBs", 0.0)
+    line = {
+        "metric": "translated airfoil: res_calc kernel GB/s of the generated code (720k cells)",
+        "value": res_gen, "unit": "GB/s", "n_gpus": 1, "steps": iters, "warmup": 0,
+        "ms_per_step": round(1e3 * secs / iters, 4) if secs else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"airfoil {I}x{J} ({mesh.ncell} cells), {args.ordering} "
+                               f"ordering, {iters} iterations, eager (rms read every update)",
+                   "plans": {lp: f"{p}:{b}" for lp, (p, b) in plans.items()},
+                   "translated_per_loop": gen, "hand_written_per_loop": hw,
+                   "translated_steps_per_s": round(iters / secs, 1) if secs else None},
+        "roofline": {"bound": "hbm", "kernel": "res_calc (generated, owner, block 128)",
+                     "achieved": res_gen, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(res_gen / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def workload_copy(op2amd, args):
     """Counter calibration: save_soln (qold = q, a plain copy) on a set of
     16.8M elements, 512 MiB read and 512 MiB written per call, far beyond the
@@ -327,13 +391,15 @@ def workload_costmodel(op2amd, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="airfoil",
-                    choices=["airfoil", "airfoil-large", "ensemble", "costmodel", "copy"],
+                    choices=["airfoil", "airfoil-large", "ensemble", "costmodel", "copy",
+                             "translated"],
                     help="airfoil: the headline 720k line (default); the others "
                          "measure the 11.5M-cell FP64/mixed config and the ensemble")
     ap.add_argument("--large-mesh", default="4800x2400")
     ap.add_argument("--large-iters", type=int, default=2000)
     ap.add_argument("--ensemble-size", type=int, default=256)
     ap.add_argument("--ensemble-iters", type=int, default=200)
+    ap.add_argument("--translated-iters", type=int, default=100)
     ap.add_argument("--cm-loops", type=int, default=200)
     ap.add_argument("--cm-epochs", type=int, default=4000)
     ap.add_argument("--cm-dump", default="", help="write the sweep records (JSON) here")
@@ -366,6 +432,8 @@ def main():
         return workload_costmodel(op2amd, args)
     if args.workload == "copy":
         return workload_copy(op2amd, args)
+    if args.workload == "translated":
+        return workload_translated(op2amd, args)
     from oracle import oracle
     I, J = (int(v) for v in args.mesh.split("x"))
     K, W = args.steps, args.warmup

@@ -55,10 +55,10 @@ typedef struct {
   int batched;     /* 1: one launch per loop for all members; 0: per member */
   int graphed;     /* batched only: time step replayed from a HIP graph */
   int member_timing; /* per-member step time with HIP events (looped mode) */
-  int autotune;      /* batched only: block sizes of the save_soln, adt_calc,
-                        res_calc and bres_calc launches picked by timing the
-                        first five steps (one candidate of 64..1024 per step;
-                        the results do not depend on the block sizes) */
+  int autotune;      /* batched only: block sizes of the save_soln, adt_calc
+                        and bres_calc launches picked by timing the first
+                        five steps (one candidate of 64..1024 per step; the
+                        results do not depend on the block sizes) */
 } op2amd_ensemble_opts;
 
 typedef struct {
@@ -73,8 +73,8 @@ typedef struct {
   double seconds;       /* wall time of iterations warmup+1..niter */
   double setup_seconds;
   long long total_cells, total_edges;
-  int blocks[4];        /* block sizes used: save_soln, adt_calc, res_calc,
-                           bres_calc */
+  int blocks[4];        /* block sizes used: save_soln, adt_calc, res_calc
+                           (its owner patch size, 128), bres_calc */
 } op2amd_ensemble_result;
 
 int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,

@@ -373,4 +373,89 @@ __device__ __forceinline__ int quad_bcast(int v) {
   return __builtin_amdgcn_mov_dpp(v, J * 0x55, 0xF, 0xF, false);
 }
 
+// ---------------------------------------------------------------------------
+// Owner-compute res_calc (the owner partition plan): per block its instance
+// records, owned cells and per-cell increment lists
+// ---------------------------------------------------------------------------
+struct OwnerArgs {
+  const int4 *inst;  // per instance: edge nodes (x, y), edge cells (z, w)
+  // compact form: per block {cell base, node base} (node base -1: the block's
+  // spans do not fit 16 bits and it reads `inst`), per instance the offsets
+  // {n0 | n1 << 16, c0 | c1 << 16} from those bases
+  const int2 *base;
+  const uint2 *inst8;
+  const int *elem_offs, *owned_offs, *owned, *csr_offs;
+  const unsigned short *csr;
+  const ushort4 *csr4;  // per owned cell: up to four increments, 0xFFFF = none
+  int max_e;
+};
+
+// Instance j of a block with bases bb: from the 8-byte record when the
+// block has one (node base >= 0, uniform over the workgroup), else the
+// 16-byte record.
+__device__ __forceinline__ int4 owner_record(const OwnerArgs &P, int2 bb, int j) {
+  if (bb.y >= 0) {
+    const uint2 r = P.inst8[j];
+    return make_int4(bb.y + (int)(r.x & 0xFFFFu), bb.y + (int)(r.x >> 16),
+                     bb.x + (int)(r.y & 0xFFFFu), bb.x + (int)(r.y >> 16));
+  }
+  return P.inst[j];
+}
+
+// Adds one owned cell's increments to acc in list order: +f for ecell[0]
+// (even entries), -f for ecell[1] (odd).
+template <class T>
+__device__ __forceinline__ void owner_apply(T *acc, const T *lf, int ent) {
+  const T *f = lf + 4 * (ent >> 1);
+  if (ent & 1) {
+    for (int d = 0; d < 4; ++d) acc[d] -= f[d];
+  } else {
+    for (int d = 0; d < 4; ++d) acc[d] += f[d];
+  }
+}
+
+// Owner-compute res_calc of block b:
+//  1. one flux per edge touching the block (edges on the block boundary are
+//     evaluated by both neighbouring blocks), from global memory, into LDS;
+//  2. each owned cell adds its edges' fluxes to res in ascending edge order
+//     (+f where it is ecell[0], -f where it is ecell[1]), as the sequential
+//     executor does, so res is bit-identical to it.
+// lf: LDS for the block's fluxes, [max_e][4].
+template <bool SOA, bool C4>
+__device__ __forceinline__ void owner_res_block(int b, const double *__restrict x,
+                                                const double *__restrict q,
+                                                const double *__restrict adt,
+                                                double *__restrict res, const OwnerArgs &P,
+                                                int sx, int sq, double *lf) {
+  const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
+  const int2 bb = P.base[b];
+  for (int j = threadIdx.x; j < ne; j += blockDim.x) {
+    const int4 r = owner_record(P, bb, e0 + j);
+    double x1[2], x2[2], q1[4], q2[4];
+    ld2<SOA>(x, r.x, sx, x1);
+    ld2<SOA>(x, r.y, sx, x2);
+    ld4<SOA>(q, r.z, sq, q1);
+    ld4<SOA>(q, r.w, sq, q2);
+    op2_k::res_flux(x1, x2, q1, q2, adt[r.z], adt[r.w], lf + 4 * j);
+  }
+  __syncthreads();
+  const int o0 = P.owned_offs[b], no = P.owned_offs[b + 1] - o0;
+  for (int i = threadIdx.x; i < no; i += blockDim.x) {
+    const int cell = P.owned[o0 + i];
+    double acc[4];
+    ld4<SOA>(res, cell, sq, acc);
+    if (C4) {
+      const ushort4 e = P.csr4[o0 + i];
+      if (e.x != 0xFFFF) owner_apply(acc, lf, e.x);
+      if (e.y != 0xFFFF) owner_apply(acc, lf, e.y);
+      if (e.z != 0xFFFF) owner_apply(acc, lf, e.z);
+      if (e.w != 0xFFFF) owner_apply(acc, lf, e.w);
+    } else {
+      for (int k = P.csr_offs[o0 + i]; k < P.csr_offs[o0 + i + 1]; ++k)
+        owner_apply(acc, lf, (int)P.csr[k]);
+    }
+    st4<SOA>(res, cell, sq, acc);
+  }
+}
+
 }  // namespace op2_dev

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "airfoil_device.h"
+#include "op2h_plan_build.h"
 
 #define ECHECK(x)                                                            \
   do {                                                                       \
@@ -33,6 +35,7 @@ namespace {
 using namespace op2_dev;
 
 constexpr int kTile = 1024;  // cells per update tile (one per thread)
+constexpr int kPatch = 128;  // cells per owner patch of res_calc (one per thread)
 
 struct SplitMix64 {
   uint64_t s;
@@ -76,54 +79,16 @@ __global__ void k_ens_adt(const double *__restrict x, const double *__restrict q
   adt[c] = a;
 }
 
-// rec[4c + k] = {n0, n1, other cell, slot} of cell c's k-th edge, edge order
+// res_calc over blocks [blo, blo + gridDim.x) of the members' owner
+// partitions: each edge evaluated once per patch it touches, each cell's
+// increments summed in edge order (op2_dev::owner_res_block), so every member
+// gets its sequential executor's bits.  LDS: fluxes [max_e][4].
 __global__ void __launch_bounds__(1024)
-    k_ens_res(const double *__restrict x, const double *__restrict q,
-              const double *__restrict adt, double *res,
-              const int4 *__restrict rec, int lo, int hi) {
-  const int g = xcd_block(kXcds) * blockDim.x + threadIdx.x;
-  const int t = g >> 2, k = g & 3;
-  const int cell = lo + t;
-  const bool valid = cell < hi;
-  const int4 rr = valid ? rec[4 * (size_t)lo + g] : make_int4(-1, -1, -1, 0);
-  double r[4] = {0.0, 0.0, 0.0, 0.0};
-  const int has = rr.x >= 0;
-  if (has) {
-    double x1[2], x2[2], qa[4], qb[4];
-    ld2<false>(x, rr.x, 0, x1);
-    ld2<false>(x, rr.y, 0, x2);
-    ld4<false>(q, cell, 0, qa);
-    ld4<false>(q, rr.z, 0, qb);
-    // operands selected so that the wave evaluates one flux (the edge's
-    // ecell[0] side is q1 whichever side this lane's cell is)
-    const bool second = rr.w != 0;
-    double q1[4], q2[4];
-    for (int d = 0; d < 4; ++d) {
-      q1[d] = second ? qb[d] : qa[d];
-      q2[d] = second ? qa[d] : qb[d];
-    }
-    const double aa = adt[cell], ab = adt[rr.z];
-    double a1 = second ? ab : aa, a2 = second ? aa : ab;
-    double r1[4] = {0.0, 0.0, 0.0, 0.0}, r2[4] = {0.0, 0.0, 0.0, 0.0};
-    op2_k::res_calc(x1, x2, q1, q2, &a1, &a2, r1, r2);
-    for (int d = 0; d < 4; ++d) r[d] = second ? r2[d] : r1[d];
-  }
-  // the quad's increments in edge order (lane 0 first), read by DPP
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  if (k == 0 && valid) ld4<false>(res, cell, 0, acc);
-  double v[4][4];
-  for (int d = 0; d < 4; ++d) {
-    v[0][d] = quad_bcast<0>(r[d]);
-    v[1][d] = quad_bcast<1>(r[d]);
-    v[2][d] = quad_bcast<2>(r[d]);
-    v[3][d] = quad_bcast<3>(r[d]);
-  }
-  const int hv[4] = {quad_bcast<0>(has), quad_bcast<1>(has), quad_bcast<2>(has),
-                     quad_bcast<3>(has)};
-  for (int j = 0; j < 4; ++j)
-    if (hv[j])
-      for (int d = 0; d < 4; ++d) acc[d] += v[j][d];
-  if (k == 0 && valid) st4<false>(res, cell, 0, acc);
+    k_ens_res_owner(const double *__restrict x, const double *__restrict q,
+                    const double *__restrict adt, double *__restrict res, OwnerArgs P,
+                    int blo) {
+  extern __shared__ double lf[];
+  owner_res_block<false, true>(blo + xcd_block(kXcds), x, q, adt, res, P, 0, 0, lf);
 }
 
 // bres_calc (bres_calc.h:1-35) with the member's own free stream `qi`
@@ -410,24 +375,68 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   res->total_cells = Nc;
   res->total_edges = Ne;
 
-  // owner-compute records of res_calc: per cell, its edges in edge order
-  std::vector<int> deg(Nc, 0);
-  for (int e = 0; e < Ne; ++e) {
-    deg[ecell[e].x]++;
-    deg[ecell[e].y]++;
+  // res_calc: each member's owner partition (patches of kPatch cells that
+  // never straddle two members), concatenated; 8-byte instance records where a
+  // patch's cells and nodes span at most 65535 indices
+  std::vector<int> mblk(M + 1, 0), owned_offs{0}, owned, elem_offs{0};
+  std::vector<int4> inst;
+  std::vector<uint2> inst8;
+  std::vector<int2> pbase;
+  std::vector<ushort4> csr4;
+  int max_e = 0;
+  {
+    std::vector<int> lmap;
+    const int inc[2] = {0, 1};
+    for (int m = 0; m < M; ++m) {
+      const int c0 = coff[m], e0 = eoff[m], nel = eoff[m + 1] - e0;
+      lmap.resize(2 * (size_t)nel);
+      for (int e = 0; e < nel; ++e) {
+        lmap[2 * (size_t)e] = ecell[e0 + e].x - c0;
+        lmap[2 * (size_t)e + 1] = ecell[e0 + e].y - c0;
+      }
+      op2h::OwnerLists L;
+      std::string err;
+      if (!op2h::owner_lists(coff[m + 1] - c0, nel, 2, lmap.data(), 2, inc, kPatch, L, err)) {
+        printf("op2hip ensemble: member %d: %s\n", m, err.c_str());
+        return 4;
+      }
+      if (L.max_incs > 4) {
+        printf("op2hip ensemble: a cell has more than 4 edges\n");
+        return 4;
+      }
+      max_e = std::max(max_e, L.max_elems);
+      for (int b = 0; b < L.nblocks; ++b) {
+        const size_t first = inst.size();
+        int cmin = INT_MAX, cmax = -1, nmin = INT_MAX, nmax = -1;
+        for (int j = L.elem_offs[b]; j < L.elem_offs[b + 1]; ++j) {
+          const int e = e0 + L.elems[j];
+          const int4 r = make_int4(edge[e].x, edge[e].y, ecell[e].x, ecell[e].y);
+          inst.push_back(r);
+          nmin = std::min(nmin, std::min(r.x, r.y));
+          nmax = std::max(nmax, std::max(r.x, r.y));
+          cmin = std::min(cmin, std::min(r.z, r.w));
+          cmax = std::max(cmax, std::max(r.z, r.w));
+        }
+        const bool fits = inst.size() > first && cmax - cmin <= 0xFFFF && nmax - nmin <= 0xFFFF;
+        pbase.push_back(fits ? make_int2(cmin, nmin) : make_int2(0, -1));
+        for (size_t j = first; j < inst.size(); ++j) {
+          const int4 r = inst[j];
+          inst8.push_back(fits ? make_uint2((unsigned)(r.x - nmin) | ((unsigned)(r.y - nmin) << 16),
+                                            (unsigned)(r.z - cmin) | ((unsigned)(r.w - cmin) << 16))
+                               : make_uint2(0u, 0u));
+        }
+        elem_offs.push_back((int)inst.size());
+        for (int i = L.owned_offs[b]; i < L.owned_offs[b + 1]; ++i) {
+          owned.push_back(c0 + L.owned[i]);
+          unsigned short v[4] = {0xFFFF, 0xFFFF, 0xFFFF, 0xFFFF};
+          for (int k = L.csr_offs[i]; k < L.csr_offs[i + 1]; ++k) v[k - L.csr_offs[i]] = L.csr[k];
+          csr4.push_back(make_ushort4(v[0], v[1], v[2], v[3]));
+        }
+        owned_offs.push_back((int)owned.size());
+      }
+      mblk[m + 1] = mblk[m] + L.nblocks;
+    }
   }
-  for (int c = 0; c < Nc; ++c)
-    if (deg[c] > 4) {
-      printf("op2hip ensemble: a cell has more than 4 edges\n");
-      return 4;
-    }
-  std::vector<int4> rec(4 * (size_t)Nc, make_int4(-1, -1, -1, 0));
-  std::fill(deg.begin(), deg.end(), 0);
-  for (int e = 0; e < Ne; ++e)
-    for (int s = 0; s < 2; ++s) {
-      int c = s ? ecell[e].y : ecell[e].x, other = s ? ecell[e].x : ecell[e].y;
-      rec[4 * (size_t)c + deg[c]++] = make_int4(edge[e].x, edge[e].y, other, s);
-    }
   // bres: boundary cells in cell order with their bedges in bedge order
   std::vector<int> bcount(Nc, 0);
   for (int b = 0; b < Nb; ++b) bcount[becell[b]]++;
@@ -473,7 +482,16 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   double *d_res = dev_alloc<double>(4 * (size_t)Nc, keep);
   double *d_adt = dev_alloc<double>(Nc, keep);
   int4 *d_cell = dev_upload(cell, keep);
-  int4 *d_rec = dev_upload(rec, keep);
+  OwnerArgs OA{};
+  OA.inst = dev_upload(inst, keep);
+  OA.base = dev_upload(pbase, keep);
+  OA.inst8 = dev_upload(inst8, keep);
+  OA.elem_offs = dev_upload(elem_offs, keep);
+  OA.owned_offs = dev_upload(owned_offs, keep);
+  OA.owned = dev_upload(owned, keep);
+  OA.csr4 = dev_upload(csr4, keep);
+  OA.max_e = max_e;
+  const size_t res_lds = sizeof(double) * 4 * (size_t)max_e;
   int2 *d_bedge = dev_upload(bedge, keep);
   int *d_bound = dev_upload(bound, keep);
   int *d_bmesh = dev_upload(bmesh, keep);
@@ -508,8 +526,9 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   int nsm = 0;
   ECHECK(hipDeviceGetAttribute(&nsm, hipDeviceAttributeMultiprocessorCount, 0));
   // block sizes of save_soln, adt_calc, res_calc, bres_calc (the defaults, or
-  // the autotuner's picks); every kernel's result is independent of them
-  int blk[4] = {256, 1024, 512, 128};
+  // the autotuner's picks); every kernel's result is independent of them.
+  // res_calc's is its owner patch size, fixed by the partition.
+  int blk[4] = {256, 1024, kPatch, 128};
   // with `ev`, each launch is bracketed by events ev[2j], ev[2j+1] in launch
   // order: copy, then adt, res, bres of each half step
   auto step_range = [&](int mlo, int mhi, hipEvent_t *ev = nullptr) {
@@ -529,8 +548,9 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       k_ens_adt<<<grid_for(c1 - c0, blk[1]), blk[1], 0, st>>>(d_x, d_q, d_adt, d_cell, c0, c1);
       mark();
       mark();
-      k_ens_res<<<grid_for(4LL * (c1 - c0), blk[2]), blk[2], 0, st>>>(d_x, d_q, d_adt, d_res,
-                                                                      d_rec, c0, c1);
+      if (mblk[mhi] > mblk[mlo])
+        k_ens_res_owner<<<mblk[mhi] - mblk[mlo], kPatch, res_lds, st>>>(d_x, d_q, d_adt, d_res,
+                                                                        OA, mblk[mlo]);
       mark();
       const int t0 = mesh_btarget_off[mlo], t1 = mesh_btarget_off[mhi];
       mark();
@@ -564,8 +584,9 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       ECHECK(hipEventCreate(&ev_b[m]));
     }
   }
-  // autotuning: step i (i < 5) runs every tuned kernel with candidate i; the
-  // fastest total per kernel over its launches wins
+  // autotuning: step i (i < 5) runs the save_soln, adt_calc and bres_calc
+  // kernels with candidate i; the fastest total per kernel over its launches
+  // wins
   const int cand[5] = {64, 128, 256, 512, 1024};
   const int ntune = (o->batched && o->autotune) ? std::min(5, o->niter) : 0;
   double tune_ms[4][5] = {};
@@ -580,7 +601,7 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       t0 = now();
     }
     if (it <= ntune) {
-      for (int k = 0; k < 4; ++k) blk[k] = cand[it - 1];
+      blk[0] = blk[1] = blk[3] = cand[it - 1];
       step_range(0, M, tev);
       history();
       ECHECK(hipStreamSynchronize(st));
@@ -592,7 +613,7 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
         tune_ms[kind[j]][it - 1] += ms;
       }
       if (it == ntune)
-        for (int k = 0; k < 4; ++k) {
+        for (int k : {0, 1, 3}) {
           int best = 0;
           for (int c = 1; c < ntune; ++c)
             if (tune_ms[k][c] < tune_ms[k][best]) best = c;

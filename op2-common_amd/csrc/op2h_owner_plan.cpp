@@ -97,6 +97,94 @@ static int grow_blocks(int nto, int nel, int dim, const int *map, int ninc,
   return nb;
 }
 
+bool owner_lists(int nto, int nel, int dim, const int *map, int ninc,
+                 const int *inc_idx, int block_cells, OwnerLists &L,
+                 std::string &err) {
+  std::vector<int> block_of;
+  const int nb = grow_blocks(nto, nel, dim, map, ninc, inc_idx, block_cells, block_of);
+  L.nblocks = nb;
+
+  // owned targets, ascending within each block
+  std::vector<int> &owned_offs = L.owned_offs;
+  owned_offs.assign(nb + 1, 0);
+  for (int t = 0; t < nto; ++t)
+    if (block_of[t] >= 0) owned_offs[block_of[t] + 1]++;
+  for (int b = 0; b < nb; ++b) owned_offs[b + 1] += owned_offs[b];
+  std::vector<int> &owned = L.owned;
+  owned.assign(owned_offs[nb], 0);
+  {
+    std::vector<int> f(owned_offs.begin(), owned_offs.end() - 1);
+    for (int t = 0; t < nto; ++t)
+      if (block_of[t] >= 0) owned[f[block_of[t]]++] = t;
+  }
+  // instances: each element once per distinct block among its targets
+  auto blocks_of_elem = [&](int e, int *out) {
+    int n = 0;
+    for (int k = 0; k < ninc; ++k) {
+      int b = block_of[map[(size_t)e * dim + inc_idx[k]]];
+      bool seen = false;
+      for (int j = 0; j < n; ++j) seen |= out[j] == b;
+      if (!seen) out[n++] = b;
+    }
+    return n;
+  };
+  std::vector<int> &elem_offs = L.elem_offs, tmp(ninc);
+  elem_offs.assign(nb + 1, 0);
+  for (int e = 0; e < nel; ++e) {
+    int n = blocks_of_elem(e, tmp.data());
+    for (int j = 0; j < n; ++j) elem_offs[tmp[j] + 1]++;
+  }
+  for (int b = 0; b < nb; ++b) elem_offs[b + 1] += elem_offs[b];
+  const int total = elem_offs[nb];
+  std::vector<int> &elems = L.elems;
+  elems.assign(total, 0);
+  {
+    std::vector<int> f(elem_offs.begin(), elem_offs.end() - 1);
+    for (int e = 0; e < nel; ++e) {
+      int n = blocks_of_elem(e, tmp.data());
+      for (int j = 0; j < n; ++j) elems[f[tmp[j]]++] = e;
+    }
+  }
+  L.max_elems = L.max_owned = 0;
+  for (int b = 0; b < nb; ++b) {
+    L.max_elems = std::max(L.max_elems, elem_offs[b + 1] - elem_offs[b]);
+    L.max_owned = std::max(L.max_owned, owned_offs[b + 1] - owned_offs[b]);
+  }
+  if ((long long)L.max_elems * ninc > 65535) {
+    err = "owner plan: a block has too many elements for 16-bit local ids";
+    return false;
+  }
+
+  // increments of each owned target, in (instance, k) order
+  std::vector<int> ownpos(nto, -1);
+  for (int i = 0; i < (int)owned.size(); ++i) ownpos[owned[i]] = i;
+  std::vector<int> &csr_offs = L.csr_offs;
+  csr_offs.assign(owned.size() + 1, 0);
+  for (int b = 0; b < nb; ++b)
+    for (int j = elem_offs[b]; j < elem_offs[b + 1]; ++j)
+      for (int k = 0; k < ninc; ++k) {
+        int t = map[(size_t)elems[j] * dim + inc_idx[k]];
+        if (block_of[t] == b) csr_offs[ownpos[t] + 1]++;
+      }
+  for (size_t i = 0; i < owned.size(); ++i) csr_offs[i + 1] += csr_offs[i];
+  std::vector<unsigned short> &csr = L.csr;
+  csr.assign(csr_offs.back(), 0);
+  {
+    std::vector<int> f(csr_offs.begin(), csr_offs.end() - 1);
+    for (int b = 0; b < nb; ++b)
+      for (int j = elem_offs[b]; j < elem_offs[b + 1]; ++j)
+        for (int k = 0; k < ninc; ++k) {
+          int t = map[(size_t)elems[j] * dim + inc_idx[k]];
+          if (block_of[t] == b)
+            csr[f[ownpos[t]]++] = (unsigned short)((j - elem_offs[b]) * ninc + k);
+        }
+  }
+  L.max_incs = 0;
+  for (size_t i = 0; i < owned.size(); ++i)
+    L.max_incs = std::max(L.max_incs, csr_offs[i + 1] - csr_offs[i]);
+  return true;
+}
+
 std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
                                                int nargs, const op_arg *args,
                                                hipStream_t stream,
@@ -135,89 +223,21 @@ std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
   }
   const int nel = set->size, nto = own->to->size, dim = own->dim;
   const int ninc = (int)inc_idx.size();
-  const int *map = own->map;
-  std::vector<int> block_of;
-  const int nb = grow_blocks(nto, nel, dim, map, ninc, inc_idx.data(),
-                             block_cells, block_of);
+  OwnerLists L;
+  if (!owner_lists(nto, nel, dim, own->map, ninc, inc_idx.data(), block_cells, L, err))
+    return nullptr;
   P.set_size = nel;
   P.to_size = nto;
   P.block_cells = block_cells;
-  P.nblocks = nb;
+  P.nblocks = L.nblocks;
   P.ninc = ninc;
-
-  // owned targets, ascending within each block
-  std::vector<int> owned_offs(nb + 1, 0);
-  for (int t = 0; t < nto; ++t)
-    if (block_of[t] >= 0) owned_offs[block_of[t] + 1]++;
-  for (int b = 0; b < nb; ++b) owned_offs[b + 1] += owned_offs[b];
-  std::vector<int> owned(owned_offs[nb]);
-  {
-    std::vector<int> f(owned_offs.begin(), owned_offs.end() - 1);
-    for (int t = 0; t < nto; ++t)
-      if (block_of[t] >= 0) owned[f[block_of[t]]++] = t;
-  }
-  // instances: each element once per distinct block among its targets
-  auto blocks_of_elem = [&](int e, int *out) {
-    int n = 0;
-    for (int k = 0; k < ninc; ++k) {
-      int b = block_of[map[(size_t)e * dim + inc_idx[k]]];
-      bool seen = false;
-      for (int j = 0; j < n; ++j) seen |= out[j] == b;
-      if (!seen) out[n++] = b;
-    }
-    return n;
-  };
-  std::vector<int> elem_offs(nb + 1, 0), tmp(ninc);
-  for (int e = 0; e < nel; ++e) {
-    int n = blocks_of_elem(e, tmp.data());
-    for (int j = 0; j < n; ++j) elem_offs[tmp[j] + 1]++;
-  }
-  for (int b = 0; b < nb; ++b) elem_offs[b + 1] += elem_offs[b];
-  const int total = elem_offs[nb];
-  std::vector<int> elems(total);
-  {
-    std::vector<int> f(elem_offs.begin(), elem_offs.end() - 1);
-    for (int e = 0; e < nel; ++e) {
-      int n = blocks_of_elem(e, tmp.data());
-      for (int j = 0; j < n; ++j) elems[f[tmp[j]]++] = e;
-    }
-  }
-  P.total = total;
-  P.redundancy = nel ? (double)total / nel : 1.0;
-  P.max_elems = P.max_owned = 0;
-  for (int b = 0; b < nb; ++b) {
-    P.max_elems = std::max(P.max_elems, elem_offs[b + 1] - elem_offs[b]);
-    P.max_owned = std::max(P.max_owned, owned_offs[b + 1] - owned_offs[b]);
-  }
-  if ((long long)P.max_elems * ninc > 65535) {
-    err = "owner plan: a block has too many elements for 16-bit local ids";
-    return nullptr;
-  }
-
-  // increments of each owned target, in (instance, k) order
-  std::vector<int> ownpos(nto, -1);
-  for (int i = 0; i < (int)owned.size(); ++i) ownpos[owned[i]] = i;
-  std::vector<int> csr_offs(owned.size() + 1, 0);
-  for (int b = 0; b < nb; ++b)
-    for (int j = elem_offs[b]; j < elem_offs[b + 1]; ++j)
-      for (int k = 0; k < ninc; ++k) {
-        int t = map[(size_t)elems[j] * dim + inc_idx[k]];
-        if (block_of[t] == b) csr_offs[ownpos[t] + 1]++;
-      }
-  for (size_t i = 0; i < owned.size(); ++i) csr_offs[i + 1] += csr_offs[i];
-  std::vector<unsigned short> csr(csr_offs.back());
-  {
-    std::vector<int> f(csr_offs.begin(), csr_offs.end() - 1);
-    for (int b = 0; b < nb; ++b)
-      for (int j = elem_offs[b]; j < elem_offs[b + 1]; ++j)
-        for (int k = 0; k < ninc; ++k) {
-          int t = map[(size_t)elems[j] * dim + inc_idx[k]];
-          if (block_of[t] == b)
-            csr[f[ownpos[t]]++] =
-                (unsigned short)((j - elem_offs[b]) * ninc + k);
-        }
-  }
-
+  P.total = (int)L.elems.size();
+  P.redundancy = nel ? (double)P.total / nel : 1.0;
+  P.max_elems = L.max_elems;
+  P.max_owned = L.max_owned;
+  const std::vector<int> &owned_offs = L.owned_offs, &owned = L.owned,
+                         &elem_offs = L.elem_offs, &elems = L.elems, &csr_offs = L.csr_offs;
+  const std::vector<unsigned short> &csr = L.csr;
   P.owned_offs = rec->mem.upload(owned_offs, stream);
   P.owned = rec->mem.upload(owned, stream);
   P.elem_offs = rec->mem.upload(elem_offs, stream);
@@ -227,9 +247,7 @@ std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
   rec->n_owned = (int)owned.size();
   rec->n_csr = (int)csr.size();
   P.n_owned = rec->n_owned;
-  P.max_incs = 0;
-  for (size_t i = 0; i < owned.size(); ++i)
-    P.max_incs = std::max(P.max_incs, csr_offs[i + 1] - csr_offs[i]);
+  P.max_incs = L.max_incs;
   OCHECK(hipStreamSynchronize(stream));
   return rec;
 }

@@ -81,6 +81,21 @@ std::unique_ptr<ColorPlanRec> build_color_plan(op_set set, int nargs,
 std::unique_ptr<GatherPlanRec> build_gather_plan(op_map map, hipStream_t stream);
 // Owner-compute partition plan (op2h_owner_plan); nullptr and `err` set when
 // the loop does not have the required shape.
+// The owner partition's lists for targets [0,nto) of a map (nel x dim,
+// row-major) incremented through the map columns inc_idx[0..ninc): blocks of
+// block_cells targets (the last may be smaller), per block its owned targets
+// (ascending) and the elements touching them (ascending), per owned target
+// its increments as 16-bit (local element * ninc + k) entries in element
+// order.  False, with err set, when a block's local ids overflow 16 bits.
+struct OwnerLists {
+  int nblocks = 0, max_elems = 0, max_owned = 0, max_incs = 0;
+  std::vector<int> owned_offs, owned, elem_offs, elems, csr_offs;
+  std::vector<unsigned short> csr;
+};
+bool owner_lists(int nto, int nel, int dim, const int *map, int ninc,
+                 const int *inc_idx, int block_cells, OwnerLists &L,
+                 std::string &err);
+
 std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
                                                int nargs, const op_arg *args,
                                                hipStream_t stream,

@@ -345,6 +345,7 @@ extern "C" void op_init(int argc, char **argv, int diags) {
     else if (!std::strncmp(a, "OP_CACHE_LINE_SIZE=", 19)) OP_cache_line_size = atoi(a + 19);
     else if (!std::strcmp(a, "OP_AUTO_SOA")) OP_auto_soa = 1;
     else if (!std::strncmp(a, "OP_MAPS_BASE_INDEX=", 19)) OP_maps_base_index = atoi(a + 19);
+    else if (!std::strcmp(a, "OP_HIP_KERNEL_TIMING")) op2h_enable_kernel_timing(1);
     else if (!std::strncmp(a, "OP_HIP_PLAN=", 12)) {
       // OP_HIP_PLAN=<loop>:<plan>[:<block>]
       std::string s(a + 12);

@@ -115,48 +115,85 @@ class LoopGen:
         f.append("  int _unused;")
         return f"struct {self.name}_args {{\n" + "\n".join(f) + "\n};\n"
 
+    def _staged(self, j, a):
+        """AoS data of arg j goes through registers when the arg reads it
+        (OP_READ, OP_RW, a direct OP_INC) and no other arg of the loop writes
+        its dat: the kernel's loads then issue together, not each after the
+        previous store through a pointer that might alias it.  OP_WRITE args
+        keep pointers into the dat, so components the kernel leaves alone
+        stay as they were."""
+        if a.acc == "OP_WRITE":
+            return False
+        for i, b in enumerate(self.L.args):
+            if i != j and isinstance(b, ArgDat) and b.dat == a.dat and b.acc != "OP_READ":
+                return False
+        return True
+
+    def _dat_arg(self, pre, post, j, a, inc=None):
+        """Code for dat arg j around the user kernel's call; returns the
+        expression passed for it.  Data goes through the registers b{j}
+        (indexed by constants after inlining, so they stay registers): the
+        increments of an indirect OP_INC (inc: "atomic" adds them to the dat
+        after the call, atomically or not as the launch says; "lds" stores
+        them to the block's LDS slots), staged args (_staged), and every arg
+        under SoA.  An AoS arg that is not staged is passed as a pointer into
+        the dat.  LAY is the kernel's template parameter (0: every
+        multi-component dat AoS, 1: all SoA, 2: mixed, each arg by its own
+        stride and always through registers), so the layout choice folds at
+        compile time."""
+        T, D = a.typ, a.dim
+        tgt = "n" if a.direct else f"A.m{j}[(size_t)n * A.md{j} + {a.idx}]"
+        soa = f"(LAY == 1 || (LAY == 2 && A.s{j}))"
+        idx = f"({soa} ? t{j} + (size_t)k * A.s{j} : (size_t)t{j} * {D} + k)"
+        opt = a.opt is not None
+        guard = f"if (A.opt{j}) " if opt else ""
+        pre.append(f"    const int t{j} = {f'A.opt{j} ? {tgt} : 0' if opt else tgt};")
+        pre.append(f"    {T} b{j}[{D}];")
+        writes = a.acc != "OP_READ"
+        if inc is not None:
+            pre.append(f"    for (int k = 0; k < {D}; ++k) b{j}[k] = ({T})0;")
+            ptr = f"b{j}"
+            if inc == "atomic":
+                post.append(f"    {guard}{{")
+                post.append(f"      if (atomic) {{")
+                post.append(f"        for (int k = 0; k < {D}; ++k) op2gen_atomic_add(A.d{j} + {idx}, b{j}[k]);")
+                post.append(f"      }} else {{")
+                post.append(f"        for (int k = 0; k < {D}; ++k) A.d{j}[{idx}] += b{j}[k];")
+                post.append(f"      }}")
+                post.append(f"    }}")
+            else:
+                post.append(f"    for (int k = 0; k < {D}; ++k) lds[((size_t)j * {inc[1]} + {inc[2]}) * {D} + k] = b{j}[k];")
+        elif self._staged(j, a):
+            pre.append(f"    {guard}for (int k = 0; k < {D}; ++k) b{j}[k] = A.d{j}[{idx}];")
+            ptr = f"b{j}"
+            if writes:
+                post.append(f"    {guard}for (int k = 0; k < {D}; ++k) A.d{j}[{idx}] = b{j}[k];")
+        else:
+            # loaded through registers even for OP_WRITE, so that components
+            # the kernel leaves alone are stored back unchanged
+            pre.append(f"    if (LAY != 0) {{")
+            pre.append(f"      {guard}for (int k = 0; k < {D}; ++k) b{j}[k] = A.d{j}[{idx}];")
+            pre.append(f"    }}")
+            ptr = f"(LAY != 0 ? b{j} : A.d{j} + (size_t)t{j} * {D})"
+            if writes:
+                post.append(f"    if (LAY != 0) {{")
+                post.append(f"      {guard}for (int k = 0; k < {D}; ++k) A.d{j}[{idx}] = b{j}[k];")
+                post.append(f"    }}")
+        return f"(A.opt{j} ? {ptr} : nullptr)" if opt else ptr
+
     # ---- device kernel ----
     def kernel(self):
         L = self.L
         pre, call, post = [], [], []
         for j, a in enumerate(L.args):
             if isinstance(a, ArgDat):
-                T, D = a.typ, a.dim
-                tgt = "n" if a.direct else f"A.m{j}[(size_t)n * A.md{j} + {a.idx}]"
-                pre.append(f"    {T} *p{j} = nullptr;")
-                pre.append(f"    {T} b{j}[{D}];")
-                pre.append(f"    const int t{j} = A.opt{j} ? {tgt} : 0;")
                 inc_ind = (not a.direct) and a.acc == "OP_INC"
-                if inc_ind:
-                    pre.append(f"    for (int k = 0; k < {D}; ++k) b{j}[k] = ({T})0;")
-                    pre.append(f"    if (A.opt{j}) p{j} = b{j};")
-                    post.append(f"    if (A.opt{j}) {{")
-                    post.append(f"      if (atomic) {{")
-                    post.append(f"        for (int k = 0; k < {D}; ++k) op2gen_atomic_add(A.d{j} + "
-                                f"(A.s{j} ? t{j} + (size_t)k * A.s{j} : (size_t)t{j} * {D} + k), b{j}[k]);")
-                    post.append(f"      }} else {{")
-                    post.append(f"        for (int k = 0; k < {D}; ++k) A.d{j}[A.s{j} ? t{j} + (size_t)k * A.s{j} "
-                                f": (size_t)t{j} * {D} + k] += b{j}[k];")
-                    post.append(f"      }}")
-                    post.append(f"    }}")
-                else:
-                    pre.append(f"    if (A.opt{j}) {{")
-                    pre.append(f"      if (A.s{j}) {{")
-                    pre.append(f"        for (int k = 0; k < {D}; ++k) b{j}[k] = A.d{j}[t{j} + (size_t)k * A.s{j}];")
-                    pre.append(f"        p{j} = b{j};")
-                    pre.append(f"      }} else {{")
-                    pre.append(f"        p{j} = A.d{j} + (size_t)t{j} * {D};")
-                    pre.append(f"      }}")
-                    pre.append(f"    }}")
-                    if a.acc != "OP_READ":
-                        post.append(f"    if (A.opt{j} && A.s{j})")
-                        post.append(f"      for (int k = 0; k < {D}; ++k) A.d{j}[t{j} + (size_t)k * A.s{j}] = b{j}[k];")
-                call.append(f"p{j}")
+                call.append(self._dat_arg(pre, post, j, a, "atomic" if inc_ind else None))
             elif isinstance(a, ArgGbl):
                 if a.acc == "OP_READ":
                     pre.append(f"    {a.typ} g{j}[{a.dim}];")
                     pre.append(f"    for (int k = 0; k < {a.dim}; ++k) g{j}[k] = A.g{j}[k];")
-                    call.append(f"A.opt{j} ? g{j} : nullptr")
+                    call.append(f"(A.opt{j} ? g{j} : nullptr)" if a.opt is not None else f"g{j}")
                 else:
                     call.append(f"r{j}")
             elif isinstance(a, ArgIdx):
@@ -184,6 +221,7 @@ class LoopGen:
                 track.append(f"    const {L.args[j].typ} before{j} = r{j}[0];")
         track_after = [f"    if (r{j}[0] != before{j}) at{j} = n;" for j in self.reds if j in self.info_of]
         body = []
+        body.append(f"template <int LAY>")
         body.append(f"__global__ void __launch_bounds__(1024)")
         body.append(f"    {self.name}_kernel({self.name}_args A, const int *__restrict elems, int start,")
         body.append(f"                 int end, int atomic, unsigned char *partials, int part_base) {{")
@@ -239,32 +277,12 @@ class LoopGen:
         pre, call, post = [], [], []
         for j, a in enumerate(L.args):
             if isinstance(a, ArgDat):
-                tgt = "n" if a.direct else f"A.m{j}[(size_t)n * A.md{j} + {a.idx}]"
-                pre.append(f"    {a.typ} *p{j} = nullptr;")
-                pre.append(f"    {a.typ} b{j}[{a.dim}];")
-                pre.append(f"    const int t{j} = A.opt{j} ? {tgt} : 0;")
-                if j in incs:
-                    k = incs.index(j)
-                    pre.append(f"    for (int k = 0; k < {a.dim}; ++k) b{j}[k] = ({a.typ})0;")
-                    pre.append(f"    p{j} = b{j};")
-                    post.append(f"    for (int k = 0; k < {D}; ++k) lds[((size_t)j * {ninc} + {k}) * {D} + k] = b{j}[k];")
-                else:
-                    pre.append(f"    if (A.opt{j}) {{")
-                    pre.append(f"      if (A.s{j}) {{")
-                    pre.append(f"        for (int k = 0; k < {a.dim}; ++k) b{j}[k] = A.d{j}[t{j} + (size_t)k * A.s{j}];")
-                    pre.append(f"        p{j} = b{j};")
-                    pre.append(f"      }} else {{")
-                    pre.append(f"        p{j} = A.d{j} + (size_t)t{j} * {a.dim};")
-                    pre.append(f"      }}")
-                    pre.append(f"    }}")
-                    if a.acc != "OP_READ":
-                        post.append(f"    if (A.opt{j} && A.s{j})")
-                        post.append(f"      for (int k = 0; k < {a.dim}; ++k) A.d{j}[t{j} + (size_t)k * A.s{j}] = b{j}[k];")
-                call.append(f"p{j}")
+                inc = ("lds", ninc, incs.index(j)) if j in incs else None
+                call.append(self._dat_arg(pre, post, j, a, inc))
             elif isinstance(a, ArgGbl):
                 pre.append(f"    {a.typ} g{j}[{a.dim}];")
                 pre.append(f"    for (int k = 0; k < {a.dim}; ++k) g{j}[k] = A.g{j}[k];")
-                call.append(f"A.opt{j} ? g{j} : nullptr")
+                call.append(f"(A.opt{j} ? g{j} : nullptr)" if a.opt is not None else f"g{j}")
             elif isinstance(a, ArgIdx):
                 if a.map:
                     pre.append(f"    int x{j} = A.m{j}[(size_t)n * A.md{j} + {a.idx}];")
@@ -272,7 +290,8 @@ class LoopGen:
                     pre.append(f"    int x{j} = n;")
                 call.append(f"&x{j}")
         w = incs[0]
-        b = [f"__global__ void __launch_bounds__(1024)",
+        b = [f"template <int LAY>",
+             f"__global__ void __launch_bounds__(1024)",
              f"    {self.name}_owner({self.name}_args A, const int *__restrict elem_offs,",
              f"                 const int *__restrict elems, const int *__restrict owned_offs,",
              f"                 const int *__restrict owned, const int *__restrict csr_offs,",
@@ -292,12 +311,12 @@ class LoopGen:
               f"  for (int i = threadIdx.x; i < no; i += blockDim.x) {{",
               f"    const int t = owned[o0 + i];",
               f"    {T} acc[{D}];",
-              f"    for (int k = 0; k < {D}; ++k) acc[k] = A.d{w}[A.s{w} ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k];",
+              f"    for (int k = 0; k < {D}; ++k) acc[k] = A.d{w}[(LAY == 1 || (LAY == 2 && A.s{w})) ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k];",
               f"    for (int c = csr_offs[o0 + i]; c < csr_offs[o0 + i + 1]; ++c) {{",
               f"      const {T} *inc = lds + (size_t)csr[c] * {D};",
               f"      for (int k = 0; k < {D}; ++k) acc[k] += inc[k];",
               f"    }}",
-              f"    for (int k = 0; k < {D}; ++k) A.d{w}[A.s{w} ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k] = acc[k];",
+              f"    for (int k = 0; k < {D}; ++k) A.d{w}[(LAY == 1 || (LAY == 2 && A.s{w})) ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k] = acc[k];",
               f"  }}",
               f"}}"]
         return "\n".join(b) + "\n"
@@ -375,6 +394,16 @@ class LoopGen:
             elif isinstance(a, ArgIdx) and a.map:
                 s.append(f"    A.m{j} = args[{j}].map_data_d;")
                 s.append(f"    A.md{j} = args[{j}].map->dim;")
+        # the kernels are instantiated for all-AoS, all-SoA and mixed layouts
+        # of the loop's multi-component dats
+        s.append(f"    int layout = -1;  // 0 AoS, 1 SoA, 2 mixed, -1 no multi-component dat")
+        for j, a in enumerate(L.args):
+            if isinstance(a, ArgDat) and a.dim > 1:
+                s.append(f"    if (args[{j}].opt) {{")
+                s.append(f"      const int l = A.s{j} != 0;")
+                s.append(f"      layout = layout < 0 || layout == l ? l : 2;")
+                s.append(f"    }}")
+
         s.append(f"    const int bs = ctx.block_size;")
         if self.owner:
             L2 = self.L
@@ -390,11 +419,16 @@ class LoopGen:
             s.append(f'        printf("op2hip: %s: owner plan needs %zu bytes of LDS; use a smaller block\\n", name, lds);')
             s.append(f"        exit(-1);")
             s.append(f"      }}")
-            s.append(f"      op2h_kernel_timer_start(&ctx);")
+            s.append(f"      void *t0 = nullptr, *t1 = nullptr;")
+            s.append(f"      const bool ext = op->nblocks && op2h_kernel_timer_events(&ctx, &t0, &t1) != 0;")
+            s.append(f"      if (!ext) op2h_kernel_timer_start(&ctx);")
             s.append(f"      if (op->nblocks)")
-            s.append(f"        {self.name}_owner<<<op->nblocks, bs, lds, st>>>(A, op->elem_offs, op->elems,")
-            s.append(f"            op->owned_offs, op->owned, op->csr_offs, op->csr);")
-            s.append(f"      op2h_kernel_timer_stop(&ctx);")
+            s.append(f"        op2gen_launch(ext, t0, t1, layout == 2 ? {self.name}_owner<2> : layout == 1 ?")
+            s.append(f"                      {self.name}_owner<1> : {self.name}_owner<0>,")
+            s.append(f"                      op->nblocks, bs, lds, st, A,")
+            s.append(f"                      op->elem_offs, op->elems, op->owned_offs, op->owned,")
+            s.append(f"                      op->csr_offs, op->csr);")
+            s.append(f"      if (!ext) op2h_kernel_timer_stop(&ctx);")
             s.append(f"      OP2GEN_CHECK(hipGetLastError());")
             s.append(f"      OP_kernels[{L2.index - 1}].transfer += (float)op2h_algorithmic_bytes(set, {len(L2.args)}, args);")
             s.append(f"      op2h_loop_end(&ctx);")
@@ -412,23 +446,33 @@ class LoopGen:
             s.append(f"    unsigned char *partials = (unsigned char *)op2h_reduct_scratch((size_t)nparts * {self.rec});")
         else:
             s.append(f"    unsigned char *partials = nullptr;")
-        s.append(f"    op2h_kernel_timer_start(&ctx);")
+        s.append(f"    auto kern = layout == 2 ? {self.name}_kernel<2> : layout == 1 ? {self.name}_kernel<1> :")
+        s.append(f"                {self.name}_kernel<0>;")
+        s.append(f"    // one launch (two with a reduction's finish) outside colouring: timed by")
+        s.append(f"    // events attached to the kernels; colour launches by an event pair")
+        s.append(f"    void *t0 = nullptr, *t1 = nullptr;")
+        s.append(f"    const bool ext = !cp && nparts > 0 && op2h_kernel_timer_events(&ctx, &t0, &t1) != 0;")
+        s.append(f"    if (!ext) op2h_kernel_timer_start(&ctx);")
         s.append(f"    if (cp) {{")
         s.append(f"      int base = 0;")
         s.append(f"      for (int c = 0; c < cp->ncolors; ++c) {{")
         s.append(f"        const int s0 = cp->col_offsets_h[c], s1 = cp->col_offsets_h[c + 1];")
         s.append(f"        const int g = (s1 - s0 + bs - 1) / bs;")
-        s.append(f"        if (g) {self.name}_kernel<<<g, bs, 0, st>>>(A, cp->col_reord, s0, s1, 0, partials, base);")
+        s.append(f"        if (g) op2gen_launch(false, nullptr, nullptr, kern, g, bs, 0, st, A, cp->col_reord,")
+        s.append(f"                             s0, s1, 0, partials, base);")
         s.append(f"        base += g;")
         s.append(f"      }}")
-        s.append(f"    }} else {{")
-        s.append(f"      {self.name}_kernel<<<nparts, bs, 0, st>>>(A, nullptr, 0, set->size, 1, partials, 0);")
+        s.append(f"    }} else if (nparts > 0) {{")
+        stop = "nullptr" if self.reds else "t1"
+        s.append(f"      op2gen_launch(ext, t0, {stop}, kern, nparts, bs, 0, st, A,")
+        s.append(f"                    nullptr, 0, set->size, 1, partials, 0);")
         s.append(f"    }}")
         if self.reds:
             slots = ", ".join(f"(unsigned char *)op2h_gbl_slot(&ctx, {j}, {self.info_of.get(j, -1)})"
                               for j in self.reds)
-            s.append(f"    {self.name}_finish<<<1, 256, 0, st>>>(partials, nparts, {slots});")
-        s.append(f"    op2h_kernel_timer_stop(&ctx);")
+            s.append(f"    op2gen_launch(ext, nullptr, t1, {self.name}_finish, 1, 256, 0, st, partials,")
+            s.append(f"                  nparts, {slots});")
+        s.append(f"    if (!ext) op2h_kernel_timer_stop(&ctx);")
         s.append(f"    OP2GEN_CHECK(hipGetLastError());")
         s.append(f"  }}")
         s.append(f"  OP_kernels[{L.index - 1}].transfer += (float)op2h_algorithmic_bytes(set, {N}, args);")
@@ -438,11 +482,14 @@ class LoopGen:
 
 
 PRELUDE = r'''#include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <tuple>
+#include <utility>
 
 #include "op2_hip.h"
 #include "op2_hip_rt.h"
@@ -458,6 +505,28 @@ PRELUDE = r'''#include <hip/hip_runtime.h>
   } while (0)
 
 namespace {
+
+// Launch with the kernel's declared argument types; with timing events
+// (op2h_kernel_timer_events) through hipExtLaunchKernel, which stamps e0 at
+// the kernel's start and e1 at its end (either may be NULL).
+template <class Tuple, std::size_t... I>
+void op2gen_arg_ptrs(Tuple &t, void **p, std::index_sequence<I...>) {
+  ((p[I] = (void *)&std::get<I>(t)), ...);
+}
+
+template <typename... KArgs, typename... Args>
+void op2gen_launch(bool ext, void *e0, void *e1, void (*kernel)(KArgs...), dim3 grid,
+                   dim3 block, size_t lds, hipStream_t st, Args... args) {
+  static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+  std::tuple<KArgs...> t{static_cast<KArgs>(args)...};
+  void *p[sizeof...(KArgs) > 0 ? sizeof...(KArgs) : 1];
+  op2gen_arg_ptrs(t, p, std::index_sequence_for<KArgs...>{});
+  if (ext)
+    OP2GEN_CHECK(hipExtLaunchKernel((const void *)kernel, grid, block, p, lds, st,
+                                    (hipEvent_t)e0, (hipEvent_t)e1, 0));
+  else
+    OP2GEN_CHECK(hipLaunchKernel((const void *)kernel, grid, block, p, lds, st));
+}
 
 template <class T> struct op2gen_lim {
   __device__ static T max() { return std::numeric_limits<T>::max(); }

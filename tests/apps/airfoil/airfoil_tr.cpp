@@ -2,7 +2,8 @@
 // dats, constants, five op_par_loops per step), for translation by
 // op2amd.translator.  Usage: airfoil_tr <new_grid.dat> <iterations> [op_init
 // tokens...].  Prints "iter rms maxerr errloc" per iteration with 17
-// significant digits, then a checksum of q.  The step is the reference
+// significant digits, the loop's wall time, op_timing_output's per-loop table
+// (kernel times with the OP_HIP_KERNEL_TIMING token), then a checksum of q.  The step is the reference
 // program's (apps/c/airfoil/airfoil_plain/dp/airfoil.cpp:221-297); the mesh
 // file format is its reader's (:88-143).
 #include <cmath>
@@ -94,6 +95,8 @@ int main(int argc, char **argv) {
   op_decl_const(1, "double", &alpha);
   op_decl_const(4, "double", qinf);
 
+  double cpu0, wall0, cpu1, wall1;
+  op_timers(&cpu0, &wall0);
   for (int iter = 1; iter <= niter; iter++) {
     op_par_loop(save_soln, "save_soln", cells,
                 op_arg_dat(p_q, -1, OP_ID, 4, "double", OP_READ),
@@ -139,6 +142,9 @@ int main(int argc, char **argv) {
     }
     std::printf("%d %.17e %.17e %d\n", iter, std::sqrt(rms / ncell), maxerr, errloc);
   }
+  op_timers(&cpu1, &wall1);
+  std::printf("time %.6f s for %d iterations\n", wall1 - wall0, niter);
+  op_timing_output();
   std::vector<double> qout(4 * ncell);
   op_fetch_data(p_q, qout.data());
   double sum = 0.0, sumsq = 0.0;

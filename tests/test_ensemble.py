@@ -56,6 +56,7 @@ def test_autotuned_block_sizes_change_no_bits():
                                 autotune=True)
     for key in ("rms", "maxerr", "errloc", "q"):
         assert np.array_equal(tuned[key], plain[key]), key
-    assert plain["blocks"] == {"save_soln": 256, "adt_calc": 1024, "res_calc": 512,
+    assert plain["blocks"] == {"save_soln": 256, "adt_calc": 1024, "res_calc": 128,
                                "bres_calc": 128}
     assert set(tuned["blocks"].values()) <= {64, 128, 256, 512, 1024}
+    assert tuned["blocks"]["res_calc"] == 128
