@@ -194,6 +194,9 @@ typedef struct {
   double build_ms;
   int n_owned;                 /* owned targets over all blocks */
   int max_incs;                /* most increments of one owned target */
+  const unsigned short *csr4;  /* [n_owned][4]: each owned target's
+                                  increments as a fixed-width list (0xFFFF
+                                  pads), when max_incs <= 4; else NULL */
 } op2h_owner_plan;
 
 const op2h_staged_plan *op2h_plan_staged(op2h_loop_ctx *ctx);

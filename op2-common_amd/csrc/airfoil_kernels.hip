@@ -417,20 +417,6 @@ __global__ void __launch_bounds__(1024)
 }
 
 
-// Fixed-width increment lists of an owner plan whose targets have at most
-// four increments each (quadrilateral cells): one 8-byte load per cell in
-// place of an offset pair and a list.
-__global__ void k_pack_owner_csr4(const int *__restrict csr_offs,
-                                  const unsigned short *__restrict csr, int n,
-                                  ushort4 *__restrict out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int k0 = csr_offs[i], cnt = csr_offs[i + 1] - k0;
-  unsigned short v[4];
-  for (int j = 0; j < 4; ++j) v[j] = j < cnt ? csr[k0 + j] : (unsigned short)0xFFFF;
-  out[i] = make_ushort4(v[0], v[1], v[2], v[3]);
-}
-
 // Per-instance records of res_calc's owner plan: {pedge[e], pecell[e]}.
 __global__ void k_pack_owner_res(const int2 *__restrict pedge,
                                  const int2 *__restrict pecell,
@@ -526,17 +512,10 @@ static OwnerArgs owner_args_res(const op2h_owner_plan *p, const op_arg *args,
   A.owned = p->owned;
   A.csr_offs = p->csr_offs;
   A.csr = p->csr;
-  A.csr4 = nullptr;
+  // the plan's fixed-width increment lists (quadrilateral cells: at most
+  // four edges each): one 8-byte load per owned cell
+  A.csr4 = (const ushort4 *)p->csr4;
   A.max_e = p->max_elems;
-  if (p->max_incs <= 4) {
-    snprintf(key, sizeof key, "res_calc.owner4.%d.%d.%d", bs, args[0].map->index,
-             args[2].map->index);
-    ushort4 *c4 = (ushort4 *)op2h_plan_user(key, sizeof(ushort4) * (size_t)p->n_owned, &fresh);
-    if (fresh && p->n_owned)
-      k_pack_owner_csr4<<<grid_for(p->n_owned, 256), 256, 0, st>>>(p->csr_offs, p->csr,
-                                                                   p->n_owned, c4);
-    A.csr4 = c4;
-  }
   return A;
 }
 

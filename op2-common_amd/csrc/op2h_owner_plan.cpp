@@ -248,6 +248,14 @@ std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
   rec->n_csr = (int)csr.size();
   P.n_owned = rec->n_owned;
   P.max_incs = L.max_incs;
+  P.csr4 = nullptr;
+  if (L.max_incs <= 4) {
+    std::vector<unsigned short> c4(4 * owned.size(), 0xFFFF);
+    for (size_t i = 0; i < owned.size(); ++i)
+      for (int k = csr_offs[i]; k < csr_offs[i + 1]; ++k) c4[4 * i + (k - csr_offs[i])] = csr[k];
+    P.csr4 = rec->mem.upload(c4, stream);
+    OCHECK(hipStreamSynchronize(stream));  // c4 is a local
+  }
   OCHECK(hipStreamSynchronize(stream));
   return rec;
 }

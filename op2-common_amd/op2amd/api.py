@@ -139,7 +139,7 @@ class op_owner_plan(C.Structure):
                 ("csr_offs", C.c_void_p), ("csr", C.c_void_p),
                 ("inc_of_arg", C.c_int * 32),
                 ("redundancy", C.c_double), ("build_ms", C.c_double),
-                ("n_owned", C.c_int), ("max_incs", C.c_int)]
+                ("n_owned", C.c_int), ("max_incs", C.c_int), ("csr4", C.c_void_p)]
 
 
 op2h_plan_owner_for = _fn("op2h_plan_owner_for", C.POINTER(op_owner_plan),

@@ -107,11 +107,11 @@ class LoopGen:
             if isinstance(a, ArgDat):
                 f.append(f"  {a.typ} *d{j}; int s{j}; int opt{j};")
                 if not a.direct:
-                    f.append(f"  const int *m{j}; int md{j};")
+                    f.append(f"  const int *m{j}; int md{j}; const int *r{j};")
             elif isinstance(a, ArgGbl) and a.acc == "OP_READ":
                 f.append(f"  {a.typ} g{j}[{a.dim}]; int opt{j};")
             elif isinstance(a, ArgIdx) and a.map:
-                f.append(f"  const int *m{j}; int md{j};")
+                f.append(f"  const int *m{j}; int md{j}; const int *r{j};")
         f.append("  int _unused;")
         return f"struct {self.name}_args {{\n" + "\n".join(f) + "\n};\n"
 
@@ -129,7 +129,7 @@ class LoopGen:
                 return False
         return True
 
-    def _dat_arg(self, pre, post, j, a, inc=None):
+    def _dat_arg(self, pre, post, j, a, inc=None, rows=False):
         """Code for dat arg j around the user kernel's call; returns the
         expression passed for it.  Data goes through the registers b{j}
         (indexed by constants after inlining, so they stay registers): the
@@ -142,7 +142,10 @@ class LoopGen:
         stride and always through registers), so the layout choice folds at
         compile time."""
         T, D = a.typ, a.dim
-        tgt = "n" if a.direct else f"A.m{j}[(size_t)n * A.md{j} + {a.idx}]"
+        # rows: the owner kernel reads instance-ordered copies of the map rows
+        row = f"A.r{j}[(size_t)(e0 + j) * A.md{j} + {a.idx}]" if rows else \
+            f"A.m{j}[(size_t)n * A.md{j} + {a.idx}]"
+        tgt = "n" if a.direct else row
         soa = f"(LAY == 1 || (LAY == 2 && A.s{j}))"
         idx = f"({soa} ? t{j} + (size_t)k * A.s{j} : (size_t)t{j} * {D} + k)"
         opt = a.opt is not None
@@ -278,14 +281,14 @@ class LoopGen:
         for j, a in enumerate(L.args):
             if isinstance(a, ArgDat):
                 inc = ("lds", ninc, incs.index(j)) if j in incs else None
-                call.append(self._dat_arg(pre, post, j, a, inc))
+                call.append(self._dat_arg(pre, post, j, a, inc, rows=True))
             elif isinstance(a, ArgGbl):
                 pre.append(f"    {a.typ} g{j}[{a.dim}];")
                 pre.append(f"    for (int k = 0; k < {a.dim}; ++k) g{j}[k] = A.g{j}[k];")
                 call.append(f"(A.opt{j} ? g{j} : nullptr)" if a.opt is not None else f"g{j}")
             elif isinstance(a, ArgIdx):
                 if a.map:
-                    pre.append(f"    int x{j} = A.m{j}[(size_t)n * A.md{j} + {a.idx}];")
+                    pre.append(f"    int x{j} = A.r{j}[(size_t)(e0 + j) * A.md{j} + {a.idx}];")
                 else:
                     pre.append(f"    int x{j} = n;")
                 call.append(f"&x{j}")
@@ -295,7 +298,8 @@ class LoopGen:
              f"    {self.name}_owner({self.name}_args A, const int *__restrict elem_offs,",
              f"                 const int *__restrict elems, const int *__restrict owned_offs,",
              f"                 const int *__restrict owned, const int *__restrict csr_offs,",
-             f"                 const unsigned short *__restrict csr) {{",
+             f"                 const unsigned short *__restrict csr,",
+             f"                 const ushort4 *__restrict csr4) {{",
              f"  extern __shared__ unsigned char lds_raw[];",
              f"  {T} *lds = reinterpret_cast<{T} *>(lds_raw);",
              f"  const int blk = blockIdx.x;",
@@ -312,9 +316,17 @@ class LoopGen:
               f"    const int t = owned[o0 + i];",
               f"    {T} acc[{D}];",
               f"    for (int k = 0; k < {D}; ++k) acc[k] = A.d{w}[(LAY == 1 || (LAY == 2 && A.s{w})) ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k];",
-              f"    for (int c = csr_offs[o0 + i]; c < csr_offs[o0 + i + 1]; ++c) {{",
-              f"      const {T} *inc = lds + (size_t)csr[c] * {D};",
-              f"      for (int k = 0; k < {D}; ++k) acc[k] += inc[k];",
+              f"    if (csr4) {{  // fixed-width lists: one 8-byte load",
+              f"      const ushort4 e = csr4[o0 + i];",
+              f"      const unsigned short es[4] = {{e.x, e.y, e.z, e.w}};",
+              f"      for (int c = 0; c < 4; ++c)",
+              f"        if (es[c] != 0xFFFF)",
+              f"          for (int k = 0; k < {D}; ++k) acc[k] += lds[(size_t)es[c] * {D} + k];",
+              f"    }} else {{",
+              f"      for (int c = csr_offs[o0 + i]; c < csr_offs[o0 + i + 1]; ++c) {{",
+              f"        const {T} *inc = lds + (size_t)csr[c] * {D};",
+              f"        for (int k = 0; k < {D}; ++k) acc[k] += inc[k];",
+              f"      }}",
               f"    }}",
               f"    for (int k = 0; k < {D}; ++k) A.d{w}[(LAY == 1 || (LAY == 2 && A.s{w})) ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k] = acc[k];",
               f"  }}",
@@ -419,6 +431,25 @@ class LoopGen:
             s.append(f'        printf("op2hip: %s: owner plan needs %zu bytes of LDS; use a smaller block\\n", name, lds);')
             s.append(f"        exit(-1);")
             s.append(f"      }}")
+            # instance-ordered copies of each map's rows, packed once per plan
+            by_map = {}
+            for j, a in enumerate(L2.args):
+                if (isinstance(a, ArgDat) and not a.direct) or (isinstance(a, ArgIdx) and a.map):
+                    by_map.setdefault(a.map, []).append(j)
+            for mp, js in by_map.items():
+                j0 = js[0]
+                s.append(f"      if (args[{j0}].opt) {{")
+                s.append(f"        char key[160];")
+                s.append(f'        snprintf(key, sizeof key, "%s.owner_rows.%d.%d", name, args[{j0}].map->index, bs);')
+                s.append(f"        int fresh = 0;")
+                s.append(f"        const int mdim = args[{j0}].map->dim;")
+                s.append(f"        int *rows = (int *)op2h_plan_user(key, sizeof(int) * (size_t)op->total * mdim, &fresh);")
+                s.append(f"        if (fresh && op->total)")
+                s.append(f"          op2gen_launch(false, nullptr, nullptr, op2gen_instance_rows, (op->total + 255) / 256, 256,")
+                s.append(f"                        0, st, op->elems, op->total, args[{j0}].map_data_d, mdim, rows);")
+                for j in js:
+                    s.append(f"        A.r{j} = rows;")
+                s.append(f"      }}")
             s.append(f"      void *t0 = nullptr, *t1 = nullptr;")
             s.append(f"      const bool ext = op->nblocks && op2h_kernel_timer_events(&ctx, &t0, &t1) != 0;")
             s.append(f"      if (!ext) op2h_kernel_timer_start(&ctx);")
@@ -427,7 +458,7 @@ class LoopGen:
             s.append(f"                      {self.name}_owner<1> : {self.name}_owner<0>,")
             s.append(f"                      op->nblocks, bs, lds, st, A,")
             s.append(f"                      op->elem_offs, op->elems, op->owned_offs, op->owned,")
-            s.append(f"                      op->csr_offs, op->csr);")
+            s.append(f"                      op->csr_offs, op->csr, (const ushort4 *)op->csr4);")
             s.append(f"      if (!ext) op2h_kernel_timer_stop(&ctx);")
             s.append(f"      OP2GEN_CHECK(hipGetLastError());")
             s.append(f"      OP_kernels[{L2.index - 1}].transfer += (float)op2h_algorithmic_bytes(set, {len(L2.args)}, args);")
@@ -526,6 +557,17 @@ void op2gen_launch(bool ext, void *e0, void *e1, void (*kernel)(KArgs...), dim3 
                                     (hipEvent_t)e0, (hipEvent_t)e1, 0));
   else
     OP2GEN_CHECK(hipLaunchKernel((const void *)kernel, grid, block, p, lds, st));
+}
+
+// Instance j's row of a map (the row of element elems[j]), packed once per
+// owner plan so that the owner kernel reads one row per instance directly.
+__global__ void op2gen_instance_rows(const int *__restrict elems, int total,
+                                     const int *__restrict map, int dim,
+                                     int *__restrict rows) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= total) return;
+  const int e = elems[j];
+  for (int d = 0; d < dim; ++d) rows[(size_t)j * dim + d] = map[(size_t)e * dim + d];
 }
 
 template <class T> struct op2gen_lim {
