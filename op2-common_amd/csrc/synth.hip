@@ -160,6 +160,55 @@ __global__ void k_syn_gather(const T *__restrict V, const T *__restrict R,
   for (int d = 0; d < D; ++d) W[(size_t)tgt * D + d] = acc[d];
 }
 
+// Owner partition (OP2H_PLAN_OWNER): one workgroup per block of owned
+// targets; every element touching the block computes its value once into LDS,
+// then each owned target adds the values of its increments in list
+// (element) order.  Entries are (local element * A + column).
+template <class T, int D>
+__global__ void __launch_bounds__(1024)
+    k_syn_owner(const T *__restrict V, const T *__restrict R,
+                const int *__restrict map, int mdim, int A, int k,
+                const int *__restrict elem_offs, const int *__restrict elems,
+                const int *__restrict owned_offs, const int *__restrict owned,
+                const ushort4 *__restrict csr4, const int *__restrict csr_offs,
+                const unsigned short *__restrict csr, T *__restrict W) {
+  extern __shared__ unsigned char lds_raw[];
+  T *lv = reinterpret_cast<T *>(lds_raw);
+  const int b = blockIdx.x;
+  const int e0 = elem_offs[b], ne = elem_offs[b + 1] - e0;
+  for (int j = threadIdx.x; j < ne; j += blockDim.x) {
+    T v[D];
+    int t[kMaxA];
+    synth_body<T, D>(V, R, map, mdim, A, elems[e0 + j], k, v, t);
+#pragma unroll
+    for (int d = 0; d < D; ++d) lv[(size_t)j * D + d] = v[d];
+  }
+  __syncthreads();
+  const int o0 = owned_offs[b], no = owned_offs[b + 1] - o0;
+  for (int i = threadIdx.x; i < no; i += blockDim.x) {
+    const int tgt = owned[o0 + i];
+    T acc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] = W[(size_t)tgt * D + d];
+    auto add = [&](int ent) {
+      const T *v = lv + (size_t)(ent / A) * D;
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc[d] += v[d];
+    };
+    if (csr4) {
+      const ushort4 e = csr4[o0 + i];
+      if (e.x != 0xFFFF) add(e.x);
+      if (e.y != 0xFFFF) add(e.y);
+      if (e.z != 0xFFFF) add(e.z);
+      if (e.w != 0xFFFF) add(e.w);
+    } else {
+      for (int c = csr_offs[o0 + i]; c < csr_offs[o0 + i + 1]; ++c) add(csr[c]);
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) W[(size_t)tgt * D + d] = acc[d];
+  }
+}
+
 int grid(long long n, int bs) { return n > 0 ? (int)((n + bs - 1) / bs) : 0; }
 
 struct Loop {
@@ -299,6 +348,19 @@ bool launch_typed(Loop &L, int plan, int bs, hipStream_t st) {
         if (inc) k_syn_staged<T, D, true><<<nb, bs, lds, st>>>(V, R, map, md, A, k, S, p->colstart_h[c], W);
         else k_syn_staged<T, D, false><<<nb, bs, lds, st>>>(V, R, map, md, A, k, S, p->colstart_h[c], W);
       }
+      return true;
+    }
+    case OP2H_PLAN_OWNER: {
+      if (!inc) return false;
+      const op2h_owner_plan *p =
+          op2h_plan_owner_for("synth", L.from, bs, (int)L.args.size(), L.args.data());
+      if (!p) return false;
+      const size_t lds = sizeof(T) * (size_t)p->max_elems * D;
+      if (lds > 64 * 1024) return false;
+      if (p->nblocks)
+        k_syn_owner<T, D><<<p->nblocks, bs, lds, st>>>(
+            V, R, map, md, A, k, p->elem_offs, p->elems, p->owned_offs, p->owned,
+            (const ushort4 *)p->csr4, p->csr_offs, p->csr, W);
       return true;
     }
     case OP2H_PLAN_GATHER: {

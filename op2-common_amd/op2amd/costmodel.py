@@ -6,7 +6,7 @@ import math
 import numpy as np
 
 from ._lib import lib
-from .api import PLAN_ATOMIC, PLAN_COLOR, PLAN_STAGED, PLAN_GATHER
+from .api import PLAN_ATOMIC, PLAN_COLOR, PLAN_STAGED, PLAN_GATHER, PLAN_OWNER
 from .mesh import ORDERINGS
 
 DIRECT = 0
@@ -74,7 +74,7 @@ def variants(access):
     blocks = BLOCKS
     if access == "read":
         return [(DIRECT, b) for b in blocks]
-    plans = [PLAN_COLOR, PLAN_GATHER] + ([PLAN_ATOMIC] if access == "inc" else [])
+    plans = [PLAN_COLOR, PLAN_GATHER] + ([PLAN_ATOMIC, PLAN_OWNER] if access == "inc" else [])
     out = [(p, b) for p in plans for b in blocks]
     out += [(PLAN_STAGED, b) for b in (128, 256, 512)]
     return out
@@ -152,7 +152,7 @@ class MLP:
 
 
 ORDER_NAMES = ("random", "rcm", "hilbert")
-PLAN_IDS = (DIRECT, PLAN_ATOMIC, PLAN_COLOR, PLAN_STAGED, PLAN_GATHER)
+PLAN_IDS = (DIRECT, PLAN_ATOMIC, PLAN_COLOR, PLAN_STAGED, PLAN_GATHER, PLAN_OWNER)
 
 
 def features(rec, plan, block):

@@ -61,7 +61,7 @@ def test_synthetic_inc_loop_agrees_across_plans(is_double, tol, arity):
     size = 0
     outs = {}
     for plan, block in [(cm.PLAN_ATOMIC, 128), (cm.PLAN_COLOR, 256), (cm.PLAN_STAGED, 128),
-                        (cm.PLAN_GATHER, 64)]:
+                        (cm.PLAN_GATHER, 64), (cm.PLAN_OWNER, 128)]:
         if not size:
             _, shape = cm.time_variants(d, [(plan, block)], reps=1)
             size = shape["target_size"] * 4
@@ -69,6 +69,8 @@ def test_synthetic_inc_loop_agrees_across_plans(is_double, tol, arity):
     ref = outs[cm.PLAN_GATHER]
     for plan, o in outs.items():
         assert np.max(np.abs(o - ref)) <= tol * np.max(np.abs(ref)), plan
+    # owner and gather both add each target's increments in element order
+    assert np.array_equal(outs[cm.PLAN_OWNER], ref)
 
 
 @pytest.mark.gpu
