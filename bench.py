@@ -415,7 +415,7 @@ def main():
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-iters", type=int, default=6)
-    ap.add_argument("--graph-steps", type=int, default=8,
+    ap.add_argument("--graph-steps", type=int, default=10,
                     help="time steps captured per graph (lowered to a divisor of --steps)")
     args = ap.parse_args()
     if args.gpus != 1:
