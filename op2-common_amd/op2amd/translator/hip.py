@@ -20,7 +20,6 @@ its own runtime, translator-v2/resources/templates/hip/loop_host.hpp.jinja):
   op_arg_info reports the payload of the first element (lowest index)
   reaching the extremum, as the sequential executor does.
 """
-import math
 
 from .op import ArgDat, ArgGbl, ArgIdx, ArgInfo
 

@@ -2,7 +2,6 @@
 bit (same permutation), with the packed one-pass level sort and the two-pass
 form, on small graphs of every shape and on the 720k-cell mesh; op_renumber
 gives the same numbering with either build."""
-import os
 import time
 
 import numpy as np
