@@ -479,15 +479,15 @@ __global__ void __launch_bounds__(256)
 // (-f)), with its instance records packed once per plan; a NULL plan or an
 // unexpected shape is fatal, as the reference's plan errors are.
 static OwnerArgs owner_args_res(const op2h_owner_plan *p, const op_arg *args,
-                                int bs, hipStream_t st) {
+                                hipStream_t st) {
   if (!p) exit(-1);
   if (p->inc_of_arg[6] != 0 || p->inc_of_arg[7] != 1 || p->ninc != 2) {
     printf("op2hip: res_calc owner plan expects res via ecell[0], ecell[1]\n");
     exit(-1);
   }
+  // packed once per plan (the plan pins the set, the maps and the block size)
   char key[96];
-  snprintf(key, sizeof key, "res_calc.owner.%d.%d.%d", bs, args[0].map->index,
-           args[2].map->index);
+  snprintf(key, sizeof key, "res_calc.owner.%p", (const void *)p);
   int fresh = 0;
   int4 *inst = (int4 *)op2h_plan_user(key, sizeof(int4) * (size_t)p->total, &fresh);
   if (fresh && p->total)
@@ -496,8 +496,7 @@ static OwnerArgs owner_args_res(const op2h_owner_plan *p, const op_arg *args,
         p->total, inst);
   // compact records: blocks per launch are at most a few hundred thousand, so
   // one workgroup per block
-  snprintf(key, sizeof key, "res_calc.owner8.%d.%d.%d", bs, args[0].map->index,
-           args[2].map->index);
+  snprintf(key, sizeof key, "res_calc.owner8.%p", (const void *)p);
   const size_t base_bytes = (sizeof(int2) * (size_t)p->nblocks + 255) / 256 * 256;
   char *c8 = (char *)op2h_plan_user(key, base_bytes + sizeof(uint2) * (size_t)p->total, &fresh);
   if (fresh && p->nblocks)
@@ -1180,7 +1179,7 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       if (ctx.plan == OP2H_PLAN_OWNER) {  // built before the timed region
         ctx.part_size = bs;
         owner_p = op2h_plan_owner(&ctx);
-        OA = owner_args_res(owner_p, args, bs, st);
+        OA = owner_args_res(owner_p, args, st);
       }
       op2h_kernel_timer_start(&ctx);
       if (ctx.plan == OP2H_PLAN_ATOMIC) {
@@ -1277,7 +1276,7 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       case OP2H_PLAN_OWNER: {
         ctx.part_size = bs;
         const op2h_owner_plan *p = op2h_plan_owner(&ctx);
-        OwnerArgs A = owner_args_res(p, args, bs, st);
+        OwnerArgs A = owner_args_res(p, args, st);
         const size_t lds = sizeof(double) * 4 * (size_t)A.max_e;
         if (lds > 160 * 1024) {
           printf("op2hip: res_calc owner plan needs %zu bytes of LDS; use a smaller block\n", lds);

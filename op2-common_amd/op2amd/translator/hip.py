@@ -439,7 +439,7 @@ class LoopGen:
                 j0 = js[0]
                 s.append(f"      if (args[{j0}].opt) {{")
                 s.append(f"        char key[160];")
-                s.append(f'        snprintf(key, sizeof key, "%s.owner_rows.%d.%d", name, args[{j0}].map->index, bs);')
+                s.append(f'        snprintf(key, sizeof key, "owner_rows.%p.%d", (const void *)op, args[{j0}].map->index);')
                 s.append(f"        int fresh = 0;")
                 s.append(f"        const int mdim = args[{j0}].map->dim;")
                 s.append(f"        int *rows = (int *)op2h_plan_user(key, sizeof(int) * (size_t)op->total * mdim, &fresh);")
