@@ -643,7 +643,7 @@ def generate(prog):
     for c in prog.consts:
         out.append(f"__constant__ {c.typ} {c.ptr}" + (f"[{c.dim}]" if c.dim > 1 else "") + ";\n")
     out.append("\n")
-    for name, text in prog.kernel_text.items():
+    for name, text in prog.device_text.items():
         out.append(_device_kernel(text) + "\n\n")
     out.append(f"}}  // namespace {ns}\n\n")
     # constants hook

@@ -94,6 +94,9 @@ class Program:
     consts: List[Const] = field(default_factory=list)  # one per name
     const_calls: List[Const] = field(default_factory=list)  # one per call
     kernel_text: dict = field(default_factory=dict)
+    # every function the device needs, user kernels and the functions of the
+    # program they call, callees before callers
+    device_text: dict = field(default_factory=dict)
 
 
 def _type(s):
@@ -145,6 +148,20 @@ def parse_arg(expr, consts):
     if kind == "op_arg_idx":
         return ArgIdx(_int(a[0], consts, "idx"), _map(a[1]))
     return ArgInfo(a[0], _int(a[1], consts, "dim"), _type(a[2]), _int(a[3], consts, "ref"))
+
+
+_NOT_CALLS = {"if", "for", "while", "switch", "return", "sizeof", "catch"}
+
+
+def _called_names(fn_text):
+    """Names called in a function body, in order of first call; member calls
+    (a.f(), p->f()) and qualified ones (std::f()) are left out."""
+    body = source.strip_comments(fn_text[fn_text.index("{"):])
+    out = []
+    for m in re.finditer(r"(?<![\w.:>])([A-Za-z_]\w*)\s*\(", body):
+        if m.group(1) not in _NOT_CALLS and m.group(1) not in out:
+            out.append(m.group(1))
+    return out
 
 
 def _kernel_source(name, text, path, seen=None):
@@ -199,6 +216,17 @@ def parse_program(path):
             if k is None:
                 raise OpError(f"definition of user kernel {kernel} not found")
             prog.kernel_text[kernel] = k
+
+    def visit(name, fn_text, stack):
+        for callee in _called_names(fn_text):
+            if callee in prog.device_text or callee in stack or callee == name:
+                continue
+            t = prog.kernel_text.get(callee) or _kernel_source(callee, text, path)
+            if t is not None:
+                visit(callee, t, stack | {name})
+        prog.device_text.setdefault(name, fn_text)
+    for name, fn_text in prog.kernel_text.items():
+        visit(name, fn_text, set())
     return prog
 
 

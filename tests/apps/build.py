@@ -18,6 +18,7 @@ PROGRAMS = {
     "idx": "functional/idx.cpp",
     "strides": "functional/strides.cpp",
     "consts": "functional/consts.cpp",
+    "helpers": "functional/helpers.cpp",
     "airfoil_tr": "airfoil/airfoil_tr.cpp",
 }
 

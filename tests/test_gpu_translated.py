@@ -28,7 +28,7 @@ def run(name, *args, timeout=120):
 @pytest.mark.parametrize("variant", [[], ["OP_AUTO_SOA"], ["OP_HIP_PLAN=*:color"],
                                      ["OP_HIP_PLAN=*:atomic"]],
                          ids=["default", "soa", "color", "atomic"])
-@pytest.mark.parametrize("name", ["gbl", "reductions", "idx", "strides", "consts"])
+@pytest.mark.parametrize("name", ["gbl", "reductions", "idx", "strides", "consts", "helpers"])
 def test_functional_program(name, variant):
     r = run(name, *variant)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]

@@ -66,9 +66,21 @@ def test_unsupported_forms_are_refused(tmp_path):
         op.parse_program(str(bad))
 
 
-@pytest.mark.parametrize("name", ["gbl", "airfoil_tr"])
+def test_called_functions_go_to_the_device_callees_first():
+    """Functions a kernel calls (from the file or a quoted include, at any
+    depth) are copied to the device once each, before their callers; calls
+    of functions the program does not define (std:: maths, members) are
+    not followed."""
+    prog = op.parse_program(str(APPS / "functional" / "helpers.cpp"))
+    assert list(prog.device_text) == ["square", "cube", "poly", "k_poly", "k_squares"]
+    assert op._called_names("void f(S s) { s.g(1); p->h(2); std::sqrt(3.0); k(4); if (x) {} }") \
+        == ["k"]
+
+
+@pytest.mark.parametrize("name", ["gbl", "airfoil_tr", "helpers"])
 def test_generated_kernels_compile_for_gfx950(name, tmp_path):
-    src = {"gbl": APPS / "functional" / "gbl.cpp", "airfoil_tr": APPS / "airfoil" / "airfoil_tr.cpp"}[name]
+    src = {"gbl": APPS / "functional" / "gbl.cpp", "airfoil_tr": APPS / "airfoil" / "airfoil_tr.cpp",
+           "helpers": APPS / "functional" / "helpers.cpp"}[name]
     paths = translate(str(src), str(tmp_path))
     cmd = ["/opt/rocm/bin/hipcc", "-O1", "-std=c++17", "--offload-arch=gfx950", "-w",
            "-I" + str(REPO / "include"), "-I" + str(src.parent), "-I" + str(tmp_path),
