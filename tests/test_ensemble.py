@@ -24,8 +24,16 @@ def test_members_deterministic_and_in_range():
 
 
 @pytest.mark.gpu
-def test_batched_equals_looped_and_each_member_equals_sequential_run():
-    sp = op2amd.ensemble.spec(**SMALL)
+@pytest.mark.parametrize("ordering", ["rcm", "random"])
+def test_batched_equals_looped_and_each_member_equals_sequential_run(ordering):
+    """Batched (graphed and eager) and looped runs give the same bits, and each
+    member equals the oracle's sequential run of that member.  With random
+    member numberings the owner patches span more than 16-bit offsets and
+    read 16-byte instance records; with RCM they fit the 8-byte form."""
+    # random: two 400x200 members (80,000 cells, spans beyond 65535)
+    cfg = SMALL if ordering == "rcm" else dict(nmesh=2, min_IJ=(400, 200), max_IJ=(400, 200),
+                                               seed=3, ordering="random")
+    sp = op2amd.ensemble.spec(**cfg)
     niter = 60
     batched = op2amd.ensemble.run(sp, niter, batched=True, graphed=True, want_q=True)
     eager = op2amd.ensemble.run(sp, niter, batched=True, graphed=False, want_q=True)
@@ -35,14 +43,15 @@ def test_batched_equals_looped_and_each_member_equals_sequential_run():
             assert np.array_equal(r[key], batched[key]), key
     off = 0
     for m, mb in enumerate(batched["members"]):
-        mesh = op2amd.mesh.generate(mb["I"], mb["J"], SMALL["seed"] + 1 + m, "rcm")
+        mesh = op2amd.mesh.generate(mb["I"], mb["J"], cfg["seed"] + 1 + m, ordering)
         ref = oracle.airfoil_run_qinf(mesh, niter, mb["qinf"])
         q = batched["q"][off:off + mesh.ncell]
         off += mesh.ncell
         assert np.array_equal(q, ref["q"]), m
         assert np.array_equal(batched["maxerr"][:, m], ref["maxerr"]), m
         assert np.array_equal(batched["errloc"][:, m], ref["errloc"]), m
-        assert np.max(np.abs(batched["rms"][:, m] / ref["rms"] - 1)) <= 1e-14, m
+        # rms: the same terms summed in another order (per-tile partials)
+        assert np.max(np.abs(batched["rms"][:, m] / ref["rms"] - 1)) <= 1e-12, m
 
 
 @pytest.mark.gpu
