@@ -589,46 +589,55 @@ __device__ __forceinline__ bool op2gen_better(T v, int at, T bv, int ba) {
   return OP == 1 ? (v < bv || (v == bv && at < ba)) : (v > bv || (v == bv && at < ba));
 }
 
-// Workgroup reduction (fixed tree: deterministic for a launch shape); the
-// result is returned to thread 0.
+// Workgroup reduction: xor-shuffle tree inside each wave, then the waves'
+// results in wave order (a fixed order for a launch shape).  Lanes past the
+// end of a partial last wave take no part.
 template <class T, int OP> __device__ T op2gen_block_reduce(T v) {
-  __shared__ T buf[1024];
-  const int t = threadIdx.x, n = blockDim.x;
-  buf[t] = v;
-  __syncthreads();
-  int p = 1;
-  while (p < n) p <<= 1;
-  for (int s = p >> 1; s > 0; s >>= 1) {
-    if (t < s && t + s < n) buf[t] = op2gen_combine<T, OP>(buf[t], buf[t + s]);
-    __syncthreads();
+  __shared__ T part[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = (blockDim.x + 63) >> 6;
+  for (int o = 32; o > 0; o >>= 1) {
+    const T u = __shfl_xor(v, o);
+    if ((t ^ o) < (int)blockDim.x) v = op2gen_combine<T, OP>(v, u);
   }
-  T r = buf[0];
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  T r = part[0];
+  for (int i = 1; i < nw; ++i) r = op2gen_combine<T, OP>(r, part[i]);
   __syncthreads();
   return r;
 }
 
 // Workgroup arg-best over (value, element) pairs: returns the winning thread.
 template <class T, int OP> __device__ int op2gen_block_argbest(T v, int at) {
-  __shared__ T bv[1024];
-  __shared__ int ba[1024], bt[1024];
-  const int t = threadIdx.x, n = blockDim.x;
-  bv[t] = v;
-  ba[t] = at;
-  bt[t] = t;
-  __syncthreads();
-  int p = 1;
-  while (p < n) p <<= 1;
-  for (int s = p >> 1; s > 0; s >>= 1) {
-    if (t < s && t + s < n && op2gen_better<T, OP>(bv[t + s], ba[t + s], bv[t], ba[t])) {
-      bv[t] = bv[t + s];
-      ba[t] = ba[t + s];
-      bt[t] = bt[t + s];
+  __shared__ T bv[16];
+  __shared__ int ba[16], bt[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = (blockDim.x + 63) >> 6;
+  int who = t;
+  for (int o = 32; o > 0; o >>= 1) {
+    const T ov = __shfl_xor(v, o);
+    const int oa = __shfl_xor(at, o), ow = __shfl_xor(who, o);
+    if ((t ^ o) < (int)blockDim.x && op2gen_better<T, OP>(ov, oa, v, at)) {
+      v = ov;
+      at = oa;
+      who = ow;
     }
-    __syncthreads();
   }
-  int w = bt[0];
+  if (lane == 0) {
+    bv[w] = v;
+    ba[w] = at;
+    bt[w] = who;
+  }
   __syncthreads();
-  return w;
+  T rv = bv[0];
+  int ra = ba[0], rw = bt[0];
+  for (int i = 1; i < nw; ++i)
+    if (op2gen_better<T, OP>(bv[i], ba[i], rv, ra)) {
+      rv = bv[i];
+      ra = ba[i];
+      rw = bt[i];
+    }
+  __syncthreads();
+  return rw;
 }
 
 }  // namespace
