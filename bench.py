@@ -362,6 +362,22 @@ def workload_costmodel(op2amd, args):
     reg_fixed = cm.regret(recs, test_ids, cm.best_fixed_choice(recs, train_ids))
     reg_train = cm.regret(recs, train_ids, model.choose)
     reg_top2 = cm.regret(recs, test_ids, lambda r: model.choose_timed(r, 2))
+    # the model inside the runtime: each airfoil loop's plan picked at its
+    # first call (op2h_set_plan_model), against the headline's exhaustive sweep
+    mesh = op2amd.mesh.generate(1200, 600, 0, "rcm")
+    n, W = 220, 20
+    model.install(top=1)
+    try:
+        picked = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W,
+                                    graph_steps=args.graph_steps)
+    finally:
+        cm.CostModel.uninstall()
+    kw = sweep(op2amd, mesh, args.sweep_iters, False)
+    swept = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, graph_steps=args.graph_steps,
+                               **kw)
+    model_rate, swept_rate = (n - W) / picked["seconds"], (n - W) / swept["seconds"]
+    log(f"  airfoil 1200x600: model plans {picked['chosen']} {model_rate:.0f} steps/s, "
+        f"swept {kw} {swept_rate:.0f} steps/s")
     line = {
         "metric": "cost-model plan selection: mean time regret vs exhaustive sweep, held-out loops",
         "value": round(float(reg_model.mean()), 4), "unit": "fraction", "n_gpus": 1,
@@ -382,7 +398,13 @@ def workload_costmodel(op2amd, args):
                    "regret_best_fixed_rule_mean": float(reg_fixed.mean()),
                    "loops_within_5pct_best_fixed_rule": float(np.mean(reg_fixed <= 0.05)),
                    "sweep_seconds": t_sweep, "train_seconds": t_train,
-                   "model_pick_seconds_per_loop": t_pick},
+                   "model_pick_seconds_per_loop": t_pick,
+                   "airfoil_1200x600_model_plans": {lp: list(v) for lp, v in
+                                                    picked["chosen"].items()},
+                   "airfoil_1200x600_model_steps_per_s": model_rate,
+                   "airfoil_1200x600_swept_plans": kw,
+                   "airfoil_1200x600_swept_steps_per_s": swept_rate,
+                   "airfoil_1200x600_model_regret": swept_rate / model_rate - 1.0},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

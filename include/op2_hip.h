@@ -293,6 +293,31 @@ int op2h_tuning_done(void);  /* 1 when every seen loop has chosen */
 int op2h_tuning_records(int max, op2h_tuning_record *out);
 int op2h_tuning_choice(const char *name, int *plan, int *block_size);
 
+/* Plan choice by a cost model (the library's MLPs, op2amd_costmodel.h) for
+ * loops left on AUTO: at a loop's first call the runtime describes the loop
+ * (set and target sizes, map arity and locality, dat dim and precision,
+ * access, arithmetic per element), builds the feature vector of every
+ * (plan, block) variant the loop implements, evaluates the members on the
+ * device and ranks the variants by their mean prediction.  top = 1 takes the
+ * best; top > 1 times the `top` best on the loop's next eager calls (the
+ * autotuner's trials) and keeps the fastest.  Features are standardised as
+ * (x - mean) / scale, nfeat values each (OP2H_MODEL_FEATURES).  n_members = 0
+ * removes the model.  The model stays installed across op_init/op_exit.
+ * Returns 0, or -1 on bad arguments. */
+#define OP2H_MODEL_FEATURES 22
+struct op2amd_mlp;
+int op2h_set_plan_model(struct op2amd_mlp *const *members, int n_members, int nfeat,
+                        const float *mean, const float *scale, int top);
+/* Arithmetic per element of loop `name`'s kernel, in multiply-add
+ * equivalents: a cost-model feature (16 when not given; clamped to 4-64,
+ * the range the model is trained on). */
+void op2h_set_loop_arith(const char *name, int per_element);
+/* The description the cost model used for loop `name` (set by its first
+ * call with a model installed): set size, target size, arity, dim,
+ * is_double, arithmetic, mean map bandwidth, access (0 read, 1 inc, 2 rw).
+ * Returns 0, or -1 when there is none. */
+int op2h_plan_model_loop(const char *name, double *desc8);
+
 /* Step capture into a HIP graph.  Between op2h_graph_begin() and
  * op2h_graph_end() every op_par_loop is recorded instead of run; gbl
  * reduction results of recorded loops are delivered to their host pointers

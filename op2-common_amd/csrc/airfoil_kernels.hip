@@ -66,8 +66,20 @@ static int airfoil_consts(int dim, char const *type, int size, const char *dat,
   return 1;
 }
 
+// Arithmetic per element of each loop's kernel in multiply-add equivalents
+// (a division or square root counted as ten), the cost model's measure of a
+// loop's compute: save_soln copies; adt_calc has a division, five square
+// roots and ~50 operations; res_calc two divisions and ~75 operations;
+// bres_calc ~40; update a division and ~12.
 static const struct AirfoilConstHook {
-  AirfoilConstHook() { op2h_register_const_hook(airfoil_consts); }
+  AirfoilConstHook() {
+    op2h_register_const_hook(airfoil_consts);
+    op2h_set_loop_arith("save_soln", 4);
+    op2h_set_loop_arith("adt_calc", 64);
+    op2h_set_loop_arith("res_calc", 60);
+    op2h_set_loop_arith("bres_calc", 40);
+    op2h_set_loop_arith("update", 20);
+  }
 } airfoil_const_hook;
 
 

@@ -28,6 +28,7 @@
 #include "op2h_reduce.h"
 #include "op2h_rcm.h"
 #include "op2h_rcm_gpu.h"
+#include "op2amd_costmodel.h"
 
 // ---------------------------------------------------------------------------
 // global state (op_lib_core.cpp:55-86)
@@ -74,6 +75,19 @@ struct TuneState {
   int trials_done = 0;
   bool done = false;
   int plan = 0, block = 0;
+};
+
+// The cost model of op2h_set_plan_model and the loop descriptions it saw.
+struct PlanModel {
+  std::vector<op2amd_mlp *> members;
+  std::vector<float> mean, scale;
+  int top = 1;
+};
+
+struct ModelLoop {
+  double set_size = 0, target_size = 0, arity = 1, dim = 1, is_double = 1, arith = 16,
+         bandwidth = 0;
+  int access = 0;  // 0 read, 1 inc, 2 rw
 };
 
 struct LoopStats {
@@ -154,6 +168,9 @@ struct Runtime {
   bool autotune = false;
   int tune_trials = 2;
   std::map<std::string, TuneState> tune;
+  PlanModel model;                          // kept across op_init/op_exit
+  std::map<std::string, int> loop_arith;    // op2h_set_loop_arith
+  std::map<std::string, ModelLoop> model_loops;
   std::vector<op2h_tuning_record> tune_records;
   hipEvent_t tune_ev[2] = {nullptr, nullptr};
   // profiling
@@ -1211,6 +1228,172 @@ extern "C" void op_decl_const_char(int dim, char const *type, int size,
   }
 }
 
+// ---- cost-model plan choice -------------------------------------------------
+namespace {
+
+// The loop as the cost model sees it (op2amd/costmodel.py describes the
+// synthetic loops it is trained on the same way): the main map is the one of
+// the first indirect written arg (else of the first indirect read one).
+ModelLoop describe_loop(const char *name, op_set set, int nargs, const op_arg *args) {
+  Runtime &R = rt();
+  ModelLoop d;
+  d.set_size = set->size;
+  op_map main = nullptr;
+  const op_arg *mainw = nullptr;
+  bool inc = false, rw = false;
+  for (int m = 0; m < nargs; ++m) {
+    const op_arg &a = args[m];
+    if (a.argtype != OP_ARG_DAT || !a.opt || !a.map || a.idx < 0) continue;
+    if (a.acc == OP_INC) inc = true;
+    if (a.acc == OP_RW || a.acc == OP_WRITE) rw = true;
+    if (a.acc != OP_READ && !mainw) mainw = &a;
+  }
+  if (mainw) {
+    main = mainw->map;
+  } else {
+    for (int m = 0; m < nargs && !main; ++m)
+      if (args[m].argtype == OP_ARG_DAT && args[m].opt && args[m].map && args[m].idx >= 0)
+        main = args[m].map;
+  }
+  d.access = inc ? 1 : rw ? 2 : 0;
+  // dim and precision: the written indirect dat, else the widest dat
+  const op_arg *ref = mainw;
+  for (int m = 0; m < nargs && !ref; ++m)
+    if (args[m].argtype == OP_ARG_DAT && args[m].opt &&
+        (!ref || args[m].dim > ref->dim))
+      ref = &args[m];
+  for (int m = 0; m < nargs; ++m)
+    if (!mainw && args[m].argtype == OP_ARG_DAT && args[m].opt && args[m].dim > ref->dim)
+      ref = &args[m];
+  if (ref) {
+    d.dim = std::max(1, ref->dim);
+    d.is_double = type_code(ref->type) == 0 ? 1.0 : 0.0;
+  }
+  if (main) {
+    d.target_size = main->to->size;
+    std::vector<int> cols;
+    for (int m = 0; m < nargs; ++m)
+      if (args[m].argtype == OP_ARG_DAT && args[m].opt && args[m].map == main &&
+          args[m].idx >= 0 && std::find(cols.begin(), cols.end(), args[m].idx) == cols.end())
+        cols.push_back(args[m].idx);
+    d.arity = std::min<int>(4, (int)cols.size());
+    double sum = 0.0;
+    for (int e = 0; e < set->size; ++e) {
+      int mx = 0;
+      for (int a : cols)
+        for (int b : cols)
+          mx = std::max(mx, std::abs(main->map[(size_t)e * main->dim + a] -
+                                     main->map[(size_t)e * main->dim + b]));
+      sum += mx;
+    }
+    d.bandwidth = set->size ? sum / set->size : 0.0;
+  } else {
+    d.target_size = set->size;
+  }
+  auto it = R.loop_arith.find(name);
+  d.arith = it == R.loop_arith.end() ? 16 : it->second;
+  return d;
+}
+
+// Feature vector of one variant, in the order of op2amd/costmodel.py's
+// features(): sizes, shape, arithmetic, access, plan one-hot (direct, atomic,
+// colour, staged, gather, owner), block (log and one-hot).
+void model_features(const ModelLoop &d, int plan, int block, float *f) {
+  int i = 0;
+  f[i++] = (float)std::log10(std::max(1.0, d.set_size));
+  f[i++] = (float)std::log10(std::max(1.0, d.target_size));
+  f[i++] = (float)(d.arity / 4.0);
+  f[i++] = (float)(std::log2(d.dim) / 3.0);
+  f[i++] = (float)d.is_double;
+  f[i++] = (float)(d.arith / 64.0);
+  f[i++] = (float)(std::log10(1.0 + d.bandwidth) / 7.0);
+  for (int a = 0; a < 3; ++a) f[i++] = d.access == a ? 1.f : 0.f;
+  for (int p = 0; p < 6; ++p) f[i++] = plan == p ? 1.f : 0.f;
+  f[i++] = (float)(std::log2((double)block) / 10.0);
+  for (int b : {64, 128, 256, 512, 1024}) f[i++] = block == b ? 1.f : 0.f;
+}
+
+// The loop's variants, best predicted first (plan 0 = direct: the loop's
+// own plan, block size only).
+std::vector<std::pair<int, int>> model_rank(const char *name, op_set set, int nargs,
+                                            const op_arg *args, unsigned plan_mask) {
+  Runtime &R = rt();
+  const ModelLoop d = describe_loop(name, set, nargs, args);
+  R.model_loops[name] = d;
+  std::vector<std::pair<int, int>> cands;
+  const int all_blocks[5] = {64, 128, 256, 512, 1024};
+  if (d.access == 0) {
+    for (int b : all_blocks) cands.push_back({0, b});
+  } else {
+    for (int p : {OP2H_PLAN_ATOMIC, OP2H_PLAN_COLOR, OP2H_PLAN_STAGED, OP2H_PLAN_GATHER,
+                  OP2H_PLAN_OWNER}) {
+      if (!((plan_mask >> p) & 1)) continue;
+      if (p == OP2H_PLAN_ATOMIC && d.access != 1) continue;  // increments only
+      if (p == OP2H_PLAN_OWNER && d.access != 1) continue;
+      for (int b : all_blocks) {
+        if (p == OP2H_PLAN_STAGED && (b < 128 || b > 512)) continue;
+        if (p == OP2H_PLAN_OWNER && b > 256) continue;  // LDS of larger patches
+        cands.push_back({p, b});
+      }
+    }
+  }
+  if (cands.empty()) return cands;
+  const int nf = OP2H_MODEL_FEATURES, n = (int)cands.size();
+  std::vector<float> X((size_t)n * nf), y(n), acc(n, 0.f);
+  for (int i = 0; i < n; ++i) {
+    float *f = &X[(size_t)i * nf];
+    model_features(d, cands[i].first, cands[i].second, f);
+    for (int k = 0; k < nf; ++k) f[k] = (f[k] - R.model.mean[k]) / R.model.scale[k];
+  }
+  for (op2amd_mlp *m : R.model.members) {
+    if (op2amd_mlp_forward(m, X.data(), n, y.data())) {
+      printf("op2hip: cost model evaluation failed for %s\n", name);
+      exit(-1);
+    }
+    for (int i = 0; i < n; ++i) acc[i] += y[i];
+  }
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return acc[a] < acc[b]; });
+  std::vector<std::pair<int, int>> ranked;
+  for (int i : order) ranked.push_back(cands[i]);
+  return ranked;
+}
+
+}  // namespace
+
+extern "C" int op2h_set_plan_model(struct op2amd_mlp *const *members, int n_members,
+                                   int nfeat, const float *mean, const float *scale,
+                                   int top) {
+  Runtime &R = rt();
+  if (n_members <= 0) {
+    R.model = PlanModel();
+    return 0;
+  }
+  if (!members || nfeat != OP2H_MODEL_FEATURES || !mean || !scale || top < 1) return -1;
+  for (int k = 0; k < nfeat; ++k)
+    if (!(scale[k] > 0.f)) return -1;
+  R.model.members.assign(members, members + n_members);
+  R.model.mean.assign(mean, mean + nfeat);
+  R.model.scale.assign(scale, scale + nfeat);
+  R.model.top = top;
+  return 0;
+}
+
+extern "C" void op2h_set_loop_arith(const char *name, int per_element) {
+  rt().loop_arith[name] = std::min(64, std::max(4, per_element));
+}
+
+extern "C" int op2h_plan_model_loop(const char *name, double *desc8) {
+  auto it = rt().model_loops.find(name);
+  if (it == rt().model_loops.end() || !desc8) return -1;
+  const ModelLoop &d = it->second;
+  const double v[8] = {d.set_size, d.target_size, d.arity, d.dim,
+                       d.is_double, d.arith, d.bandwidth, (double)d.access};
+  std::copy(v, v + 8, desc8);
+  return 0;
+}
+
 extern "C" void op2h_loop_begin(op2h_loop_ctx *ctx, int kernel_idx,
                                 const char *name, op_set set, int nargs,
                                 op_arg *args, int def_plan, int def_block) {
@@ -1260,8 +1443,33 @@ extern "C" void op2h_loop_begin_mask(op2h_loop_ctx *ctx, int kernel_idx,
   LoopConfig c = R.all_cfg;
   auto it = R.loop_cfg.find(name);
   if (it != R.loop_cfg.end()) c = it->second;
-  if (R.autotune && c.plan == OP2H_PLAN_AUTO && c.block_size == 0 && set->size > 0) {
+  const bool with_model = !R.model.members.empty();
+  if ((R.autotune || with_model) && c.plan == OP2H_PLAN_AUTO && c.block_size == 0 &&
+      set->size > 0) {
     TuneState &ts = R.tune[name];
+    if (ts.cands.empty() && with_model && !R.capturing) {
+      // the cost model ranks the variants: the best one, or the `top` best to
+      // be timed by the autotuner's trials
+      std::vector<std::pair<int, int>> ranked = model_rank(name, set, nargs, args, plan_mask);
+      for (auto &v : ranked)
+        if (v.first == 0) v.first = def_plan;  // conflict-free: the loop's own plan
+      if (!ranked.empty()) {
+        const size_t k = std::min(ranked.size(), (size_t)R.model.top);
+        ts.cands.assign(ranked.begin(), ranked.begin() + k);
+        ts.best_ms.assign(ts.cands.size(), 1e300);
+        if (k == 1) {
+          ts.done = true;
+          ts.plan = ts.cands[0].first;
+          ts.block = ts.cands[0].second;
+        }
+      }
+    }
+    if (ts.cands.empty() && R.capturing) {  // first seen inside a capture
+      ctx->plan = c.plan == OP2H_PLAN_AUTO ? def_plan : c.plan;
+      ctx->block_size = def_block;
+      ctx->part_size = ctx->block_size;
+      return;
+    }
     if (ts.cands.empty()) {
       int nplans = 0;
       for (int p = 1; p < OP2H_PLAN_COUNT; ++p) nplans += (plan_mask >> p) & 1;

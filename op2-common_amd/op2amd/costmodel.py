@@ -53,6 +53,27 @@ lib.op2amd_mlp_train.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c
                                  C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]
 
 
+_INSTALLED = None
+lib.op2h_set_plan_model.restype = C.c_int
+lib.op2h_set_plan_model.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float),
+                                    C.POINTER(C.c_float), C.c_int]
+lib.op2h_set_loop_arith.argtypes = [C.c_char_p, C.c_int]
+lib.op2h_plan_model_loop.restype = C.c_int
+lib.op2h_plan_model_loop.argtypes = [C.c_char_p, C.POINTER(C.c_double)]
+
+
+def model_loop(name):
+    """The runtime's description of loop `name` at its model-driven first
+    call: dict of set_size, target_size, arity, dim, is_double, k,
+    mean_bandwidth, access (None if the loop has none)."""
+    d = (C.c_double * 8)()
+    if lib.op2h_plan_model_loop(name.encode(), d):
+        return None
+    return {"set_size": d[0], "target_size": d[1], "arity": int(d[2]), "dim": int(d[3]),
+            "is_double": int(d[4]), "k": d[5], "mean_bandwidth": d[6],
+            "access": ("read", "inc", "rw")[int(d[7])]}
+
+
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
@@ -157,13 +178,14 @@ PLAN_IDS = (DIRECT, PLAN_ATOMIC, PLAN_COLOR, PLAN_STAGED, PLAN_GATHER, PLAN_OWNE
 
 def features(rec, plan, block):
     """Feature vector of one (loop, variant): sizes, shape, arithmetic and the
-    variant itself."""
-    f = [math.log10(rec["set_size"]), math.log10(rec["target_size"]), rec["arity"] / 4.0,
-         math.log2(rec["dim"]) / 3.0, float(rec["is_double"]), rec["k"] / 64.0,
-         math.log10(1.0 + rec["mean_bandwidth"]) / 7.0,
+    variant itself.  Everything in it is known for a real loop at its first
+    call (the numbering enters only through the map's mean bandwidth), so the
+    runtime builds the same vector (op2h_set_plan_model)."""
+    f = [math.log10(max(1, rec["set_size"])), math.log10(max(1, rec["target_size"])),
+         rec["arity"] / 4.0, math.log2(rec["dim"]) / 3.0, float(rec["is_double"]),
+         rec["k"] / 64.0, math.log10(1.0 + rec["mean_bandwidth"]) / 7.0,
          float(rec["access"] == "read"), float(rec["access"] == "inc"),
          float(rec["access"] == "rw")]
-    f += [float(rec["ordering"] == o) for o in ORDER_NAMES]
     f += [float(plan == p) for p in PLAN_IDS]
     # block size both as a magnitude and one-hot: its effect is not monotonic
     f += [math.log2(block) / 10.0]
@@ -316,6 +338,27 @@ class CostModel:
 
     def choose(self, rec):
         return self.ranked(rec)[0]
+
+    def install(self, top=1):
+        """Make the runtime pick the plan of every loop left on AUTO with this
+        model at the loop's first call (op2h_set_plan_model): the best
+        predicted variant, or with top > 1 the fastest of the `top` best,
+        timed on the loop's next calls.  The model stays installed until
+        uninstall()."""
+        global _INSTALLED
+        handles = (C.c_void_p * len(self.nets))(*[n.h for n in self.nets])
+        mu = np.ascontiguousarray(self.mu, np.float32)
+        sd = np.ascontiguousarray(self.sd, np.float32)
+        if lib.op2h_set_plan_model(handles, len(self.nets), len(mu), _fp(mu), _fp(sd), top):
+            raise ValueError("the runtime refused the model (feature count or scale)")
+        _INSTALLED = (self, handles, mu, sd)  # keeps the networks alive
+        return self
+
+    @staticmethod
+    def uninstall():
+        global _INSTALLED
+        lib.op2h_set_plan_model(None, 0, 0, None, None, 1)
+        _INSTALLED = None
 
     def choose_timed(self, rec, k):
         """Model-pruned tuning: time only the k best predicted variants (here

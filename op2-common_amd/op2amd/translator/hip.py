@@ -21,6 +21,9 @@ its own runtime, translator-v2/resources/templates/hip/loop_host.hpp.jinja):
   reaching the extremum, as the sequential executor does.
 """
 
+import re
+
+from . import source
 from .op import ArgDat, ArgGbl, ArgIdx, ArgInfo
 
 PLAN_ATOMIC, PLAN_COLOR, PLAN_OWNER = 1, 2, 5
@@ -43,9 +46,33 @@ def _device_kernel(text):
     return "__device__ " + text.lstrip()
 
 
+def _arith_estimate(kernel_text):
+    """Multiply-add equivalents per element of a user kernel, for the cost
+    model: its arithmetic operators, a division or a call of sqrt/pow/exp/
+    log/trigonometry counted as ten, an operator inside a `for` loop's body
+    as four (loops over components are short); clamped to 4-64."""
+    body = source.strip_comments(kernel_text[kernel_text.index("{"):])
+    body = re.sub(r"->|\+\+|--", " ", body)
+
+    def ops(t):
+        return len(re.findall(r"[*+\-]", t)) + 10 * (
+            len(re.findall(r"/", t)) +
+            len(re.findall(r"\b(?:sqrt|pow|exp|log|sin|cos|tan|atan2?)\w*\s*\(", t)))
+    total = ops(body)
+    for m in re.finditer(r"\bfor\s*\(", body):
+        close = source.match_close(body, m.end() - 1)
+        k = close + 1
+        while k < len(body) and body[k] in " \t\r\n":
+            k += 1
+        inner = body[k:source.match_close(body, k) + 1] if k < len(body) and body[k] == "{" \
+            else body[k:body.find(";", k) + 1]
+        total += 3 * ops(inner)
+    return max(4, min(64, total))
+
+
 class LoopGen:
-    def __init__(self, loop, ns):
-        self.L, self.ns = loop, ns
+    def __init__(self, loop, ns, arith=16):
+        self.L, self.ns, self.arith = loop, ns, arith
         self.info_of = {}  # gbl arg index -> info arg index
         for j, a in enumerate(loop.args):
             if isinstance(a, ArgInfo):
@@ -383,6 +410,11 @@ class LoopGen:
         sig = ", ".join(["char const *name", "op_set set"] + [f"op_arg arg{j}" for j in range(N)])
         s = [f'extern "C" void {L.stub}({sig}) {{']
         s.append(f"  op_arg args[{max(N, 1)}] = {{{', '.join(f'arg{j}' for j in range(N)) or '{}'}}};")
+        s.append(f"  static bool arith_given = false;  // the cost model's arithmetic feature")
+        s.append(f"  if (!arith_given) {{")
+        s.append(f"    op2h_set_loop_arith(name, {self.arith});")
+        s.append(f"    arith_given = true;")
+        s.append(f"  }}")
         s.append(f"  op2h_loop_ctx ctx;")
         s.append(f"  op2h_loop_begin_mask(&ctx, {L.index - 1}, name, set, {N}, args, {self.def_plan}, 256, "
                  f"{self.mask}u);")
@@ -666,7 +698,7 @@ def generate(prog):
     out.append(f"static const struct {ns}_hook {{\n  {ns}_hook() {{ op2h_register_const_hook({ns}_consts); }}\n"
                f"}} {ns}_hook_v;\n\n")
     for loop in prog.loops:
-        g = LoopGen(loop, ns)
+        g = LoopGen(loop, ns, _arith_estimate(prog.kernel_text[loop.kernel]))
         out.append(f"// ---- loop {loop.index}: {loop.kernel} ----\n")
         out.append("namespace {\n")
         out.append(g.args_struct())

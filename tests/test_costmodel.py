@@ -100,3 +100,40 @@ def test_model_picked_plans_beat_the_static_default():
     assert r_model.mean() < r_default.mean()
     # on its own training loops the model is near the sweep's best
     assert cm.regret(recs, train, model.choose).mean() < 0.1
+
+
+@pytest.mark.gpu
+def test_runtime_picks_each_loop_plan_with_the_model():
+    """An installed model picks each AUTO loop's plan at the loop's first
+    call, on the device: the runtime describes res_calc as this module does
+    (sizes, arity, dim, precision, access, arithmetic, the map's mean
+    bandwidth), its pick is the model's best prediction over the variants
+    res_calc implements, and the run keeps the reference trajectory."""
+    from conftest import golden, pointwise_rel_err
+    cm = op2amd.costmodel
+    recs = cm.sweep(cm.random_descs(12, 3, n_lo=1e4, n_hi=6e4), reps=1)
+    model = cm.CostModel(members=2, seed=1).fit(recs, list(range(len(recs))), epochs=300)
+    g = golden("airfoil_120x60_random_a1")
+    mesh = op2amd.mesh.generate(int(g["I"]), int(g["J"]), 0, str(g["ordering"]))
+    model.install(top=1)
+    try:
+        r = op2amd.airfoil.run(mesh, 200, mode="tuned")
+        d = cm.model_loop("res_calc")
+        dd = cm.model_loop("save_soln")
+    finally:
+        cm.CostModel.uninstall()
+    e = mesh.ecell.astype(np.int64)
+    assert (d["set_size"], d["target_size"], d["arity"], d["dim"], d["is_double"], d["access"],
+            d["k"]) == (mesh.nedge, mesh.ncell, 2, 4, 1, "inc", 60)
+    assert abs(d["mean_bandwidth"] - np.abs(e[:, 0] - e[:, 1]).mean()) <= 1e-9 * d["mean_bandwidth"]
+    assert (dd["access"], dd["target_size"], dd["mean_bandwidth"]) == ("read", mesh.ncell, 0.0)
+    cands = [(p, b) for p in (cm.PLAN_ATOMIC, cm.PLAN_COLOR, cm.PLAN_STAGED, cm.PLAN_GATHER,
+                              cm.PLAN_OWNER)
+             for b in cm.BLOCKS
+             if not (p == cm.PLAN_STAGED and not 128 <= b <= 512) and not (p == cm.PLAN_OWNER and b > 256)]
+    pred = model.predict([cm.features(d, p, b) for p, b in cands])
+    chosen = tuple(r["chosen"]["res_calc"])
+    assert chosen in cands
+    assert pred[cands.index(chosen)] <= pred.min() + 1e-4
+    assert r["chosen"]["save_soln"][1] in cm.BLOCKS
+    assert pointwise_rel_err(r["rms"], g["rms"][:200]) <= 1e-10
