@@ -191,7 +191,8 @@ class LoopGen:
                 post.append(f"      }}")
                 post.append(f"    }}")
             else:
-                post.append(f"    for (int k = 0; k < {D}; ++k) lds[((size_t)j * {inc[1]} + {inc[2]}) * {D} + k] = b{j}[k];")
+                # the chunk's slots, component-major: slot threadIdx.x * ninc + which
+                post.append(f"    for (int k = 0; k < {D}; ++k) lds[(size_t)k * lds_s + threadIdx.x * {inc[1]} + {inc[2]}] = b{j}[k];")
         elif self._staged(j, a):
             pre.append(f"    {guard}for (int k = 0; k < {D}; ++k) b{j}[k] = A.d{j}[{idx}];")
             ptr = f"b{j}"
@@ -294,9 +295,16 @@ class LoopGen:
         return "\n".join(body) + "\n"
 
     def owner_kernel(self):
-        """Owner-partition variant: phase 1 evaluates each instance of the
-        block once with its increments in registers and stores them in LDS;
-        phase 2 adds each owned target's increments in list (element) order."""
+        """Owner-partition variant.  A block owns at most blockDim targets (the
+        plan's patches hold at most that many) and walks its instances in
+        chunks of blockDim: each thread evaluates one instance of the chunk
+        with its increments in registers and stores them in the chunk's LDS
+        slots (component-major, so a wave's stores are consecutive); then each
+        owned target adds the entries of its list that fall in the chunk.  The
+        lists are ascending and chunks are taken in order, so every target
+        sums its increments in element order, as the sequential executor
+        does.  LDS is blockDim * ninc * dim values whatever the patch's
+        perimeter.  Blocks take patches in XCD-contiguous order."""
         if not self.owner:
             return ""
         L = self.L
@@ -319,6 +327,7 @@ class LoopGen:
                     pre.append(f"    int x{j} = n;")
                 call.append(f"&x{j}")
         w = incs[0]
+        at = f"(LAY == 1 || (LAY == 2 && A.s{w})) ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k"
         b = [f"template <int LAY>",
              f"__global__ void __launch_bounds__(1024)",
              f"    {self.name}_owner({self.name}_args A, const int *__restrict elem_offs,",
@@ -328,34 +337,49 @@ class LoopGen:
              f"                 const ushort4 *__restrict csr4) {{",
              f"  extern __shared__ unsigned char lds_raw[];",
              f"  {T} *lds = reinterpret_cast<{T} *>(lds_raw);",
-             f"  const int blk = blockIdx.x;",
+             f"  const int lds_s = (int)blockDim.x * {ninc};  // slots per chunk",
+             f"  const int blk = op2gen_xcd_block();",
              f"  const int e0 = elem_offs[blk], ne = elem_offs[blk + 1] - e0;",
-             f"  for (int j = threadIdx.x; j < ne; j += blockDim.x) {{",
-             f"    const int n = elems[e0 + j];"]
-        b += pre
-        b.append(f"    {self.ns}::{L.kernel}({', '.join(call)});")
-        b += post
-        b += [f"  }}",
-              f"  __syncthreads();",
-              f"  const int o0 = owned_offs[blk], no = owned_offs[blk + 1] - o0;",
-              f"  for (int i = threadIdx.x; i < no; i += blockDim.x) {{",
-              f"    const int t = owned[o0 + i];",
-              f"    {T} acc[{D}];",
-              f"    for (int k = 0; k < {D}; ++k) acc[k] = A.d{w}[(LAY == 1 || (LAY == 2 && A.s{w})) ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k];",
-              f"    if (csr4) {{  // fixed-width lists: one 8-byte load",
-              f"      const ushort4 e = csr4[o0 + i];",
-              f"      const unsigned short es[4] = {{e.x, e.y, e.z, e.w}};",
-              f"      for (int c = 0; c < 4; ++c)",
-              f"        if (es[c] != 0xFFFF)",
-              f"          for (int k = 0; k < {D}; ++k) acc[k] += lds[(size_t)es[c] * {D} + k];",
-              f"    }} else {{",
-              f"      for (int c = csr_offs[o0 + i]; c < csr_offs[o0 + i + 1]; ++c) {{",
-              f"        const {T} *inc = lds + (size_t)csr[c] * {D};",
-              f"        for (int k = 0; k < {D}; ++k) acc[k] += inc[k];",
+             f"  const int o0 = owned_offs[blk], no = owned_offs[blk + 1] - o0;",
+             f"  const bool own = (int)threadIdx.x < no;",
+             f"  const int t = own ? owned[o0 + threadIdx.x] : 0;",
+             f"  {T} acc[{D}];",
+             f"  unsigned short es[4] = {{0xFFFF, 0xFFFF, 0xFFFF, 0xFFFF}};",
+             f"  int c = 0, c1 = 0;",
+             f"  if (own) {{",
+             f"    for (int k = 0; k < {D}; ++k) acc[k] = A.d{w}[{at}];",
+             f"    if (csr4) {{  // fixed-width lists: one 8-byte load",
+             f"      const ushort4 e = csr4[o0 + threadIdx.x];",
+             f"      es[0] = e.x; es[1] = e.y; es[2] = e.z; es[3] = e.w;",
+             f"    }} else {{",
+             f"      c = csr_offs[o0 + threadIdx.x];",
+             f"      c1 = csr_offs[o0 + threadIdx.x + 1];",
+             f"    }}",
+             f"  }}",
+             f"  for (int base = 0; base < ne; base += blockDim.x) {{",
+             f"    const int j = base + (int)threadIdx.x;",
+             f"    if (j < ne) {{",
+             f"      const int n = elems[e0 + j];"]
+        b += ["  " + ln for ln in pre]
+        b.append(f"      {self.ns}::{L.kernel}({', '.join(call)});")
+        b += ["  " + ln for ln in post]
+        b += [f"    }}",
+              f"    __syncthreads();",
+              f"    if (own) {{",
+              f"      const int lo = base * {ninc}, hi = lo + lds_s;",
+              f"      if (csr4) {{",
+              f"        for (int q = 0; q < 4; ++q)",
+              f"          if (es[q] != 0xFFFF && es[q] >= lo && es[q] < hi)",
+              f"            for (int k = 0; k < {D}; ++k) acc[k] += lds[(size_t)k * lds_s + (es[q] - lo)];",
+              f"      }} else {{",
+              f"        for (; c < c1 && (int)csr[c] < hi; ++c)",
+              f"          for (int k = 0; k < {D}; ++k) acc[k] += lds[(size_t)k * lds_s + ((int)csr[c] - lo)];",
               f"      }}",
               f"    }}",
-              f"    for (int k = 0; k < {D}; ++k) A.d{w}[(LAY == 1 || (LAY == 2 && A.s{w})) ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k] = acc[k];",
+              f"    __syncthreads();",
               f"  }}",
+              f"  if (own)",
+              f"    for (int k = 0; k < {D}; ++k) A.d{w}[{at}] = acc[k];",
               f"}}"]
         return "\n".join(b) + "\n"
 
@@ -457,9 +481,10 @@ class LoopGen:
             s.append(f"      ctx.part_size = bs;")
             s.append(f"      const op2h_owner_plan *op = op2h_plan_owner(&ctx);")
             s.append(f"      if (!op) exit(-1);")
-            s.append(f"      const size_t lds = (size_t)op->max_elems * {len(incs)} * {D} * sizeof({T});")
-            s.append(f"      if (lds > 160 * 1024) {{")
-            s.append(f'        printf("op2hip: %s: owner plan needs %zu bytes of LDS; use a smaller block\\n", name, lds);')
+            s.append(f"      const size_t lds = (size_t)bs * {len(incs)} * {D} * sizeof({T});")
+            s.append(f"      if (lds > 160 * 1024 || op->max_owned > bs) {{")
+            s.append(f'        printf("op2hip: %s: owner plan with %d-target blocks needs %zu bytes of LDS; use a smaller block\\n",')
+            s.append(f"               name, op->max_owned, lds);")
             s.append(f"        exit(-1);")
             s.append(f"      }}")
             # instance-ordered copies of each map's rows, packed once per plan
@@ -599,6 +624,19 @@ __global__ void op2gen_instance_rows(const int *__restrict elems, int total,
   if (j >= total) return;
   const int e = elems[j];
   for (int d = 0; d < dim; ++d) rows[(size_t)j * dim + d] = map[(size_t)e * dim + d];
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each
+// with its own L2.  Logical block ids give each XCD one contiguous range, so
+// neighbouring patches of a locally numbered mesh share an L2.  A bijection of
+// [0, gridDim.x).
+__device__ __forceinline__ int op2gen_xcd_block() {
+  constexpr int nxcd = 8;
+  const int b = blockIdx.x, n = gridDim.x;
+  if (n < nxcd) return b;
+  const int q = n / nxcd, r = n % nxcd;
+  const int x = b % nxcd, i = b / nxcd;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
 template <class T> struct op2gen_lim {

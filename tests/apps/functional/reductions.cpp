@@ -80,6 +80,35 @@ int main(int argc, char **argv) {
     ft_check(std::fabs(got[i] - want3[i]) < 1e-4, "indirect inc dim 3", i);
   std::printf("indirect_inc3 passed\n");
 
+  // many increments per target: 600 spokes join one of three hubs (nodes 0,
+  // 48, 96) to node s mod 144, so a hub takes over 200 increments and each
+  // block of the owner partition walks its spokes in several chunks; the
+  // weights are multiples of 1/4, so every sum is exact in float
+  const int ns = 600;
+  std::vector<int> s2n(2 * ns);
+  std::vector<float> sw(ns);
+  for (int e = 0; e < ns; ++e) {
+    s2n[2 * e] = 48 * (e % 3);
+    s2n[2 * e + 1] = e % nn;
+    sw[e] = 0.25f * (e % 5);
+  }
+  op_set spokes = op_decl_set(ns, "spokes");
+  op_map ms = op_decl_map(spokes, nodes, 2, s2n.data(), "spoke_nodes");
+  op_dat swd = op_decl_dat(spokes, 1, "float", sw.data(), "sw");
+  op_fetch_data(v1, got.data());
+  std::vector<double> wants(got.begin(), got.begin() + nn);
+  for (int e = 0; e < ns; ++e) {
+    wants[s2n[2 * e]] += sw[e];
+    wants[s2n[2 * e + 1]] += sw[e];
+  }
+  op_par_loop(edge_to_nodes1, "spokes_to_nodes", spokes,
+              op_arg_dat(v1, 0, ms, 1, "float", OP_INC), op_arg_dat(v1, 1, ms, 1, "float", OP_INC),
+              op_arg_dat(swd, -1, OP_ID, 1, "float", OP_READ));
+  op_fetch_data(v1, got.data());
+  for (int i = 0; i < nn; ++i)
+    ft_check(got[i] == (float)wants[i], "many increments per target", i);
+  std::printf("hub_inc passed\n");
+
   op_par_loop(self_inc1, "self_inc1", edges, op_arg_dat(w1, -1, OP_ID, 1, "float", OP_READ),
               op_arg_dat(a1, -1, OP_ID, 1, "float", OP_INC));
   op_fetch_data(a1, got.data());
