@@ -134,10 +134,12 @@ class LoopGen:
                 f.append(f"  {a.typ} *d{j}; int s{j}; int opt{j};")
                 if not a.direct:
                     f.append(f"  const int *m{j}; int md{j}; const int *r{j};")
+                    f.append(f"  const unsigned short *h{j}; const int *hb{j};")
             elif isinstance(a, ArgGbl) and a.acc == "OP_READ":
                 f.append(f"  {a.typ} g{j}[{a.dim}]; int opt{j};")
             elif isinstance(a, ArgIdx) and a.map:
                 f.append(f"  const int *m{j}; int md{j}; const int *r{j};")
+                f.append(f"  const unsigned short *h{j}; const int *hb{j};")
         f.append("  int _unused;")
         return f"struct {self.name}_args {{\n" + "\n".join(f) + "\n};\n"
 
@@ -169,7 +171,7 @@ class LoopGen:
         compile time."""
         T, D = a.typ, a.dim
         # rows: the owner kernel reads instance-ordered copies of the map rows
-        row = f"A.r{j}[(size_t)(e0 + j) * A.md{j} + {a.idx}]" if rows else \
+        row = f"op2gen_row(A.r{j}, A.h{j}, hb{j}, (size_t)(e0 + j) * A.md{j} + {a.idx})" if rows else \
             f"A.m{j}[(size_t)n * A.md{j} + {a.idx}]"
         tgt = "n" if a.direct else row
         soa = f"(LAY == 1 || (LAY == 2 && A.s{j}))"
@@ -322,7 +324,7 @@ class LoopGen:
                 call.append(f"(A.opt{j} ? g{j} : nullptr)" if a.opt is not None else f"g{j}")
             elif isinstance(a, ArgIdx):
                 if a.map:
-                    pre.append(f"    int x{j} = A.r{j}[(size_t)(e0 + j) * A.md{j} + {a.idx}];")
+                    pre.append(f"    int x{j} = op2gen_row(A.r{j}, A.h{j}, hb{j}, (size_t)(e0 + j) * A.md{j} + {a.idx});")
                 else:
                     pre.append(f"    int x{j} = n;")
                 call.append(f"&x{j}")
@@ -340,7 +342,11 @@ class LoopGen:
              f"  const int lds_s = (int)blockDim.x * {ninc};  // slots per chunk",
              f"  const int blk = op2gen_xcd_block();",
              f"  const int e0 = elem_offs[blk], ne = elem_offs[blk + 1] - e0;",
-             f"  const int o0 = owned_offs[blk], no = owned_offs[blk + 1] - o0;",
+             f"  const int o0 = owned_offs[blk], no = owned_offs[blk + 1] - o0;"]
+        for j, a in enumerate(L.args):
+            if (isinstance(a, ArgDat) and not a.direct) or (isinstance(a, ArgIdx) and a.map):
+                b.append(f"  const int hb{j} = A.hb{j} ? A.hb{j}[blk] : -1;  // -1: int rows")
+        b += [
              f"  const bool own = (int)threadIdx.x < no;",
              f"  const int t = own ? owned[o0 + threadIdx.x] : 0;",
              f"  {T} acc[{D}];",
@@ -493,18 +499,32 @@ class LoopGen:
                 if (isinstance(a, ArgDat) and not a.direct) or (isinstance(a, ArgIdx) and a.map):
                     by_map.setdefault(a.map, []).append(j)
             for mp, js in by_map.items():
-                j0 = js[0]
-                s.append(f"      if (args[{j0}].opt) {{")
-                s.append(f"        char key[160];")
-                s.append(f'        snprintf(key, sizeof key, "owner_rows.%p.%d", (const void *)op, args[{j0}].map->index);')
-                s.append(f"        int fresh = 0;")
-                s.append(f"        const int mdim = args[{j0}].map->dim;")
-                s.append(f"        int *rows = (int *)op2h_plan_user(key, sizeof(int) * (size_t)op->total * mdim, &fresh);")
-                s.append(f"        if (fresh && op->total)")
-                s.append(f"          op2gen_launch(false, nullptr, nullptr, op2gen_instance_rows, (op->total + 255) / 256, 256,")
-                s.append(f"                        0, st, op->elems, op->total, args[{j0}].map_data_d, mdim, rows);")
+                s.append(f"      {{")
+                s.append(f"        const op_arg *ra = nullptr;  // the first enabled arg on this map")
                 for j in js:
-                    s.append(f"        A.r{j} = rows;")
+                    s.append(f"        if (!ra && args[{j}].opt) ra = &args[{j}];")
+                s.append(f"        if (ra) {{")
+                s.append(f"          char key[160];")
+                s.append(f'          snprintf(key, sizeof key, "owner_rows.%p.%d", (const void *)op, ra->map->index);')
+                s.append(f"          int fresh = 0;")
+                s.append(f"          const int mdim = ra->map->dim;")
+                s.append(f"          const size_t nr = (size_t)op->total * mdim;")
+                s.append(f"          // [int rows][int base per block][ushort rows][pad]")
+                s.append(f"          unsigned char *mem = (unsigned char *)op2h_plan_user(")
+                s.append(f"              key, 6 * nr + 4 * (size_t)op->nblocks + 16, &fresh);")
+                s.append(f"          int *rows = (int *)mem, *base = rows + nr;")
+                s.append(f"          unsigned short *rows16 = (unsigned short *)(base + op->nblocks);")
+                s.append(f"          if (fresh && op->total) {{")
+                s.append(f"            op2gen_launch(false, nullptr, nullptr, op2gen_instance_rows, (op->total + 255) / 256, 256,")
+                s.append(f"                          0, st, op->elems, op->total, ra->map_data_d, mdim, rows);")
+                s.append(f"            op2gen_launch(false, nullptr, nullptr, op2gen_rows16, op->nblocks, 256, 0, st,")
+                s.append(f"                          op->elem_offs, rows, mdim, base, rows16);")
+                s.append(f"          }}")
+                for j in js:
+                    s.append(f"          A.r{j} = rows;")
+                    s.append(f"          A.h{j} = rows16;")
+                    s.append(f"          A.hb{j} = op->total ? base : nullptr;")
+                s.append(f"        }}")
                 s.append(f"      }}")
             s.append(f"      void *t0 = nullptr, *t1 = nullptr;")
             s.append(f"      const bool ext = op->nblocks && op2h_kernel_timer_events(&ctx, &t0, &t1) != 0;")
@@ -637,6 +657,55 @@ __device__ __forceinline__ int op2gen_xcd_block() {
   const int q = n / nxcd, r = n % nxcd;
   const int x = b % nxcd, i = b / nxcd;
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+// Per block b of an owner plan (one workgroup of 256 threads each): base[b]
+// = the least entry of the block's instance rows when every entry lies within
+// 65535 of it, rows16 then holding entry - base, which halves the bytes an
+// owner kernel reads per row; else base[b] = -1 and the block reads the int
+// rows.
+__global__ void __launch_bounds__(256)
+    op2gen_rows16(const int *__restrict elem_offs, const int *__restrict rows, int dim,
+                  int *__restrict base, unsigned short *__restrict rows16) {
+  __shared__ int s_mn[4], s_mx[4];
+  const int b = blockIdx.x;
+  const size_t lo = (size_t)elem_offs[b] * dim, hi = (size_t)elem_offs[b + 1] * dim;
+  int mn = 0x7fffffff, mx = -1;
+  for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    mn = min(mn, rows[i]);
+    mx = max(mx, rows[i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = min(mn, __shfl_xor(mn, o, 64));
+    mx = max(mx, __shfl_xor(mx, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_mn[threadIdx.x >> 6] = mn;
+    s_mx[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  mn = min(min(s_mn[0], s_mn[1]), min(s_mn[2], s_mn[3]));
+  mx = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));
+  const bool narrow = hi == lo || (long long)mx - mn < 65535;
+  if (threadIdx.x == 0) base[b] = hi == lo ? 0 : narrow ? mn : -1;
+  if (narrow)
+    for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+      rows16[i] = (unsigned short)(rows[i] - mn);
+}
+
+// Entry i of an owner kernel's instance rows: from the 16-bit rows when the
+// block has a base (uniform over the workgroup), else from the int rows.  One
+// 32-bit load either way (the aligned word holding h[i]; the 16-bit rows
+// start 4-byte aligned and are padded), its address selected without a
+// branch, so the loads of all rows of an instance issue together.
+__device__ __forceinline__ int op2gen_row(const int *__restrict r,
+                                          const unsigned short *__restrict h, int base,
+                                          size_t i) {
+  const bool narrow = base >= 0;
+  const unsigned *w = narrow ? reinterpret_cast<const unsigned *>(h + (i & ~(size_t)1))
+                             : reinterpret_cast<const unsigned *>(r + i);
+  const unsigned v = *w;
+  return narrow ? base + (int)((i & 1) ? v >> 16 : v & 0xFFFFu) : (int)v;
 }
 
 template <class T> struct op2gen_lim {

@@ -109,6 +109,38 @@ int main(int argc, char **argv) {
     ft_check(got[i] == (float)wants[i], "many increments per target", i);
   std::printf("hub_inc passed\n");
 
+  // targets far apart in numbering: on 140000 nodes, 500 links join
+  // 137e mod 140000 to a node 70001 further on, 500 more join neighbours, so
+  // some blocks of the owner partition span more than 65535 node numbers and
+  // others do not
+  const int nf = 140000, nl = 1000;
+  std::vector<int> l2n(2 * nl);
+  std::vector<float> lw(nl), fv(nf, 0.5f);
+  for (int e = 0; e < nl; ++e) {
+    const int a = e < 500 ? (137 * e) % nf : 90000 + e;
+    l2n[2 * e] = a;
+    l2n[2 * e + 1] = e < 500 ? (a + 70001) % nf : a + 1;
+    lw[e] = 0.25f * (1 + e % 3);
+  }
+  op_set far_nodes = op_decl_set(nf, "far_nodes");
+  op_set links = op_decl_set(nl, "links");
+  op_map ml = op_decl_map(links, far_nodes, 2, l2n.data(), "link_nodes");
+  op_dat lwd = op_decl_dat(links, 1, "float", lw.data(), "lw");
+  op_dat fvd = op_decl_dat(far_nodes, 1, "float", fv.data(), "fv");
+  std::vector<double> wantf(fv.begin(), fv.end());
+  for (int e = 0; e < nl; ++e) {
+    wantf[l2n[2 * e]] += lw[e];
+    wantf[l2n[2 * e + 1]] += lw[e];
+  }
+  op_par_loop(edge_to_nodes1, "links_to_nodes", links,
+              op_arg_dat(fvd, 0, ml, 1, "float", OP_INC), op_arg_dat(fvd, 1, ml, 1, "float", OP_INC),
+              op_arg_dat(lwd, -1, OP_ID, 1, "float", OP_READ));
+  std::vector<float> gotf(nf);
+  op_fetch_data(fvd, gotf.data());
+  for (int i = 0; i < nf; ++i)
+    ft_check(gotf[i] == (float)wantf[i], "targets far apart in numbering", i);
+  std::printf("far_inc passed\n");
+
   op_par_loop(self_inc1, "self_inc1", edges, op_arg_dat(w1, -1, OP_ID, 1, "float", OP_READ),
               op_arg_dat(a1, -1, OP_ID, 1, "float", OP_INC));
   op_fetch_data(a1, got.data());
