@@ -98,6 +98,14 @@ class LoopGen:
                 off += _pad8(ia.dim * _size(ia.typ)) + 8
         self.rec = max(off, 8)
         self.owner = self._owner_eligible()
+        # args on one map read its rows through the first of them, so the
+        # compiler sees one base pointer and can merge the loads of a row
+        # (optional args keep their own, which is null when they are off)
+        self.rep, first = {}, {}
+        for j, a in enumerate(loop.args):
+            mapped = (isinstance(a, ArgDat) and not a.direct) or (isinstance(a, ArgIdx) and a.map)
+            if mapped and getattr(a, "opt", None) is None:
+                self.rep[j] = first.setdefault(a.map, j)
         if loop.indirect_writes():
             self.def_plan, self.mask = PLAN_COLOR, 1 << PLAN_COLOR
         elif loop.indirect_incs():
@@ -171,8 +179,9 @@ class LoopGen:
         compile time."""
         T, D = a.typ, a.dim
         # rows: the owner kernel reads instance-ordered copies of the map rows
-        row = f"op2gen_row(A.r{j}, A.h{j}, hb{j}, (size_t)(e0 + j) * A.md{j} + {a.idx})" if rows else \
-            f"A.m{j}[(size_t)n * A.md{j} + {a.idx}]"
+        m = self.rep.get(j, j)
+        row = f"op2gen_row(A.r{m}, A.h{m}, hb{m}, (size_t)(e0 + j) * A.md{m} + {a.idx})" if rows else \
+            f"A.m{m}[(size_t)n * A.md{m} + {a.idx}]"
         tgt = "n" if a.direct else row
         soa = f"(LAY == 1 || (LAY == 2 && A.s{j}))"
         idx = f"({soa} ? t{j} + (size_t)k * A.s{j} : (size_t)t{j} * {D} + k)"
@@ -230,7 +239,8 @@ class LoopGen:
                     call.append(f"r{j}")
             elif isinstance(a, ArgIdx):
                 if a.map:
-                    pre.append(f"    int x{j} = A.m{j}[(size_t)n * A.md{j} + {a.idx}];")
+                    m = self.rep.get(j, j)
+                    pre.append(f"    int x{j} = A.m{m}[(size_t)n * A.md{m} + {a.idx}];")
                 else:
                     pre.append(f"    int x{j} = n;")
                 call.append(f"&x{j}")
@@ -324,7 +334,8 @@ class LoopGen:
                 call.append(f"(A.opt{j} ? g{j} : nullptr)" if a.opt is not None else f"g{j}")
             elif isinstance(a, ArgIdx):
                 if a.map:
-                    pre.append(f"    int x{j} = op2gen_row(A.r{j}, A.h{j}, hb{j}, (size_t)(e0 + j) * A.md{j} + {a.idx});")
+                    m = self.rep.get(j, j)
+                    pre.append(f"    int x{j} = op2gen_row(A.r{m}, A.h{m}, hb{m}, (size_t)(e0 + j) * A.md{m} + {a.idx});")
                 else:
                     pre.append(f"    int x{j} = n;")
                 call.append(f"&x{j}")
