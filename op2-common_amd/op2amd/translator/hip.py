@@ -267,7 +267,8 @@ class LoopGen:
         body.append(f"__global__ void __launch_bounds__(1024)")
         body.append(f"    {self.name}_kernel({self.name}_args A, const int *__restrict elems, int start,")
         body.append(f"                 int end, int atomic, unsigned char *partials, int part_base) {{")
-        body.append(f"  const int tid = start + (int)(blockIdx.x * blockDim.x + threadIdx.x);")
+        body.append(f"  const int blk = op2gen_xcd_block();  // XCD-contiguous element ranges")
+        body.append(f"  const int tid = start + blk * (int)blockDim.x + (int)threadIdx.x;")
         body.append(f"  const bool live = tid < end;")
         body.append(f"  const int n = live ? (elems ? elems[tid] : tid) : 0;")
         body += red_decl
@@ -279,7 +280,7 @@ class LoopGen:
         body += post
         body.append(f"  }}")
         if self.reds:
-            body.append(f"  unsigned char *rec = partials + (size_t)(part_base + blockIdx.x) * {self.rec};")
+            body.append(f"  unsigned char *rec = partials + (size_t)(part_base + blk) * {self.rec};")
             for j in self.reds:
                 a = L.args[j]
                 op = {"OP_INC": "0", "OP_MIN": "1", "OP_MAX": "2"}[a.acc]
@@ -405,7 +406,7 @@ class LoopGen:
             return ""
         L = self.L
         params = ", ".join(f"unsigned char *slot{j}" for j in self.reds)
-        b = [f"__global__ void __launch_bounds__(256)",
+        b = [f"__global__ void __launch_bounds__(1024)",
              f"    {self.name}_finish(const unsigned char *__restrict partials, int nparts, {params}) {{"]
         for j in self.reds:
             a = L.args[j]
@@ -588,7 +589,7 @@ class LoopGen:
         if self.reds:
             slots = ", ".join(f"(unsigned char *)op2h_gbl_slot(&ctx, {j}, {self.info_of.get(j, -1)})"
                               for j in self.reds)
-            s.append(f"    op2gen_launch(ext, nullptr, t1, {self.name}_finish, 1, 256, 0, st, partials,")
+            s.append(f"    op2gen_launch(ext, nullptr, t1, {self.name}_finish, 1, 1024, 0, st, partials,")
             s.append(f"                  nparts, {slots});")
         s.append(f"    if (!ext) op2h_kernel_timer_stop(&ctx);")
         s.append(f"    OP2GEN_CHECK(hipGetLastError());")
