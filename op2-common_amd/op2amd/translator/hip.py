@@ -280,7 +280,12 @@ class LoopGen:
         body += post
         body.append(f"  }}")
         if self.reds:
-            body.append(f"  unsigned char *rec = partials + (size_t)(part_base + blk) * {self.rec};")
+            # block epilogue: each wave reduces by shuffles, one barrier, then
+            # thread 0 combines the waves in wave order (the order of
+            # op2gen_block_reduce) and writes the block's partial record
+            body.append(f"  {{")
+            body.append(f"    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;")
+            fin = []
             for j in self.reds:
                 a = L.args[j]
                 op = {"OP_INC": "0", "OP_MIN": "1", "OP_MAX": "2"}[a.acc]
@@ -290,20 +295,44 @@ class LoopGen:
                     ia = L.args[ij]
                     io = o + _pad8(_size(a.typ))
                     ao = io + _pad8(ia.dim * _size(ia.typ))
-                    body.append(f"  {{")
-                    body.append(f"    {a.typ} bv = r{j}[0]; int ba = at{j};")
-                    body.append(f"    const int win = op2gen_block_argbest<{a.typ}, {op}>(bv, ba);")
-                    body.append(f"    if (threadIdx.x == (unsigned)win) {{")
-                    body.append(f"      *({a.typ} *)(rec + {o}) = r{j}[0];")
-                    body.append(f"      for (int k = 0; k < {ia.dim}; ++k) (({ia.typ} *)(rec + {io}))[k] = v{ij}[k];")
-                    body.append(f"      *(int *)(rec + {ao}) = at{j};")
+                    body.append(f"    __shared__ {a.typ} s_v{j}[16];")
+                    body.append(f"    __shared__ int s_a{j}[16];")
+                    body.append(f"    __shared__ {ia.typ} s_i{j}[16][{ia.dim}];")
+                    body.append(f"    {{")
+                    body.append(f"      {a.typ} bv = r{j}[0];")
+                    body.append(f"      int ba = at{j};")
+                    body.append(f"      const int win = op2gen_wave_argbest<{a.typ}, {op}>(bv, ba);")
+                    body.append(f"      if (lane == win) {{")
+                    body.append(f"        s_v{j}[w] = bv;")
+                    body.append(f"        s_a{j}[w] = ba;")
+                    body.append(f"        for (int k = 0; k < {ia.dim}; ++k) s_i{j}[w][k] = v{ij}[k];")
+                    body.append(f"      }}")
                     body.append(f"    }}")
-                    body.append(f"  }}")
+                    fin.append(f"      {{")
+                    fin.append(f"        int bw = 0;")
+                    fin.append(f"        for (int i = 1; i < nw; ++i)")
+                    fin.append(f"          if (op2gen_better<{a.typ}, {op}>(s_v{j}[i], s_a{j}[i], s_v{j}[bw], s_a{j}[bw])) bw = i;")
+                    fin.append(f"        *({a.typ} *)(rec + {o}) = s_v{j}[bw];")
+                    fin.append(f"        for (int k = 0; k < {ia.dim}; ++k) (({ia.typ} *)(rec + {io}))[k] = s_i{j}[bw][k];")
+                    fin.append(f"        *(int *)(rec + {ao}) = s_a{j}[bw];")
+                    fin.append(f"      }}")
                 else:
-                    body.append(f"  for (int k = 0; k < {a.dim}; ++k) {{")
-                    body.append(f"    const {a.typ} v = op2gen_block_reduce<{a.typ}, {op}>(r{j}[k]);")
-                    body.append(f"    if (threadIdx.x == 0) (({a.typ} *)(rec + {o}))[k] = v;")
-                    body.append(f"  }}")
+                    body.append(f"    __shared__ {a.typ} s_r{j}[16][{a.dim}];")
+                    body.append(f"    for (int k = 0; k < {a.dim}; ++k) {{")
+                    body.append(f"      const {a.typ} v = op2gen_wave_reduce<{a.typ}, {op}>(r{j}[k]);")
+                    body.append(f"      if (lane == 0) s_r{j}[w][k] = v;")
+                    body.append(f"    }}")
+                    fin.append(f"      for (int k = 0; k < {a.dim}; ++k) {{")
+                    fin.append(f"        {a.typ} acc = s_r{j}[0][k];")
+                    fin.append(f"        for (int i = 1; i < nw; ++i) acc = op2gen_combine<{a.typ}, {op}>(acc, s_r{j}[i][k]);")
+                    fin.append(f"        (({a.typ} *)(rec + {o}))[k] = acc;")
+                    fin.append(f"      }}")
+            body.append(f"    __syncthreads();")
+            body.append(f"    if (threadIdx.x == 0) {{")
+            body.append(f"      unsigned char *rec = partials + (size_t)(part_base + blk) * {self.rec};")
+            body += fin
+            body.append(f"    }}")
+            body.append(f"  }}")
         body.append("}")
         return "\n".join(body) + "\n"
 
@@ -402,12 +431,16 @@ class LoopGen:
         return "\n".join(b) + "\n"
 
     def finish(self):
+        """One workgroup combines the per-block partial records: a single pass
+        over the records gathers every reduction's per-thread value (records
+        in ascending order per thread), then one workgroup reduction each."""
         if not self.reds:
             return ""
         L = self.L
         params = ", ".join(f"unsigned char *slot{j}" for j in self.reds)
         b = [f"__global__ void __launch_bounds__(1024)",
              f"    {self.name}_finish(const unsigned char *__restrict partials, int nparts, {params}) {{"]
+        decl, load, fin = [], [], []
         for j in self.reds:
             a = L.args[j]
             op = {"OP_INC": "0", "OP_MIN": "1", "OP_MAX": "2"}[a.acc]
@@ -418,31 +451,37 @@ class LoopGen:
                 ia = L.args[self.info_of[j]]
                 io = o + _pad8(_size(a.typ))
                 ao = io + _pad8(ia.dim * _size(ia.typ))
-                b.append(f"  {{")
-                b.append(f"    {a.typ} bv = {ident}; int ba = 0x7fffffff, bp = -1;")
-                b.append(f"    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {{")
-                b.append(f"      const unsigned char *rec = partials + (size_t)p * {self.rec};")
-                b.append(f"      const {a.typ} v = *(const {a.typ} *)(rec + {o});")
-                b.append(f"      const int at = *(const int *)(rec + {ao});")
-                b.append(f"      if (op2gen_better<{a.typ}, {op}>(v, at, bv, ba)) {{ bv = v; ba = at; bp = p; }}")
-                b.append(f"    }}")
-                b.append(f"    const int win = op2gen_block_argbest<{a.typ}, {op}>(bv, ba);")
-                b.append(f"    if (threadIdx.x == (unsigned)win) {{")
-                b.append(f"      *({a.typ} *)slot{j} = bv;")
-                b.append(f"      if (bp >= 0) for (int k = 0; k < {ia.dim}; ++k)")
-                b.append(f"        (({ia.typ} *)(slot{j} + {_size(a.typ)}))[k] = "
-                         f"((const {ia.typ} *)(partials + (size_t)bp * {self.rec} + {io}))[k];")
-                b.append(f"    }}")
-                b.append(f"  }}")
+                decl.append(f"  {a.typ} bv{j} = {ident}; int ba{j} = 0x7fffffff, bp{j} = -1;")
+                load.append(f"    {{")
+                load.append(f"      const {a.typ} v = *(const {a.typ} *)(rec + {o});")
+                load.append(f"      const int at = *(const int *)(rec + {ao});")
+                load.append(f"      if (op2gen_better<{a.typ}, {op}>(v, at, bv{j}, ba{j})) {{ bv{j} = v; ba{j} = at; bp{j} = p; }}")
+                load.append(f"    }}")
+                fin.append(f"  {{")
+                fin.append(f"    const int win = op2gen_block_argbest<{a.typ}, {op}>(bv{j}, ba{j});")
+                fin.append(f"    if (threadIdx.x == (unsigned)win) {{")
+                fin.append(f"      *({a.typ} *)slot{j} = bv{j};")
+                fin.append(f"      if (bp{j} >= 0) for (int k = 0; k < {ia.dim}; ++k)")
+                fin.append(f"        (({ia.typ} *)(slot{j} + {_size(a.typ)}))[k] = "
+                           f"((const {ia.typ} *)(partials + (size_t)bp{j} * {self.rec} + {io}))[k];")
+                fin.append(f"    }}")
+                fin.append(f"  }}")
             else:
-                b.append(f"  for (int k = 0; k < {a.dim}; ++k) {{")
-                b.append(f"    {a.typ} acc = {ident};")
-                b.append(f"    for (int p = threadIdx.x; p < nparts; p += blockDim.x)")
-                b.append(f"      acc = op2gen_combine<{a.typ}, {op}>(acc, ((const {a.typ} *)(partials + "
-                         f"(size_t)p * {self.rec} + {o}))[k]);")
-                b.append(f"    const {a.typ} v = op2gen_block_reduce<{a.typ}, {op}>(acc);")
-                b.append(f"    if (threadIdx.x == 0) (({a.typ} *)slot{j})[k] = v;")
-                b.append(f"  }}")
+                decl.append(f"  {a.typ} acc{j}[{a.dim}];")
+                decl.append(f"  for (int k = 0; k < {a.dim}; ++k) acc{j}[k] = {ident};")
+                load.append(f"    for (int k = 0; k < {a.dim}; ++k)")
+                load.append(f"      acc{j}[k] = op2gen_combine<{a.typ}, {op}>(acc{j}[k], "
+                            f"((const {a.typ} *)(rec + {o}))[k]);")
+                fin.append(f"  for (int k = 0; k < {a.dim}; ++k) {{")
+                fin.append(f"    const {a.typ} v = op2gen_block_reduce<{a.typ}, {op}>(acc{j}[k]);")
+                fin.append(f"    if (threadIdx.x == 0) (({a.typ} *)slot{j})[k] = v;")
+                fin.append(f"  }}")
+        b += decl
+        b.append(f"  for (int p = threadIdx.x; p < nparts; p += blockDim.x) {{")
+        b.append(f"    const unsigned char *rec = partials + (size_t)p * {self.rec};")
+        b += load
+        b.append(f"  }}")
+        b += fin
         b.append("}")
         return "\n".join(b) + "\n"
 
@@ -756,6 +795,39 @@ template <class T, int OP> __device__ T op2gen_block_reduce(T v) {
   for (int i = 1; i < nw; ++i) r = op2gen_combine<T, OP>(r, part[i]);
   __syncthreads();
   return r;
+}
+
+// Xor-shuffle tree inside a wave (the first stage of op2gen_block_reduce);
+// lanes past the end of a partial last wave take no part.
+template <class T, int OP> __device__ __forceinline__ T op2gen_wave_reduce(T v) {
+  const int t = threadIdx.x;
+  if ((blockDim.x & 63) == 0) {  // whole waves (uniform): no lane guard
+    for (int o = 32; o > 0; o >>= 1) v = op2gen_combine<T, OP>(v, __shfl_xor(v, o));
+    return v;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const T u = __shfl_xor(v, o);
+    if ((t ^ o) < (int)blockDim.x) v = op2gen_combine<T, OP>(v, u);
+  }
+  return v;
+}
+
+// Wave arg-best over (value, element) pairs: v and at become the wave's best
+// pair; returns the lane that held it, as lane 0 sees it (lanes tied on an
+// equal pair may each name themselves), so exactly one lane is named.
+template <class T, int OP> __device__ __forceinline__ int op2gen_wave_argbest(T &v, int &at) {
+  const int t = threadIdx.x;
+  const bool whole = (blockDim.x & 63) == 0;  // uniform
+  int who = t & 63;
+  for (int o = 32; o > 0; o >>= 1) {
+    const T ov = __shfl_xor(v, o);
+    const int oa = __shfl_xor(at, o), ow = __shfl_xor(who, o);
+    const bool take = (whole || (t ^ o) < (int)blockDim.x) && op2gen_better<T, OP>(ov, oa, v, at);
+    v = take ? ov : v;
+    at = take ? oa : at;
+    who = take ? ow : who;
+  }
+  return __shfl(who, 0);
 }
 
 // Workgroup arg-best over (value, element) pairs: returns the winning thread.
