@@ -253,12 +253,25 @@ def workload_translated(op2amd, args):
     with tempfile.TemporaryDirectory() as d:
         grid = os.path.join(d, "new_grid.dat")
         mesh.write(grid)
-        log(f"translated airfoil: {I}x{J}, {iters} iterations")
+        log(f"translated airfoil: {I}x{J}, {iters} iterations, kernel timing on")
         r = subprocess.run([str(exe), grid, str(iters)] + tokens, capture_output=True, text=True,
                            timeout=600)
-    if r.returncode != 0:
-        raise SystemExit(f"airfoil_tr failed ({r.returncode}): {r.stdout[-2000:]}{r.stderr[-2000:]}")
-    secs = None
+        if r.returncode != 0:
+            raise SystemExit(f"airfoil_tr failed ({r.returncode}): {r.stdout[-2000:]}{r.stderr[-2000:]}")
+        # the step rate without the per-loop timing events (which wait for
+        # each kernel), as an application would run it; the program's clock
+        # also covers the plans built at each loop's first call, so the run is
+        # long enough to amortise them
+        iters2 = max(iters, 2000)
+        log(f"translated airfoil: {I}x{J}, {iters2} iterations, kernel timing off")
+        r2 = subprocess.run([str(exe), grid, str(iters2)] + tokens[1:], capture_output=True,
+                            text=True, timeout=600)
+        if r2.returncode != 0:
+            raise SystemExit(f"airfoil_tr failed ({r2.returncode}): {r2.stdout[-2000:]}{r2.stderr[-2000:]}")
+    secs = secs_untimed = None
+    for line in r2.stdout.splitlines():
+        if line.startswith("time "):
+            secs_untimed = float(line.split()[1])
     gen = {}
     for line in r.stdout.splitlines():
         if line.startswith("time "):
@@ -280,14 +293,16 @@ def workload_translated(op2amd, args):
     line = {
         "metric": "translated airfoil: res_calc kernel GB/s of the generated code (720k cells)",
         "value": res_gen, "unit": "GB/s", "n_gpus": 1, "steps": iters, "warmup": 0,
-        "ms_per_step": round(1e3 * secs / iters, 4) if secs else None,
+        "ms_per_step": round(1e3 * secs_untimed / iters2, 4) if secs_untimed else None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"airfoil {I}x{J} ({mesh.ncell} cells), {args.ordering} "
                                f"ordering, {iters} iterations, eager (rms read every update)",
                    "plans": {lp: f"{p}:{b}" for lp, (p, b) in plans.items()},
                    "translated_per_loop": gen, "hand_written_per_loop": hw,
-                   "translated_steps_per_s": round(iters / secs, 1) if secs else None},
+                   "translated_steps_per_s": round(iters2 / secs_untimed, 1) if secs_untimed else None,
+                   "translated_steps_per_s_iterations": iters2,
+                   "translated_steps_per_s_kernel_timing": round(iters / secs, 1) if secs else None},
         "roofline": {"bound": "hbm", "kernel": "res_calc (generated, owner, block 128)",
                      "achieved": res_gen, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(res_gen / HBM_PEAK_GBS, 4), "traffic": None},
