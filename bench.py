@@ -268,10 +268,20 @@ def workload_translated(op2amd, args):
                             text=True, timeout=600)
         if r2.returncode != 0:
             raise SystemExit(f"airfoil_tr failed ({r2.returncode}): {r2.stdout[-2000:]}{r2.stderr[-2000:]}")
-    secs = secs_untimed = None
+        # and as launches of a captured 10-step graph after one eager step
+        iters3 = iters2 + 1
+        log(f"translated airfoil: {I}x{J}, {iters3} iterations, graphs of 10 steps")
+        r3 = subprocess.run([str(exe), grid, str(iters3), "graph=10"] + tokens[1:],
+                            capture_output=True, text=True, timeout=600)
+        if r3.returncode != 0:
+            raise SystemExit(f"airfoil_tr failed ({r3.returncode}): {r3.stdout[-2000:]}{r3.stderr[-2000:]}")
+    secs = secs_untimed = secs_graph = None
     for line in r2.stdout.splitlines():
         if line.startswith("time "):
             secs_untimed = float(line.split()[1])
+    for line in r3.stdout.splitlines():
+        if line.startswith("time "):
+            secs_graph = float(line.split()[1])
     gen = {}
     for line in r.stdout.splitlines():
         if line.startswith("time "):
@@ -302,6 +312,7 @@ def workload_translated(op2amd, args):
                    "translated_per_loop": gen, "hand_written_per_loop": hw,
                    "translated_steps_per_s": round(iters2 / secs_untimed, 1) if secs_untimed else None,
                    "translated_steps_per_s_iterations": iters2,
+                   "translated_graphed_steps_per_s": round(iters3 / secs_graph, 1) if secs_graph else None,
                    "translated_steps_per_s_kernel_timing": round(iters / secs, 1) if secs else None},
         "roofline": {"bound": "hbm", "kernel": "res_calc (generated, owner, block 128)",
                      "achieved": res_gen, "peak": HBM_PEAK_GBS, "unit": "GB/s",

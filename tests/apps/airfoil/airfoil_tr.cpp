@@ -1,14 +1,18 @@
 // The airfoil time-stepping program written against the OP2 API (sets, maps,
 // dats, constants, five op_par_loops per step), for translation by
-// op2amd.translator.  Usage: airfoil_tr <new_grid.dat> <iterations> [op_init
-// tokens...].  Prints "iter rms maxerr errloc" per iteration with 17
+// op2amd.translator.  Usage: airfoil_tr <new_grid.dat> <iterations> [graph=G]
+// [op_init tokens...].  Prints "iter rms maxerr errloc" per iteration with 17
 // significant digits, the loop's wall time, op_timing_output's per-loop table
 // (kernel times with the OP_HIP_KERNEL_TIMING token), then a checksum of q.  The step is the reference
 // program's (apps/c/airfoil/airfoil_plain/dp/airfoil.cpp:221-297); the mesh
-// file format is its reader's (:88-143).
+// file format is its reader's (:88-143).  With graph=G (built for the HIP
+// backend) the first iteration runs eagerly and the rest as launches of a
+// captured graph of up to G steps (a divisor of the remaining iterations),
+// their rms/maxerr/errloc read from the graph history.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "op_seq.h"
@@ -23,7 +27,13 @@ int main(int argc, char **argv) {
     return 2;
   }
   std::vector<char *> init_argv{argv[0]};
-  for (int i = 3; i < argc; ++i) init_argv.push_back(argv[i]);
+  int graph_steps = 0;
+  for (int i = 3; i < argc; ++i) {
+    if (!std::strncmp(argv[i], "graph=", 6))
+      graph_steps = std::atoi(argv[i] + 6);
+    else
+      init_argv.push_back(argv[i]);
+  }
   op_init((int)init_argv.size(), init_argv.data(), 0);
   const int niter = std::atoi(argv[2]);
 
@@ -97,12 +107,12 @@ int main(int argc, char **argv) {
 
   double cpu0, wall0, cpu1, wall1;
   op_timers(&cpu0, &wall0);
-  for (int iter = 1; iter <= niter; iter++) {
+  double rms = 0.0, maxerr = 0.0;
+  int errloc = 0;
+  auto step = [&]() {
     op_par_loop(save_soln, "save_soln", cells,
                 op_arg_dat(p_q, -1, OP_ID, 4, "double", OP_READ),
                 op_arg_dat(p_qold, -1, OP_ID, 4, "double", OP_WRITE));
-    double rms = 0.0, maxerr = 0.0;
-    int errloc = 0;
     for (int k = 0; k < 2; k++) {
       op_par_loop(adt_calc, "adt_calc", cells,
                   op_arg_dat(p_x, 0, pcell, 2, "double", OP_READ),
@@ -140,7 +150,49 @@ int main(int argc, char **argv) {
                   op_arg_idx(-1, OP_ID),
                   op_arg_info(&errloc, 1, "int", 5));
     }
-    std::printf("%d %.17e %.17e %d\n", iter, std::sqrt(rms / ncell), maxerr, errloc);
+  };
+  if (graph_steps <= 0 || niter < 2) {
+    for (int iter = 1; iter <= niter; iter++) {
+      step();
+      std::printf("%d %.17e %.17e %d\n", iter, std::sqrt(rms / ncell), maxerr, errloc);
+    }
+  } else {
+#ifdef OP2_HIP_H
+    // the first step eagerly (plans are built at each loop's first call),
+    // then graph launches of G steps; each launch appends its 2 x G updates'
+    // rms, maxerr, errloc (one 8-byte word each) to the device history
+    step();
+    std::printf("%d %.17e %.17e %d\n", 1, std::sqrt(rms / ncell), maxerr, errloc);
+    const int rest = niter - 1;
+    int G = graph_steps < rest ? graph_steps : rest;
+    while (rest % G) --G;
+    if (op2h_graph_begin() != 0) return 3;
+    for (int k = 0; k < G; ++k) step();
+    if (op2h_graph_end() != 0) return 3;
+    const int launches = rest / G;
+    op2h_graph_launch(launches);
+    op2h_device_sync();
+    std::vector<double> hist((size_t)launches * 6 * G);
+    int nrec = 0;
+    const int words = op2h_graph_history(launches, hist.data(), &nrec);
+    if (words != 6 * G || nrec != launches) {
+      std::printf("graph history: %d records of %d words\n", nrec, words);
+      return 4;
+    }
+    for (int i = 0; i < rest; ++i) {
+      const double *h = &hist[(size_t)i * 6 + 3];  // the step's second update
+      // delivered as (host value at capture, 0) op (device result)
+      const double mx = h[1] > 0.0 ? h[1] : 0.0;
+      int loc = 0;
+      if (h[1] > 0.0) std::memcpy(&loc, &h[2], sizeof(int));
+      std::printf("%d %.17e %.17e %d\n", i + 2, std::sqrt(h[0] / ncell), mx, loc);
+    }
+    op2h_graph_sync();
+    op2h_graph_destroy();
+#else
+    std::printf("graph=G needs the HIP backend\n");
+    return 2;
+#endif
   }
   op_timers(&cpu1, &wall1);
   std::printf("time %.6f s for %d iterations\n", wall1 - wall0, niter);

@@ -4,7 +4,10 @@
   SoA strides, constants) with the default plans (owner partition for
   increments), with every dat in SoA layout (OP_AUTO_SOA), and with
   coloured or atomic increments;
-- the translated airfoil program against the reference-pinned trajectory."""
+- the translated airfoil program against the reference-pinned trajectory,
+  eagerly on each plan and as graph launches of 9 captured steps (the rest of
+  the 999 iterations after an eager one), rms/maxerr/errloc from the graph
+  history."""
 import subprocess
 
 import numpy as np
@@ -36,8 +39,8 @@ def test_functional_program(name, variant):
 
 
 @pytest.mark.parametrize("variant", [[], ["OP_HIP_PLAN=*:atomic"], ["OP_HIP_PLAN=*:color"],
-                                     ["OP_AUTO_SOA"]],
-                         ids=["owner", "atomic", "color", "soa"])
+                                     ["OP_AUTO_SOA"], ["graph=10"]],
+                         ids=["owner", "atomic", "color", "soa", "graphed"])
 def test_translated_airfoil_matches_reference_trajectory(variant, tmp_path):
     g = golden("airfoil_120x60_random_a1")
     mesh = op2amd.mesh.generate(int(g["I"]), int(g["J"]), 0, str(g["ordering"]))
