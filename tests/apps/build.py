@@ -23,7 +23,7 @@ PROGRAMS = {
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = ["-O2", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off",
+FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
