@@ -87,3 +87,31 @@ def test_generated_kernels_compile_for_gfx950(name, tmp_path):
            "-x", "hip", "-c", paths["kernels"], "-o", str(tmp_path / "k.o")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+def test_generated_airfoil_kernel_shapes(tmp_path):
+    """The shapes the generated kernels rely on: args on one map read its rows
+    through the first such arg (one pointer, so a row loads at once); the
+    owner kernel walks its instances in chunks of blockDim with LDS slots per
+    chunk and reads instance rows through op2gen_row (16-bit where a block
+    allows); every element kernel takes XCD-ordered blocks; update's partial
+    records come from one barrier per block."""
+    paths = translate(str(APPS / "airfoil" / "airfoil_tr.cpp"), str(tmp_path))
+    k = open(paths["kernels"]).read()
+    # adt_calc: the four x args through pcell share A.m0
+    for i in range(4):
+        assert f"const int t{i} = A.m0[(size_t)n * A.md0 + {i}];" in k
+    # res_calc owner kernel: pedge rows through arg 0, pecell rows through arg 2
+    owner = k[k.index("op2gen_airfoil_tr_3_owner(") :]
+    owner = owner[: owner.index("\n}\n")]
+    assert "op2gen_row(A.r0, A.h0, hb0, (size_t)(e0 + j) * A.md0 + 1)" in owner
+    for i in (3, 4, 5, 6, 7):
+        assert f"const int t{i} = op2gen_row(A.r2, A.h2, hb2," in owner
+    assert "for (int base = 0; base < ne; base += blockDim.x)" in owner
+    assert "lds[(size_t)k * lds_s + threadIdx.x * 2 + 1] = b7[k];" in owner
+    assert owner.count("__syncthreads();") == 2 and "op2gen_xcd_block()" in owner
+    upd = k[k.index("op2gen_airfoil_tr_5_kernel(") :]
+    upd = upd[: upd.index("\n}\n")]
+    assert upd.count("__syncthreads();") == 1 and "op2gen_xcd_block()" in upd
+    stub = open(paths["kernels"]).read()
+    assert "op2gen_rows16, op->nblocks, 256" in stub
