@@ -172,10 +172,15 @@ int main(int argc, char **argv) {
     const int launches = rest / G;
     op2h_graph_launch(launches);
     op2h_device_sync();
-    std::vector<double> hist((size_t)launches * 6 * G);
     int nrec = 0;
-    const int words = op2h_graph_history(launches, hist.data(), &nrec);
-    if (words != 6 * G || nrec != launches) {
+    const int words = op2h_graph_history(0, nullptr, &nrec);  // the record length
+    if (words != 6 * G) {
+      std::printf("graph history: records of %d words\n", words);
+      return 4;
+    }
+    std::vector<double> hist((size_t)launches * words);
+    op2h_graph_history(launches, hist.data(), &nrec);
+    if (nrec != launches) {
       std::printf("graph history: %d records of %d words\n", nrec, words);
       return 4;
     }
