@@ -283,8 +283,9 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
     // per-iteration gbl results from the device history; one record per
     // launch, per step [rms_k0][max_k0][loc_k0][rms_k1][max_k1][loc_k1]
     int nrec = 0;
+    const int words = launches > 0 ? op2h_graph_history(0, nullptr, &nrec) : 6 * G;
     std::vector<double> hist(std::max(launches, 1) * 6 * (size_t)G);
-    int words = op2h_graph_history(launches, hist.data(), &nrec);
+    if (words == 6 * G && launches > 0) op2h_graph_history(launches, hist.data(), &nrec);
     if (launches > 0 && (words != 6 * G || nrec != launches)) {
       printf("op2amd_airfoil_run: graph history has %d records of %d words\n",
              nrec, words);
