@@ -316,7 +316,9 @@ def workload_translated(op2amd, args):
                    "translated_steps_per_s_kernel_timing": round(iters / secs, 1) if secs else None},
         "roofline": {"bound": "hbm", "kernel": "res_calc (generated, owner, block 128)",
                      "achieved": res_gen, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(res_gen / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(res_gen / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic("res_calc/generated-owner/128",
+                                                 f"{I}x{J}/{args.ordering}")},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
