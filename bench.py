@@ -7,16 +7,22 @@ bres_calc, update with the rms reduction) over the whole 1200x600 mesh
   plain    the reference's execution model on this backend: LDS-staged,
            two-level-coloured indirect loops, device sync and gbl read-back
            after every loop;
+  reference design, best
+           the same execution model with the fastest of the reference's own
+           schemes (staged or flat-coloured, any block size) per indirect loop;
   tuned    per-loop plan and block size picked by timing every variant on the
-           same mesh (exhaustive sweep);
+           same mesh (exhaustive sweep), run eagerly (sync where a gbl result
+           is needed);
   graphed  the tuned plans with each time step replayed from one HIP graph,
            rms/maxerr/errloc read back from a device history at the end.
 
 Two input orderings are measured (both named by the workload): the headline
 uses the RCM-renumbered mesh; the randomly numbered mesh is reported beside
-it, as given and after the runtime's op_renumber.  Parity: the graphed rms
-trajectory against the plain one at every iteration, and against the
-oracle's OpenMP run for the first iterations (max relative error).
+it, as given and after the runtime's op_renumber.  Parity: every run's rms at
+every one of the W + K iterations (warm-up, eager and graph-replayed alike)
+and its final q against the oracle's OpenMP run of the same mesh, relative
+error at most 1e-10; otherwise the line carries value null and the exit
+status is 1.
 """
 import argparse
 import json
@@ -33,6 +39,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "airfoil steps/s, 720k cells: tuned/graphed vs plain HIP at rms rel err ≤1e-10"
+PARITY_TOL = 1e-10  # rms (every iteration) and final q, relative, vs the oracle
 PLANS = ("atomic", "color", "staged", "gather", "owner")
 
 
@@ -68,8 +75,11 @@ def rel(a, b):
 
 def sweep(op2amd, mesh, iters, renumber):
     """Exhaustive sweep: per indirect loop, every plan x block size, timed
-    per call with HIP events; then the block size of the direct loops."""
-    best = {}
+    per call with HIP events; then the block size of the direct loops.
+    Returns the fastest plans, and the fastest among the reference's own
+    schemes (LDS-staged two-level colouring, flat colouring) per indirect
+    loop under "_reference_design"."""
+    best, ref_best = {}, {}
     for loop in ("res_calc", "bres_calc"):
         times = {}
         for plan in PLANS:
@@ -81,6 +91,7 @@ def sweep(op2amd, mesh, iters, renumber):
                                        kernel_timing=True, renumber=renumber, **kw)
                 times[(plan, bs)] = r["loop_kernel_ms"][loop] / max(r["loop_calls"][loop], 1)
         best[loop] = min(times, key=times.get)
+        ref_best[loop] = min((k for k in times if k[0] in ("staged", "color")), key=times.get)
         log(f"  sweep {loop}: " + ", ".join(
             f"{p}/{b}={t * 1e3:.1f}us" for (p, b), t in sorted(times.items(), key=lambda kv: kv[1])))
     direct = {}
@@ -93,7 +104,11 @@ def sweep(op2amd, mesh, iters, renumber):
     bd = min(direct, key=direct.get)
     return dict(plan_res=best["res_calc"][0], block_res=best["res_calc"][1],
                 plan_bres=best["bres_calc"][0], block_bres=best["bres_calc"][1],
-                block_direct=bd, block_adt=bd)
+                block_direct=bd, block_adt=bd,
+                _reference_design=dict(plan_res=ref_best["res_calc"][0],
+                                       block_res=ref_best["res_calc"][1],
+                                       plan_bres=ref_best["bres_calc"][0],
+                                       block_bres=ref_best["bres_calc"][1]))
 
 
 def parse_plans(text):
@@ -111,24 +126,48 @@ def parse_plans(text):
 
 
 def measure(op2amd, mesh, K, W, args, renumber=False, plans=None, plain_too=True):
+    """Step rates of the plain backend, the best of the reference's own
+    schemes with the reference's per-loop syncs, the tuned plans run eagerly,
+    and the tuned plans replayed from graphs; all over the same W + K
+    iterations, whose trajectories are returned for the parity check."""
     n = K + W
-    out = {}
+    out, runs = {}, {}
     if plain_too:
         plain = op2amd.airfoil.run(mesh, n, mode="plain", warmup=W, renumber=renumber)
         out["plain_steps_per_s"] = K / plain["seconds"]
+        runs["plain"] = plain
         log(f"  plain: {out['plain_steps_per_s']:.1f} steps/s")
-    kw = plans or sweep(op2amd, mesh, args.sweep_iters, renumber)
+    kw = dict(plans or sweep(op2amd, mesh, args.sweep_iters, renumber))
+    ref_design = kw.pop("_reference_design", None)
     out["tuned_plans"] = kw
+    if plain_too and ref_design:
+        # the reference's schemes at their best block sizes, synced per loop
+        rd = op2amd.airfoil.run(mesh, n, mode="plain", warmup=W, renumber=renumber,
+                                **ref_design)
+        out["reference_design_best_plans"] = ref_design
+        out["reference_design_best_steps_per_s"] = K / rd["seconds"]
+        runs["reference_design_best"] = rd
+        log(f"  best staged/colour plans {ref_design}: "
+            f"{out['reference_design_best_steps_per_s']:.1f} steps/s")
+    eager = op2amd.airfoil.run(mesh, n, mode="tuned", warmup=W, renumber=renumber, **kw)
+    out["tuned_eager_steps_per_s"] = K / eager["seconds"]
+    runs["tuned_eager"] = eager
+    log(f"  tuned plans, eager: {out['tuned_eager_steps_per_s']:.1f} steps/s")
     graphed = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, renumber=renumber,
                                  graph_steps=args.graph_steps, **kw)
+    runs["graphed"] = graphed
     out["graphed_steps_per_s"] = K / graphed["seconds"]
     out["steps_per_graph_launch"] = graphed["graph_steps"]
     # host-to-device work outside the timed region: declarations (copies and
     # uploads of every dat and map), plan builds, graph capture
     out["graphed_setup_seconds"] = graphed.get("setup_seconds")
     log(f"  graphed {kw}: {out['graphed_steps_per_s']:.1f} steps/s")
+    out["graph_gain_vs_tuned_eager"] = out["graphed_steps_per_s"] / out["tuned_eager_steps_per_s"]
     if plain_too:
         out["speedup_vs_plain"] = out["graphed_steps_per_s"] / out["plain_steps_per_s"]
+        if "reference_design_best_steps_per_s" in out:
+            out["speedup_vs_reference_design_best"] = (
+                out["graphed_steps_per_s"] / out["reference_design_best_steps_per_s"])
         out["rms_rel_err_graphed_vs_plain"] = rel(graphed["rms"], plain["rms"])
         out["q_rel_err_graphed_vs_plain"] = float(
             np.max(np.abs(graphed["q"] - plain["q"])) / np.max(np.abs(plain["q"])))
@@ -143,8 +182,19 @@ def measure(op2amd, mesh, K, W, args, renumber=False, plans=None, plain_too=True
     out["per_loop"] = per_loop
     out["step_bytes"] = sum(timed["loop_bytes"][lp] * (1 if lp == "save_soln" else 2)
                             for lp in op2amd.airfoil.LOOPS)
-    out["_graphed"] = graphed
+    out["_runs"] = runs
     return out
+
+
+def parity(runs, orc):
+    """Per run: the largest relative difference of rms from the oracle's over
+    every iteration (warm-up, eager and graph-replayed ones alike), and of
+    the final q, relative to max |q|."""
+    res = {}
+    for name, r in runs.items():
+        q_err = float(np.max(np.abs(r["q"] - orc["q"])) / np.max(np.abs(orc["q"])))
+        res[name] = {"rms_rel_err_max": rel(r["rms"], orc["rms"]), "q_rel_err": q_err}
+    return res
 
 
 def workload_large(op2amd, args):
@@ -502,30 +552,32 @@ def main():
     log(f"headline: {args.ordering} ordering")
     head = measure(op2amd, mesh, K, W, args, plans=plans)
     value = head["graphed_steps_per_s"]
-    graphed = head.pop("_graphed")
+    head_runs = head.pop("_runs")
 
-    others = {}
+    others, other_runs = {}, {}
     if not args.only_headline:
         other = "random" if args.ordering == "rcm" else "rcm"
         log(f"{other} ordering, as given")
         # the headline's plans (a full sweep per mesh would triple the run)
+        plans_other = dict(head["tuned_plans"])
+        if "reference_design_best_plans" in head:
+            plans_other["_reference_design"] = head["reference_design_best_plans"]
         others[f"{other}_ordering"] = measure(op2amd, meshes[other], K, W, args,
-                                              plans=head["tuned_plans"])
-        others[f"{other}_ordering"].pop("_graphed")
+                                              plans=plans_other)
+        other_runs[f"{other}_ordering"] = others[f"{other}_ordering"].pop("_runs")
         if other == "random":
             log("random ordering after op_renumber(pecell)")
             r = measure(op2amd, meshes["random"], K, W, args, renumber=True,
-                        plans=head["tuned_plans"])
-            r.pop("_graphed")
+                        plans=plans_other)
+            other_runs["random_ordering_op_renumber"] = r.pop("_runs")
             others["random_ordering_op_renumber"] = r
 
-    # parity of the headline against the oracle's OpenMP executor
+    # parity of every run over all W + K iterations against the oracle's
+    # OpenMP executor on the same mesh; its run doubles as the CPU baseline's
+    # sample when that is long enough
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    ncheck = 5
-    orc = oracle.airfoil_run(mesh, ncheck, 1, "openmp", nthreads=nthreads)
-    err_vs_oracle = rel(graphed["rms"][:ncheck], orc["rms"])
-
-    cpu = None
+    n_all = K + W
+    iters = n_all
     if not args.no_cpu:
         def cpu_call(n):
             t = time.time()
@@ -535,8 +587,29 @@ def main():
         # allocations of each call cancel; then a ~20 s sample
         t_short = cpu_call(2)
         one = max((cpu_call(12) - t_short) / 10, 1e-4)
-        iters = int(max(10, min(20000, 20.0 / one)))
-        cpu_run = oracle.airfoil_run(mesh, iters, 1, "openmp", nthreads=nthreads)
+        iters = max(n_all, int(max(10, min(20000, 20.0 / one))))
+    log(f"oracle: OpenMP executor, {iters} iterations on {nthreads} threads")
+    cpu_run = oracle.airfoil_run(mesh, iters, 1, "openmp", nthreads=nthreads)
+    orc = {"rms": cpu_run["rms"][:n_all]}
+    if iters == n_all:
+        orc["q"] = cpu_run["q"]
+    else:
+        orc["q"] = oracle.airfoil_run(mesh, n_all, 1, "openmp", nthreads=nthreads)["q"]
+    checks = {args.ordering: parity(head_runs, orc)}
+    for name, runs in other_runs.items():
+        if name == "random_ordering_op_renumber" or name.startswith("random"):
+            m_o = meshes["random"]
+        else:
+            m_o = meshes["rcm"]
+        o2 = oracle.airfoil_run(m_o, n_all, 1, "openmp", nthreads=nthreads)
+        checks[name] = parity(runs, {"rms": o2["rms"], "q": o2["q"]})
+    worst = max(v for c in checks.values() for r in c.values() for v in r.values())
+    parity_ok = bool(worst <= PARITY_TOL)
+    log(f"parity vs the oracle over {n_all} iterations: worst {worst:.2e} "
+        f"({'ok' if parity_ok else 'FAILED'})")
+
+    cpu = None
+    if not args.no_cpu:
         secs = float(sum(cpu_run["loop_seconds"]))  # the time-step loops only
         # per-loop CPU time and GB/s on the GPU line's byte model
         calls = {"save_soln": 1, "adt_calc": 2, "res_calc": 2, "bres_calc": 2, "update": 2}
@@ -583,7 +656,10 @@ def main():
                         f"{args.ordering} ordering, FP64; one step = save_soln + 2 x "
                         "(adt_calc, res_calc, bres_calc, update)",
             "mesh": args.mesh, "ordering": args.ordering,
-            "rms_rel_err_vs_oracle_first_steps": err_vs_oracle,
+            "parity_vs_oracle": {
+                "oracle": f"OpenMP executor on {nthreads} threads, same mesh and ordering, "
+                          f"all {n_all} iterations (warm-up, eager and graph-replayed)",
+                "tolerance": PARITY_TOL, "worst": worst, "ok": parity_ok, "runs": checks},
             **{k: v for k, v in head.items()},
             **others,
         },
@@ -599,7 +675,13 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if not parity_ok:
+        # a rate whose results differ from the oracle's is not reported
+        out["value"] = None
+        out["ms_per_step"] = None
     print(json.dumps(out), flush=True)
+    if not parity_ok:
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -202,7 +202,8 @@ void op_diagnostic_output(void);
 int op_is_root(void);
 void op_rank(int *rank);
 
-/* ---- profiling C API (op_profile.h:197-212); kernels timed with HIP events ---- */
+/* ---- profiling C API (op_profile.h:197-212): a timing tree with levels and
+ * JSON output, documented in op_profile.h ---- */
 void op_profile_start(const char *name);
 void op_profile_enter(const char *name);
 void op_profile_enter_kernel(const char *name, const char *target,
@@ -211,6 +212,18 @@ void op_profile_next(const char *name);
 void op_profile_exit(void);
 void op_profile_end(void);
 void op_profile_output(void);
+void op_profile_output_json(const char *filename);
+
+/* ---- host memory (op_lib_core.h:423-426, op_lib_core.cpp:1388-1420) ----
+ * op_malloc returns memory aligned to OP2_ALIGNMENT bytes (its size rounded up
+ * to a multiple of it; size 0 gives malloc(0)); op_calloc zeroes it;
+ * op_realloc keeps the alignment; all of it may be released with op_free or
+ * free. */
+#define OP2_ALIGNMENT 64
+void *op_malloc(size_t size);
+void *op_calloc(size_t num, size_t size);
+void *op_realloc(void *ptr, size_t size);
+void op_free(void *ptr);
 
 /* ---- renumbering (op2/src/externlib/op_renumber.cpp:224-505) ----
  * Reorders every set reachable from base->to for locality: base->to by

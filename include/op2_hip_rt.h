@@ -265,6 +265,69 @@ double op2h_algorithmic_bytes(op_set set, int nargs, const op_arg *args);
 /* Where plans are coloured: 1 on the GPU (default), 0 on the host. */
 void op2h_set_plan_builder(int on_device);
 
+/* ---- plan checks (op_plan_check, op2/src/core/op_rt_support.cpp:129-295) ----
+ * With OP_diags > 0 the runtime checks every plan it builds on the host and
+ * stops the program (exit(-1), after the messages) when a check fails: a
+ * plan with a colour conflict or a missing increment gives wrong sums
+ * without any other sign.  The checks are the reference's (element totals,
+ * block offsets, block permutation, staging-list offsets and order, local ids
+ * leading back to the map) plus the colouring checks its stub leaves empty
+ * (op_rt_support.cpp:290-294), and the analogous ones for the owner-compute
+ * and gather plans.  Each check below works on a host image of a plan,
+ * returns the number of failed checks and prints
+ * " *** OP_plan_check: <what> error" for each (OP_diags > 6: "<what> OK" for
+ * those that pass, as the reference does). */
+
+/* One indirect access of a loop, for the conflict checks: element e touches
+ * target table[e*dim + idx] (in [0,to_size)) of the loop's dat number `dat`
+ * (dats numbered from 0 within the loop); writes = 1 for OP_INC and OP_RW. */
+typedef struct {
+  const int *table;
+  int dim, idx, to_size, dat, writes;
+} op2h_check_access;
+
+/* Host image of a staged (two-level coloured) plan: the fields of
+ * op2h_staged_plan, with colstart[ncolors+1] the first blkmap position of
+ * each block colour and, per map slot, the map's table, dim and target set
+ * size; per loc slot its map slot and map index. */
+typedef struct {
+  int set_size, nblocks, ncolors;
+  const int *colstart, *blkmap, *offset, *nelems, *nthrcol;
+  const unsigned char *thrcol;
+  int nmaps;
+  const int *map_table[OP2H_MAX_ARGS];
+  int map_dim[OP2H_MAX_ARGS], map_to_size[OP2H_MAX_ARGS];
+  const int *ind_offs[OP2H_MAX_ARGS], *ind_sizes[OP2H_MAX_ARGS], *ind_map[OP2H_MAX_ARGS];
+  int nlocs;
+  int loc_map_slot[OP2H_MAX_ARGS], loc_idx[OP2H_MAX_ARGS];
+  const unsigned short *loc_map[OP2H_MAX_ARGS];
+} op2h_staged_plan_host;
+int op2h_check_staged_plan(const op2h_staged_plan_host *p, int nacc,
+                           const op2h_check_access *acc);
+
+/* Flat colouring: col_offsets[ncolors+1], col_reord[set_size]. */
+int op2h_check_color_plan(int set_size, int ncolors, const int *col_offsets,
+                          const int *col_reord, int nacc,
+                          const op2h_check_access *acc);
+
+/* Host image of an owner-compute plan (op2h_owner_plan) with the increment
+ * map's table (set_size x map_dim) and the map index of each increment;
+ * csr4 may be NULL. */
+typedef struct {
+  int set_size, to_size, nblocks, ninc;
+  const int *map_table;
+  int map_dim;
+  int inc_idx[OP2H_MAX_ARGS];
+  const int *owned_offs, *owned, *elem_offs, *elems, *csr_offs;
+  const unsigned short *csr, *csr4;
+} op2h_owner_plan_host;
+int op2h_check_owner_plan(const op2h_owner_plan_host *p);
+
+/* Gather plan (op2h_gather_plan) of a map (from_size x dim, row-major). */
+int op2h_check_gather_plan(int from_size, int to_size, int dim, const int *map,
+                           int ntargets, const int *offsets, const int *entries,
+                           const int *targets);
+
 #ifdef __cplusplus
 }
 #endif

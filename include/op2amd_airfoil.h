@@ -25,7 +25,9 @@ extern "C" {
 #endif
 
 #define OP2AMD_MODE_PLAIN 0   /* sync + gbl read-back after every loop,
-                                 default plans (LDS-staged, coloured) */
+                                 default plans (LDS-staged, coloured), or the
+                                 staged/coloured plans the options name (any
+                                 other plan: return code 5) */
 #define OP2AMD_MODE_TUNED 1   /* per-loop plans from the options, sync only
                                  where a gbl result is needed */
 #define OP2AMD_MODE_GRAPHED 2 /* TUNED plans, each time step replayed from a

@@ -188,8 +188,22 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
   }
 
   if (o->mode == OP2AMD_MODE_PLAIN) {
+    // the reference's execution model: a device sync after every loop; the
+    // indirect loops on their default (LDS-staged, two-level coloured) plan
+    // unless a staged or coloured variant is named
     op2h_set_sync_mode(OP2H_SYNC_EVERY_LOOP);
     op2h_set_loop_plan(nullptr, OP2H_PLAN_AUTO, 0);
+    for (auto pl : {std::make_pair("res_calc", std::make_pair(o->plan_res, o->block_res)),
+                    std::make_pair("bres_calc", std::make_pair(o->plan_bres, o->block_bres))})
+      if (pl.second.first != OP2H_PLAN_AUTO || pl.second.second != 0) {
+        if (pl.second.first != OP2H_PLAN_AUTO && pl.second.first != OP2H_PLAN_STAGED &&
+            pl.second.first != OP2H_PLAN_COLOR) {
+          printf("op2amd_airfoil_run: plain mode runs staged or coloured plans only\n");
+          op_exit();
+          return 5;
+        }
+        op2h_set_loop_plan(pl.first, pl.second.first, pl.second.second);
+      }
   } else if (o->autotune > 0) {
     op2h_set_sync_mode(OP2H_SYNC_GBL_ONLY);
     op2h_set_loop_plan(nullptr, OP2H_PLAN_AUTO, 0);

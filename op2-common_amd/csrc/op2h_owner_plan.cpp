@@ -217,6 +217,32 @@ std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
     err = "owner plan: the loop has no indirect increments";
     return nullptr;
   }
+  // Elements on a block boundary are evaluated by both blocks: every other
+  // effect of an element must be one that a second evaluation repeats
+  // harmlessly, and no element may read the dat being incremented.
+  for (int m = 0; m < nargs; ++m) {
+    const op_arg &a = args[m];
+    if (!a.opt) continue;
+    if (a.argtype == OP_ARG_GBL && a.acc != OP_READ) {
+      err = "owner plan: global reductions are not supported (elements on block "
+            "boundaries are evaluated twice)";
+      return nullptr;
+    }
+    if (a.argtype == OP_ARG_INFO) {
+      err = "owner plan: op_arg_info is not supported";
+      return nullptr;
+    }
+    if (a.argtype != OP_ARG_DAT) continue;
+    if (a.dat == owndat && a.acc != OP_INC) {
+      err = "owner plan: the incremented dat is also read by the loop";
+      return nullptr;
+    }
+    if (!a.map && (a.acc == OP_RW || a.acc == OP_INC)) {
+      err = "owner plan: direct OP_RW/OP_INC args are not supported (elements on "
+            "block boundaries are evaluated twice)";
+      return nullptr;
+    }
+  }
   if (block_cells < 1 || block_cells > 4096) {
     err = "owner plan: block size must be in [1, 4096]";
     return nullptr;

@@ -117,6 +117,17 @@ int gpu_colour_blocks_and_threads(const LoopShape &sh, int nblocks,
                                   int *blkcol_d, unsigned char *thrcol_d,
                                   int *nthrcol_d, hipStream_t stream);
 
+// Plan checks (op2h_plan_check.cpp) of a freshly built plan: the plan is read
+// back and checked on the host; on a failure the messages are printed and the
+// program exits with -1.
+void check_staged_or_exit(const StagedPlanRec &rec, const char *name, op_set set,
+                          int nargs, const op_arg *args);
+void check_color_or_exit(const ColorPlanRec &rec, const char *name, op_set set,
+                         int nargs, const op_arg *args);
+void check_owner_or_exit(const OwnerPlanRec &rec, const char *name, int nargs,
+                         const op_arg *args);
+void check_gather_or_exit(const GatherPlanRec &rec, op_map map);
+
 int download_staged(const op2h_staged_plan *p, int *thrcol, int *blkcol,
                     int *offset, int *nelems);
 int download_color(const op2h_color_plan *p, int *color_of_elem);

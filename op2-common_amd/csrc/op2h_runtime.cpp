@@ -25,6 +25,7 @@
 
 #include <rocprofiler-sdk-roctx/roctx.h>
 #include "op2h_plan_build.h"
+#include "op2h_profile.h"
 #include "op2h_reduce.h"
 #include "op2h_rcm.h"
 #include "op2h_rcm_gpu.h"
@@ -173,9 +174,10 @@ struct Runtime {
   std::map<std::string, ModelLoop> model_loops;
   std::vector<op2h_tuning_record> tune_records;
   hipEvent_t tune_ev[2] = {nullptr, nullptr};
-  // profiling
-  std::vector<std::pair<std::string, double>> prof_stack;
-  std::map<std::string, double> prof_totals;
+  // the current loop's node in the op_profile tree (loops do not nest)
+  bool prof_loop = false;
+  double prof_t0 = 0.0, prof_comp_t0 = 0.0, prof_kms0 = 0.0;
+  bool timing_on() const { return kernel_timing || (prof_loop && op2h::profile_level() >= 3); }
 };
 
 Runtime &rt() {
@@ -866,43 +868,30 @@ extern "C" void op_diagnostic_output(void) {
 }
 
 extern "C" int op_is_root(void) { return 1; }
-extern "C" void op_rank(int *rank) { *rank = 0; }
 
-// ---- profiling C API: a flat per-name wall-clock tree (op_profile.h:197-212) ----
-extern "C" void op_profile_start(const char *name) { op_profile_enter(name); }
-// Profile regions and op_par_loops are also roctx ranges, so a rocprofv3
-// --marker-trace run attributes kernels to them.
-extern "C" void op_profile_enter(const char *name) {
-  op2h_device_sync();
-  rt().prof_stack.push_back({name, wall_now()});
-  roctxRangePushA(name);
+// ---- host memory with OP2_ALIGNMENT (op_lib_core.cpp:1388-1420) ----
+extern "C" void *op_malloc(size_t size) {
+  if (size == 0) return malloc(0);
+  const size_t a = OP2_ALIGNMENT;
+  return aligned_alloc(a, (size + a - 1) / a * a);
 }
-extern "C" void op_profile_enter_kernel(const char *name, const char *,
-                                        const char *) {
-  op_profile_enter(name);
+extern "C" void *op_calloc(size_t num, size_t size) {
+  const size_t bytes = num * size;
+  if (size != 0 && bytes / size != num) return nullptr;  // overflow
+  void *p = op_malloc(bytes);
+  if (p) std::memset(p, 0, bytes);
+  return p;
 }
-extern "C" void op_profile_next(const char *name) {
-  op_profile_exit();
-  op_profile_enter(name);
+extern "C" void *op_realloc(void *ptr, size_t size) {
+  void *p = realloc(ptr, size);
+  if (!p || ((uintptr_t)p % OP2_ALIGNMENT) == 0) return p;
+  void *q = op_malloc(size);  // realloc moved it off the alignment
+  if (q) std::memcpy(q, p, size);
+  free(p);
+  return q;
 }
-extern "C" void op_profile_exit(void) {
-  Runtime &R = rt();
-  if (R.prof_stack.empty()) return;
-  op2h_device_sync();
-  auto top = R.prof_stack.back();
-  R.prof_stack.pop_back();
-  roctxRangePop();
-  std::string path;
-  for (auto &p : R.prof_stack) path += p.first + "/";
-  R.prof_totals[path + top.first] += wall_now() - top.second;
-}
-extern "C" void op_profile_end(void) {
-  while (!rt().prof_stack.empty()) op_profile_exit();
-}
-extern "C" void op_profile_output(void) {
-  for (auto &kv : rt().prof_totals)
-    op_printf("%-40s %10.4f s\n", kv.first.c_str(), kv.second);
-}
+extern "C" void op_free(void *ptr) { free(ptr); }
+extern "C" void op_rank(int *rank) { *rank = 0; }
 
 // ---- single-rank no-ops (op_dummy_singlenode.cpp:43-208) ----
 extern "C" void op_partition(const char *, const char *, op_set, op_map, op_dat) {}
@@ -1404,42 +1393,12 @@ extern "C" void op2h_loop_begin(op2h_loop_ctx *ctx, int kernel_idx,
                        def_plan == OP2H_PLAN_STAGED ? all : 1u << def_plan);
 }
 
-extern "C" void op2h_loop_begin_mask(op2h_loop_ctx *ctx, int kernel_idx,
-                                     const char *name, op_set set, int nargs,
-                                     op_arg *args, int def_plan, int def_block,
-                                     unsigned plan_mask) {
-  Runtime &R = rt();
-  ensure_init();
-  std::memset(ctx, 0, sizeof(*ctx));
-  roctxRangePushA(name);
-  ctx->kernel_idx = kernel_idx;
-  ctx->name = name;
-  ctx->set = set;
-  ctx->nargs = nargs;
-  ctx->args = args;
-  ctx->stream = R.stream;
-  ctx->capturing = R.capturing ? 1 : 0;
-  op_timing_realloc(kernel_idx);
-  OP_kernels[kernel_idx].name = name;
-  OP_kernels[kernel_idx].count += 1;
-  ctx->wall0 = wall_now();
-  if (OP_diags > 0) {
-    int ninds = 0;
-    for (int m = 0; m < nargs; ++m) op_arg_check(set, m, args[m], &ninds, name);
-  }
-  if (OP_diags > 2) printf(" kernel routine: %s\n", name);
-  for (int m = 0; m < nargs; ++m) {
-    op_arg &a = args[m];
-    if (a.argtype == OP_ARG_DAT && a.opt && a.dat && a.dat->dirty_hd == 1) {
-      if (R.capturing) {
-        printf("op2hip: dat %s changed on the host during graph capture\n",
-               a.dat->name);
-        exit(-1);
-      }
-      upload_dat(a.dat);
-    }
-    if (a.argtype == OP_ARG_DAT && a.dat) a.data_d = a.dat->data_d;
-  }
+namespace {
+// Plan and block size of one call: a recorded per-loop or all-loop setting,
+// else the autotuner's or the cost model's choice, else the stub's default.
+void choose_plan(Runtime &R, op2h_loop_ctx *ctx, const char *name, op_set set,
+                 int nargs, op_arg *args, int def_plan, int def_block,
+                 unsigned plan_mask) {
   LoopConfig c = R.all_cfg;
   auto it = R.loop_cfg.find(name);
   if (it != R.loop_cfg.end()) c = it->second;
@@ -1518,6 +1477,55 @@ extern "C" void op2h_loop_begin_mask(op2h_loop_ctx *ctx, int kernel_idx,
                                         : def_block);
   ctx->part_size = OP_part_size > 0 ? OP_part_size : ctx->block_size;
 }
+}  // namespace
+
+extern "C" void op2h_loop_begin_mask(op2h_loop_ctx *ctx, int kernel_idx,
+                                     const char *name, op_set set, int nargs,
+                                     op_arg *args, int def_plan, int def_block,
+                                     unsigned plan_mask) {
+  Runtime &R = rt();
+  ensure_init();
+  std::memset(ctx, 0, sizeof(*ctx));
+  roctxRangePushA(name);
+  ctx->kernel_idx = kernel_idx;
+  ctx->name = name;
+  ctx->set = set;
+  ctx->nargs = nargs;
+  ctx->args = args;
+  ctx->stream = R.stream;
+  ctx->capturing = R.capturing ? 1 : 0;
+  op_timing_realloc(kernel_idx);
+  OP_kernels[kernel_idx].name = name;
+  OP_kernels[kernel_idx].count += 1;
+  R.prof_loop = !R.capturing && op2h::profile_loops_on();
+  if (R.prof_loop) op2h_device_sync();
+  ctx->wall0 = wall_now();
+  if (OP_diags > 0) {
+    int ninds = 0;
+    for (int m = 0; m < nargs; ++m) op_arg_check(set, m, args[m], &ninds, name);
+  }
+  if (OP_diags > 2) printf(" kernel routine: %s\n", name);
+  for (int m = 0; m < nargs; ++m) {
+    op_arg &a = args[m];
+    if (a.argtype == OP_ARG_DAT && a.opt && a.dat && a.dat->dirty_hd == 1) {
+      if (R.capturing) {
+        printf("op2hip: dat %s changed on the host during graph capture\n",
+               a.dat->name);
+        exit(-1);
+      }
+      upload_dat(a.dat);
+    }
+    if (a.argtype == OP_ARG_DAT && a.dat) a.data_d = a.dat->data_d;
+  }
+  choose_plan(R, ctx, name, set, nargs, args, def_plan, def_block, plan_mask);
+  if (R.prof_loop) {
+    if (op2h::profile_level() >= 3) {
+      resolve_timers();
+      R.prof_kms0 = R.stats[kernel_idx].kernel_ms;
+    }
+    R.prof_comp_t0 = wall_now();
+  }
+}
 
 extern "C" void op2h_kernel_timer_start(op2h_loop_ctx *ctx) {
   Runtime &R = rt();
@@ -1529,7 +1537,7 @@ extern "C" void op2h_kernel_timer_start(op2h_loop_ctx *ctx) {
     HIPCHECK(hipEventRecord(R.tune_ev[0], R.stream));
     return;
   }
-  if (!R.kernel_timing || R.capturing) return;
+  if (!R.timing_on() || R.capturing) return;
   PendingTimer t{ctx->kernel_idx, take_event(), take_event()};
   HIPCHECK(hipEventRecord(t.a, R.stream));
   R.pending.push_back(t);
@@ -1539,7 +1547,7 @@ extern "C" int op2h_kernel_timer_events(op2h_loop_ctx *ctx, void **start,
                                         void **stop) {
   Runtime &R = rt();
   *start = *stop = nullptr;
-  if (ctx->tuning || !R.kernel_timing || R.capturing) return 0;
+  if (ctx->tuning || !R.timing_on() || R.capturing) return 0;
   PendingTimer t{ctx->kernel_idx, take_event(), take_event()};
   R.pending.push_back(t);
   *start = t.a;
@@ -1583,7 +1591,7 @@ extern "C" void op2h_kernel_timer_stop(op2h_loop_ctx *ctx) {
     }
     return;
   }
-  if (!R.kernel_timing || R.capturing || R.pending.empty()) return;
+  if (!R.timing_on() || R.capturing || R.pending.empty()) return;
   (void)ctx;
   HIPCHECK(hipEventRecord(R.pending.back().b, R.stream));
 }
@@ -1647,9 +1655,31 @@ extern "C" void op2h_loop_end(op2h_loop_ctx *ctx) {
                   with_info ? ctx->args[info].size : 0);
     }
   }
+  if (R.prof_loop) {
+    op2h_device_sync();
+    double kms = -1.0;
+    if (op2h::profile_level() >= 3) {
+      resolve_timers();
+      kms = R.stats[ctx->kernel_idx].kernel_ms - R.prof_kms0;
+    }
+    const double t = wall_now();
+    static const char *plan_name[OP2H_PLAN_COUNT] = {"auto",   "atomic", "color",
+                                                     "staged", "gather", "owner"};
+    bool indirect = false;
+    for (int m = 0; m < ctx->nargs; ++m)
+      indirect |= ctx->args[m].argtype == OP_ARG_DAT && ctx->args[m].opt &&
+                  ctx->args[m].map != nullptr;
+    const char *variant = !indirect ? "direct"
+                          : ctx->plan > 0 && ctx->plan < OP2H_PLAN_COUNT ? plan_name[ctx->plan]
+                                                                          : "auto";
+    op2h::profile_loop(ctx->name, variant, (long long)((t - ctx->wall0) * 1e9),
+                       (long long)((t - R.prof_comp_t0) * 1e9), kms);
+    R.prof_loop = false;
+  }
   double t1 = wall_now();
   OP_kernels[ctx->kernel_idx].time += t1 - ctx->wall0;
   OP_kernels[ctx->kernel_idx].times[0] += t1 - ctx->wall0;
+  roctxRangePop();
 }
 
 extern "C" int op2h_dat_soa_stride(op_dat dat) {
@@ -1824,6 +1854,7 @@ extern "C" const op2h_staged_plan *op2h_plan_staged_for(const char *name,
                                      R.plan_on_device, R.stream);
   rec->signature = sig;
   rec->plan.build_ms = (wall_now() - t0) * 1e3;
+  if (OP_diags > 0) op2h::check_staged_or_exit(*rec, name, set, nargs, args);
   R.staged.push_back(std::move(rec));
   return &R.staged.back()->plan;
 }
@@ -1850,6 +1881,7 @@ extern "C" const op2h_color_plan *op2h_plan_color_for(const char *name,
   auto rec = op2h::build_color_plan(set, nargs, args, R.plan_on_device, R.stream);
   rec->signature = sig;
   rec->plan.build_ms = (wall_now() - t0) * 1e3;
+  if (OP_diags > 0) op2h::check_color_or_exit(*rec, name, set, nargs, args);
   R.color.push_back(std::move(rec));
   return &R.color.back()->plan;
 }
@@ -1870,6 +1902,7 @@ extern "C" const op2h_gather_plan *op2h_plan_gather(op_map map) {
   double t0 = wall_now();
   auto rec = op2h::build_gather_plan(map, R.stream);
   rec->plan.build_ms = (wall_now() - t0) * 1e3;
+  if (OP_diags > 0) op2h::check_gather_or_exit(*rec, map);
   auto *p = &rec->plan;
   R.gather[map] = std::move(rec);
   return p;
@@ -1896,6 +1929,7 @@ extern "C" const op2h_owner_plan *op2h_plan_owner_for(const char *name,
   }
   rec->signature = sig;
   rec->plan.build_ms = (wall_now() - t0) * 1e3;
+  if (OP_diags > 0) op2h::check_owner_or_exit(*rec, name, nargs, args);
   R.owner.push_back(std::move(rec));
   return &R.owner.back()->plan;
 }

@@ -167,6 +167,44 @@ def owner_partition(nto, table, inc_idx, block_cells):
     return nb, out
 
 
+class op2h_check_access(C.Structure):
+    _fields_ = [("table", C.POINTER(C.c_int)), ("dim", C.c_int), ("idx", C.c_int),
+                ("to_size", C.c_int), ("dat", C.c_int), ("writes", C.c_int)]
+
+
+_ip = C.POINTER(C.c_int)
+
+
+class op2h_staged_plan_host(C.Structure):
+    _fields_ = [("set_size", C.c_int), ("nblocks", C.c_int), ("ncolors", C.c_int),
+                ("colstart", _ip), ("blkmap", _ip), ("offset", _ip), ("nelems", _ip),
+                ("nthrcol", _ip), ("thrcol", C.POINTER(C.c_ubyte)), ("nmaps", C.c_int),
+                ("map_table", _ip * 32), ("map_dim", C.c_int * 32),
+                ("map_to_size", C.c_int * 32), ("ind_offs", _ip * 32),
+                ("ind_sizes", _ip * 32), ("ind_map", _ip * 32), ("nlocs", C.c_int),
+                ("loc_map_slot", C.c_int * 32), ("loc_idx", C.c_int * 32),
+                ("loc_map", C.POINTER(C.c_ushort) * 32)]
+
+
+class op2h_owner_plan_host(C.Structure):
+    _fields_ = [("set_size", C.c_int), ("to_size", C.c_int), ("nblocks", C.c_int),
+                ("ninc", C.c_int), ("map_table", _ip), ("map_dim", C.c_int),
+                ("inc_idx", C.c_int * 32), ("owned_offs", _ip), ("owned", _ip),
+                ("elem_offs", _ip), ("elems", _ip), ("csr_offs", _ip),
+                ("csr", C.POINTER(C.c_ushort)), ("csr4", C.POINTER(C.c_ushort))]
+
+
+op2h_check_staged_plan = _fn("op2h_check_staged_plan", C.c_int,
+                             C.POINTER(op2h_staged_plan_host), C.c_int,
+                             C.POINTER(op2h_check_access))
+op2h_check_color_plan = _fn("op2h_check_color_plan", C.c_int, C.c_int, C.c_int, _ip, _ip,
+                            C.c_int, C.POINTER(op2h_check_access))
+op2h_check_owner_plan = _fn("op2h_check_owner_plan", C.c_int,
+                            C.POINTER(op2h_owner_plan_host))
+op2h_check_gather_plan = _fn("op2h_check_gather_plan", C.c_int, C.c_int, C.c_int, C.c_int,
+                             _ip, C.c_int, _ip, _ip, _ip)
+
+
 op2h_rcm_order = _fn("op2h_rcm_order", C.c_int, C.c_int, C.POINTER(C.c_int),
                      C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int))
 
@@ -197,6 +235,7 @@ op2h_staged_plan_download = _fn("op2h_staged_plan_download", C.c_int,
                                 C.c_void_p, C.c_void_p, C.c_void_p)
 op2h_color_plan_download = _fn("op2h_color_plan_download", C.c_int,
                                C.POINTER(op_color_plan), C.c_void_p)
+op2h_plan_gather = _fn("op2h_plan_gather", C.c_void_p, op_map)
 
 
 def init(*tokens, diags=0):

@@ -1,6 +1,7 @@
 """Translate the OP2 programs under tests/apps with op2amd.translator and
-compile them for gfx950 against the in-tree libop2hip.so.  Outputs go to
-tests/apps/_build/<program>/ (the executable has the program's name).
+compile them for gfx950 against the in-tree libop2hip.so; where the
+reference tree is present, also its unmodified airfoil application.  Outputs
+go to tests/apps/_build/<program>/ (the executable has the program's name).
 
     python tests/apps/build.py [program ...]
 """
@@ -22,6 +23,15 @@ PROGRAMS = {
     "airfoil_tr": "airfoil/airfoil_tr.cpp",
 }
 
+# The reference's own airfoil application, translated unchanged, where the
+# reference tree is present (this container).  Only its executable is kept:
+# the generated sources and objects, which carry the application's text, are
+# removed after the link.
+REFERENCE = "/root/reference"
+REF_PROGRAMS = {
+    "airfoil_ref": "apps/c/airfoil/airfoil_plain/dp/airfoil.cpp",
+}
+
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -29,7 +39,10 @@ FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off",
 
 def build(name):
     from op2amd.translator import translate
-    src = os.path.join(HERE, PROGRAMS[name])
+    if name in REF_PROGRAMS:
+        src = os.path.join(REFERENCE, REF_PROGRAMS[name])
+    else:
+        src = os.path.join(HERE, PROGRAMS[name])
     out = os.path.join(HERE, "_build", name)
     paths = translate(src, out)
     inc = ["-I" + os.path.join(REPO, "include"), "-I" + os.path.dirname(src), "-I" + out]
@@ -44,9 +57,13 @@ def build(name):
     subprocess.run([HIPCC, "--offload-arch=gfx950", *objs, "-L" + lib, "-lop2hip",
                     "-Wl,-rpath," + lib, "-Wl,-rpath,$ORIGIN/../../../../op2-common_amd/lib",
                     "-o", exe], check=True)
+    if name in REF_PROGRAMS:
+        for p in [*paths.values(), *objs]:
+            os.remove(p)
     return exe
 
 
 if __name__ == "__main__":
-    for n in sys.argv[1:] or PROGRAMS:
+    names = sys.argv[1:] or [*PROGRAMS, *(REF_PROGRAMS if os.path.isdir(REFERENCE) else [])]
+    for n in names:
         print(build(n), flush=True)

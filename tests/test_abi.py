@@ -8,7 +8,7 @@ import pytest
 
 from conftest import REPO
 
-HEADERS = ["op2_hip.h", "op2_hip_rt.h", "op2amd_mesh.h", "op2amd_airfoil.h",
+HEADERS = ["op2_hip.h", "op_profile.h", "op2_hip_rt.h", "op2amd_mesh.h", "op2amd_airfoil.h",
            "op2amd_ensemble.h", "op2amd_synth.h", "op2amd_costmodel.h"]
 
 
@@ -42,6 +42,8 @@ def test_drop_in_symbols_present():
               "op_decl_const_char", "op_arg_dat", "op_arg_gbl_char", "op_arg_idx",
               "op_arg_info_char", "op_fetch_data_char", "op_get_size",
               "op_print_dat_to_binfile", "op_timing_output", "op_mpi_halo_exchanges",
+              "op_malloc", "op_calloc", "op_realloc", "op_free", "op_fetch_data_idx_char",
+              "op_profile_start", "op_profile_end", "op_profile_output",
               "op_par_loop_airfoil_1_save_soln", "op_par_loop_airfoil_2_adt_calc",
               "op_par_loop_airfoil_3_res_calc", "op_par_loop_airfoil_4_bres_calc",
               "op_par_loop_airfoil_5_update"]:

@@ -33,3 +33,20 @@ def test_roofline_traffic_from_profiles():
     assert gen and 1.0 <= gen / 109.44e6 <= 1.1
     assert large and 1.0 <= large / 1751.0e6 <= 1.1
     assert b.profiled_traffic("res_calc/owner/128", "no-such-mesh") is None
+
+
+def test_parity_check_flags_a_wrong_iteration():
+    """The bench's parity check covers every iteration: one rms off by 1e-9
+    relative anywhere in the trajectory, or q off by 1e-9, fails it."""
+    import numpy as np
+    b = load_bench()
+    orc = {"rms": np.linspace(1e-3, 1e-4, 520), "q": np.linspace(1.0, 2.0, 40).reshape(10, 4)}
+    good = {"rms": orc["rms"] * (1 + 1e-14), "q": orc["q"] + 1e-15}
+    late = {"rms": orc["rms"].copy(), "q": orc["q"]}
+    late["rms"][517] *= 1 + 1e-9
+    qbad = {"rms": orc["rms"], "q": orc["q"].copy()}
+    qbad["q"][3, 2] += 2e-9 * 2.0
+    res = b.parity({"good": good, "late": late, "qbad": qbad}, orc)
+    assert max(res["good"].values()) <= b.PARITY_TOL
+    assert res["late"]["rms_rel_err_max"] > b.PARITY_TOL
+    assert res["qbad"]["q_rel_err"] > b.PARITY_TOL and res["qbad"]["rms_rel_err_max"] == 0.0

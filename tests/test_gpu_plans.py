@@ -198,15 +198,67 @@ def test_owner_plan_invariants(ordering, block):
 
 
 def test_owner_plan_refuses_unsupported_loops():
-    """Loops whose increments use two maps cannot be partitioned by owner."""
+    """Loops the owner partition cannot run: increments through two maps, and
+    loops with effects that evaluating a boundary element twice would repeat
+    (a gbl reduction, a direct OP_RW) or that read the incremented dat."""
+    import ctypes as C
     mesh = op2amd.mesh.generate(20, 10, 0, "structured")
     q, res = inputs(mesh, 0)
     app = App(mesh, q, res)
     try:
-        R, I = api.OP_READ, api.OP_INC
+        R, I, RW = api.OP_READ, api.OP_INC, api.OP_RW
         d = app.d
-        args = (api.op_arg * 2)(d(app.res, 0, app.pecell, 4, "double", I),
-                                d(app.x, 0, app.pedge, 2, "double", I))
+        inc = lambda: d(app.res, 0, app.pecell, 4, "double", I)  # noqa: E731
+        args = (api.op_arg * 2)(inc(), d(app.x, 0, app.pedge, 2, "double", I))
         assert not api.op2h_plan_owner_for(b"two_maps", app.edges, 64, 2, args)
+        total = np.zeros(1)
+        gbl = api.op_arg_gbl_char(total.ctypes.data_as(C.c_void_p), 1, b"double", 8, I)
+        args = (api.op_arg * 2)(inc(), gbl)
+        assert not api.op2h_plan_owner_for(b"with_gbl_inc", app.edges, 64, 2, args)
+        ew = np.zeros(mesh.nedge)
+        edat = api.decl_dat(app.edges, 1, "double", ew, "p_ew")
+        args = (api.op_arg * 2)(inc(), d(edat, -1, None, 1, "double", RW))
+        assert not api.op2h_plan_owner_for(b"with_direct_rw", app.edges, 64, 2, args)
+        args = (api.op_arg * 2)(inc(), d(app.res, 1, app.pecell, 4, "double", R))
+        assert not api.op2h_plan_owner_for(b"reads_res", app.edges, 64, 2, args)
+        # a gbl read and a direct write are fine
+        args = (api.op_arg * 3)(inc(), api.op_arg_gbl_char(total.ctypes.data_as(C.c_void_p), 1,
+                                                          b"double", 8, R),
+                                d(edat, -1, None, 1, "double", api.OP_WRITE))
+        assert api.op2h_plan_owner_for(b"read_gbl_write_direct", app.edges, 64, 3, args)
     finally:
         app.close()
+
+
+@pytest.mark.parametrize("ordering", ["random", "rcm"])
+def test_runtime_plan_checks_pass_on_built_plans(ordering, capfd):
+    """With OP_diags > 0 the runtime checks every plan it builds (and exits
+    on a failure); at OP_diags > 6 each passing check prints OK."""
+    import ctypes as C
+    mesh = op2amd.mesh.generate(120, 60, 0, ordering)
+    q, res = inputs(mesh, 0)
+    app = App(mesh, q, res)
+    diags = C.c_int.in_dll(api.lib, "OP_diags")
+    try:
+        diags.value = 7
+        for on_dev in (0, 1):
+            api.op2h_set_plan_builder(on_dev)
+            for part in (64, 256):
+                name = f"res_calc_{on_dev}_{part}".encode()
+                assert api.op2h_plan_staged_for(name, app.edges, part, 8, res_args(app))
+            assert api.op2h_plan_color_for(f"res_calc_{on_dev}".encode(), app.edges, 8,
+                                           res_args(app))
+        for block in (64, 256):
+            assert api.op2h_plan_owner_for(b"res_calc", app.edges, block, 8, res_args(app))
+        assert api.op2h_plan_gather(app.pecell)
+    finally:
+        diags.value = 0
+        api.op2h_set_plan_builder(1)
+        app.close()
+    out = capfd.readouterr().out
+    assert "error" not in out
+    assert out.count("OP_plan_check: thread colouring OK") == 4
+    assert out.count("OP_plan_check: block colouring OK") == 4
+    assert out.count("OP_plan_check: element colouring OK") == 2
+    assert out.count("OP_plan_check: increment completeness OK") == 2
+    assert out.count("OP_plan_check: gather entries OK") == 1
