@@ -265,6 +265,21 @@ double op2h_algorithmic_bytes(op_set set, int nargs, const op_arg *args);
 /* Where plans are coloured: 1 on the GPU (default), 0 on the host. */
 void op2h_set_plan_builder(int on_device);
 
+/* ---- streaming ceiling (measurement) ----
+ * The HBM streaming rate of copy kernels of several shapes, the ceiling the
+ * loops' achieved bandwidth is read against beside the 8 TB/s peak.
+ * variant: 0 one 16-byte vector per thread; 1 four per thread; 2 four per
+ * thread, non-temporal loads and stores; 3 eight per thread; 4 a persistent
+ * grid (8 workgroups per CU) striding four vectors per iteration; 5 read
+ * only (four per thread); 6 write only (four per thread).  bytes: size of the
+ * source and of the destination buffer (at least 1 MiB; allocated here).
+ * gbs: bytes moved (read + written) per second / 1e9 over the fastest of
+ * `reps` timed calls; us_per_call (may be NULL): that call's time.
+ * Returns 0, -1 on bad arguments, -2 on a HIP error. */
+#define OP2H_STREAM_VARIANTS 7
+int op2h_stream_probe(size_t bytes, int variant, int reps, double *gbs,
+                      double *us_per_call);
+
 /* ---- plan checks (op_plan_check, op2/src/core/op_rt_support.cpp:129-295) ----
  * With OP_diags > 0 the runtime checks every plan it builds on the host and
  * stops the program (exit(-1), after the messages) when a check fails: a

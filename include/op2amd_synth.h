@@ -19,6 +19,8 @@
 
 #include <stdint.h>
 
+#include "op2amd_mesh.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -56,6 +58,19 @@ int op2amd_synth_time(const op2amd_synth_desc *d, int nvar, const int *plan,
  * in `out` (target_size*dim or set_size*dim values).  For tests. */
 int op2amd_synth_run_once(const op2amd_synth_desc *d, int plan, int block,
                           double *out, int max_out);
+
+/* Agreement of plans: from the same seeded dat values, run the reference
+ * variant (ref_plan, ref_block) once, then each of the `nvar` variants once,
+ * and compare each result (W for INC/RW, the direct output for READ) with the
+ * reference's on the device: max_rel_dev[i] = max |w - w_ref| / max |w_ref|
+ * (NaN if any difference is NaN), mismatches[i] = elements whose bits differ;
+ * both -1 for a variant that does not apply.  `mesh` (may be NULL: generated
+ * from the descriptor) is a mesh of the descriptor's shape, read only.
+ * Returns 0, 1 for a bad descriptor, 2 if the reference variant does not
+ * apply. */
+int op2amd_synth_check(const op2amd_synth_desc *d, const op2amd_mesh *mesh, int ref_plan,
+                       int ref_block, int nvar, const int *plan, const int *block,
+                       double *max_rel_dev, long long *mismatches);
 
 #ifdef __cplusplus
 }

@@ -211,8 +211,51 @@ __global__ void __launch_bounds__(1024)
 
 int grid(long long n, int bs) { return n > 0 ? (int)((n + bs - 1) / bs) : 0; }
 
+__device__ __forceinline__ unsigned long long bits_of(double x) {
+  return (unsigned long long)__double_as_longlong(x);
+}
+__device__ __forceinline__ unsigned long long bits_of(float x) { return __float_as_uint(x); }
+
+// Per workgroup of 256: max |a - ref|, max |ref| and the number of elements
+// whose bits differ, into part[3 * blockIdx.x ..]; a NaN difference makes
+// the first maximum NaN (comparisons written so that NaN wins).
+template <class T>
+__global__ void __launch_bounds__(256) k_syn_compare(const T *__restrict a,
+                                                     const T *__restrict ref, size_t n,
+                                                     double *__restrict part) {
+  __shared__ double s_d[256], s_r[256], s_c[256];
+  double dmax = 0.0, rmax = 0.0, cnt = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double x = (double)a[i], r = (double)ref[i];
+    const double d = fabs(x - r), ar = fabs(r);
+    if (!(d <= dmax)) dmax = d;
+    if (!(ar <= rmax)) rmax = ar;
+    cnt += bits_of(a[i]) != bits_of(ref[i]) ? 1.0 : 0.0;
+  }
+  s_d[threadIdx.x] = dmax;
+  s_r[threadIdx.x] = rmax;
+  s_c[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double od = s_d[threadIdx.x + w], orr = s_r[threadIdx.x + w];
+      if (!(od <= s_d[threadIdx.x])) s_d[threadIdx.x] = od;
+      if (!(orr <= s_r[threadIdx.x])) s_r[threadIdx.x] = orr;
+      s_c[threadIdx.x] += s_c[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[3 * (size_t)blockIdx.x] = s_d[0];
+    part[3 * (size_t)blockIdx.x + 1] = s_r[0];
+    part[3 * (size_t)blockIdx.x + 2] = s_c[0];
+  }
+}
+
 struct Loop {
   op2amd_mesh mesh{};
+  bool own_mesh = true;  // generated here (freed at teardown)
   op_set from = nullptr, to = nullptr;
   op_map map = nullptr;
   op_dat V = nullptr, R = nullptr, W = nullptr, OUT = nullptr;
@@ -234,11 +277,16 @@ std::vector<T> seeded(size_t n, uint64_t seed) {
   return v;
 }
 
-bool setup(const op2amd_synth_desc *d, Loop &L) {
+bool setup(const op2amd_synth_desc *d, Loop &L, const op2amd_mesh *given = nullptr) {
   if (d->arity < 1 || d->arity > kMaxA || !(d->dim == 1 || d->dim == 2 || d->dim == 4 || d->dim == 8) ||
       !(d->access == OP_READ || d->access == OP_INC || d->access == OP_RW))
     return false;
-  if (op2amd_mesh_generate(d->I, d->J, d->seed, d->ordering, &L.mesh)) return false;
+  if (given) {
+    L.mesh = *given;
+    L.own_mesh = false;
+  } else if (op2amd_mesh_generate(d->I, d->J, d->seed, d->ordering, &L.mesh)) {
+    return false;
+  }
   char prog[] = "synth";
   char *argv[1] = {prog};
   op_init(1, argv, 0);
@@ -294,7 +342,7 @@ bool setup(const op2amd_synth_desc *d, Loop &L) {
 
 void teardown(Loop &L) {
   op_exit();
-  op2amd_mesh_free(&L.mesh);
+  if (L.own_mesh) op2amd_mesh_free(&L.mesh);
 }
 
 template <class T, int D>
@@ -463,6 +511,63 @@ extern "C" int op2amd_synth_run_once(const op2amd_synth_desc *d, int plan, int b
   op_fetch_data_char(res, (char *)buf.data());
   for (size_t i = 0; i < n; ++i)
     out[i] = L.is_double ? ((double *)buf.data())[i] : (double)((float *)buf.data())[i];
+  teardown(L);
+  return 0;
+}
+
+extern "C" int op2amd_synth_check(const op2amd_synth_desc *d, const op2amd_mesh *mesh,
+                                  int ref_plan, int ref_block, int nvar, const int *plan,
+                                  const int *block, double *max_rel_dev, long long *mismatches) {
+  Loop L;
+  if (!setup(d, L, mesh)) return 1;
+  hipStream_t st = (hipStream_t)op2h_stream();
+  op_dat res = L.access == OP_READ ? L.OUT : L.W;
+  const size_t n = (size_t)res->set->size * L.D;
+  const size_t es = L.is_double ? 8 : 4;
+  auto reset = [&]() {  // W back to its seeded values
+    if (L.access != OP_READ && n)
+      SCHECK(hipMemcpyAsync(L.W->data_d, L.w0.data(), n * es, hipMemcpyHostToDevice, st));
+  };
+  reset();
+  if (!launch(L, ref_plan, ref_block)) {
+    teardown(L);
+    return 2;
+  }
+  void *ref = nullptr;
+  const int nb = (int)std::min<size_t>(1024, std::max<size_t>(1, (n + 255) / 256));
+  double *part = nullptr;
+  SCHECK(hipMalloc(&ref, std::max<size_t>(n * es, 8)));
+  SCHECK(hipMalloc(&part, sizeof(double) * 3 * nb));
+  if (n) SCHECK(hipMemcpyAsync(ref, res->data_d, n * es, hipMemcpyDeviceToDevice, st));
+  std::vector<double> hp(3 * (size_t)nb);
+  for (int v = 0; v < nvar; ++v) {
+    max_rel_dev[v] = -1.0;
+    mismatches[v] = -1;
+    reset();
+    if (!launch(L, plan[v], block[v])) continue;
+    if (!n) {
+      max_rel_dev[v] = 0.0;
+      mismatches[v] = 0;
+      continue;
+    }
+    if (L.is_double)
+      k_syn_compare<double><<<nb, 256, 0, st>>>((const double *)res->data_d, (const double *)ref, n, part);
+    else
+      k_syn_compare<float><<<nb, 256, 0, st>>>((const float *)res->data_d, (const float *)ref, n, part);
+    SCHECK(hipGetLastError());
+    SCHECK(hipMemcpyAsync(hp.data(), part, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, st));
+    SCHECK(hipStreamSynchronize(st));
+    double dmax = 0.0, rmax = 0.0, cnt = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      if (!(hp[3 * b] <= dmax)) dmax = hp[3 * b];
+      if (!(hp[3 * b + 1] <= rmax)) rmax = hp[3 * b + 1];
+      cnt += hp[3 * b + 2];
+    }
+    max_rel_dev[v] = rmax > 0.0 ? dmax / rmax : dmax;
+    mismatches[v] = (long long)cnt;
+  }
+  SCHECK(hipFree(ref));
+  SCHECK(hipFree(part));
   teardown(L);
   return 0;
 }

@@ -113,6 +113,36 @@ def time_variants(desc, var, reps=3):
                       "mean_bandwidth": shape.mean_bandwidth}
 
 
+lib.op2amd_synth_check.restype = C.c_int
+lib.op2amd_synth_check.argtypes = [C.POINTER(Desc), C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+
+
+def desc_mesh(desc):
+    """The generator mesh a descriptor's loop runs over."""
+    from .mesh import generate, ORDERINGS as _O
+    name = {v: k for k, v in _O.items()}[desc.ordering]
+    return generate(desc.I, desc.J, desc.seed, name)
+
+
+def check_plans(desc, var, ref=(PLAN_GATHER, 128), mesh=None):
+    """Run the reference variant and each (plan, block) of `var` once from the
+    same seeded values; per variant (max |w - w_ref| / max |w_ref|, elements
+    whose bits differ), (-1, -1) where the variant does not apply.  `mesh`:
+    the descriptor's mesh if already generated (desc_mesh)."""
+    n = len(var)
+    plan = (C.c_int * n)(*[p for p, _ in var])
+    block = (C.c_int * n)(*[b for _, b in var])
+    dev = (C.c_double * n)()
+    mis = (C.c_longlong * n)()
+    mp = C.byref(mesh.c()) if mesh is not None else None
+    rc = lib.op2amd_synth_check(C.byref(desc), mp, ref[0], ref[1], n, plan, block, dev, mis)
+    if rc:
+        raise RuntimeError(f"synthetic plan check failed ({rc})")
+    return list(zip(list(dev), list(mis)))
+
+
 def run_once(desc, plan, block, size):
     out = np.zeros(size)
     rc = lib.op2amd_synth_run_once(C.byref(desc), plan, block,
@@ -217,7 +247,7 @@ def sweep(descs, reps=5, log=None):
     return recs
 
 
-def random_descs(count, seed, n_lo=1e4, n_hi=4e6):
+def random_descs(count, seed, n_lo=1e4, n_hi=1e7):
     """Seeded loop descriptions: set sizes log-uniform in [n_lo, n_hi]."""
     rng = np.random.default_rng(seed)
     out = []

@@ -466,10 +466,66 @@ static int run_with_consts(const op2amd_mesh *m, int niter, const orc_consts &c,
                    bres_acc, bres_bytes, bres_to, part_size, &bres_plan.p);
   }
 
+  // exec 2: the sequential executor's results, evaluated in parallel.  Each
+  // interior edge's two increments are evaluated once into a scratch pair
+  // started at -0.0 (so it holds exactly +f and -f: -0.0 + f == f and
+  // -0.0 - f == -f for every f, signed zeros included); each cell then adds its
+  // increments in the sequential order (interior edges ascending, then
+  // boundary edges ascending), and x - f == x + (-f) exactly.  The update's
+  // per-component squares are summed and compared serially in element order.
+  std::vector<int> inc_off, inc_ent;  // per cell: 2*edge+side, then bedge ids
+  std::vector<double> eflux, bflux, sq;
+  if (exec == 2) {
+    inc_off.assign(nc + 1, 0);
+    for (size_t i = 0; i < 2 * (size_t)m->nedge; ++i) inc_off[m->ecell[i] + 1]++;
+    for (int b = 0; b < m->nbedge; ++b) inc_off[m->becell[b] + 1]++;
+    for (int n = 0; n < nc; ++n) inc_off[n + 1] += inc_off[n];
+    inc_ent.resize(inc_off[nc]);
+    std::vector<int> fill(inc_off.begin(), inc_off.end() - 1);
+    for (size_t i = 0; i < 2 * (size_t)m->nedge; ++i) inc_ent[fill[m->ecell[i]]++] = (int)i;
+    for (int b = 0; b < m->nbedge; ++b)
+      inc_ent[fill[m->becell[b]]++] = -1 - b;  // after every interior edge
+    eflux.resize(8 * (size_t)m->nedge);
+    bflux.resize(4 * (size_t)m->nbedge);
+    sq.resize(4 * (size_t)nc);
+  }
+  auto res_bres_ordered = [&]() {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int e = 0; e < m->nedge; ++e) {
+      double *f = &eflux[8 * (size_t)e];
+      for (int k = 0; k < 8; ++k) f[k] = -0.0;
+      const int *en = m->edge + 2 * (size_t)e, *ec = m->ecell + 2 * (size_t)e;
+      k_res_calc(c, m->x + 2 * (size_t)en[0], m->x + 2 * (size_t)en[1],
+                 q.data() + 4 * (size_t)ec[0], q.data() + 4 * (size_t)ec[1],
+                 adt.data() + ec[0], adt.data() + ec[1], f, f + 4);
+    }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int b = 0; b < m->nbedge; ++b) {
+      double *f = &bflux[4 * (size_t)b];
+      for (int k = 0; k < 4; ++k) f[k] = -0.0;
+      const int *bn = m->bedge + 2 * (size_t)b;
+      const int cc = m->becell[b];
+      k_bres_calc(c, m->x + 2 * (size_t)bn[0], m->x + 2 * (size_t)bn[1],
+                  q.data() + 4 * (size_t)cc, adt.data() + cc, f, m->bound + b);
+    }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int n = 0; n < nc; ++n) {
+      double *r = &res[4 * (size_t)n];
+      for (int j = inc_off[n]; j < inc_off[n + 1]; ++j) {
+        const int ent = inc_ent[j];
+        const double *f = ent >= 0 ? &eflux[4 * (size_t)ent] : &bflux[4 * (size_t)(-1 - ent)];
+        for (int k = 0; k < 4; ++k) r[k] += f[k];
+      }
+    }
+  };
+
   for (int iter = 1; iter <= niter; ++iter) {
     double t0 = now();
     if (exec == 0) {
       orc_save_soln(nc, q.data(), qold.data());
+    } else if (exec == 2) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+      for (int n = 0; n < nc; ++n) k_save_soln(&q[4 * (size_t)n], &qold[4 * (size_t)n]);
     } else {
 #pragma omp parallel for num_threads(nthreads)
       for (int t = 0; t < nthreads; ++t) {
@@ -484,13 +540,18 @@ static int run_with_consts(const op2amd_mesh *m, int niter, const orc_consts &c,
       t0 = now();
       if (exec == 0)
         adt_range(m, c, q.data(), adt.data(), 0, nc);
-      else
+      else if (exec == 2) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int n = 0; n < nc; ++n) adt_range(m, c, q.data(), adt.data(), n, n + 1);
+      } else
         run_blocks(adt_plan.p, nthreads,
                    [&](int lo, int hi) { adt_range(m, c, q.data(), adt.data(), lo, hi); });
       double t1 = now();
       tl[1] += t1 - t0;
       if (exec == 0)
         res_range(m, c, q.data(), adt.data(), res.data(), 0, m->nedge);
+      else if (exec == 2)
+        res_bres_ordered();  // both loops: timed as res_calc
       else
         run_blocks(res_plan.p, nthreads, [&](int lo, int hi) {
           res_range(m, c, q.data(), adt.data(), res.data(), lo, hi);
@@ -499,7 +560,7 @@ static int run_with_consts(const op2amd_mesh *m, int niter, const orc_consts &c,
       tl[2] += t2 - t1;
       if (exec == 0)
         bres_range(m, c, q.data(), adt.data(), res.data(), 0, m->nbedge);
-      else
+      else if (exec == 1)
         run_blocks(bres_plan.p, nthreads, [&](int lo, int hi) {
           bres_range(m, c, q.data(), adt.data(), res.data(), lo, hi);
         });
@@ -511,6 +572,27 @@ static int run_with_consts(const op2amd_mesh *m, int niter, const orc_consts &c,
       if (exec == 0) {
         update_range(qold.data(), q.data(), res.data(), adt.data(), &rms,
                      &maxerr, &errloc, 0, nc);
+      } else if (exec == 2) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int n = 0; n < nc; ++n) {  // update.h:1-19 per element
+          const double adti = 1.0 / adt[n];
+          for (int k = 0; k < 4; ++k) {
+            const size_t i = 4 * (size_t)n + k;
+            const double del = adti * res[i];
+            q[i] = qold[i] - del;
+            res[i] = 0.0;
+            sq[i] = del * del;
+          }
+        }
+        for (int n = 0; n < nc; ++n)
+          for (int k = 0; k < 4; ++k) {
+            const double sqdel = sq[4 * (size_t)n + k];
+            rms += sqdel;
+            if (sqdel > maxerr) {
+              maxerr = sqdel;
+              errloc = n;
+            }
+          }
       } else {
         // openmp/loop_host.hpp.jinja:183-202,221-245,300-324: per-thread
         // partials 64 elements apart, so no two threads write one cache line

@@ -59,7 +59,11 @@ void orc_update(int ncell, const double *qold, double *q, double *res,
 
 /* Full airfoil run (airfoil.cpp:219-297).
  *   exec 0: sequential;  exec 1: OpenMP executor with `nthreads` threads
- *   (<=0: all) and plan part size `part_size` (0: the reference heuristic).
+ *   (<=0: all) and plan part size `part_size` (0: the reference heuristic);
+ *   exec 2: the sequential executor's results bit for bit, evaluated with
+ *   `nthreads` threads (each edge's increments evaluated once, each cell
+ *   adding its own in the sequential order, the update's reductions summed
+ *   serially in element order); its res_calc time includes bres_calc.
  * Outputs per iteration (arrays of niter, any may be NULL): the raw rms sum of
  * the first update call, the normalised rms after the second
  * (sqrt(rms/ncell)), maxerr and errloc of the second.  q_out: final q.

@@ -127,6 +127,13 @@ def update(qold, res, adt, rms=0.0, maxerr=0.0, errloc=0):
     return q, res, r.value, m.value, e.value
 
 
+# "seq": the sequential executor; "openmp": the OpenMP executor (block colours
+# in sequence, blocks in parallel); "seqpar": the sequential executor's results
+# bit for bit, evaluated on `nthreads` threads (tests/test_oracle.py pins it
+# to "seq").
+EXECS = {"seq": 0, "openmp": 1, "seqpar": 2}
+
+
 def airfoil_run(mesh, niter, alpha_mode=1, exec_="seq", nthreads=0, part_size=0):
     rms_k0 = np.zeros(niter)
     rms = np.zeros(niter)
@@ -135,7 +142,7 @@ def airfoil_run(mesh, niter, alpha_mode=1, exec_="seq", nthreads=0, part_size=0)
     q = np.zeros((mesh.ncell, 4))
     secs = np.zeros(5)
     rc = lib().orc_airfoil_run(C.byref(mesh.c()), niter, alpha_mode,
-                               {"seq": 0, "openmp": 1}[exec_], nthreads, part_size,
+                               EXECS[exec_], nthreads, part_size,
                                _p(rms_k0), _p(rms), _p(maxerr),
                                errloc.ctypes.data_as(C.POINTER(C.c_int)), _p(q),
                                _p(secs))
@@ -154,7 +161,7 @@ def airfoil_run_qinf(mesh, niter, qinf, exec_="seq", nthreads=0):
     q = np.zeros((mesh.ncell, 4))
     qi = np.ascontiguousarray(qinf, dtype=np.float64)
     rc = lib().orc_airfoil_run_qinf(C.byref(mesh.c()), niter, _p(qi),
-                                    {"seq": 0, "openmp": 1}[exec_], nthreads, 0,
+                                    EXECS[exec_], nthreads, 0,
                                     _p(rms_k0), _p(rms), _p(maxerr),
                                     errloc.ctypes.data_as(C.POINTER(C.c_int)), _p(q),
                                     None)

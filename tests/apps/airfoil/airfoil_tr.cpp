@@ -8,7 +8,8 @@
 // file format is its reader's (:88-143).  With graph=G (built for the HIP
 // backend) the first iteration runs eagerly and the rest as launches of a
 // captured graph of up to G steps (a divisor of the remaining iterations),
-// their rms/maxerr/errloc read from the graph history.
+// their rms/maxerr/errloc read from the graph history.  With dump=<file> the
+// final q is also written with op_print_dat_to_binfile (airfoil.cpp:303).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -28,9 +29,12 @@ int main(int argc, char **argv) {
   }
   std::vector<char *> init_argv{argv[0]};
   int graph_steps = 0;
+  const char *dump = nullptr;
   for (int i = 3; i < argc; ++i) {
     if (!std::strncmp(argv[i], "graph=", 6))
       graph_steps = std::atoi(argv[i] + 6);
+    else if (!std::strncmp(argv[i], "dump=", 5))
+      dump = argv[i] + 5;
     else
       init_argv.push_back(argv[i]);
   }
@@ -210,6 +214,7 @@ int main(int argc, char **argv) {
     sumsq += v * v;
   }
   std::printf("q %.17e %.17e\n", sum, sumsq);
+  if (dump) op_print_dat_to_binfile(p_q, dump);
   op_exit();
   return 0;
 }

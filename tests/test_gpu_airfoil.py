@@ -41,6 +41,16 @@ def test_airfoil_120x60_random_1000_iterations(case):
     assert pointwise_rel_err(r["rms_k0"], g["rms_k0_raw"]) <= TOL
     assert pointwise_rel_err(r["maxerr"], g["maxerr"]) <= TOL
     assert rel_err(r["q"], g["q"]) <= TOL
+    if case.get("plan_res") in ("owner", "gather") and case.get("plan_bres") in ("owner", "gather"):
+        # increments summed in the sequential order: the reference's bits
+        assert np.array_equal(r["q"], g["q"])
+        assert np.array_equal(r["maxerr"], g["maxerr"])
+        assert np.array_equal(r["errloc"], g["errloc"])
+    else:
+        # another summation order: where the maximum came out identical, it
+        # was found at the same cell
+        same = r["maxerr"] == g["maxerr"]
+        assert np.array_equal(r["errloc"][same], g["errloc"][same])
 
 
 def test_airfoil_rcm_graphed():
@@ -60,6 +70,7 @@ def test_owner_plan_reproduces_sequential_order(block):
                            plan_res="owner", block_res=block, plan_bres="gather")
     assert np.array_equal(r["q"], g["q"])
     assert np.array_equal(r["maxerr"], g["maxerr"])
+    assert np.array_equal(r["errloc"], g["errloc"])
 
 
 def test_gather_plan_reproduces_sequential_order():

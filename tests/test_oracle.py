@@ -152,3 +152,26 @@ def test_colouring_checker_rejects_a_planted_conflict():
               if c0 & set(m.ecell[e]))
     thrcol[e1] = thrcol[e0]
     assert oracle.colouring_conflicts(m.ecell, thrcol, blk) > 0
+
+
+@pytest.mark.parametrize("I,J,ordering,niter,alpha", [(120, 60, "random", 200, 1),
+                                                      (120, 60, "rcm", 200, 1),
+                                                      (24, 12, "random", 300, 1),
+                                                      (24, 12, "structured", 50, 0)])
+@pytest.mark.parametrize("nthreads", [1, 3, 8])
+def test_seqpar_is_the_sequential_executor_bit_for_bit(I, J, ordering, niter, alpha, nthreads):
+    """The parallel evaluation of the sequential executor ("seqpar": each
+    edge's increments evaluated once, each cell adding its own in edge order,
+    reductions summed serially) gives the sequential executor's bits: every
+    per-iteration output and the final q."""
+    m = op2amd.mesh.generate(I, J, 0, ordering)
+    a = oracle.airfoil_run(m, niter, alpha, "seq")
+    b = oracle.airfoil_run(m, niter, alpha, "seqpar", nthreads=nthreads)
+    for k in ("rms_k0", "rms", "maxerr", "errloc", "q"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_seqpar_matches_unmodified_reference_app():
+    g = golden("airfoil_24x12_random_app_a0")
+    r = oracle.airfoil_run(mesh_of(g), 1000, 0, "seqpar", nthreads=4)
+    assert np.array_equal(r["q"], g["q"])
