@@ -71,6 +71,34 @@ void op2h_kernel_timer_stop(op2h_loop_ctx *ctx);
  * Otherwise returns 0 with both NULL. */
 int op2h_kernel_timer_events(op2h_loop_ctx *ctx, void **start, void **stop);
 
+/* Deferred launches, for loop-chain fusion.  Inside a captured graph a stub
+ * may leave its kernel launch pending, op2h_defer(tag, flush, arg), so that
+ * the next stub can fold it into its own launch: that stub announces it with
+ * op2h_accept_deferred(tag) before op2h_loop_begin and then takes it over
+ * with op2h_take_deferred(tag, &arg) (1, with the deferred arg; 0 when
+ * nothing is pending, or, after flushing it, when what is pending has
+ * another tag).  Everything else flushes the pending launch
+ * first (calls flush(arg)): any other op2h_loop_begin, op2h_device_sync,
+ * fetches and dumps, graph begin and end, op_renumber, op_exit.  At most one
+ * launch is pending.  op2h_fusion_enabled() is 1 when a stub may defer: the
+ * call is being captured, fusion is on (op2h_set_fusion, default on; a
+ * setting kept across op_init/op_exit) and neither kernel timing, the
+ * autotuner, per-loop syncs nor per-loop profiling needs the loop on its
+ * own. */
+typedef void (*op2h_flush_fn)(void *arg);
+void op2h_defer(const char *tag, op2h_flush_fn flush, void *arg);
+void op2h_accept_deferred(const char *tag);
+int op2h_take_deferred(const char *tag, void **arg);
+void op2h_flush_deferred(void);
+void op2h_set_fusion(int on);
+int op2h_fusion_enabled(void);
+/* Pending launches taken over by a later stub since the library loaded. */
+unsigned long long op2h_fused_count(void);
+
+/* A counter advanced whenever the runtime drops its plans (op_exit,
+ * op_renumber): data a stub derives from a plan is valid while it holds. */
+unsigned op2h_plan_epoch(void);
+
 /* Device layout of a dat: 0 AoS, else the SoA stride in elements. */
 int op2h_dat_soa_stride(op_dat dat);
 
@@ -209,6 +237,13 @@ const op2h_owner_plan *op2h_plan_owner(op2h_loop_ctx *ctx);
 const op2h_owner_plan *op2h_plan_owner_for(const char *name, op_set set,
                                            int block_cells, int nargs,
                                            op_arg *args);
+/* As op2h_plan_owner, but under transparent renumbering a block's instances
+ * stay in the device's element order (each target's increments still follow
+ * the application's): for stubs that evaluate all of a block's instances
+ * before summing any increment, where the device order keeps neighbouring
+ * elements in neighbouring lanes.  Without transparent renumbering it is
+ * op2h_plan_owner. */
+const op2h_owner_plan *op2h_plan_owner_local(op2h_loop_ctx *ctx);
 /* Copy an owner plan's block structure back to the host (any pointer may be
  * NULL): owned_offs [nblocks+1], owned, elem_offs [nblocks+1], elems,
  * csr_offs, csr.  Returns 0. */
@@ -256,6 +291,20 @@ const op2h_color_plan *op2h_plan_color_for(const char *name, op_set set,
  * dropped by op_renumber.  *fresh is 1 when the caller must fill it (before
  * any graph capture). */
 void *op2h_plan_user(const char *key, size_t bytes, int *fresh);
+
+/* Under transparent renumbering (op_init token OP_HIP_RENUMBER or
+ * OP_HIP_RENUMBER=<map>): before the first loop the runtime renumbers the
+ * sets as op_renumber(<map>) would (default map: the first declared one of
+ * dimension >= 2 from the largest set), while the application keeps its own
+ * numbering: op_fetch_data, op_fetch_data_idx and the op_print_dat_to_*
+ * dumps give rows in the application's order, op_arg_idx and op_arg_info
+ * payloads carry the application's indices, and increments that the
+ * sequential order fixes (owner and gather plans) keep the application's
+ * element order, so results are those of the unrenumbered run.  An explicit
+ * op_renumber call is then a no-op.  This returns, on the device, the
+ * application's index of each element of `set` (old_of_new), or NULL when
+ * the set is not renumbered that way. */
+const int *op2h_set_old_index(op_set set);
 
 /* Compulsory ("algorithmic") bytes of one call of a loop: every dat element
  * and map entry the loop touches, counted once; RW and INC count a read and a
@@ -327,7 +376,10 @@ int op2h_check_color_plan(int set_size, int ncolors, const int *col_offsets,
 
 /* Host image of an owner-compute plan (op2h_owner_plan) with the increment
  * map's table (set_size x map_dim) and the map index of each increment;
- * csr4 may be NULL. */
+ * csr4 may be NULL.  elem_key (may be NULL): the element order each
+ * target's increments follow is ascending elem_key[element] (the
+ * application's numbering
+ * under transparent renumbering) instead of ascending element. */
 typedef struct {
   int set_size, to_size, nblocks, ninc;
   const int *map_table;
@@ -335,13 +387,17 @@ typedef struct {
   int inc_idx[OP2H_MAX_ARGS];
   const int *owned_offs, *owned, *elem_offs, *elems, *csr_offs;
   const unsigned short *csr, *csr4;
+  const int *elem_key;
+  int instances_by_key; /* 1: a block's instances follow elem_key's order
+                           too (else ascending element) */
 } op2h_owner_plan_host;
 int op2h_check_owner_plan(const op2h_owner_plan_host *p);
 
-/* Gather plan (op2h_gather_plan) of a map (from_size x dim, row-major). */
+/* Gather plan (op2h_gather_plan) of a map (from_size x dim, row-major);
+ * elem_key as for the owner plan. */
 int op2h_check_gather_plan(int from_size, int to_size, int dim, const int *map,
                            int ntargets, const int *offsets, const int *entries,
-                           const int *targets);
+                           const int *targets, const int *elem_key);
 
 #ifdef __cplusplus
 }

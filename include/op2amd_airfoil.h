@@ -47,7 +47,10 @@ typedef struct {
   int plan_on_device;          /* 1: colour plans on the GPU */
   int renumber;                /* 1: op_renumber(pecell) after declarations,
                                   as airfoil_hdf5/dp/airfoil.cpp:155 does;
-                                  results are reported in the input numbering */
+                                  results are reported in the input numbering;
+                                  2: the same numbering on the device only
+                                  (op_init token OP_HIP_RENUMBER=pecell), the
+                                  application keeping its own */
   int mixed;                   /* 1: p_adt and p_res declared "float": fluxes
                                   and adt in FP32, q/qold/update/rms in FP64 */
   int autotune;                /* >0: runtime autotuning with this many timed

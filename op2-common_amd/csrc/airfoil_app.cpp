@@ -92,10 +92,13 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
   if (!m || !o || !r || o->niter < 1 || o->warmup < 0 || o->warmup > o->niter)
     return 1;
   double t_setup0 = now();
-  std::string soa_tok = "OP_AUTO_SOA";
+  std::string soa_tok = "OP_AUTO_SOA", ren_tok = "OP_HIP_RENUMBER=pecell";
   char prog[] = "airfoil";
-  char *argv[2] = {prog, &soa_tok[0]};
-  op_init(o->soa ? 2 : 1, argv, 0);
+  char *argv[3] = {prog, nullptr, nullptr};
+  int argc = 1;
+  if (o->soa) argv[argc++] = &soa_tok[0];
+  if (o->renumber == 2) argv[argc++] = &ren_tok[0];  // transparent renumbering
+  op_init(argc, argv, 0);
   op2h_set_plan_builder(o->plan_on_device);
   op2h_enable_kernel_timing(o->kernel_timing);
 
@@ -146,7 +149,7 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
   op_decl_const2("mach", 1, "double", &mach);
   op_decl_const2("alpha", 1, "double", &alpha);
   op_decl_const2("qinf", 4, "double", qinf);
-  if (o->renumber) op_renumber(A.pecell);
+  if (o->renumber == 1) op_renumber(A.pecell);
   std::vector<int> cell_new_of_old(m->ncell);
   op2h_set_permutation(A.cells, cell_new_of_old.data());
   std::vector<int> cell_old_of_new(m->ncell);
@@ -336,7 +339,7 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
     const std::string base(o->dump);
     op_print_dat_to_txtfile(A.p_q, (base + ".dat").c_str());
     op_print_dat_to_binfile(A.p_q, (base + ".bin").c_str());
-    if (o->renumber) {
+    if (o->renumber == 1) {
       FILE *f = std::fopen((base + ".perm").c_str(), "w");
       if (!f) {
         printf("can't open file %s.perm\n", o->dump);
@@ -347,7 +350,7 @@ extern "C" int op2amd_airfoil_run(const op2amd_mesh *m, const op2amd_airfoil_opt
       std::fclose(f);
     }
   }
-  if (r->errloc && o->renumber)
+  if (r->errloc && o->renumber == 1)
     for (int it = 0; it < o->niter; ++it)
       if (r->errloc[it] >= 0 && r->errloc[it] < m->ncell)
         r->errloc[it] = cell_old_of_new[r->errloc[it]];

@@ -458,4 +458,67 @@ __device__ __forceinline__ void owner_res_block(int b, const double *__restrict 
   }
 }
 
+// res_calc and bres_calc fused into one owner-compute launch (loop-chain
+// fusion inside a captured step): each owned cell's fixed-width record lists
+// its interior increments, then its boundary edges as 0x8000 | j, j indexing
+// the block's list of boundary edges.  The interior fluxes are summed first,
+// in edge order, then the boundary fluxes, added by bres_calc itself into the
+// running sum in boundary-edge order: the order of res_calc then bres_calc in
+// the sequential executor, so res is bit-identical to it.
+struct BoundaryArgs {
+  const ushort4 *rec;   // per owned cell: up to four entries, 0xFFFF = none
+  const int *offs;      // [nblocks+1] per block: its boundary edges' start
+  const int *edges;     // boundary edge of each (block, j)
+  const int2 *pbedge;   // boundary edge -> its two nodes
+  const int *bound;     // boundary edge -> 1 wall, else far field
+};
+
+template <bool SOA>
+__device__ __forceinline__ void owner_res_bres_block(int b, const double *__restrict x,
+                                                     const double *__restrict q,
+                                                     const double *__restrict adt,
+                                                     double *__restrict res,
+                                                     const OwnerArgs &P, const BoundaryArgs &B,
+                                                     int sx, int sq, double *lf) {
+  const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
+  const int2 bb = P.base[b];
+  for (int j = threadIdx.x; j < ne; j += blockDim.x) {
+    const int4 r = owner_record(P, bb, e0 + j);
+    double x1[2], x2[2], q1[4], q2[4];
+    ld2<SOA>(x, r.x, sx, x1);
+    ld2<SOA>(x, r.y, sx, x2);
+    ld4<SOA>(q, r.z, sq, q1);
+    ld4<SOA>(q, r.w, sq, q2);
+    op2_k::res_flux(x1, x2, q1, q2, adt[r.z], adt[r.w], lf + 4 * j);
+  }
+  __syncthreads();
+  const int o0 = P.owned_offs[b], no = P.owned_offs[b + 1] - o0;
+  const int bb0 = B.offs[b];
+  for (int i = threadIdx.x; i < no; i += blockDim.x) {
+    const int cell = P.owned[o0 + i];
+    double acc[4];
+    ld4<SOA>(res, cell, sq, acc);
+    const ushort4 e = B.rec[o0 + i];
+    const unsigned short ent[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (ent[k] == 0xFFFF) continue;
+      if (ent[k] & 0x8000) {
+        const int be = B.edges[bb0 + (ent[k] & 0x7FFF)];
+        const int2 bn = B.pbedge[be];
+        double x1[2], x2[2], q1[4];
+        ld2<SOA>(x, bn.x, sx, x1);
+        ld2<SOA>(x, bn.y, sx, x2);
+        ld4<SOA>(q, cell, sq, q1);
+        const double a1 = adt[cell];
+        const int bnd = B.bound[be];
+        op2_k::bres_calc(x1, x2, q1, &a1, acc, &bnd);
+      } else {
+        owner_apply(acc, lf, (int)ent[k]);
+      }
+    }
+    st4<SOA>(res, cell, sq, acc);
+  }
+}
+
 }  // namespace op2_dev

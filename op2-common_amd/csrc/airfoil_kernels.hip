@@ -27,7 +27,10 @@
 #include <cstdio>
 #include <cstring>
 
+#include <map>
+
 #include "op2_hip_rt.h"
+#include "op2h_plan_build.h"
 #include "airfoil_device.h"
 #include "op2h_reduce.h"
 
@@ -717,7 +720,7 @@ __global__ void __launch_bounds__(1024)
     k_update(const double *__restrict qold, double *__restrict q,
              double *__restrict res, const double *__restrict adt, int n, int s,
              double *__restrict part_sum, double *__restrict part_max,
-             int *__restrict part_loc) {
+             int *__restrict part_loc, const int *__restrict oldidx) {
   __shared__ double s_sum[16];
   __shared__ double s_max[16];
   __shared__ int s_loc[16];
@@ -729,7 +732,9 @@ __global__ void __launch_bounds__(1024)
     int loc = ml.loc;
     ld4<SOA>(qold, c, s, qo);
     ld4<SOA>(res, c, s, r);
-    op2_k::update(qo, qn, r, &a, &rms, &mx, &c, &loc);
+    // op_arg_idx: the application's element index (transparent renumbering)
+    int ci = oldidx ? oldidx[c] : c;
+    op2_k::update(qo, qn, r, &a, &rms, &mx, &ci, &loc);
     ml.v = mx;
     ml.loc = loc;
     st4<SOA>(q, c, s, qn);
@@ -1020,7 +1025,7 @@ __global__ void __launch_bounds__(1024)
     k_update_mixed(const double *__restrict qold, double *__restrict q,
                    float *__restrict res, const float *__restrict adt, int n, int s,
                    double *__restrict part_sum, double *__restrict part_max,
-                   int *__restrict part_loc) {
+                   int *__restrict part_loc, const int *__restrict oldidx) {
   __shared__ double s_sum[16];
   __shared__ double s_max[16];
   __shared__ int s_loc[16];
@@ -1034,7 +1039,9 @@ __global__ void __launch_bounds__(1024)
     ld4<SOA>(qold, c, s, qo);
     ld4s<SOA>(res, c, s, rf);
     for (int d = 0; d < 4; ++d) r[d] = (double)rf[d];
-    op2_k::update(qo, qn, r, &a, &rms, &mx, &c, &loc);
+    // op_arg_idx: the application's element index (transparent renumbering)
+    int ci = oldidx ? oldidx[c] : c;
+    op2_k::update(qo, qn, r, &a, &rms, &mx, &ci, &loc);
     ml.v = mx;
     ml.loc = loc;
     st4<SOA>(q, c, s, qn);
@@ -1102,23 +1109,304 @@ void account(int kernel_idx, op_set set, int nargs, const op_arg *args) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Loop-chain fusion inside a captured step.  The stubs leave launches pending
+// (op2h_defer) and the next stub folds them into its own:
+//   - the direct loops of one cell set run per cell in one kernel: an update
+//     (and/or a save_soln) is held until the adt_calc that follows and then
+//     launched as [update] [save_soln] adt_calc per cell (adt depends on its
+//     own cell's q only);
+//   - an owner-plan res_calc is held until the bres_calc that follows and
+//     both run as one owner launch, each owned cell adding its boundary
+//     fluxes after its interior ones.
+// Every cell sees the operations of the unfused loops in the same order, and
+// the update keeps its grid, block and partials, so every result is the
+// unfused one bit for bit.  Anything else flushes a pending launch as it was.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <bool SOA, bool UPD, bool SAVE>
+__global__ void __launch_bounds__(1024)
+    k_direct_chain(const double *qold, double *__restrict q, double *__restrict res,
+                   double *adt, double *qold_out, const double *__restrict x,
+                   const int *__restrict pcell, int n, int s, int sx,
+                   double *__restrict part_sum, double *__restrict part_max,
+                   int *__restrict part_loc, const int *__restrict oldidx) {
+  // qold (read by update, written by save_soln) and adt (read by update,
+  // written by adt_calc) are read before they are written, by the same
+  // thread for the same cell: not restrict-qualified
+  __shared__ double s_sum[16];
+  __shared__ double s_max[16];
+  __shared__ int s_loc[16];
+  double rms = 0.0;
+  MaxLoc ml{-INFINITY, 0x7fffffff};
+  const int stride = gridDim.x * blockDim.x;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+    double qn[4];
+    if (UPD) {
+      double qo[4], r[4], a = adt[c], mx = ml.v;
+      int loc = ml.loc;
+      ld4<SOA>(qold, c, s, qo);
+      ld4<SOA>(res, c, s, r);
+      int ci = oldidx ? oldidx[c] : c;
+      op2_k::update(qo, qn, r, &a, &rms, &mx, &ci, &loc);
+      ml.v = mx;
+      ml.loc = loc;
+      st4<SOA>(q, c, s, qn);
+      st4<SOA>(res, c, s, r);
+    } else {
+      ld4<SOA>(q, c, s, qn);
+    }
+    if (SAVE) st4<SOA>(qold_out, c, s, qn);
+    const int4 cn = reinterpret_cast<const int4 *>(pcell)[c];
+    double x1[2], x2[2], x3[2], x4[2], a2;
+    ld2<SOA>(x, cn.x, sx, x1);
+    ld2<SOA>(x, cn.y, sx, x2);
+    ld2<SOA>(x, cn.z, sx, x3);
+    ld2<SOA>(x, cn.w, sx, x4);
+    op2_k::adt_calc(x1, x2, x3, x4, qn, &a2);
+    adt[c] = a2;
+  }
+  if (!UPD) return;
+  // the partials of k_update, by the same tree
+  rms = wave_sum(rms);
+  ml = wave_maxloc(ml);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_sum[wave] = rms;
+    s_max[wave] = ml.v;
+    s_loc[wave] = ml.loc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bsum = 0.0;
+    MaxLoc bm{-INFINITY, 0x7fffffff};
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) {
+      bsum += s_sum[w];
+      bm = maxloc_combine(bm, MaxLoc{s_max[w], s_loc[w]});
+    }
+    part_sum[blockIdx.x] = bsum;
+    part_max[blockIdx.x] = bm.v;
+    part_loc[blockIdx.x] = bm.loc;
+  }
+}
+
+template <bool SOA>
+__global__ void __launch_bounds__(1024)
+    k_res_bres_owner(const double *__restrict x, const double *__restrict q,
+                     const double *__restrict adt, double *__restrict res, OwnerArgs P,
+                     BoundaryArgs B, int sx, int sq) {
+  extern __shared__ double lf[];
+  owner_res_bres_block<SOA>(xcd_block(kXcds), x, q, adt, res, P, B, sx, sq, lf);
+}
+
+// A pending run of direct loops over one cell set: [update] [save_soln].
+struct PendingDirect {
+  bool upd = false, save = false, soa = false;
+  op_set set = nullptr;
+  hipStream_t st = nullptr;
+  int n = 0, s = 0;
+  // update (its partials handed to the graph's history kernel)
+  int g = 0, bs = 0;
+  const double *qold = nullptr;
+  double *q = nullptr, *res = nullptr, *adt = nullptr;
+  double *part_sum = nullptr, *part_max = nullptr;
+  int *part_loc = nullptr;
+  const int *oldidx = nullptr;
+  // save_soln: src -> dst, n2 16-byte vectors when run on its own
+  const double *save_src = nullptr;
+  double *save_dst = nullptr;
+  int save_n2 = 0, save_bs = 256;
+};
+PendingDirect g_direct;
+
+void flush_direct(void *) {
+  PendingDirect P = g_direct;
+  g_direct = PendingDirect();
+  if (P.upd)
+    launch_k(false, nullptr, nullptr, P.soa ? k_update<true> : k_update<false>, P.g, P.bs, 0,
+             P.st, P.qold, P.q, P.res, (const double *)P.adt, P.n, P.s, P.part_sum,
+             P.part_max, P.part_loc, P.oldidx);
+  if (P.save)
+    launch_k(false, nullptr, nullptr, k_save_soln, grid_for(P.save_n2, P.save_bs), P.save_bs, 0,
+             P.st, (const double2 *)P.save_src, (double2 *)P.save_dst, P.save_n2);
+}
+
+// A pending owner-plan res_calc.
+struct PendingRes {
+  bool soa = false;
+  hipStream_t st = nullptr;
+  const op2h_owner_plan *plan = nullptr;
+  OwnerArgs A{};
+  const double *x = nullptr, *q = nullptr, *adt = nullptr;
+  double *res = nullptr;
+  int sx = 0, sq = 0, bs = 0;
+  size_t lds = 0;
+  op_map pecell = nullptr;
+};
+PendingRes g_res;
+PendingRes g_last_res;  // the latest owner-plan res_calc, pending or not
+
+void launch_res_owner(const PendingRes &P) {
+  if (P.plan->nblocks)
+    launch_k(false, nullptr, nullptr,
+             P.A.csr4 ? (P.soa ? k_res_owner<true, true> : k_res_owner<false, true>)
+                      : (P.soa ? k_res_owner<true, false> : k_res_owner<false, false>),
+             P.plan->nblocks, P.bs, P.lds, P.st, P.x, P.q, P.adt, P.res, P.A, P.sx, P.sq);
+}
+
+void flush_res(void *) {
+  PendingRes P = g_res;
+  g_res = PendingRes();
+  launch_res_owner(P);
+}
+
+// The fused res_calc + bres_calc records for an owner plan and the boundary
+// map (bedge -> cell), built on the host once per plan: each owned cell's
+// interior increments, then its boundary edges (ascending; under transparent
+// renumbering in the application's order).  Null when a cell has more than
+// four increments in all, or the local ids do not fit 15 bits.
+struct BoundaryRec {
+  unsigned epoch = 0;  // op2h_plan_epoch() when built
+  bool ok = false;
+  BoundaryArgs B{};
+};
+
+// `build` false: only a record built before (inside a capture nothing may
+// be allocated or uploaded).
+const BoundaryArgs *boundary_args(const op2h_owner_plan *p, op_map pbecell, op_map pbedge,
+                                  op_dat bound, hipStream_t st, bool build) {
+  static std::map<std::pair<const void *, int>, BoundaryRec> cache;
+  auto key = std::make_pair((const void *)p, pbecell->index);
+  auto it = cache.find(key);
+  if (it != cache.end() && it->second.epoch == op2h_plan_epoch()) {
+    if (!it->second.ok) return nullptr;
+    it->second.B.bound = (const int *)bound->data_d;
+    it->second.B.pbedge = (const int2 *)pbedge->map_d;
+    return &it->second.B;
+  }
+  if (!build) return nullptr;
+  BoundaryRec &slot = cache[key];
+  slot = BoundaryRec();
+  slot.epoch = op2h_plan_epoch();
+  if (!p->csr4 || pbecell->dim != 1 || (long long)p->max_elems * p->ninc >= 0x8000) return nullptr;
+  const int nb = p->nblocks, no = p->n_owned;
+  std::vector<int> oo(nb + 1), ow(std::max(no, 1)), eo(nb + 1), el(std::max(p->total, 1)),
+      co(no + 1);
+  op2h_owner_plan_download(p, oo.data(), ow.data(), eo.data(), el.data(), co.data(), nullptr);
+  std::vector<unsigned short> csr(std::max(co[no], 1));
+  op2h_owner_plan_download(p, nullptr, nullptr, nullptr, nullptr, nullptr, csr.data());
+  // boundary edges per cell, in order
+  const int nbe = pbecell->from->size, ncell = p->to_size;
+  const int *key_be = op2h::old_index_host(pbecell->from);
+  std::vector<int> bo(ncell + 1, 0), order(nbe);
+  for (int e = 0; e < nbe; ++e) bo[pbecell->map[e] + 1]++;
+  for (int c = 0; c < ncell; ++c) bo[c + 1] += bo[c];
+  for (int e = 0; e < nbe; ++e) order[e] = e;
+  if (key_be)
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return key_be[a] < key_be[b]; });
+  std::vector<int> fill(bo.begin(), bo.end() - 1), bl(std::max(nbe, 1));
+  for (int e : order) bl[fill[pbecell->map[e]]++] = e;
+  std::vector<ushort4> rec(std::max(no, 1));
+  std::vector<int> offs(nb + 1, 0), edges;
+  for (int b = 0; b < nb; ++b) {
+    offs[b] = (int)edges.size();
+    for (int i = oo[b]; i < oo[b + 1]; ++i) {
+      unsigned short ent[4] = {0xFFFF, 0xFFFF, 0xFFFF, 0xFFFF};
+      int k = 0;
+      for (int j = co[i]; j < co[i + 1]; ++j) {
+        if (k == 4) return nullptr;
+        ent[k++] = csr[j];
+      }
+      const int cell = ow[i];
+      for (int j = bo[cell]; j < bo[cell + 1]; ++j) {
+        const int local = (int)edges.size() - offs[b];
+        if (k == 4 || local >= 0x7FFF) return nullptr;
+        ent[k++] = (unsigned short)(0x8000 | local);
+        edges.push_back(bl[j]);
+      }
+      rec[i] = make_ushort4(ent[0], ent[1], ent[2], ent[3]);
+    }
+  }
+  offs[nb] = (int)edges.size();
+  if (edges.size() != (size_t)nbe) return nullptr;  // a boundary edge of an unowned cell
+  char name[96];
+  snprintf(name, sizeof name, "res_bres.fused.%p.%d", (const void *)p, pbecell->index);
+  const size_t b_rec = ((size_t)8 * std::max(no, 1) + 255) / 256 * 256;
+  const size_t b_offs = ((size_t)4 * (nb + 1) + 255) / 256 * 256;
+  int fresh = 0;
+  char *buf = (char *)op2h_plan_user(name, b_rec + b_offs + 4 * std::max<size_t>(edges.size(), 1),
+                                     &fresh);
+  KCHECK(hipMemcpyAsync(buf, rec.data(), 8 * (size_t)no, hipMemcpyHostToDevice, st));
+  KCHECK(hipMemcpyAsync(buf + b_rec, offs.data(), 4 * (size_t)(nb + 1), hipMemcpyHostToDevice, st));
+  if (!edges.empty())
+    KCHECK(hipMemcpyAsync(buf + b_rec + b_offs, edges.data(), 4 * edges.size(),
+                          hipMemcpyHostToDevice, st));
+  KCHECK(hipStreamSynchronize(st));
+  BoundaryArgs &B = slot.B;
+  B.rec = (const ushort4 *)buf;
+  B.offs = (const int *)(buf + b_rec);
+  B.edges = (const int *)(buf + b_rec + b_offs);
+  B.pbedge = (const int2 *)pbedge->map_d;
+  B.bound = (const int *)bound->data_d;
+  slot.ok = true;
+  return &B;
+}
+
+}  // namespace
+
 extern "C" void op_par_loop_airfoil_1_save_soln(char const *name, op_set set,
                                                 op_arg arg0, op_arg arg1) {
   op_arg args[2] = {arg0, arg1};
   op2h_loop_ctx ctx;
+  op2h_accept_deferred("direct_chain");
   op2h_loop_begin(&ctx, 0, name, set, 2, args, OP2H_PLAN_ATOMIC, 256);
   hipStream_t st = (hipStream_t)ctx.stream;
+  const bool fuse = op2h_fusion_enabled() != 0;
+  bool held = op2h_take_deferred("direct_chain", nullptr) != 0;
+  if (held && !fuse) {  // launched first, in its place in the order
+    flush_direct(nullptr);
+    held = false;
+  }
   if (set->size > 0) {
     int stride = 0;
     bool soa = loop_soa(args, 2, &stride);
     // identical layouts: the copy covers the whole device allocation
     long long ndouble = soa ? 4LL * set->stride : 4LL * set->size;
     int n2 = (int)(ndouble / 2);
+    PendingDirect &P = g_direct;
+    if (held && !(P.upd && !P.save && P.set == set && P.soa == soa &&
+                  P.q == (double *)args[0].data_d)) {
+      flush_direct(nullptr);  // not the update this copy follows
+      held = false;
+    }
+    if (fuse) {  // held for the adt_calc that follows
+      if (!held) {
+        P = PendingDirect();
+        P.set = set;
+        P.soa = soa;
+        P.st = st;
+        P.n = set->size;
+        P.s = soa ? set->stride : 0;
+      }
+      P.save = true;
+      P.save_src = (const double *)args[0].data_d;
+      P.save_dst = (double *)args[1].data_d;
+      P.save_n2 = n2;
+      P.save_bs = ctx.block_size;
+      op2h_defer("direct_chain", flush_direct, nullptr);
+      account(0, set, 2, args);
+      op2h_loop_end(&ctx);
+      return;
+    }
     LoopTimer tm(&ctx);
     launch_k(tm.ext, tm.e0, tm.e1, k_save_soln, grid_for(n2, ctx.block_size), ctx.block_size,
              0, st, (const double2 *)args[0].data_d, (double2 *)args[1].data_d, n2);
     tm.done();
     KCHECK(hipGetLastError());
+  } else if (held) {
+    flush_direct(nullptr);
   }
   account(0, set, 2, args);
   op2h_loop_end(&ctx);
@@ -1130,8 +1418,14 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
                                                op_arg arg4, op_arg arg5) {
   op_arg args[6] = {arg0, arg1, arg2, arg3, arg4, arg5};
   op2h_loop_ctx ctx;
+  op2h_accept_deferred("direct_chain");
   op2h_loop_begin(&ctx, 1, name, set, 6, args, OP2H_PLAN_ATOMIC, 256);
   hipStream_t st = (hipStream_t)ctx.stream;
+  bool held = op2h_take_deferred("direct_chain", nullptr) != 0;
+  if (held && !op2h_fusion_enabled()) {
+    flush_direct(nullptr);
+    held = false;
+  }
   if (set->size > 0) {
     int dummy;
     bool soa = loop_soa(args, 6, &dummy);
@@ -1141,6 +1435,28 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
     double *adt = (double *)args[5].data_d;
     const int *pcell = args[0].map->map_d;
     int g = grid_for(set->size, ctx.block_size);
+    if (held) {
+      PendingDirect P = g_direct;
+      const double *qsrc = P.upd ? P.q : P.save_src;
+      if (P.set == set && P.soa == soa && (soa ? sq == P.s : true) && qsrc == q &&
+          !is_float(args[5]) && (!P.upd || P.adt == adt)) {
+        g_direct = PendingDirect();
+        // [update] [save_soln] adt_calc per cell: the update's grid and block
+        // (its partials), else adt_calc's
+        const int gg = P.upd ? P.g : g, bb = P.upd ? P.bs : ctx.block_size;
+        auto kern = P.upd ? (P.save ? (soa ? k_direct_chain<true, true, true> : k_direct_chain<false, true, true>)
+                                    : (soa ? k_direct_chain<true, true, false> : k_direct_chain<false, true, false>))
+                          : (soa ? k_direct_chain<true, false, true> : k_direct_chain<false, false, true>);
+        launch_k(false, nullptr, nullptr, kern, gg, bb, 0, st, P.qold, P.q ? P.q : (double *)q,
+                 P.res, adt, P.save_dst, x, pcell, set->size, sq, sx, P.part_sum, P.part_max,
+                 P.part_loc, P.oldidx);
+        KCHECK(hipGetLastError());
+        account(1, set, 6, args);
+        op2h_loop_end(&ctx);
+        return;
+      }
+      flush_direct(nullptr);
+    }
     LoopTimer tm(&ctx);
     if (is_float(args[5])) {
       float *adtf = (float *)args[5].data_d;
@@ -1152,6 +1468,8 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
     }
     tm.done();
     KCHECK(hipGetLastError());
+  } else if (held) {
+    flush_direct(nullptr);
   }
   account(1, set, 6, args);
   op2h_loop_end(&ctx);
@@ -1190,7 +1508,7 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       OwnerArgs OA{};
       if (ctx.plan == OP2H_PLAN_OWNER) {  // built before the timed region
         ctx.part_size = bs;
-        owner_p = op2h_plan_owner(&ctx);
+        owner_p = op2h_plan_owner_local(&ctx);
         OA = owner_args_res(owner_p, args, st);
       }
       op2h_kernel_timer_start(&ctx);
@@ -1201,6 +1519,10 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       } else if (ctx.plan == OP2H_PLAN_OWNER) {
         const int nblocks = owner_p->nblocks;
         const size_t lds = sizeof(float) * 4 * (size_t)OA.max_e;
+        if (lds > 160 * 1024) {
+          printf("op2hip: res_calc owner plan needs %zu bytes of LDS; use a smaller block\n", lds);
+          exit(-1);
+        }
         if (nblocks) {
           auto kern = OA.csr4 ? (soa ? k_res_owner_mixed<true, true> : k_res_owner_mixed<false, true>)
                               : (soa ? k_res_owner_mixed<true, false> : k_res_owner_mixed<false, false>);
@@ -1287,12 +1609,32 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       }
       case OP2H_PLAN_OWNER: {
         ctx.part_size = bs;
-        const op2h_owner_plan *p = op2h_plan_owner(&ctx);
+        const op2h_owner_plan *p = op2h_plan_owner_local(&ctx);
         OwnerArgs A = owner_args_res(p, args, st);
         const size_t lds = sizeof(double) * 4 * (size_t)A.max_e;
         if (lds > 160 * 1024) {
           printf("op2hip: res_calc owner plan needs %zu bytes of LDS; use a smaller block\n", lds);
           exit(-1);
+        }
+        PendingRes R;
+        R.soa = soa;
+        R.st = st;
+        R.plan = p;
+        R.A = A;
+        R.x = x;
+        R.q = q;
+        R.adt = adt;
+        R.res = res;
+        R.sx = sx;
+        R.sq = sq;
+        R.bs = bs;
+        R.lds = lds;
+        R.pecell = args[6].map;
+        g_last_res = R;
+        if (op2h_fusion_enabled()) {  // held for the bres_calc that follows
+          g_res = R;
+          op2h_defer("res_owner", flush_res, nullptr);
+          break;
         }
         LoopTimer tm(&ctx);
         if (p->nblocks)
@@ -1353,8 +1695,14 @@ extern "C" void op_par_loop_airfoil_4_bres_calc(char const *name, op_set set,
                                                 op_arg arg4, op_arg arg5) {
   op_arg args[6] = {arg0, arg1, arg2, arg3, arg4, arg5};
   op2h_loop_ctx ctx;
+  op2h_accept_deferred("res_owner");
   op2h_loop_begin(&ctx, 3, name, set, 6, args, OP2H_PLAN_STAGED, 256);
   hipStream_t st = (hipStream_t)ctx.stream;
+  bool held = op2h_take_deferred("res_owner", nullptr) != 0;
+  if (held && !op2h_fusion_enabled()) {
+    flush_res(nullptr);
+    held = false;
+  }
   if (set->size > 0) {
     int dummy;
     bool soa = loop_soa(args, 6, &dummy);
@@ -1367,6 +1715,29 @@ extern "C" void op_par_loop_airfoil_4_bres_calc(char const *name, op_set set,
     const int2 *pbedge = (const int2 *)args[0].map->map_d;
     const int *pbecell = args[2].map->map_d;
     const int bs = ctx.block_size;
+    // the res_calc this loop follows, on the same dats: the two as one owner
+    // launch (inside a capture), its boundary records built beforehand
+    const PendingRes &L = held ? g_res : g_last_res;
+    const bool same = L.plan && L.x == x && L.q == q && L.adt == adt && L.res == res &&
+                      L.soa == soa && L.sx == sx && L.sq == sq && !is_float(args[4]) &&
+                      args[2].map->to == L.pecell->to && args[4].map == args[2].map;
+    const BoundaryArgs *B = same ? boundary_args(L.plan, args[2].map, args[0].map, args[5].dat,
+                                                 st, !ctx.capturing)
+                                 : nullptr;
+    if (held) {
+      if (B) {
+        PendingRes P = g_res;
+        g_res = PendingRes();
+        if (P.plan->nblocks)
+          launch_k(false, nullptr, nullptr, soa ? k_res_bres_owner<true> : k_res_bres_owner<false>,
+                   P.plan->nblocks, P.bs, P.lds, st, x, q, adt, res, P.A, *B, sx, sq);
+        KCHECK(hipGetLastError());
+        account(3, set, 6, args);
+        op2h_loop_end(&ctx);
+        return;
+      }
+      flush_res(nullptr);
+    }
     if (is_float(args[4])) {
       const float *adtf = (const float *)args[3].data_d;
       float *resf = (float *)args[4].data_d;
@@ -1459,6 +1830,8 @@ extern "C" void op_par_loop_airfoil_4_bres_calc(char const *name, op_set set,
       }
     }
     KCHECK(hipGetLastError());
+  } else if (held) {
+    flush_res(nullptr);
   }
   account(3, set, 6, args);
   op2h_loop_end(&ctx);
@@ -1499,21 +1872,47 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
     part_max = part_sum + g;
     part_loc = (int *)(part_max + g);
   }
-  LoopTimer tm(&ctx);
   if (is_float(args[2]) != is_float(args[3])) {
     printf("op2hip: update needs res and adt both FP32 (mixed mode) or both FP64\n");
     exit(-1);
   }
+  if (deferred && !is_float(args[2]) && op2h_fusion_enabled() && set->size > 0) {
+    // held for the save_soln / adt_calc that follow (its partials are
+    // finished by the graph's history kernel either way)
+    PendingDirect &P = g_direct;
+    P = PendingDirect();
+    P.upd = true;
+    P.set = set;
+    P.soa = soa;
+    P.st = st;
+    P.n = set->size;
+    P.s = s;
+    P.g = g;
+    P.bs = bs;
+    P.qold = (const double *)args[0].data_d;
+    P.q = (double *)args[1].data_d;
+    P.res = (double *)args[2].data_d;
+    P.adt = (double *)args[3].data_d;
+    P.part_sum = part_sum;
+    P.part_max = part_max;
+    P.part_loc = part_loc;
+    P.oldidx = op2h_set_old_index(set);
+    op2h_defer("direct_chain", flush_direct, nullptr);
+    account(4, set, 8, args);
+    op2h_loop_end(&ctx);
+    return;
+  }
+  LoopTimer tm(&ctx);
   if (is_float(args[2])) {
     launch_k(tm.ext, tm.e0, nullptr, soa ? k_update_mixed<true> : k_update_mixed<false>, g, bs,
              0, st, (const double *)args[0].data_d, (double *)args[1].data_d,
              (float *)args[2].data_d, (const float *)args[3].data_d, set->size, s, part_sum,
-             part_max, part_loc);
+             part_max, part_loc, op2h_set_old_index(set));
   } else {
     launch_k(tm.ext, tm.e0, nullptr, soa ? k_update<true> : k_update<false>, g, bs, 0, st,
              (const double *)args[0].data_d, (double *)args[1].data_d,
              (double *)args[2].data_d, (const double *)args[3].data_d, set->size, s,
-             part_sum, part_max, part_loc);
+             part_sum, part_max, part_loc, op2h_set_old_index(set));
   }
   if (!deferred)
     launch_k(tm.ext, nullptr, tm.e1, k_update_finish, 1, kFinishThreads, 0, st, part_sum,

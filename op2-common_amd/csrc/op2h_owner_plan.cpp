@@ -12,6 +12,7 @@
 // order is the sequential executor's (op_seq.h:918-1073), so results are
 // bit-identical to it.
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -36,7 +37,12 @@ namespace op2h {
 // when a block's search runs dry before block_cells targets (a fragment cut
 // off by earlier blocks, or no adjacency at all) it continues from the next
 // unassigned seed, so every block but the last holds exactly block_cells
-// targets.  Returns the block count.
+// targets.  When the numbering is local (the targets of one element lie
+// within a bandwidth w much below 2^15) a block only takes targets within
+// 65535 - 2w - 1024 of its first seed, so that every target its elements
+// reach lies within 16 bits of the block's lowest (stubs then read 16-bit
+// offsets instead of full indices): the window cuts a block short only when
+// the growth would have left it.  Returns the block count.
 static int grow_blocks(int nto, int nel, int dim, const int *map, int ninc,
                        const int *inc_idx, int block_cells,
                        std::vector<int> &block_of) {
@@ -61,6 +67,19 @@ static int grow_blocks(int nto, int nel, int dim, const int *map, int ninc,
         if (l != k && u != t) adj[fill[t]++] = u;
       }
     }
+  // the bandwidth of the numbering: the widest spread of one element's targets
+  long long w = 0;
+  for (int e = 0; e < nel; ++e) {
+    int lo = INT_MAX, hi = -1;
+    for (int k = 0; k < ninc; ++k) {
+      const int t = map[(size_t)e * dim + inc_idx[k]];
+      lo = std::min(lo, t);
+      hi = std::max(hi, t);
+    }
+    w = std::max(w, (long long)hi - lo);
+  }
+  const long long window = 65535 - 2 * w - 1024;
+  const long long span = window >= 32768 ? window : 0;  // 0: no window
   block_of.assign(nto, -1);
   std::vector<int> mark(nto, -1), queue;
   queue.reserve(4 * (size_t)block_cells + 16);
@@ -75,9 +94,11 @@ static int grow_blocks(int nto, int nel, int dim, const int *map, int ninc,
     queue.clear();
     size_t head = 0;
     int count = 0;
+    const long long s0 = seed;  // the block's lowest target
     while (count < block_cells) {
       if (head == queue.size()) {
         if (!next_seed()) break;
+        if (span && seed - s0 > span) break;
         mark[seed] = nb;
         queue.push_back(seed);
       }
@@ -86,7 +107,7 @@ static int grow_blocks(int nto, int nel, int dim, const int *map, int ninc,
       ++count;
       for (int j = deg[c]; j < deg[c + 1]; ++j) {
         int u = adj[j];
-        if (block_of[u] < 0 && mark[u] != nb) {
+        if (block_of[u] < 0 && mark[u] != nb && (!span || u - s0 <= span)) {
           mark[u] = nb;
           queue.push_back(u);
         }
@@ -99,7 +120,7 @@ static int grow_blocks(int nto, int nel, int dim, const int *map, int ninc,
 
 bool owner_lists(int nto, int nel, int dim, const int *map, int ninc,
                  const int *inc_idx, int block_cells, OwnerLists &L,
-                 std::string &err) {
+                 std::string &err, const int *elem_key, bool key_instances) {
   std::vector<int> block_of;
   const int nb = grow_blocks(nto, nel, dim, map, ninc, inc_idx, block_cells, block_of);
   L.nblocks = nb;
@@ -139,8 +160,16 @@ bool owner_lists(int nto, int nel, int dim, const int *map, int ninc,
   std::vector<int> &elems = L.elems;
   elems.assign(total, 0);
   {
+    // instances in element order: ascending element, or ascending key (the
+    // application's element order) when a key is given, so that a block's
+    // instances and each target's increments follow the same order
+    std::vector<int> order(nel);
+    for (int e = 0; e < nel; ++e) order[e] = e;
+    if (elem_key && key_instances)
+      std::sort(order.begin(), order.end(),
+                [&](int a, int b) { return elem_key[a] < elem_key[b]; });
     std::vector<int> f(elem_offs.begin(), elem_offs.end() - 1);
-    for (int e = 0; e < nel; ++e) {
+    for (int e : order) {
       int n = blocks_of_elem(e, tmp.data());
       for (int j = 0; j < n; ++j) elems[f[tmp[j]]++] = e;
     }
@@ -179,6 +208,16 @@ bool owner_lists(int nto, int nel, int dim, const int *map, int ninc,
             csr[f[ownpos[t]]++] = (unsigned short)((j - elem_offs[b]) * ninc + k);
         }
   }
+  if (elem_key) {  // each target's increments in (key of the element, k) order
+    for (int b = 0; b < nb; ++b)
+      for (int i = owned_offs[b]; i < owned_offs[b + 1]; ++i)
+        std::sort(csr.begin() + csr_offs[i], csr.begin() + csr_offs[i + 1],
+                  [&](unsigned short x, unsigned short y) {
+                    const int kx = elem_key[elems[elem_offs[b] + x / ninc]];
+                    const int ky = elem_key[elems[elem_offs[b] + y / ninc]];
+                    return kx != ky ? kx < ky : x % ninc < y % ninc;
+                  });
+  }
   L.max_incs = 0;
   for (size_t i = 0; i < owned.size(); ++i)
     L.max_incs = std::max(L.max_incs, csr_offs[i + 1] - csr_offs[i]);
@@ -188,7 +227,7 @@ bool owner_lists(int nto, int nel, int dim, const int *map, int ninc,
 std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
                                                int nargs, const op_arg *args,
                                                hipStream_t stream,
-                                               std::string &err) {
+                                               std::string &err, bool key_instances) {
   auto rec = std::make_unique<OwnerPlanRec>();
   op2h_owner_plan &P = rec->plan;
   // the increment args: all through one map into one dat
@@ -250,8 +289,12 @@ std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
   const int nel = set->size, nto = own->to->size, dim = own->dim;
   const int ninc = (int)inc_idx.size();
   OwnerLists L;
-  if (!owner_lists(nto, nel, dim, own->map, ninc, inc_idx.data(), block_cells, L, err))
+  // under transparent renumbering the application's element order fixes the
+  // order of each target's increments
+  if (!owner_lists(nto, nel, dim, own->map, ninc, inc_idx.data(), block_cells, L, err,
+                   old_index_host(set), key_instances))
     return nullptr;
+  rec->key_instances = key_instances;
   P.set_size = nel;
   P.to_size = nto;
   P.block_cells = block_cells;

@@ -348,7 +348,13 @@ std::unique_ptr<GatherPlanRec> build_gather_plan(op_map map, hipStream_t stream)
     if (count[t + 1]) targets.push_back(t);
   for (int t = 0; t < to; ++t) count[t + 1] += count[t];
   std::vector<int> entries((size_t)from * dim), fill(count.begin(), count.end() - 1);
-  for (int e = 0; e < from; ++e)  // element order, then slot order
+  // element order, then slot order; under transparent renumbering, the
+  // application's element order
+  const int *key = old_index_host(map->from);
+  std::vector<int> order(from);
+  for (int e = 0; e < from; ++e) order[e] = e;
+  if (key) std::sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  for (int e : order)
     for (int k = 0; k < dim; ++k) {
       int t = map->map[(size_t)e * dim + k];
       entries[fill[t]++] = e * dim + k;

@@ -41,6 +41,7 @@ struct OwnerPlanRec {
   std::string signature;
   op2h_owner_plan plan{};
   int n_owned = 0, n_csr = 0;
+  bool key_instances = true;  // instances in the application's element order
   DeviceAllocs mem;
 };
 
@@ -59,6 +60,11 @@ struct LoopShape {
 };
 
 LoopShape loop_shape(op_set set, int nargs, const op_arg *args);
+
+// Under transparent renumbering (OP_HIP_RENUMBER): the application's index of
+// each element of a renumbered set (old_of_new), else nullptr.  Plans that
+// sum increments in element order use it to keep the application's order.
+const int *old_index_host(op_set set);
 
 // Greedy first-fit colouring in element order with 32 colours per pass
 // (op_rt_support.cpp:684-754 for elements, :798-884 for blocks).
@@ -92,14 +98,20 @@ struct OwnerLists {
   std::vector<int> owned_offs, owned, elem_offs, elems, csr_offs;
   std::vector<unsigned short> csr;
 };
+// elem_key (may be nullptr): the order in which one target's increments are
+// listed is ascending (elem_key[element], k) instead of (element, k); with
+// key_instances a block's instances follow that order too (else ascending
+// element, which keeps neighbouring elements in neighbouring lanes).
 bool owner_lists(int nto, int nel, int dim, const int *map, int ninc,
                  const int *inc_idx, int block_cells, OwnerLists &L,
-                 std::string &err);
+                 std::string &err, const int *elem_key = nullptr,
+                 bool key_instances = true);
 
 std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
                                                int nargs, const op_arg *args,
                                                hipStream_t stream,
-                                               std::string &err);
+                                               std::string &err,
+                                               bool key_instances = true);
 int download_owner(const OwnerPlanRec &r, int *owned_offs, int *owned,
                    int *elem_offs, int *elems, int *csr_offs,
                    unsigned short *csr);

@@ -262,10 +262,13 @@ extern "C" int op2h_check_owner_plan(const op2h_owner_plan_host *p) {
   // instances ascending and in range; increments decode to their target, in
   // element order, each (element, increment) listed exactly once
   ok = true;
+  auto order_of = [&](int e) {
+    return p->elem_key && p->instances_by_key ? p->elem_key[e] : e;
+  };
   for (int b = 0; b < nb; ++b)
     for (int i = p->elem_offs[b]; i < p->elem_offs[b + 1]; ++i)
       ok &= p->elems[i] >= 0 && p->elems[i] < nel &&
-            (i == p->elem_offs[b] || p->elems[i] > p->elems[i - 1]);
+            (i == p->elem_offs[b] || order_of(p->elems[i]) > order_of(p->elems[i - 1]));
   r.check(ok, "instances");
   if (r.errors) return r.errors;
   bool lists = true, complete = true;
@@ -275,14 +278,18 @@ extern "C" int op2h_check_owner_plan(const op2h_owner_plan_host *p) {
     used.assign(ni, 0);
     for (int o = p->owned_offs[b]; o < p->owned_offs[b + 1]; ++o) {
       const int t = p->owned[o];
-      int last = -1;
+      long long last = -1;
       for (int j = p->csr_offs[o]; j < p->csr_offs[o + 1]; ++j) {
         const int ent = p->csr[j], inst = ent / ninc, k = ent % ninc;
-        if (inst >= ni || ent <= last) {
+        // element order: the element's own index, or its key when given
+        const long long pos =
+            inst < ni ? (p->elem_key ? (long long)p->elem_key[p->elems[i0 + inst]] : inst) * ninc + k
+                      : -1;
+        if (inst >= ni || pos <= last) {
           lists = false;
           continue;
         }
-        last = ent;
+        last = pos;
         used[inst] = 1;
         lists &= p->map_table[(size_t)p->elems[i0 + inst] * p->map_dim + p->inc_idx[k]] == t;
         if (p->csr4 && j - p->csr_offs[o] < 4)
@@ -304,7 +311,7 @@ extern "C" int op2h_check_owner_plan(const op2h_owner_plan_host *p) {
 
 extern "C" int op2h_check_gather_plan(int from_size, int to_size, int dim, const int *map,
                                       int ntargets, const int *offsets, const int *entries,
-                                      const int *targets) {
+                                      const int *targets, const int *elem_key) {
   Report r;
   bool ok = from_size >= 0 && to_size >= 0 && dim >= 1 && ntargets >= 0 &&
             offsets[0] == 0 && offsets[ntargets] == from_size * dim;
@@ -317,7 +324,11 @@ extern "C" int op2h_check_gather_plan(int from_size, int to_size, int dim, const
   for (int i = 0; i < ntargets; ++i)
     for (int j = offsets[i]; j < offsets[i + 1]; ++j)
       ok &= entries[j] >= 0 && entries[j] < from_size * dim &&
-            (j == offsets[i] || entries[j] > entries[j - 1]) && map[entries[j]] == targets[i];
+            map[entries[j]] == targets[i] &&
+            (j == offsets[i] ||
+             (elem_key ? (long long)elem_key[entries[j] / dim] * dim + entries[j] % dim >
+                             (long long)elem_key[entries[j - 1] / dim] * dim + entries[j - 1] % dim
+                       : entries[j] > entries[j - 1]));
   r.check(ok, "gather entries");
   return r.errors;
 }
@@ -444,6 +455,9 @@ void check_owner_or_exit(const OwnerPlanRec &rec, const char *name, int nargs,
   h.csr_offs = co.data();
   h.csr = csr.data();
   h.csr4 = P.csr4 ? c4.data() : nullptr;
+  for (int m = 0; m < nargs; ++m)
+    if (P.inc_of_arg[m] >= 0) h.elem_key = old_index_host(args[m].map->from);
+  h.instances_by_key = rec.key_instances ? 1 : 0;
   verdict(op2h_check_owner_plan(&h), "owner", name);
 }
 
@@ -454,7 +468,7 @@ void check_gather_or_exit(const GatherPlanRec &rec, op_map map) {
   auto ent = fetch(P.entries, (size_t)P.from_size * P.dim);
   auto tg = fetch(P.targets, (size_t)P.ntargets);
   verdict(op2h_check_gather_plan(P.from_size, P.to_size, P.dim, map->map, P.ntargets,
-                                 off.data(), ent.data(), tg.data()),
+                                 off.data(), ent.data(), tg.data(), old_index_host(map->from)),
           "gather", map->name);
 }
 

@@ -164,6 +164,13 @@ struct Runtime {
   int history_capacity = 0;  // launches
   std::map<op_map, long long> distinct_targets;  // per map, for byte counts
   std::vector<std::vector<int>> set_new_of_old;  // composed op_renumber perms
+  // transparent renumbering (OP_HIP_RENUMBER): before the first loop the
+  // sets are renumbered as op_renumber does, but the application keeps its
+  // own numbering: fetches, dumps and element indices are translated
+  bool transparent = false, transparent_done = false;
+  std::string transparent_map;                    // base map named by the token
+  std::vector<std::vector<int>> set_old_of_new;   // per renumbered set
+  std::vector<int *> old_index_d;                 // device copies of those
   std::map<std::string, std::pair<void *, size_t>> user_plans;
   // autotuning
   bool autotune = false;
@@ -174,6 +181,14 @@ struct Runtime {
   std::map<std::string, ModelLoop> model_loops;
   std::vector<op2h_tuning_record> tune_records;
   hipEvent_t tune_ev[2] = {nullptr, nullptr};
+  // a kernel launch a stub left pending for the next stub to fuse
+  // (op2h_defer); flushed by everything that is not that stub
+  op2h_flush_fn pend_fn = nullptr;
+  void *pend_arg = nullptr;
+  std::string pend_tag, accept_tag;
+  bool fusion = true;  // op2h_set_fusion; kept across op_init/op_exit
+  unsigned plan_epoch = 1;  // advanced whenever the plans are dropped
+  unsigned long long fused = 0;  // pending launches taken over by a later stub
   // the current loop's node in the op_profile tree (loops do not nest)
   bool prof_loop = false;
   double prof_t0 = 0.0, prof_comp_t0 = 0.0, prof_kms0 = 0.0;
@@ -183,6 +198,19 @@ struct Runtime {
 Runtime &rt() {
   static Runtime R;
   return R;
+}
+
+const char *app_rows(op_dat dat, std::vector<char> &buf);
+void transparent_renumber();
+
+// The device numbering of a set under transparent renumbering (new_of_old),
+// or null when the set keeps the application's.
+const int *device_number(op_set set) {
+  Runtime &R = rt();
+  if (!R.transparent || !R.transparent_done || (size_t)set->index >= R.set_new_of_old.size() ||
+      R.set_new_of_old[set->index].empty())
+    return nullptr;
+  return R.set_new_of_old[set->index].data();
 }
 
 void ensure_init() {
@@ -365,6 +393,10 @@ extern "C" void op_init(int argc, char **argv, int diags) {
     else if (!std::strcmp(a, "OP_AUTO_SOA")) OP_auto_soa = 1;
     else if (!std::strncmp(a, "OP_MAPS_BASE_INDEX=", 19)) OP_maps_base_index = atoi(a + 19);
     else if (!std::strcmp(a, "OP_HIP_KERNEL_TIMING")) op2h_enable_kernel_timing(1);
+    else if (!std::strcmp(a, "OP_HIP_RENUMBER") || !std::strncmp(a, "OP_HIP_RENUMBER=", 16)) {
+      rt().transparent = true;
+      rt().transparent_map = a[15] == '=' ? std::string(a + 16) : std::string();
+    }
     else if (!std::strncmp(a, "OP_HIP_PLAN=", 12)) {
       // OP_HIP_PLAN=<loop>:<plan>[:<block>]
       std::string s(a + 12);
@@ -393,10 +425,12 @@ extern "C" void op_init_soa(int argc, char **argv, int diags, int soa) {
 extern "C" void op_exit(void) {
   Runtime &R = rt();
   if (!R.initialised) return;
+  op2h_flush_deferred();
   resolve_timers();
   HIPCHECK(hipStreamSynchronize(R.stream));
   op2h_graph_destroy();
   R.staged.clear();
+  ++R.plan_epoch;
   R.color.clear();
   R.owner.clear();
   R.gather.clear();
@@ -440,6 +474,12 @@ extern "C" void op_exit(void) {
   R.stats.clear();
   R.distinct_targets.clear();
   R.set_new_of_old.clear();
+  for (int *p : R.old_index_d)
+    if (p) (void)hipFree(p);
+  R.old_index_d.clear();
+  R.set_old_of_new.clear();
+  R.transparent = R.transparent_done = false;
+  R.transparent_map.clear();
   R.tune.clear();
   R.autotune = false;
   if (R.tune_ev[0]) (void)hipEventDestroy(R.tune_ev[0]);
@@ -509,6 +549,16 @@ extern "C" op_map op_decl_map(op_set from, op_set to, int dim, idx_l_t *imap,
              m->map[i]);
       exit(-1);
     }
+  // declared after transparent renumbering: from the application's numbering
+  // to the device's (entries through the target set's, rows through the
+  // source set's)
+  if (const int *pt = device_number(to))
+    for (size_t i = 0; i < n; ++i) m->map[i] = pt[m->map[i]];
+  if (const int *pf = device_number(from)) {
+    std::vector<int> tmp(m->map, m->map + n);
+    for (int i = 0; i < from->size; ++i)
+      std::memcpy(m->map + (size_t)pf[i] * dim, tmp.data() + (size_t)i * dim, dim * sizeof(int));
+  }
   HIPCHECK(hipMalloc(&m->map_d, std::max(n, (size_t)1) * sizeof(int)));
   HIPCHECK(hipMemcpy(m->map_d, m->map, n * sizeof(int), hipMemcpyHostToDevice));
   rt().maps.push_back(m);
@@ -541,7 +591,14 @@ extern "C" op_dat op_decl_dat_char(op_set set, int dim, char const *type,
   d->size = dim * size;
   size_t bytes = (size_t)d->size * set->size;
   d->data = (char *)std::calloc(std::max(bytes, (size_t)1), 1);
-  if (data) std::memcpy(d->data, data, bytes);
+  if (data) {
+    if (const int *p = device_number(set)) {  // rows to the device numbering
+      for (int i = 0; i < set->size; ++i)
+        std::memcpy(d->data + (size_t)p[i] * d->size, data + (size_t)i * d->size, d->size);
+    } else {
+      std::memcpy(d->data, data, bytes);
+    }
+  }
   d->name = copy_str(name);
   d->type = copy_str(type);
   d->dirtybit = 1;
@@ -731,8 +788,10 @@ extern "C" void op_fetch_data_char(op_dat dat, char *usr_ptr) {
     printf("op_fetch_data error -- not allowed during graph capture\n");
     exit(-1);
   }
+  op2h_flush_deferred();
   if (dat->dirty_hd == 2) download_dat(dat);
-  std::memcpy(usr_ptr, dat->data, (size_t)dat->size * dat->set->size);
+  std::vector<char> buf;
+  std::memcpy(usr_ptr, app_rows(dat, buf), (size_t)dat->size * dat->set->size);
 }
 
 extern "C" void op_fetch_data_idx_char(op_dat dat, char *usr_ptr, int low,
@@ -742,8 +801,10 @@ extern "C" void op_fetch_data_idx_char(op_dat dat, char *usr_ptr, int low,
            dat->set->name);
     exit(-1);
   }
+  op2h_flush_deferred();
   if (dat->dirty_hd == 2) download_dat(dat);
-  std::memcpy(usr_ptr, dat->data + (size_t)low * dat->size,
+  std::vector<char> buf;
+  std::memcpy(usr_ptr, app_rows(dat, buf) + (size_t)low * dat->size,
               (size_t)(high - low + 1) * dat->size);
 }
 
@@ -751,6 +812,7 @@ extern "C" idx_g_t op_get_size(op_set set) { return set->size; }
 
 extern "C" void op_print_dat_to_binfile(op_dat dat, const char *file_name) {
   // op_lib_core.cpp:1283-1307: int count, int dim, then raw AoS data
+  op2h_flush_deferred();
   if (dat->dirty_hd == 2) download_dat(dat);
   FILE *fp = std::fopen(file_name, "wb");
   if (!fp) {
@@ -758,9 +820,11 @@ extern "C" void op_print_dat_to_binfile(op_dat dat, const char *file_name) {
     exit(2);
   }
   int count = dat->set->size, dim = dat->dim;
+  std::vector<char> buf;
+  const char *rows = app_rows(dat, buf);
   if (std::fwrite(&count, sizeof(int), 1, fp) < 1 ||
       std::fwrite(&dim, sizeof(int), 1, fp) < 1 ||
-      std::fwrite(dat->data, dat->size, count, fp) < (size_t)count) {
+      std::fwrite(rows, dat->size, count, fp) < (size_t)count) {
     printf("error writing to %s\n", file_name);
     exit(2);
   }
@@ -769,6 +833,7 @@ extern "C" void op_print_dat_to_binfile(op_dat dat, const char *file_name) {
 
 extern "C" void op_print_dat_to_txtfile(op_dat dat, const char *file_name) {
   // op_lib_core.cpp:1309-1369
+  op2h_flush_deferred();
   if (dat->dirty_hd == 2) download_dat(dat);
   FILE *fp = std::fopen(file_name, "w");
   if (!fp) {
@@ -777,17 +842,19 @@ extern "C" void op_print_dat_to_txtfile(op_dat dat, const char *file_name) {
   }
   std::fprintf(fp, "%d %d\n", dat->set->size, dat->dim);
   int tc = type_code(dat->type);
+  std::vector<char> buf;
+  const char *rows = app_rows(dat, buf);
   for (int i = 0; i < dat->set->size; i++) {
     for (int j = 0; j < dat->dim; j++) {
       size_t k = (size_t)i * dat->dim + j;
       if (tc == 0) {
-        double v = ((double *)dat->data)[k];
+        double v = ((const double *)rows)[k];
         if (v == -0.0) v = +0.0;
         std::fprintf(fp, " %+2.15lE", v);
       } else if (tc == 1) {
-        std::fprintf(fp, " %+f", ((float *)dat->data)[k]);
+        std::fprintf(fp, " %+f", ((const float *)rows)[k]);
       } else if (tc == 2) {
-        std::fprintf(fp, " %+d", ((int *)dat->data)[k]);
+        std::fprintf(fp, " %+d", ((const int *)rows)[k]);
       } else {
         printf("Unknown type %s, cannot be written to file %s\n", dat->type,
                file_name);
@@ -967,8 +1034,10 @@ struct Renumber {
 
 }  // namespace
 
-extern "C" void op_renumber(op_map base) {
+namespace {
+void renumber_impl(op_map base) {
   Runtime &R = rt();
+  op2h_flush_deferred();
   if (R.capturing) {
     printf("op_renumber error -- not allowed during graph capture\n");
     exit(-1);
@@ -1071,6 +1140,7 @@ extern "C" void op_renumber(op_map base) {
   for (op_dat d : R.dats)
     if (d) upload_dat(d);
   R.staged.clear();
+  ++R.plan_epoch;
   R.color.clear();
   R.owner.clear();
   R.gather.clear();
@@ -1086,9 +1156,92 @@ extern "C" void op_renumber(op_map base) {
   }
 }
 
+// OP_HIP_RENUMBER: renumber every set reachable from the base map, once,
+// before the first loop, and keep the inverse permutations for translating
+// back to the application's numbering.  The base map is the one the token
+// names, else the first declared map of dimension >= 2 from the largest set.
+void transparent_renumber() {
+  Runtime &R = rt();
+  R.transparent_done = true;
+  op_map base = nullptr;
+  for (op_map m : R.maps) {
+    if (!R.transparent_map.empty()) {
+      if (R.transparent_map == m->name) base = m;
+    } else if (m->dim >= 2 && (!base || m->from->size > base->from->size)) {
+      base = m;
+    }
+  }
+  if (!base) {
+    if (!R.transparent_map.empty()) {
+      printf("op2hip: OP_HIP_RENUMBER names map %s, which is not declared\n",
+             R.transparent_map.c_str());
+      exit(-1);
+    }
+    return;  // no map to renumber by
+  }
+  if (R.capturing) {
+    printf("op2hip: OP_HIP_RENUMBER: the first loop must run before graph capture\n");
+    exit(-1);
+  }
+  renumber_impl(base);
+  R.set_old_of_new.assign(R.sets.size(), {});
+  R.old_index_d.assign(R.sets.size(), nullptr);
+  for (op_set st : R.sets) {
+    if ((size_t)st->index >= R.set_new_of_old.size() || R.set_new_of_old[st->index].empty())
+      continue;
+    const std::vector<int> &p = R.set_new_of_old[st->index];
+    std::vector<int> &q = R.set_old_of_new[st->index];
+    q.assign(p.size(), 0);
+    for (size_t i = 0; i < p.size(); ++i) q[p[i]] = (int)i;
+    HIPCHECK(hipMalloc(&R.old_index_d[st->index], std::max<size_t>(q.size(), 1) * sizeof(int)));
+    HIPCHECK(hipMemcpy(R.old_index_d[st->index], q.data(), q.size() * sizeof(int),
+                       hipMemcpyHostToDevice));
+  }
+}
+
+// Host data of a dat in the application's numbering: the dat's own rows, or,
+// under transparent renumbering, a permuted copy in `buf`.
+const char *app_rows(op_dat dat, std::vector<char> &buf) {
+  Runtime &R = rt();
+  if (!R.transparent || (size_t)dat->set->index >= R.set_new_of_old.size() ||
+      R.set_new_of_old[dat->set->index].empty())
+    return dat->data;
+  const std::vector<int> &p = R.set_new_of_old[dat->set->index];
+  buf.resize((size_t)dat->size * dat->set->size);
+  for (int i = 0; i < dat->set->size; ++i)
+    std::memcpy(&buf[(size_t)i * dat->size], dat->data + (size_t)p[i] * dat->size, dat->size);
+  return buf.data();
+}
+}  // namespace
+
+extern "C" void op_renumber(op_map base) {
+  if (rt().transparent) {  // the device already works in its own numbering
+    if (OP_diags > 0)
+      op_printf("op_renumber: OP_HIP_RENUMBER is on, the call is a no-op\n");
+    return;
+  }
+  renumber_impl(base);
+}
+
+extern "C" const int *op2h_set_old_index(op_set set) {
+  Runtime &R = rt();
+  if (!R.transparent || (size_t)set->index >= R.old_index_d.size()) return nullptr;
+  return R.old_index_d[set->index];
+}
+
+namespace op2h {
+const int *old_index_host(op_set set) {
+  Runtime &R = rt();
+  if (!R.transparent || (size_t)set->index >= R.set_old_of_new.size() ||
+      R.set_old_of_new[set->index].empty())
+    return nullptr;
+  return R.set_old_of_new[set->index].data();
+}
+}  // namespace op2h
+
 extern "C" int op2h_set_permutation(op_set set, int *new_of_old) {
   Runtime &R = rt();
-  if ((size_t)set->index < R.set_new_of_old.size() &&
+  if (!R.transparent && (size_t)set->index < R.set_new_of_old.size() &&
       !R.set_new_of_old[set->index].empty()) {
     std::copy(R.set_new_of_old[set->index].begin(), R.set_new_of_old[set->index].end(),
               new_of_old);
@@ -1185,7 +1338,59 @@ extern "C" void *op2h_stream(void) {
   return rt().stream;
 }
 
+extern "C" void op2h_flush_deferred(void) {
+  Runtime &R = rt();
+  if (!R.pend_fn) return;
+  op2h_flush_fn fn = R.pend_fn;
+  void *arg = R.pend_arg;
+  R.pend_fn = nullptr;
+  R.pend_arg = nullptr;
+  R.pend_tag.clear();
+  fn(arg);
+}
+
+extern "C" void op2h_defer(const char *tag, op2h_flush_fn flush, void *arg) {
+  Runtime &R = rt();
+  op2h_flush_deferred();  // at most one pending launch
+  R.pend_fn = flush;
+  R.pend_arg = arg;
+  R.pend_tag = tag ? tag : "";
+}
+
+extern "C" void op2h_accept_deferred(const char *tag) { rt().accept_tag = tag ? tag : ""; }
+
+extern "C" int op2h_take_deferred(const char *tag, void **arg) {
+  Runtime &R = rt();
+  if (arg) *arg = nullptr;
+  if (!R.pend_fn) return 0;
+  if (!tag || R.pend_tag != tag) {
+    op2h_flush_deferred();
+    return 0;
+  }
+  if (arg) *arg = R.pend_arg;
+  R.pend_fn = nullptr;
+  R.pend_arg = nullptr;
+  R.pend_tag.clear();
+  ++R.fused;
+  return 1;
+}
+
+extern "C" unsigned long long op2h_fused_count(void) { return rt().fused; }
+
+extern "C" void op2h_set_fusion(int on) { rt().fusion = on != 0; }
+
+extern "C" unsigned op2h_plan_epoch(void) { return rt().plan_epoch; }
+
+extern "C" int op2h_fusion_enabled(void) {
+  Runtime &R = rt();
+  // fused launches only inside a captured graph, where no per-loop result,
+  // time or tuning sample is taken between the loops
+  return R.fusion && R.capturing && !R.kernel_timing && !R.autotune &&
+         R.sync_mode != OP2H_SYNC_EVERY_LOOP && !op2h::profile_loops_on();
+}
+
 extern "C" void op2h_device_sync(void) {
+  op2h_flush_deferred();
   if (!rt().initialised || rt().capturing) return;
   HIPCHECK(hipStreamSynchronize(rt().stream));
 }
@@ -1485,6 +1690,9 @@ extern "C" void op2h_loop_begin_mask(op2h_loop_ctx *ctx, int kernel_idx,
                                      unsigned plan_mask) {
   Runtime &R = rt();
   ensure_init();
+  if (R.transparent && !R.transparent_done) transparent_renumber();
+  if (R.pend_fn && R.accept_tag != R.pend_tag) op2h_flush_deferred();
+  R.accept_tag.clear();
   std::memset(ctx, 0, sizeof(*ctx));
   roctxRangePushA(name);
   ctx->kernel_idx = kernel_idx;
@@ -1908,12 +2116,15 @@ extern "C" const op2h_gather_plan *op2h_plan_gather(op_map map) {
   return p;
 }
 
-extern "C" const op2h_owner_plan *op2h_plan_owner_for(const char *name,
-                                                      op_set set,
-                                                      int block_cells,
-                                                      int nargs, op_arg *args) {
+namespace {
+const op2h_owner_plan *owner_plan_for(const char *name, op_set set, int block_cells,
+                                      int nargs, op_arg *args, bool key_instances) {
   Runtime &R = rt();
-  std::string sig = arg_signature(name, set, nargs, args, block_cells);
+  // device-ordered instances make a plan of their own only where the two
+  // orders differ (transparent renumbering)
+  if (!op2h::old_index_host(set)) key_instances = true;
+  std::string sig = arg_signature(name, set, nargs, args, block_cells) +
+                    (key_instances ? "" : "/local");
   for (auto &p : R.owner)
     if (p->signature == sig) return &p->plan;
   if (R.capturing) {
@@ -1922,7 +2133,8 @@ extern "C" const op2h_owner_plan *op2h_plan_owner_for(const char *name,
   }
   double t0 = wall_now();
   std::string err;
-  auto rec = op2h::build_owner_plan(set, block_cells, nargs, args, R.stream, err);
+  auto rec = op2h::build_owner_plan(set, block_cells, nargs, args, R.stream, err,
+                                    key_instances);
   if (!rec) {
     printf("op2hip: %s: %s\n", name, err.c_str());
     return nullptr;
@@ -1933,10 +2145,21 @@ extern "C" const op2h_owner_plan *op2h_plan_owner_for(const char *name,
   R.owner.push_back(std::move(rec));
   return &R.owner.back()->plan;
 }
+}  // namespace
+
+extern "C" const op2h_owner_plan *op2h_plan_owner_for(const char *name, op_set set,
+                                                      int block_cells, int nargs,
+                                                      op_arg *args) {
+  return owner_plan_for(name, set, block_cells, nargs, args, true);
+}
 
 extern "C" const op2h_owner_plan *op2h_plan_owner(op2h_loop_ctx *ctx) {
   return op2h_plan_owner_for(ctx->name, ctx->set, ctx->part_size, ctx->nargs,
                              ctx->args);
+}
+
+extern "C" const op2h_owner_plan *op2h_plan_owner_local(op2h_loop_ctx *ctx) {
+  return owner_plan_for(ctx->name, ctx->set, ctx->part_size, ctx->nargs, ctx->args, false);
 }
 
 extern "C" int op2h_owner_plan_download(const op2h_owner_plan *p,
@@ -2051,6 +2274,7 @@ extern "C" int op2h_graph_begin(void) {
   Runtime &R = rt();
   ensure_init();
   if (R.capturing) return -1;
+  op2h_flush_deferred();
   op2h_graph_destroy();
   resolve_timers();
   HIPCHECK(hipStreamSynchronize(R.stream));
@@ -2083,6 +2307,7 @@ extern "C" int op2h_graph_begin(void) {
 extern "C" int op2h_graph_end(void) {
   Runtime &R = rt();
   if (!R.capturing) return -1;
+  op2h_flush_deferred();  // the last pending launch belongs to the graph
   size_t words = 0;
   HistSources src;
   std::memset(&src, 0, sizeof(src));

@@ -44,11 +44,14 @@ def run(mesh, niter, mode="plain", warmup=0, alpha_mode=1, soa=False,
         renumber=False, mixed=False, autotune=0, graph_steps=1, dump=None):
     """Run the airfoil app; returns a dict of numpy arrays and timings.  With
     `dump` (a path prefix) the final q is also written to <dump>.dat and
-    <dump>.bin, and a renumbered run's cell permutation to <dump>.perm."""
+    <dump>.bin, and a renumbered run's cell permutation to <dump>.perm.
+    renumber: False; True (op_renumber(pecell), the application sees the new
+    numbering); "transparent" (the device alone works in that numbering)."""
     o = Opts(niter, warmup, alpha_mode, MODES[mode], int(soa),
              PLAN_NAMES[plan_res], block_res, PLAN_NAMES[plan_bres], block_bres,
              block_direct, block_adt, int(kernel_timing), int(plan_on_device),
-             int(renumber), int(mixed), int(autotune), int(graph_steps),
+             {False: 0, True: 1, "transparent": 2}[renumber], int(mixed), int(autotune),
+             int(graph_steps),
              None if dump is None else str(dump).encode())
     rms = np.zeros(niter)
     rms_k0 = np.zeros(niter)

@@ -191,7 +191,8 @@ class op2h_owner_plan_host(C.Structure):
                 ("ninc", C.c_int), ("map_table", _ip), ("map_dim", C.c_int),
                 ("inc_idx", C.c_int * 32), ("owned_offs", _ip), ("owned", _ip),
                 ("elem_offs", _ip), ("elems", _ip), ("csr_offs", _ip),
-                ("csr", C.POINTER(C.c_ushort)), ("csr4", C.POINTER(C.c_ushort))]
+                ("csr", C.POINTER(C.c_ushort)), ("csr4", C.POINTER(C.c_ushort)),
+                ("elem_key", _ip), ("instances_by_key", C.c_int)]
 
 
 op2h_check_staged_plan = _fn("op2h_check_staged_plan", C.c_int,
@@ -202,7 +203,10 @@ op2h_check_color_plan = _fn("op2h_check_color_plan", C.c_int, C.c_int, C.c_int, 
 op2h_check_owner_plan = _fn("op2h_check_owner_plan", C.c_int,
                             C.POINTER(op2h_owner_plan_host))
 op2h_check_gather_plan = _fn("op2h_check_gather_plan", C.c_int, C.c_int, C.c_int, C.c_int,
-                             _ip, C.c_int, _ip, _ip, _ip)
+                             _ip, C.c_int, _ip, _ip, _ip, _ip)
+op2h_set_old_index = _fn("op2h_set_old_index", C.c_void_p, op_set)
+op2h_set_fusion = _fn("op2h_set_fusion", None, C.c_int)
+op2h_fused_count = _fn("op2h_fused_count", C.c_ulonglong)
 
 
 op2h_rcm_order = _fn("op2h_rcm_order", C.c_int, C.c_int, C.POINTER(C.c_int),

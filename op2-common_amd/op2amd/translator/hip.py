@@ -148,6 +148,11 @@ class LoopGen:
             elif isinstance(a, ArgIdx) and a.map:
                 f.append(f"  const int *m{j}; int md{j}; const int *r{j};")
                 f.append(f"  const unsigned short *h{j}; const int *hb{j};")
+            if isinstance(a, ArgIdx):
+                # the application's index of each element of the indexed set
+                # under transparent renumbering (OP_HIP_RENUMBER), else null
+                f.append(f"  const int *oi{j};")
+        f.append("  const int *on;  // the loop set's application indices (OP_HIP_RENUMBER), or null")
         f.append("  int _unused;")
         return f"struct {self.name}_args {{\n" + "\n".join(f) + "\n};\n"
 
@@ -243,6 +248,7 @@ class LoopGen:
                     pre.append(f"    int x{j} = A.m{m}[(size_t)n * A.md{m} + {a.idx}];")
                 else:
                     pre.append(f"    int x{j} = n;")
+                pre.append(f"    if (A.oi{j}) x{j} = A.oi{j}[x{j}];")
                 call.append(f"&x{j}")
             elif isinstance(a, ArgInfo):
                 call.append(f"v{j}")
@@ -261,7 +267,11 @@ class LoopGen:
         for j in self.reds:
             if j in self.info_of:
                 track.append(f"    const {L.args[j].typ} before{j} = r{j}[0];")
-        track_after = [f"    if (r{j}[0] != before{j}) at{j} = n;" for j in self.reds if j in self.info_of]
+        # the element a MIN/MAX result came from, for ties: the application's
+        # index under transparent renumbering, so the first element in its
+        # numbering wins as in the sequential executor
+        track_after = [f"    if (r{j}[0] != before{j}) at{j} = A.on ? A.on[n] : n;"
+                       for j in self.reds if j in self.info_of]
         body = []
         body.append(f"template <int LAY>")
         body.append(f"__global__ void __launch_bounds__(1024)")
@@ -368,6 +378,7 @@ class LoopGen:
                     pre.append(f"    int x{j} = op2gen_row(A.r{m}, A.h{m}, hb{m}, (size_t)(e0 + j) * A.md{m} + {a.idx});")
                 else:
                     pre.append(f"    int x{j} = n;")
+                pre.append(f"    if (A.oi{j}) x{j} = A.oi{j}[x{j}];")
                 call.append(f"&x{j}")
         w = incs[0]
         at = f"(LAY == 1 || (LAY == 2 && A.s{w})) ? t + (size_t)k * A.s{w} : (size_t)t * {D} + k"
@@ -503,6 +514,7 @@ class LoopGen:
         s.append(f"    hipStream_t st = (hipStream_t)ctx.stream;")
         s.append(f"    {self.name}_args A;")
         s.append(f"    memset(&A, 0, sizeof(A));")
+        s.append(f"    A.on = op2h_set_old_index(set);")
         for j, a in enumerate(L.args):
             if isinstance(a, ArgDat):
                 s.append(f"    A.opt{j} = args[{j}].opt;")
@@ -518,6 +530,9 @@ class LoopGen:
             elif isinstance(a, ArgIdx) and a.map:
                 s.append(f"    A.m{j} = args[{j}].map_data_d;")
                 s.append(f"    A.md{j} = args[{j}].map->dim;")
+            if isinstance(a, ArgIdx):
+                tgt = f"args[{j}].map->to" if a.map else "set"
+                s.append(f"    A.oi{j} = op2h_set_old_index({tgt});")
         # the kernels are instantiated for all-AoS, all-SoA and mixed layouts
         # of the loop's multi-component dats
         s.append(f"    int layout = -1;  // 0 AoS, 1 SoA, 2 mixed, -1 no multi-component dat")

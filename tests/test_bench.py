@@ -46,7 +46,12 @@ def test_parity_check_flags_a_wrong_iteration():
     late["rms"][517] *= 1 + 1e-9
     qbad = {"rms": orc["rms"], "q": orc["q"].copy()}
     qbad["q"][3, 2] += 2e-9 * 2.0
-    res = b.parity({"good": good, "late": late, "qbad": qbad}, orc)
+    nan = {"rms": orc["rms"].copy(), "q": orc["q"].copy()}
+    nan["rms"][100] = np.nan
+    nan["q"][0, 0] = np.nan
+    res = b.parity({"good": good, "late": late, "qbad": qbad, "nan": nan}, orc)
     assert max(res["good"].values()) <= b.PARITY_TOL
     assert res["late"]["rms_rel_err_max"] > b.PARITY_TOL
     assert res["qbad"]["q_rel_err"] > b.PARITY_TOL and res["qbad"]["rms_rel_err_max"] == 0.0
+    assert not res["nan"]["rms_rel_err_max"] <= b.PARITY_TOL
+    assert not res["nan"]["q_rel_err"] <= b.PARITY_TOL

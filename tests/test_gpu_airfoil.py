@@ -169,3 +169,70 @@ def test_graphed_reductions_equal_eager_ones():
     assert np.array_equal(eager["maxerr"], graphed["maxerr"])
     assert np.array_equal(eager["errloc"], graphed["errloc"])
     assert np.array_equal(eager["q"], graphed["q"])
+
+
+@pytest.mark.parametrize("case", [
+    dict(mode="graphed", plan_res="owner", block_res=128, plan_bres="gather", graph_steps=8),
+    dict(mode="tuned", plan_res="gather", plan_bres="gather", soa=True),
+    dict(mode="plain"),
+    dict(mode="graphed", plan_res="owner", block_res=128, plan_bres="atomic")],
+    ids=["owner-gather-graphed", "gather-soa", "plain", "owner-atomic-graphed"])
+def test_transparent_renumbering_keeps_the_application_numbering(case, tmp_path):
+    """OP_HIP_RENUMBER=pecell: the device works in the RCM numbering, the
+    application (the randomly numbered mesh) keeps its own: q fetched and
+    dumped, and every iteration's errloc, are in the application's numbering;
+    with owner/gather increments (summed in the application's element order)
+    q and maxerr are the reference's bits, and no permutation file is
+    written."""
+    g = golden("airfoil_120x60_random_a1")
+    r = op2amd.airfoil.run(mesh_of(g), 1000, alpha_mode=1, renumber="transparent",
+                           dump=tmp_path / "q", **case)
+    assert pointwise_rel_err(r["rms"], g["rms"]) <= TOL
+    assert rel_err(r["q"], g["q"]) <= TOL
+    assert not (tmp_path / "q.perm").exists()
+    raw = (tmp_path / "q.bin").read_bytes()
+    dumped = np.frombuffer(raw[8:], dtype=np.float64).reshape(-1, 4)
+    assert np.array_equal(dumped, r["q"])
+    if case.get("plan_bres") == "gather":
+        assert np.array_equal(r["q"], g["q"])
+        assert np.array_equal(r["maxerr"], g["maxerr"])
+        assert np.array_equal(r["errloc"], g["errloc"])
+    else:
+        same = r["maxerr"] == g["maxerr"]
+        assert np.array_equal(r["errloc"][same], g["errloc"][same])
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(soa=True), dict(renumber="transparent"),
+                                dict(plan_bres="atomic")],
+                         ids=["aos", "soa", "renumbered", "bres-atomic"])
+def test_fused_loop_chains_equal_the_unfused_loops(kw):
+    """Inside a captured step the direct loops of a cell run per cell in one
+    kernel ([update] [save_soln] adt_calc) and res_calc + bres_calc in one
+    owner launch: every iteration's rms, maxerr and errloc and the final q
+    are the bits of the unfused graph, and q is the reference's bits (with
+    bres_calc on atomics the eager warm-up steps sum corner cells' boundary
+    fluxes in the order the atomics land: q within 1e-10)."""
+    import op2amd.api as api
+    g = golden("airfoil_120x60_random_a1")
+    base = dict(mode="graphed", plan_res="owner", block_res=128, plan_bres="gather",
+                graph_steps=10, warmup=10)
+    base.update(kw)
+    n0 = api.op2h_fused_count()
+    fused = op2amd.airfoil.run(mesh_of(g), 1000, **base)
+    n1 = api.op2h_fused_count()
+    # captured once: per step two direct chains and two res_calc + bres_calc
+    assert n1 - n0 >= 4 * fused["graph_steps"]
+    api.op2h_set_fusion(0)
+    try:
+        unfused = op2amd.airfoil.run(mesh_of(g), 1000, **base)
+    finally:
+        api.op2h_set_fusion(1)
+    assert api.op2h_fused_count() == n1
+    if kw.get("plan_bres") != "atomic":
+        assert np.array_equal(fused["q"], g["q"])
+        assert np.array_equal(fused["errloc"], g["errloc"])
+        for k in ("rms", "rms_k0", "maxerr", "errloc", "q"):
+            assert np.array_equal(fused[k], unfused[k]), k
+    else:  # the eager warm-up steps add corner cells' boundary fluxes by atomics
+        assert rel_err(fused["q"], g["q"]) <= TOL
+    assert pointwise_rel_err(fused["rms"], g["rms"]) <= TOL

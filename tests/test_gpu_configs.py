@@ -67,8 +67,9 @@ def test_config_b_720k_1000_iterations(ordering):
     assert np.array_equal(exact["maxerr"], seq["maxerr"])
     assert np.array_equal(exact["errloc"], seq["errloc"])
     # rms: the same squares summed by a tree instead of in element order
-    assert pointwise_rel_err(exact["rms"], seq["rms"]) <= 1e-13
-    assert pointwise_rel_err(exact["rms_k0"], seq["rms_k0"]) <= 1e-13
+    # (measured 2.7e-13 over 2.88M terms)
+    assert pointwise_rel_err(exact["rms"], seq["rms"]) <= 2e-12
+    assert pointwise_rel_err(exact["rms_k0"], seq["rms_k0"]) <= 2e-12
 
 
 @pytest.fixture(scope="module")
@@ -104,7 +105,8 @@ def test_config_c_11m_loops_bit_exact(mesh_c, plan, block):
     assert np.array_equal(res2, res2_o)
     qn_o, rn_o, rms_o, mx_o, loc_o = oracle.update(q, res2_o, adt_o, 0.0, 0.0, 0)
     assert np.array_equal(qn, qn_o) and np.array_equal(rn, rn_o)
-    assert abs(rms.value - rms_o) <= 1e-13 * rms_o
+    # the same 46M squares summed by a tree (measured 2.8e-13 relative)
+    assert abs(rms.value - rms_o) <= 2e-12 * rms_o
     assert maxerr.value == mx_o and errloc.value == loc_o
 
 
@@ -112,8 +114,8 @@ def test_config_c_11m_mixed_precision_2000_iterations(mesh_c):
     """FP32 fluxes, adt and res with FP64 q, update and reductions, against
     the FP64 run of the same plans (whose per-call results equal the
     sequential oracle's bit for bit, above) over 2000 iterations.  Measured on
-    MI355X: rms (pointwise, every iteration) and q errors are printed; the
-    bounds hold them with a margin."""
+    MI355X: rms 1.7e-6 (pointwise max over the iterations), 1.6e-6 at the
+    last, q 1.9e-6 (relative to max |q|); the bounds leave a 5x margin."""
     common = dict(plan_res="owner", block_res=256, plan_bres="atomic", block_bres=128,
                   block_direct=1024, block_adt=1024, graph_steps=10)
     fp64 = op2amd.airfoil.run(mesh_c, 2000, mode="graphed", warmup=20, **common)
@@ -123,7 +125,7 @@ def test_config_c_11m_mixed_precision_2000_iterations(mesh_c):
     e_q = rel_err(mixed["q"], fp64["q"])
     print(f"config C mixed vs FP64: rms max {e_rms:.3e}, rms last {e_last:.3e}, q {e_q:.3e}")
     assert np.all(np.isfinite(mixed["rms"])) and np.all(np.isfinite(mixed["q"]))
-    assert e_rms <= 1e-4
+    assert e_rms <= 1e-5
     assert e_last <= 1e-5
     assert e_q <= 1e-5
 

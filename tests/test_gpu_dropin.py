@@ -52,7 +52,8 @@ def read_bin(path):
     return np.frombuffer(raw[8:], dtype=np.float64).reshape(count, dim)
 
 
-@pytest.mark.parametrize("tokens", [[], ["OP_AUTO_SOA"]], ids=["aos", "soa"])
+@pytest.mark.parametrize("tokens", [[], ["OP_AUTO_SOA"], ["OP_HIP_RENUMBER=pecell"]],
+                         ids=["aos", "soa", "renumbered"])
 def test_unmodified_reference_app_bit_identical(tmp_path, tokens):
     g, mesh, out = run_app(tmp_path, *tokens)
     q = read_bin(tmp_path / "out_grid_seq.bin")

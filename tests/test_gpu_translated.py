@@ -10,7 +10,8 @@
   history; the final q read back from its op_print_dat_to_binfile dump.  The
   owner-partition plans (the default, and the graphed run) sum each cell's
   increments in the sequential order: their q and every iteration's maxerr
-  and errloc are the reference's bits.  Atomic and coloured increments sum
+  and errloc are the reference's bits, also when the device works in its own
+  RCM numbering (OP_HIP_RENUMBER) while the program keeps the mesh's.  Atomic and coloured increments sum
   in another order: q within 1e-10, and errloc wherever maxerr came out
   identical."""
 import subprocess
@@ -34,8 +35,8 @@ def run(name, *args, timeout=120):
 
 
 @pytest.mark.parametrize("variant", [[], ["OP_AUTO_SOA"], ["OP_HIP_PLAN=*:color"],
-                                     ["OP_HIP_PLAN=*:atomic"]],
-                         ids=["default", "soa", "color", "atomic"])
+                                     ["OP_HIP_PLAN=*:atomic"], ["OP_HIP_RENUMBER"]],
+                         ids=["default", "soa", "color", "atomic", "renumbered"])
 @pytest.mark.parametrize("name", ["gbl", "reductions", "idx", "strides", "consts", "helpers"])
 def test_functional_program(name, variant):
     r = run(name, *variant)
@@ -44,8 +45,10 @@ def test_functional_program(name, variant):
 
 
 @pytest.mark.parametrize("variant", [[], ["OP_HIP_PLAN=*:atomic"], ["OP_HIP_PLAN=*:color"],
-                                     ["OP_AUTO_SOA"], ["graph=10"]],
-                         ids=["owner", "atomic", "color", "soa", "graphed"])
+                                     ["OP_AUTO_SOA"], ["graph=10"], ["OP_HIP_RENUMBER=pecell"],
+                                     ["graph=10", "OP_HIP_RENUMBER=pecell"]],
+                         ids=["owner", "atomic", "color", "soa", "graphed", "renumbered",
+                              "renumbered-graphed"])
 def test_translated_airfoil_matches_reference_trajectory(variant, tmp_path):
     g = golden("airfoil_120x60_random_a1")
     mesh = op2amd.mesh.generate(int(g["I"]), int(g["J"]), 0, str(g["ordering"]))
@@ -63,7 +66,7 @@ def test_translated_airfoil_matches_reference_trajectory(variant, tmp_path):
     raw = dump.read_bytes()
     count, dim = np.frombuffer(raw[:8], dtype=np.int32)
     q = np.frombuffer(raw[8:], dtype=np.float64).reshape(count, dim)
-    exact = variant in ([], ["graph=10"])
+    exact = not any(v.startswith(("OP_HIP_PLAN", "OP_AUTO_SOA")) for v in variant)
     if exact:
         assert np.array_equal(q, g["q"])
         assert np.array_equal(maxerr, g["maxerr"])  # printed with 17 digits: exact

@@ -273,8 +273,15 @@ def test_gather_plan_checked(mesh, capfd):
     offs = np.concatenate([[0], np.cumsum(np.bincount(flat, minlength=mesh.ncell)[targets])])
     offs = offs.astype(np.int32)
     args = (mesh.nedge, mesh.ncell, 2, iptr(flat), len(targets), iptr(offs))
-    assert api.op2h_check_gather_plan(*args, iptr(order), iptr(targets)) == 0
+    assert api.op2h_check_gather_plan(*args, iptr(order), iptr(targets), None) == 0
     bad = order.copy()
     bad[0], bad[offs[1]] = bad[offs[1]], bad[0]  # an entry under another cell
-    assert api.op2h_check_gather_plan(*args, iptr(bad), iptr(targets)) == 1
+    assert api.op2h_check_gather_plan(*args, iptr(bad), iptr(targets), None) == 1
     assert "OP_plan_check: gather entries error" in capfd.readouterr().out
+    # with an element key the entries must follow the key's order instead
+    key = np.random.default_rng(3).permutation(mesh.nedge).astype(np.int32)
+    keyed = np.concatenate([sorted(order[offs[i]:offs[i + 1]], key=lambda e: (key[e // 2], e % 2))
+                            for i in range(len(targets))]).astype(np.int32)
+    assert api.op2h_check_gather_plan(*args, iptr(keyed), iptr(targets), iptr(key)) == 0
+    assert api.op2h_check_gather_plan(*args, iptr(order), iptr(targets), iptr(key)) == 1
+    capfd.readouterr()
