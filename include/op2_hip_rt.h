@@ -329,6 +329,34 @@ void op2h_set_plan_builder(int on_device);
 int op2h_stream_probe(size_t bytes, int variant, int reps, double *gbs,
                       double *us_per_call);
 
+/* ---- the update loop without allocating ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the update loop of op_par_loop_airfoil_5_update (update.h:1-19)
+ * on caller-owned device arrays of ncell cells (qold, q, res: 4 doubles per
+ * cell, AoS when soa_stride is 0, else SoA with that stride; adt: 1 double),
+ * with the same kernels, so the same bits for the same block size.  The
+ * loop's own reductions over the cells (identity-started: rms sum 0, maxerr
+ * -infinity) go to the caller's device struct `result`, on `stream` (NULL:
+ * the default stream); the caller combines them with its own values.  It
+ * allocates nothing: its per-workgroup partials (at most 2048, 20 bytes each)
+ * live in a 40 KiB device array of the library, one per device, shared by all
+ * calls on that device — calls that overlap on one device race.  No memset
+ * and no host synchronisation, so the call can be captured into a graph.
+ * block: 64..1024, a multiple of 64; ncell: any non-negative int.
+ * Returns 0, or a negative code after printing the reason: -1 bad arguments,
+ * -2 no current device or its scratch address, -3 partials beyond the
+ * scratch, -4 launch failure. */
+typedef struct {
+  double rms;    /* sum over the cells of del^2 */
+  double maxerr; /* the largest del^2 */
+  int errloc;    /* the lowest cell reaching it */
+  int pad;
+} op2h_update_result;
+int op2h_update_noalloc(int ncell, int soa_stride, int block, const double *qold,
+                        double *q, double *res, const double *adt,
+                        op2h_update_result *result, void *stream);
+
 /* ---- plan checks (op_plan_check, op2/src/core/op_rt_support.cpp:129-295) ----
  * With OP_diags > 0 the runtime checks every plan it builds on the host and
  * stops the program (exit(-1), after the messages) when a check fails: a

@@ -1867,8 +1867,8 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
   if (!deferred) {
     slot_rms = (double *)op2h_gbl_slot(&ctx, 4, -1);
     slot_max = (double *)op2h_gbl_slot(&ctx, 5, 7);
-    char *scratch = (char *)op2h_reduct_scratch((size_t)g * 24);
-    part_sum = (double *)scratch;
+    char *partials = (char *)op2h_reduct_scratch((size_t)g * 24);
+    part_sum = (double *)partials;
     part_max = part_sum + g;
     part_loc = (int *)(part_max + g);
   }
@@ -1921,4 +1921,73 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
   KCHECK(hipGetLastError());
   account(4, set, 8, args);
   op2h_loop_end(&ctx);
+}
+
+// ---------------------------------------------------------------------------
+// The update loop on caller-owned device arrays without allocating: the same
+// kernels as op_par_loop_airfoil_5_update (same workgroup count for a block
+// size, so the same reduction tree and the same bits), with the per-workgroup
+// partials in a file-scope device array instead of runtime-allocated memory.
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, serving loops, harnesses that forbid it).  Every call on one device
+// shares that device's array: calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kNoAllocMaxParts = 2048;  // the stub's workgroup cap
+constexpr size_t kNoAllocScratchBytes = (size_t)kNoAllocMaxParts * (8 + 8 + 4);  // 40 KiB
+constexpr int kNoAllocMaxDevices = 64;
+void *g_scratch_addr[kNoAllocMaxDevices];  // device address of `scratch`, per device
+}  // namespace
+
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+extern "C" int op2h_update_noalloc(int ncell, int soa_stride, int block, const double *qold,
+                                   double *q, double *res, const double *adt,
+                                   op2h_update_result *result, void *stream) {
+  if (ncell < 0 || block < 64 || block > 1024 || block % 64 != 0 || !result ||
+      (ncell > 0 && (!qold || !q || !res || !adt)) ||
+      (soa_stride != 0 && soa_stride < ncell)) {
+    printf("op2h_update_noalloc: bad arguments (ncell %d, soa_stride %d, block %d)\n", ncell,
+           soa_stride, block);
+    return -1;
+  }
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices) {
+    printf("op2h_update_noalloc: no usable current device\n");
+    return -2;
+  }
+  if (!g_scratch_addr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      printf("op2h_update_noalloc: hipGetSymbolAddress failed: %s\n", hipGetErrorString(e));
+      return -2;
+    }
+    g_scratch_addr[dev] = ptr;
+  }
+  const int g = std::max(1, std::min(grid_for(ncell, block), kNoAllocMaxParts));
+  if ((size_t)g * (8 + 8 + 4) > kNoAllocScratchBytes) {
+    printf("op2h_update_noalloc: %d partials exceed the %zu-byte scratch\n", g,
+           kNoAllocScratchBytes);
+    return -3;
+  }
+  double *part_sum = (double *)g_scratch_addr[dev];
+  double *part_max = part_sum + g;
+  int *part_loc = (int *)(part_max + g);
+  hipStream_t st = (hipStream_t)stream;
+  // every partial the finish kernel reads is written by this call's k_update
+  if (soa_stride)
+    k_update<true><<<g, block, 0, st>>>(qold, q, res, adt, ncell, soa_stride, part_sum,
+                                        part_max, part_loc, nullptr);
+  else
+    k_update<false><<<g, block, 0, st>>>(qold, q, res, adt, ncell, 0, part_sum, part_max,
+                                         part_loc, nullptr);
+  k_update_finish<<<1, kFinishThreads, 0, st>>>(part_sum, part_max, part_loc, g, &result->rms,
+                                               &result->maxerr);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    printf("op2h_update_noalloc: launch failed: %s\n", hipGetErrorString(e));
+    return -4;
+  }
+  return 0;
 }
