@@ -69,6 +69,37 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+STREAM_VARIANTS = ("copy, one 16-byte vector per thread", "copy, four per thread",
+                   "copy, four per thread, non-temporal", "copy, eight per thread",
+                   "copy, persistent grid striding four per iteration",
+                   "read only, four per thread", "write only, four per thread")
+
+
+def stream_ceiling(op2amd, size=1 << 30, reps=20):
+    """The achievable HBM streaming rate on this device: copy (and read-only,
+    write-only) kernels of several shapes over 1 GiB buffers, the fastest of
+    `reps` calls each (op2h_stream_probe).  Returns (best copy GB/s, all)."""
+    import ctypes as C
+    lib = op2amd._lib.lib
+    lib.op2h_stream_probe.restype = C.c_int
+    lib.op2h_stream_probe.argtypes = [C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
+    res = {}
+    for v, name in enumerate(STREAM_VARIANTS):
+        g, us = C.c_double(), C.c_double()
+        if lib.op2h_stream_probe(size, v, reps, C.byref(g), C.byref(us)) == 0:
+            res[name] = round(g.value, 1)
+    best = max(v for k, v in res.items() if k.startswith("copy"))
+    return best, res
+
+
+def with_fractions(per_loop, ceiling):
+    for v in per_loop.values():
+        v["frac_of_peak"] = round(v["achieved_GBs"] / HBM_PEAK_GBS, 4)
+        v["frac_of_stream_ceiling"] = round(v["achieved_GBs"] / ceiling, 4)
+    return per_loop
+
+
 def rel(a, b):
     a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
     return float(np.max(np.abs(a - b) / np.abs(b)))
@@ -210,8 +241,21 @@ def workload_large(op2amd, args):
     mesh = op2amd.mesh.generate(I, J, 0, args.ordering)
     n, W = args.large_iters, 20
     out = {}
-    common = dict(plan_res="owner", block_res=256, plan_bres="atomic", block_bres=128,
-                  block_direct=1024, block_adt=1024)
+    # block sizes: owner patches of 256 or 512 cells, direct loops 256..1024,
+    # each timed per call with launch-attached events
+    def per_call(kw, loop):
+        t = op2amd.airfoil.run(mesh, 8, mode="tuned", warmup=2, kernel_timing=True, **kw)
+        return t["loop_kernel_ms"][loop] / max(t["loop_calls"][loop], 1)
+    base = dict(plan_res="owner", plan_bres="atomic", block_bres=128)
+    res_t = {b: per_call(dict(base, block_res=b), "res_calc") for b in (256, 512)}
+    block_res = min(res_t, key=res_t.get)
+    adt_t = {b: per_call(dict(base, block_res=block_res, block_adt=b), "adt_calc")
+             for b in (256, 512, 1024)}
+    block_adt = min(adt_t, key=adt_t.get)
+    log(f"  owner patch size: {res_t}; adt_calc block: {adt_t}")
+    out["block_sweep_ms"] = {"res_calc": res_t, "adt_calc": adt_t}
+    common = dict(plan_res="owner", block_res=block_res, plan_bres="atomic", block_bres=128,
+                  block_direct=1024, block_adt=block_adt)
     common["graph_steps"] = args.graph_steps
     ref = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, **common)
     out["fp64_steps_per_s"] = (n - W) / ref["seconds"]
@@ -228,6 +272,10 @@ def workload_large(op2amd, args):
                  "achieved_GBs": round(t["loop_bytes"][lp] / (1e-3 * t["loop_kernel_ms"][lp] /
                                                               max(t["loop_calls"][lp], 1)) / 1e9, 1)}
             for lp in op2amd.airfoil.LOOPS}
+    ceiling, ceiling_all = stream_ceiling(op2amd)
+    out["stream_ceiling_GBs"], out["stream_probe_GBs"] = ceiling, ceiling_all
+    for label in ("fp64", "mixed"):
+        with_fractions(out[f"{label}_per_loop"], ceiling)
     dom = out["fp64_per_loop"]["res_calc"]
     line = {
         "metric": "airfoil steps/s, 11.5M cells, FP64 vs FP32-flux mixed mode",
@@ -553,8 +601,11 @@ def main():
             f"({time.time() - t0:.1f}s)")
     mesh = meshes[args.ordering]
 
+    ceiling, ceiling_all = stream_ceiling(op2amd)
+    log(f"streaming ceiling: {ceiling:.1f} GB/s ({ceiling_all})")
     log(f"headline: {args.ordering} ordering")
     head = measure(op2amd, mesh, K, W, args, plans=plans)
+    with_fractions(head["per_loop"], ceiling)
     value = head["graphed_steps_per_s"]
     head_runs = head.pop("_runs")
 
@@ -666,6 +717,7 @@ def main():
                         f"{args.ordering} ordering, FP64; one step = save_soln + 2 x "
                         "(adt_calc, res_calc, bres_calc, update)",
             "mesh": args.mesh, "ordering": args.ordering,
+            "stream_ceiling_GBs": ceiling, "stream_probe_GBs": ceiling_all,
             "parity_vs_oracle": {
                 "oracle": f"OpenMP executor on {nthreads} threads, same mesh and ordering, "
                           f"all {n_all} iterations (warm-up, eager and graph-replayed)",
