@@ -92,6 +92,18 @@ int op2h_take_deferred(const char *tag, void **arg);
 void op2h_flush_deferred(void);
 void op2h_set_fusion(int on);
 int op2h_fusion_enabled(void);
+/* Non-temporal hints for a kernel's single-use streams (element-indexed
+ * reads and writes no other element of the launch touches): with them the
+ * lines bypass the caches' normal retention, which pays once the data a
+ * loop streams exceeds the 256 MiB last-level cache (its next pass could not
+ * hit it anyway, and the lines the kernel does reuse, such as gathered
+ * nodes, stay cached), and costs a loop whose data would otherwise still be
+ * cached for the next loop.  op2h_nontemporal(bytes) is 1 when a stub
+ * streaming `bytes` should use them: by size (the default, mode -1), never
+ * (0) or always (1), set by op2h_set_nontemporal or the op_init token
+ * OP_HIP_NONTEMPORAL=auto|off|on (a setting kept across op_init/op_exit). */
+void op2h_set_nontemporal(int mode);
+int op2h_nontemporal(size_t stream_bytes);
 /* Pending launches taken over by a later stub since the library loaded. */
 unsigned long long op2h_fused_count(void);
 

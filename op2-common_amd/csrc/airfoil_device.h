@@ -314,6 +314,36 @@ __device__ __forceinline__ void st4(double *__restrict p, int n, int s,
   }
 }
 
+// Non-temporal forms of the element-indexed (single-use) accesses, for
+// loops whose streams exceed the last-level cache
+typedef double nt_double2 __attribute__((ext_vector_type(2)));
+typedef int nt_int4 __attribute__((ext_vector_type(4)));
+
+template <bool SOA>
+__device__ __forceinline__ void ld4_nt(const double *__restrict p, int n, int s,
+                                       double *v) {
+  if (SOA) {
+    v[0] = __builtin_nontemporal_load(p + n);
+    v[1] = __builtin_nontemporal_load(p + (size_t)s + n);
+    v[2] = __builtin_nontemporal_load(p + 2 * (size_t)s + n);
+    v[3] = __builtin_nontemporal_load(p + 3 * (size_t)s + n);
+  } else {
+    const nt_double2 a =
+        __builtin_nontemporal_load(reinterpret_cast<const nt_double2 *>(p) + 2 * (size_t)n);
+    const nt_double2 b =
+        __builtin_nontemporal_load(reinterpret_cast<const nt_double2 *>(p) + 2 * (size_t)n + 1);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+  }
+}
+
+__device__ __forceinline__ int4 ld_int4_nt(const int *__restrict p, int n) {
+  const nt_int4 t = __builtin_nontemporal_load(reinterpret_cast<const nt_int4 *>(p) + n);
+  return make_int4(t.x, t.y, t.z, t.w);
+}
+
 template <bool SOA>
 __device__ __forceinline__ void add4(double *__restrict p, int n, int s,
                                      const double *v) {

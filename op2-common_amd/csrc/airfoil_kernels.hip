@@ -136,23 +136,27 @@ __global__ void k_save_soln(const double2 *__restrict q, double2 *__restrict qol
 }
 
 // ---------------------------------------------------------------------------
-// adt_calc: one thread per cell, nodes gathered through pcell
+// adt_calc: one thread per cell, nodes gathered through pcell.  NT: the
+// cell's own streams (its pcell row, q, adt) non-temporal, the gathered
+// nodes (shared with the neighbouring cells) cached as usual.
 // ---------------------------------------------------------------------------
-template <bool SOA>
+template <bool SOA, bool NT>
 __global__ void k_adt_calc(const double *__restrict x, const double *__restrict q,
                            double *__restrict adt, const int *__restrict pcell,
                            int n, int sx, int sq) {
   int c = xcd_block(kXcds) * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  int4 cn = reinterpret_cast<const int4 *>(pcell)[c];
+  const int4 cn = NT ? ld_int4_nt(pcell, c) : reinterpret_cast<const int4 *>(pcell)[c];
   double x1[2], x2[2], x3[2], x4[2], qc[4], a;
+  if (NT) ld4_nt<SOA>(q, c, sq, qc);
+  else ld4<SOA>(q, c, sq, qc);
   ld2<SOA>(x, cn.x, sx, x1);
   ld2<SOA>(x, cn.y, sx, x2);
   ld2<SOA>(x, cn.z, sx, x3);
   ld2<SOA>(x, cn.w, sx, x4);
-  ld4<SOA>(q, c, sq, qc);
   op2_k::adt_calc(x1, x2, x3, x4, qc, &a);
-  adt[c] = a;
+  if (NT) __builtin_nontemporal_store(a, adt + c);
+  else adt[c] = a;
 }
 
 // ---------------------------------------------------------------------------
@@ -1463,8 +1467,12 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
       launch_k(tm.ext, tm.e0, tm.e1, soa ? k_adt_calc_mixed<true> : k_adt_calc_mixed<false>, g,
                ctx.block_size, 0, st, x, q, adtf, pcell, set->size, sx, sq);
     } else {
-      launch_k(tm.ext, tm.e0, tm.e1, soa ? k_adt_calc<true> : k_adt_calc<false>, g,
-               ctx.block_size, 0, st, x, q, adt, pcell, set->size, sx, sq);
+      // streams: pcell rows, q and adt
+      const bool nt = op2h_nontemporal((size_t)set->size * 56) != 0;
+      launch_k(tm.ext, tm.e0, tm.e1,
+               nt ? (soa ? k_adt_calc<true, true> : k_adt_calc<false, true>)
+                  : (soa ? k_adt_calc<true, false> : k_adt_calc<false, false>),
+               g, ctx.block_size, 0, st, x, q, adt, pcell, set->size, sx, sq);
     }
     tm.done();
     KCHECK(hipGetLastError());

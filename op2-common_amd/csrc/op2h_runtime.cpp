@@ -187,6 +187,7 @@ struct Runtime {
   void *pend_arg = nullptr;
   std::string pend_tag, accept_tag;
   bool fusion = true;  // op2h_set_fusion; kept across op_init/op_exit
+  int nontemporal = -1;  // op2h_set_nontemporal: -1 by size, 0 never, 1 always
   unsigned plan_epoch = 1;  // advanced whenever the plans are dropped
   unsigned long long fused = 0;  // pending launches taken over by a later stub
   // the current loop's node in the op_profile tree (loops do not nest)
@@ -393,6 +394,8 @@ extern "C" void op_init(int argc, char **argv, int diags) {
     else if (!std::strcmp(a, "OP_AUTO_SOA")) OP_auto_soa = 1;
     else if (!std::strncmp(a, "OP_MAPS_BASE_INDEX=", 19)) OP_maps_base_index = atoi(a + 19);
     else if (!std::strcmp(a, "OP_HIP_KERNEL_TIMING")) op2h_enable_kernel_timing(1);
+    else if (!std::strncmp(a, "OP_HIP_NONTEMPORAL=", 19))
+      op2h_set_nontemporal(!std::strcmp(a + 19, "on") ? 1 : !std::strcmp(a + 19, "off") ? 0 : -1);
     else if (!std::strcmp(a, "OP_HIP_RENUMBER") || !std::strncmp(a, "OP_HIP_RENUMBER=", 16)) {
       rt().transparent = true;
       rt().transparent_map = a[15] == '=' ? std::string(a + 16) : std::string();
@@ -1378,6 +1381,17 @@ extern "C" int op2h_take_deferred(const char *tag, void **arg) {
 extern "C" unsigned long long op2h_fused_count(void) { return rt().fused; }
 
 extern "C" void op2h_set_fusion(int on) { rt().fusion = on != 0; }
+
+// MI355X's last-level cache (the Infinity Cache, memory-side, shared by all
+// XCDs): a stream larger than this cannot be served from it on the next pass.
+static constexpr size_t kLastLevelCacheBytes = size_t(256) << 20;
+
+extern "C" void op2h_set_nontemporal(int mode) { rt().nontemporal = mode < 0 ? -1 : mode != 0; }
+
+extern "C" int op2h_nontemporal(size_t stream_bytes) {
+  const int m = rt().nontemporal;
+  return m < 0 ? stream_bytes > kLastLevelCacheBytes : m;
+}
 
 extern "C" unsigned op2h_plan_epoch(void) { return rt().plan_epoch; }
 
