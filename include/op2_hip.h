@@ -358,25 +358,6 @@ void op2h_device_sync(void);
 /* Version string of the library. */
 const char *op2h_version(void);
 
-/* ---- generated loop stubs for the airfoil application ----
- * Argument lists exactly as airfoil.cpp:225-277 passes them. */
-void op_par_loop_airfoil_1_save_soln(char const *name, op_set set, op_arg arg0,
-                                     op_arg arg1);
-void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set, op_arg arg0,
-                                    op_arg arg1, op_arg arg2, op_arg arg3,
-                                    op_arg arg4, op_arg arg5);
-void op_par_loop_airfoil_3_res_calc(char const *name, op_set set, op_arg arg0,
-                                    op_arg arg1, op_arg arg2, op_arg arg3,
-                                    op_arg arg4, op_arg arg5, op_arg arg6,
-                                    op_arg arg7);
-void op_par_loop_airfoil_4_bres_calc(char const *name, op_set set, op_arg arg0,
-                                     op_arg arg1, op_arg arg2, op_arg arg3,
-                                     op_arg arg4, op_arg arg5);
-void op_par_loop_airfoil_5_update(char const *name, op_set set, op_arg arg0,
-                                  op_arg arg1, op_arg arg2, op_arg arg3,
-                                  op_arg arg4, op_arg arg5, op_arg arg6,
-                                  op_arg arg7);
-
 #ifdef __cplusplus
 } /* extern "C" */
 

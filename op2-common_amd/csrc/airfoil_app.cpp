@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "op2_hip_airfoil.h"
 #include "op2_hip_rt.h"
 
 namespace {

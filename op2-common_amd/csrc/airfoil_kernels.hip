@@ -29,6 +29,7 @@
 
 #include <map>
 
+#include "op2_hip_airfoil.h"
 #include "op2_hip_rt.h"
 #include "op2h_plan_build.h"
 #include "airfoil_device.h"

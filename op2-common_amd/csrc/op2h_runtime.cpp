@@ -575,9 +575,11 @@ extern "C" op_map op_decl_map_long(op_set from, op_set to, int dim,
   return op_decl_map(from, to, dim, tmp.data(), name);
 }
 
-extern "C" op_dat op_decl_dat_char(op_set set, int dim, char const *type,
-                                   int size, char *data, char const *name) {
-  // op_lib_core.cpp:502-576 (always copies: OP_NO_REALLOC is not supported)
+namespace {
+// op_lib_core.cpp:502-576 (always copies: OP_NO_REALLOC is not supported);
+// `temp`: zeroed on the device directly, nothing uploaded
+op_dat decl_dat(op_set set, int dim, char const *type, int size, char *data,
+                char const *name, bool temp) {
   ensure_init();
   if (!set) {
     printf("op_decl_dat error -- invalid set for data: %s\n", name);
@@ -606,19 +608,47 @@ extern "C" op_dat op_decl_dat_char(op_set set, int dim, char const *type,
   d->type = copy_str(type);
   d->dirtybit = 1;
   rt().dats.push_back(d);
-  upload_dat(d);
+  if (!temp) {
+    upload_dat(d);
+  } else {
+    Runtime &R = rt();
+    const size_t dbytes =
+        std::max((size_t)d->size * (size_t)std::max(set->stride, 1), (size_t)64);
+    HIPCHECK(hipMalloc(&d->data_d, dbytes));
+    HIPCHECK(hipMemsetAsync(d->data_d, 0, dbytes, R.stream));
+    HIPCHECK(hipStreamSynchronize(R.stream));
+    d->dirty_hd = 0;
+  }
   return d;
 }
+}  // namespace
 
-extern "C" op_dat op_decl_dat_temp_char(op_set set, int dim, char const *type,
-                                        int size, char const *name) {
-  return op_decl_dat_char(set, dim, type, size, nullptr, name);
+extern "C" op_dat op_decl_dat_char(op_set set, int dim, char const *type,
+                                   int size, char *data, char const *name) {
+  return decl_dat(set, dim, type, size, data, name, false);
 }
 
+// op_lib_core.cpp:643-680: a temporary dat may not share its name (with
+// its set, dim, type and element size) with a live dat; it starts zeroed
+// (op_decl_dat_core's calloc), here on the device without a host copy.
+extern "C" op_dat op_decl_dat_temp_char(op_set set, int dim, char const *type,
+                                        int size, char const *name) {
+  ensure_init();
+  for (op_dat d : rt().dats)
+    if (d && d->set == set && d->dim == dim && d->size == dim * size &&
+        !std::strcmp(d->name, name) && !std::strcmp(d->type, type)) {
+      printf("op_dat with name %s already exists, cannot create temporary op_dat\n ", name);
+      exit(2);
+    }
+  return decl_dat(set, dim, type, size, nullptr, name, true);
+}
+
+// 1 when the dat was found and freed, -1 otherwise (op_lib_core.cpp:657-680)
 extern "C" int op_free_dat_temp_char(op_dat dat) {
   Runtime &R = rt();
   for (auto &d : R.dats)
-    if (d == dat) {
+    if (d && d == dat) {
+      op2h_flush_deferred();
       HIPCHECK(hipStreamSynchronize(R.stream));
       if (d->data_d) (void)hipFree(d->data_d);
       std::free(d->data);
@@ -628,7 +658,7 @@ extern "C" int op_free_dat_temp_char(op_dat dat) {
       d = nullptr;
       return 1;
     }
-  return 0;
+  return -1;
 }
 
 // ===========================================================================
@@ -2044,15 +2074,26 @@ extern "C" double op2h_algorithmic_bytes(op_set set, int nargs,
 // plans
 // ===========================================================================
 namespace {
+// A plan depends on the loop's set, maps, indices and access modes, on which
+// arguments name the same dat, and on the dats' shapes, not on the dats
+// themselves: the key names each dat by the first argument that passes it,
+// so a loop over temporary dats declared afresh every iteration
+// (op_decl_dat_temp) finds the plan its first iteration built.
 std::string arg_signature(const char *name, op_set set, int nargs,
                           const op_arg *args, int part) {
   std::string s = std::string(name) + "|" + std::to_string(set->index) + "|" +
                   std::to_string(part);
   for (int m = 0; m < nargs; ++m) {
     const op_arg &a = args[m];
-    s += "|" + std::to_string(a.argtype) + "," +
-         std::to_string(a.dat ? a.dat->index : -1) + "," +
-         std::to_string(a.map ? a.map->index : -1) + "," + std::to_string(a.idx) +
+    int first = -1;
+    if (a.dat)
+      for (int k = 0; k <= m && first < 0; ++k)
+        if (args[k].dat == a.dat) first = k;
+    s += "|" + std::to_string(a.argtype) + "," + std::to_string(first) + "," +
+         (a.dat ? std::to_string(a.dat->set->index) + ":" + std::to_string(a.dat->dim) +
+                      ":" + std::to_string(a.dat->size) + ":" + std::to_string(dat_is_soa(a.dat))
+                : std::string("-")) +
+         "," + std::to_string(a.map ? a.map->index : -1) + "," + std::to_string(a.idx) +
          "," + std::to_string(a.acc) + "," + std::to_string(a.opt);
   }
   return s;

@@ -20,6 +20,7 @@ PROGRAMS = {
     "strides": "functional/strides.cpp",
     "consts": "functional/consts.cpp",
     "helpers": "functional/helpers.cpp",
+    "tempdats": "functional/tempdats.cpp",
     "airfoil_tr": "airfoil/airfoil_tr.cpp",
 }
 
@@ -30,6 +31,10 @@ PROGRAMS = {
 REFERENCE = "/root/reference"
 REF_PROGRAMS = {
     "airfoil_ref": "apps/c/airfoil/airfoil_plain/dp/airfoil.cpp",
+    # single precision throughout, rms-only update
+    "airfoil_ref_sp": "apps/c/airfoil/airfoil_plain/sp/airfoil.cpp",
+    # res, adt and qold as temporary dats declared and freed every iteration
+    "airfoil_ref_tempdats": "apps/c/airfoil/airfoil_tempdats/dp/airfoil.cpp",
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

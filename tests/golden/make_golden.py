@@ -75,6 +75,24 @@ def unmodified_app_case(name, I, J, ordering):
     print("wrote", name)
 
 
+def app_history_case(name, exe, I, J, ordering):
+    """An unmodified reference app variant whose only result is its printed
+    history (" <iter>  <rms>" every 100 iterations, 1000 iterations fixed):
+    the single-precision app and the temporary-dat app."""
+    mesh = op2amd.mesh.generate(I, J, 0, ordering)
+    with tempfile.TemporaryDirectory() as d:
+        mesh.write(pathlib.Path(d) / "new_grid.dat")
+        out = subprocess.run([str(REF / exe)], cwd=d, check=True, capture_output=True,
+                             text=True).stdout
+    rows = [l.split() for l in out.splitlines()
+            if len(l.split()) == 2 and l.split()[0].isdigit()]
+    lines = np.array([" ".join(r) for r in rows])
+    np.savez_compressed(HERE / f"{name}.npz", I=I, J=J, ordering=ordering, niter=1000,
+                        iters=np.array([int(r[0]) for r in rows], np.int32),
+                        rms=np.array([float(r[1]) for r in rows]), lines=lines)
+    print("wrote", name, lines[-1])
+
+
 def plan_case(name, I, J, ordering, part_size):
     mesh = op2amd.mesh.generate(I, J, 0, ordering)
     out = {"I": I, "J": J, "ordering": ordering, "part_size": part_size}
@@ -115,17 +133,30 @@ def plan_case(name, I, J, ordering, part_size):
     print("wrote", name)
 
 
+CASES = {
+    "airfoil_120x60_random_a1":
+        lambda n: harness_case(n, 120, 60, "random", 1000, 1, False),
+    "airfoil_120x60_rcm_a1": lambda n: harness_case(n, 120, 60, "rcm", 1000, 1, False),
+    "airfoil_24x12_random_a1": lambda n: harness_case(n, 24, 12, "random", 300, 1, True),
+    "airfoil_24x12_structured_a0":
+        lambda n: harness_case(n, 24, 12, "structured", 50, 0, False),
+    "airfoil_24x12_random_app_a0": lambda n: unmodified_app_case(n, 24, 12, "random"),
+    "airfoil_120x60_random_app_sp":
+        lambda n: app_history_case(n, "airfoil_sp_seq", 120, 60, "random"),
+    "airfoil_120x60_random_app_tempdats":
+        lambda n: app_history_case(n, "airfoil_tempdats_seq", 120, 60, "random"),
+    "plan_60x30_random_p0": lambda n: plan_case(n, 60, 30, "random", 0),
+    "plan_60x30_rcm_p64": lambda n: plan_case(n, 60, 30, "rcm", 64),
+}
+
+
 def main():
+    """python tests/golden/make_golden.py [case ...] (default: every case)"""
     if not pathlib.Path("/root/reference").exists():
         sys.exit("the reference tree is not available here")
     subprocess.run(["make", "-s", "-C", str(REPO / "oracle"), "ref"], check=True)
-    harness_case("airfoil_120x60_random_a1", 120, 60, "random", 1000, 1, False)
-    harness_case("airfoil_120x60_rcm_a1", 120, 60, "rcm", 1000, 1, False)
-    harness_case("airfoil_24x12_random_a1", 24, 12, "random", 300, 1, True)
-    harness_case("airfoil_24x12_structured_a0", 24, 12, "structured", 50, 0, False)
-    unmodified_app_case("airfoil_24x12_random_app_a0", 24, 12, "random")
-    plan_case("plan_60x30_random_p0", 60, 30, "random", 0)
-    plan_case("plan_60x30_rcm_p64", 60, 30, "rcm", 64)
+    for name in sys.argv[1:] or CASES:
+        CASES[name](name)
 
 
 if __name__ == "__main__":

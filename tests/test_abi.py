@@ -8,7 +8,7 @@ import pytest
 
 from conftest import REPO
 
-HEADERS = ["op2_hip.h", "op_profile.h", "op2_hip_rt.h", "op2amd_mesh.h", "op2amd_airfoil.h",
+HEADERS = ["op2_hip.h", "op2_hip_airfoil.h", "op_profile.h", "op2_hip_rt.h", "op2amd_mesh.h", "op2amd_airfoil.h",
            "op2amd_ensemble.h", "op2amd_synth.h", "op2amd_costmodel.h"]
 
 

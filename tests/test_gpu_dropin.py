@@ -16,7 +16,13 @@ Checked against the oracle and the reference's own output:
 - the op_profile tree the application asks for (airfoil.cpp:215,299,313) at
   OP_PROFILE_LEVEL=3, written to OP_PROFILE_JSON_OUTPUT: one loop node per
   op_par_loop with the application's call counts, each holding its kernels'
-  device time."""
+  device time.
+
+Two more of the reference's airfoil applications run the same way,
+unmodified: the single-precision one (airfoil_plain/sp) and the one whose
+work arrays are temporary dats (airfoil_tempdats/dp); their printed
+histories are checked against the reference's sequential builds of the same
+sources on the same mesh."""
 import json
 import os
 import subprocess
@@ -44,6 +50,22 @@ def run_app(tmp_path, *tokens, env_extra=None):
                        text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     return g, mesh, r.stdout
+
+
+def run_variant(tmp_path, exe, golden_name, *tokens):
+    """A reference app variant on its golden's mesh; its printed history."""
+    path = REPO / "tests" / "apps" / "_build" / exe / exe
+    assert path.exists(), f"{path} missing: run __graft_entry__.build() where the reference is present"
+    g = golden(golden_name)
+    mesh = op2amd.mesh.generate(int(g["I"]), int(g["J"]), 0, str(g["ordering"]))
+    mesh.write(str(tmp_path / "new_grid.dat"))
+    print("running", exe, *tokens, flush=True)
+    r = subprocess.run([str(path), *tokens], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    rows = [l.split() for l in r.stdout.splitlines()
+            if len(l.split()) == 2 and l.split()[0].isdigit()]
+    return g, mesh, r.stdout, rows
 
 
 def read_bin(path):
@@ -94,3 +116,34 @@ def test_unmodified_reference_app_profile_tree(tmp_path):
     total = sum(c["clock"]["total"] for c in tree["children"])
     assert total <= tree["clock"]["total"]
     assert "%kern" in out
+
+
+@pytest.mark.parametrize("tokens", [[], ["OP_AUTO_SOA"]], ids=["aos", "soa"])
+def test_reference_tempdats_app(tmp_path, tokens):
+    """apps/c/airfoil/airfoil_tempdats/dp/airfoil.cpp, unmodified: res, adt and
+    qold are temporary dats declared zeroed and freed every iteration
+    (:225-307), the update reduces rms only.  Its printed history (rms every
+    100 iterations, 6 significant digits) against the reference's sequential
+    build of the same app on the same mesh (golden), and against the
+    oracle's rms; no free reported an error."""
+    g, mesh, out, rows = run_variant(tmp_path, "airfoil_ref_tempdats",
+                                     "airfoil_120x60_random_app_tempdats", *tokens)
+    assert "cannot be removed" not in out
+    assert [int(r[0]) for r in rows] == list(g["iters"])
+    rms = np.array([float(r[1]) for r in rows])
+    assert np.all(np.abs(rms - g["rms"]) <= 1e-5 * g["rms"]), (rms, g["rms"])
+    ref = oracle.airfoil_run(mesh, 1000, 0, "seq")
+    assert np.all(np.abs(rms - ref["rms"][g["iters"] - 1]) <= 1e-5 * rms)
+
+
+def test_reference_single_precision_app(tmp_path):
+    """apps/c/airfoil/airfoil_plain/sp/airfoil.cpp, unmodified: float dats,
+    constants and rms reduction.  On the reference's free stream the flow is
+    steady and the printed rms is float rounding noise, which after 1000
+    iterations reproduces only if every step's float arithmetic does: the
+    history against the reference's sequential build of the same app on the
+    same mesh (golden), to 1e-4 relative (the rms sum's own order differs)."""
+    g, _, out, rows = run_variant(tmp_path, "airfoil_ref_sp", "airfoil_120x60_random_app_sp")
+    assert [int(r[0]) for r in rows] == list(g["iters"])
+    rms = np.array([float(r[1]) for r in rows])
+    assert np.all(np.abs(rms - g["rms"]) <= 1e-4 * g["rms"]), (rms, g["rms"])

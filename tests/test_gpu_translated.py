@@ -1,7 +1,7 @@
 """Programs translated by op2amd.translator and built by tests/apps/build.py
 (run by __graft_entry__.build()), on the GPU:
 - the functional programs (global reductions, dat increments, op_arg_idx,
-  SoA strides, constants) with the default plans (owner partition for
+  SoA strides, constants, temporary dats) with the default plans (owner partition for
   increments), with every dat in SoA layout (OP_AUTO_SOA), and with
   coloured or atomic increments;
 - the translated airfoil program against the reference-pinned trajectory,
@@ -37,7 +37,8 @@ def run(name, *args, timeout=120):
 @pytest.mark.parametrize("variant", [[], ["OP_AUTO_SOA"], ["OP_HIP_PLAN=*:color"],
                                      ["OP_HIP_PLAN=*:atomic"], ["OP_HIP_RENUMBER"]],
                          ids=["default", "soa", "color", "atomic", "renumbered"])
-@pytest.mark.parametrize("name", ["gbl", "reductions", "idx", "strides", "consts", "helpers"])
+@pytest.mark.parametrize("name", ["gbl", "reductions", "idx", "strides", "consts", "helpers",
+                                  "tempdats"])
 def test_functional_program(name, variant):
     r = run(name, *variant)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
