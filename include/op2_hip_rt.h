@@ -92,18 +92,24 @@ int op2h_take_deferred(const char *tag, void **arg);
 void op2h_flush_deferred(void);
 void op2h_set_fusion(int on);
 int op2h_fusion_enabled(void);
-/* Non-temporal hints for a kernel's single-use streams (element-indexed
- * reads and writes no other element of the launch touches): with them the
- * lines bypass the caches' normal retention, which pays once the data a
- * loop streams exceeds the 256 MiB last-level cache (its next pass could not
- * hit it anyway, and the lines the kernel does reuse, such as gathered
- * nodes, stay cached), and costs a loop whose data would otherwise still be
- * cached for the next loop.  op2h_nontemporal(bytes) is 1 when a stub
- * streaming `bytes` should use them: by size (the default, mode -1), never
- * (0) or always (1), set by op2h_set_nontemporal or the op_init token
- * OP_HIP_NONTEMPORAL=auto|off|on (a setting kept across op_init/op_exit). */
-void op2h_set_nontemporal(int mode);
-int op2h_nontemporal(size_t stream_bytes);
+/* Streaming or cache-resident: whether the data a loop streams (`bytes`,
+ * its element-indexed reads and writes) exceeds the 256 MiB last-level
+ * cache, so that the next loop cannot find it there.  op2h_streaming(bytes)
+ * is 1 when it does: by size (the default, mode -1), always (1) or never
+ * (0), set by op2h_set_streaming or the op_init token
+ * OP_HIP_STREAMING=auto|on|off (a setting kept across op_init/op_exit).
+ * Stubs choose by it:
+ *   - non-temporal hints on a loop's single-use streams (adt_calc's pcell
+ *     rows, q and adt): streaming data gains L2 room for what the kernel
+ *     does reuse (the gathered nodes), cache-resident data would lose its
+ *     place in the last-level cache for the next loop;
+ *   - the cells of the update (and of the direct loops fused with it) per
+ *     workgroup: cache-resident, contiguous chunks of about three cells per
+ *     thread on XCD-ordered workgroups (neighbouring cells and the nodes
+ *     they gather in one XCD's L2); streaming, a grid-stride sweep of at
+ *     most 2048 workgroups moving through the set together. */
+void op2h_set_streaming(int mode);
+int op2h_streaming(size_t stream_bytes);
 /* Pending launches taken over by a later stub since the library loaded. */
 unsigned long long op2h_fused_count(void);
 

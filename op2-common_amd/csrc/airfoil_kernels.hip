@@ -720,19 +720,37 @@ __device__ __forceinline__ MaxLoc wave_maxloc(MaxLoc m) {
   return m;
 }
 
+// The cells a workgroup of the update (or of a direct chain fused with it)
+// takes, and the partial it writes: with chunk > 0 the contiguous cells
+// [L chunk, (L+1) chunk) of the workgroup the XCD-ordered id L names, its
+// threads striding by the workgroup size; with chunk 0 a grid-stride sweep
+// from the workgroup's own id.  The same cells in the same order per
+// partial, so every kernel launched with one shape reduces the same bits.
+struct CellRange {
+  int L, c, end, step;
+};
+__device__ __forceinline__ CellRange cell_range(int n, int chunk) {
+  if (chunk > 0) {
+    const int L = xcd_block(kXcds);
+    return {L, L * chunk + (int)threadIdx.x, min(n, (L + 1) * chunk), (int)blockDim.x};
+  }
+  return {(int)blockIdx.x, (int)(blockIdx.x * blockDim.x + threadIdx.x), n,
+          (int)(gridDim.x * blockDim.x)};
+}
+
 template <bool SOA>
 __global__ void __launch_bounds__(1024)
     k_update(const double *__restrict qold, double *__restrict q,
              double *__restrict res, const double *__restrict adt, int n, int s,
              double *__restrict part_sum, double *__restrict part_max,
-             int *__restrict part_loc, const int *__restrict oldidx) {
+             int *__restrict part_loc, const int *__restrict oldidx, int chunk) {
   __shared__ double s_sum[16];
   __shared__ double s_max[16];
   __shared__ int s_loc[16];
   double rms = 0.0;
   MaxLoc ml{-INFINITY, 0x7fffffff};
-  const int stride = gridDim.x * blockDim.x;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+  const CellRange cr = cell_range(n, chunk);
+  for (int c = cr.c; c < cr.end; c += cr.step) {
     double qo[4], qn[4], r[4], a = adt[c], mx = ml.v;
     int loc = ml.loc;
     ld4<SOA>(qold, c, s, qo);
@@ -762,9 +780,9 @@ __global__ void __launch_bounds__(1024)
       bs += s_sum[w];
       bm = maxloc_combine(bm, MaxLoc{s_max[w], s_loc[w]});
     }
-    part_sum[blockIdx.x] = bs;
-    part_max[blockIdx.x] = bm.v;
-    part_loc[blockIdx.x] = bm.loc;
+    part_sum[cr.L] = bs;
+    part_max[cr.L] = bm.v;
+    part_loc[cr.L] = bm.loc;
   }
 }
 
@@ -1136,7 +1154,7 @@ __global__ void __launch_bounds__(1024)
                    double *adt, double *qold_out, const double *__restrict x,
                    const int *__restrict pcell, int n, int s, int sx,
                    double *__restrict part_sum, double *__restrict part_max,
-                   int *__restrict part_loc, const int *__restrict oldidx) {
+                   int *__restrict part_loc, const int *__restrict oldidx, int chunk) {
   // qold (read by update, written by save_soln) and adt (read by update,
   // written by adt_calc) are read before they are written, by the same
   // thread for the same cell: not restrict-qualified
@@ -1145,8 +1163,9 @@ __global__ void __launch_bounds__(1024)
   __shared__ int s_loc[16];
   double rms = 0.0;
   MaxLoc ml{-INFINITY, 0x7fffffff};
-  const int stride = gridDim.x * blockDim.x;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+  // the update's cells per workgroup (its partials are the update's)
+  const CellRange cr = cell_range(n, chunk);
+  for (int c = cr.c; c < cr.end; c += cr.step) {
     double qn[4];
     if (UPD) {
       double qo[4], r[4], a = adt[c], mx = ml.v;
@@ -1191,9 +1210,9 @@ __global__ void __launch_bounds__(1024)
       bsum += s_sum[w];
       bm = maxloc_combine(bm, MaxLoc{s_max[w], s_loc[w]});
     }
-    part_sum[blockIdx.x] = bsum;
-    part_max[blockIdx.x] = bm.v;
-    part_loc[blockIdx.x] = bm.loc;
+    part_sum[cr.L] = bsum;
+    part_max[cr.L] = bm.v;
+    part_loc[cr.L] = bm.loc;
   }
 }
 
@@ -1207,13 +1226,30 @@ __global__ void __launch_bounds__(1024)
 }
 
 // A pending run of direct loops over one cell set: [update] [save_soln].
+// The update's launch shape: g workgroups, one reduction partial each
+// (at most 2048, the finish kernels' and the no-allocation scratch's limit).
+// Cache-resident data (op2h_streaming false for its 136 bytes per cell): g
+// contiguous chunks of about three cells per thread, on XCD-ordered
+// workgroups; streaming data: one cell per thread up to 2048 workgroups, a
+// grid-stride sweep beyond.  chunk 0 = grid-stride.
+struct UpdateShape {
+  int g, chunk;
+};
+UpdateShape update_shape(int n, int bs) {
+  if (n > 0 && !op2h_streaming((size_t)n * 136)) {
+    const int g = std::max(1, std::min(grid_for(n, 3 * bs), 2048));
+    return {g, (n + g - 1) / g};
+  }
+  return {std::max(1, std::min(grid_for(n, bs), 2048)), 0};
+}
+
 struct PendingDirect {
   bool upd = false, save = false, soa = false;
   op_set set = nullptr;
   hipStream_t st = nullptr;
   int n = 0, s = 0;
   // update (its partials handed to the graph's history kernel)
-  int g = 0, bs = 0;
+  int g = 0, bs = 0, chunk = 0;
   const double *qold = nullptr;
   double *q = nullptr, *res = nullptr, *adt = nullptr;
   double *part_sum = nullptr, *part_max = nullptr;
@@ -1232,7 +1268,7 @@ void flush_direct(void *) {
   if (P.upd)
     launch_k(false, nullptr, nullptr, P.soa ? k_update<true> : k_update<false>, P.g, P.bs, 0,
              P.st, P.qold, P.q, P.res, (const double *)P.adt, P.n, P.s, P.part_sum,
-             P.part_max, P.part_loc, P.oldidx);
+             P.part_max, P.part_loc, P.oldidx, P.chunk);
   if (P.save)
     launch_k(false, nullptr, nullptr, k_save_soln, grid_for(P.save_n2, P.save_bs), P.save_bs, 0,
              P.st, (const double2 *)P.save_src, (double2 *)P.save_dst, P.save_n2);
@@ -1446,15 +1482,16 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
       if (P.set == set && P.soa == soa && (soa ? sq == P.s : true) && qsrc == q &&
           !is_float(args[5]) && (!P.upd || P.adt == adt)) {
         g_direct = PendingDirect();
-        // [update] [save_soln] adt_calc per cell: the update's grid and block
+        // [update] [save_soln] adt_calc per cell: the update's launch shape
         // (its partials), else adt_calc's
         const int gg = P.upd ? P.g : g, bb = P.upd ? P.bs : ctx.block_size;
+        const int chunk = P.upd ? P.chunk : 0;
         auto kern = P.upd ? (P.save ? (soa ? k_direct_chain<true, true, true> : k_direct_chain<false, true, true>)
                                     : (soa ? k_direct_chain<true, true, false> : k_direct_chain<false, true, false>))
                           : (soa ? k_direct_chain<true, false, true> : k_direct_chain<false, false, true>);
         launch_k(false, nullptr, nullptr, kern, gg, bb, 0, st, P.qold, P.q ? P.q : (double *)q,
                  P.res, adt, P.save_dst, x, pcell, set->size, sq, sx, P.part_sum, P.part_max,
-                 P.part_loc, P.oldidx);
+                 P.part_loc, P.oldidx, chunk);
         KCHECK(hipGetLastError());
         account(1, set, 6, args);
         op2h_loop_end(&ctx);
@@ -1469,7 +1506,7 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
                ctx.block_size, 0, st, x, q, adtf, pcell, set->size, sx, sq);
     } else {
       // streams: pcell rows, q and adt
-      const bool nt = op2h_nontemporal((size_t)set->size * 56) != 0;
+      const bool nt = op2h_streaming((size_t)set->size * 56) != 0;
       launch_k(tm.ext, tm.e0, tm.e1,
                nt ? (soa ? k_adt_calc<true, true> : k_adt_calc<false, true>)
                   : (soa ? k_adt_calc<true, false> : k_adt_calc<false, false>),
@@ -1859,10 +1896,8 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
   bool soa = loop_soa(args, 8, &dummy);
   int s = set_stride(args[0]);
   const int bs = ctx.block_size;
-  // enough workgroups to fill 256 CUs several times over; each thread
-  // strides over the set when it is larger
-  int g = grid_for(set->size, bs);
-  g = std::max(1, std::min(g, 2048));
+  const UpdateShape shape = update_shape(set->size, bs);
+  const int g = shape.g;
   // inside a captured graph the partials go to the runtime, whose history
   // kernel finishes both reductions of every update of the step in one pass
   // (same tree as k_update_finish); otherwise finish them here
@@ -1898,6 +1933,7 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
     P.s = s;
     P.g = g;
     P.bs = bs;
+    P.chunk = shape.chunk;
     P.qold = (const double *)args[0].data_d;
     P.q = (double *)args[1].data_d;
     P.res = (double *)args[2].data_d;
@@ -1921,7 +1957,7 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
     launch_k(tm.ext, tm.e0, nullptr, soa ? k_update<true> : k_update<false>, g, bs, 0, st,
              (const double *)args[0].data_d, (double *)args[1].data_d,
              (double *)args[2].data_d, (const double *)args[3].data_d, set->size, s,
-             part_sum, part_max, part_loc, op2h_set_old_index(set));
+             part_sum, part_max, part_loc, op2h_set_old_index(set), shape.chunk);
   }
   if (!deferred)
     launch_k(tm.ext, nullptr, tm.e1, k_update_finish, 1, kFinishThreads, 0, st, part_sum,
@@ -1974,7 +2010,8 @@ extern "C" int op2h_update_noalloc(int ncell, int soa_stride, int block, const d
     }
     g_scratch_addr[dev] = ptr;
   }
-  const int g = std::max(1, std::min(grid_for(ncell, block), kNoAllocMaxParts));
+  const UpdateShape shape = update_shape(ncell, block);  // the stub's
+  const int g = shape.g;
   if ((size_t)g * (8 + 8 + 4) > kNoAllocScratchBytes) {
     printf("op2h_update_noalloc: %d partials exceed the %zu-byte scratch\n", g,
            kNoAllocScratchBytes);
@@ -1987,10 +2024,10 @@ extern "C" int op2h_update_noalloc(int ncell, int soa_stride, int block, const d
   // every partial the finish kernel reads is written by this call's k_update
   if (soa_stride)
     k_update<true><<<g, block, 0, st>>>(qold, q, res, adt, ncell, soa_stride, part_sum,
-                                        part_max, part_loc, nullptr);
+                                        part_max, part_loc, nullptr, shape.chunk);
   else
     k_update<false><<<g, block, 0, st>>>(qold, q, res, adt, ncell, 0, part_sum, part_max,
-                                         part_loc, nullptr);
+                                         part_loc, nullptr, shape.chunk);
   k_update_finish<<<1, kFinishThreads, 0, st>>>(part_sum, part_max, part_loc, g, &result->rms,
                                                &result->maxerr);
   const hipError_t e = hipGetLastError();

@@ -187,7 +187,7 @@ struct Runtime {
   void *pend_arg = nullptr;
   std::string pend_tag, accept_tag;
   bool fusion = true;  // op2h_set_fusion; kept across op_init/op_exit
-  int nontemporal = -1;  // op2h_set_nontemporal: -1 by size, 0 never, 1 always
+  int streaming = -1;  // op2h_set_streaming: -1 by size, 0 never, 1 always
   unsigned plan_epoch = 1;  // advanced whenever the plans are dropped
   unsigned long long fused = 0;  // pending launches taken over by a later stub
   // the current loop's node in the op_profile tree (loops do not nest)
@@ -394,8 +394,8 @@ extern "C" void op_init(int argc, char **argv, int diags) {
     else if (!std::strcmp(a, "OP_AUTO_SOA")) OP_auto_soa = 1;
     else if (!std::strncmp(a, "OP_MAPS_BASE_INDEX=", 19)) OP_maps_base_index = atoi(a + 19);
     else if (!std::strcmp(a, "OP_HIP_KERNEL_TIMING")) op2h_enable_kernel_timing(1);
-    else if (!std::strncmp(a, "OP_HIP_NONTEMPORAL=", 19))
-      op2h_set_nontemporal(!std::strcmp(a + 19, "on") ? 1 : !std::strcmp(a + 19, "off") ? 0 : -1);
+    else if (!std::strncmp(a, "OP_HIP_STREAMING=", 17))
+      op2h_set_streaming(!std::strcmp(a + 17, "on") ? 1 : !std::strcmp(a + 17, "off") ? 0 : -1);
     else if (!std::strcmp(a, "OP_HIP_RENUMBER") || !std::strncmp(a, "OP_HIP_RENUMBER=", 16)) {
       rt().transparent = true;
       rt().transparent_map = a[15] == '=' ? std::string(a + 16) : std::string();
@@ -1416,10 +1416,10 @@ extern "C" void op2h_set_fusion(int on) { rt().fusion = on != 0; }
 // XCDs): a stream larger than this cannot be served from it on the next pass.
 static constexpr size_t kLastLevelCacheBytes = size_t(256) << 20;
 
-extern "C" void op2h_set_nontemporal(int mode) { rt().nontemporal = mode < 0 ? -1 : mode != 0; }
+extern "C" void op2h_set_streaming(int mode) { rt().streaming = mode < 0 ? -1 : mode != 0; }
 
-extern "C" int op2h_nontemporal(size_t stream_bytes) {
-  const int m = rt().nontemporal;
+extern "C" int op2h_streaming(size_t stream_bytes) {
+  const int m = rt().streaming;
   return m < 0 ? stream_bytes > kLastLevelCacheBytes : m;
 }
 

@@ -207,8 +207,8 @@ op2h_check_gather_plan = _fn("op2h_check_gather_plan", C.c_int, C.c_int, C.c_int
 op2h_set_old_index = _fn("op2h_set_old_index", C.c_void_p, op_set)
 op2h_set_fusion = _fn("op2h_set_fusion", None, C.c_int)
 op2h_fused_count = _fn("op2h_fused_count", C.c_ulonglong)
-op2h_set_nontemporal = _fn("op2h_set_nontemporal", None, C.c_int)
-op2h_nontemporal = _fn("op2h_nontemporal", C.c_int, C.c_size_t)
+op2h_set_streaming = _fn("op2h_set_streaming", None, C.c_int)
+op2h_streaming = _fn("op2h_streaming", C.c_int, C.c_size_t)
 
 
 op2h_rcm_order = _fn("op2h_rcm_order", C.c_int, C.c_int, C.POINTER(C.c_int),

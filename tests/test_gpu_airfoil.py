@@ -241,21 +241,27 @@ def test_fused_loop_chains_equal_the_unfused_loops(kw):
 @pytest.mark.parametrize("kw", [dict(mode="tuned"), dict(mode="tuned", soa=True),
                                 dict(mode="graphed", graph_steps=10)],
                          ids=["eager", "soa", "graphed"])
-def test_nontemporal_streams_same_bits(kw):
-    """adt_calc with its single-use streams (pcell rows, q, adt) non-temporal,
-    as the runtime picks for meshes whose streams exceed the last-level cache
-    (here forced on at 120x60): the reference's bits, as without the hints."""
+def test_streaming_shapes_same_bits(kw):
+    """The launch choices the runtime makes for data beyond the last-level
+    cache, forced on at 120x60 (op2h_set_streaming(1)): adt_calc with its
+    single-use streams non-temporal, the update (and the direct loops fused
+    with it) as a grid-stride sweep instead of XCD-ordered chunks.  q and
+    errloc are the reference's bits either way; rms within 1e-10 (its
+    partials follow the launch shape)."""
     import op2amd.api as api
     g = golden("airfoil_120x60_random_a1")
-    assert api.op2h_nontemporal(1 << 20) == 0 and api.op2h_nontemporal(1 << 30) == 1
-    api.op2h_set_nontemporal(1)
-    try:
-        assert api.op2h_nontemporal(0) == 1
-        r = op2amd.airfoil.run(mesh_of(g), 1000, plan_res="owner", block_res=128,
-                               plan_bres="gather", **kw)
-    finally:
-        api.op2h_set_nontemporal(-1)
-    assert api.op2h_nontemporal(1 << 20) == 0
-    assert np.array_equal(r["q"], g["q"])
-    assert np.array_equal(r["errloc"], g["errloc"])
-    assert pointwise_rel_err(r["rms"], g["rms"]) <= TOL
+    assert api.op2h_streaming(1 << 20) == 0 and api.op2h_streaming(1 << 30) == 1
+    runs = {}
+    for mode in (1, 0):
+        api.op2h_set_streaming(mode)
+        try:
+            assert api.op2h_streaming(0) == mode
+            runs[mode] = op2amd.airfoil.run(mesh_of(g), 1000, plan_res="owner", block_res=128,
+                                            plan_bres="gather", **kw)
+        finally:
+            api.op2h_set_streaming(-1)
+    assert api.op2h_streaming(1 << 20) == 0
+    for r in runs.values():
+        assert np.array_equal(r["q"], g["q"])
+        assert np.array_equal(r["errloc"], g["errloc"])
+        assert pointwise_rel_err(r["rms"], g["rms"]) <= TOL
