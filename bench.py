@@ -160,8 +160,9 @@ def parse_plans(text):
 def measure(op2amd, mesh, K, W, args, renumber=False, plans=None, plain_too=True):
     """Step rates of the plain backend, the best of the reference's own
     schemes with the reference's per-loop syncs, the tuned plans run eagerly,
-    and the tuned plans replayed from graphs; all over the same W + K
-    iterations, whose trajectories are returned for the parity check."""
+    and the tuned plans replayed from graphs, with and without loop-chain
+    fusion; all over the same W + K iterations, whose trajectories are
+    returned for the parity check."""
     n = K + W
     out, runs = {}, {}
     if plain_too:
@@ -194,6 +195,18 @@ def measure(op2amd, mesh, K, W, args, renumber=False, plans=None, plain_too=True
     # uploads of every dat and map), plan builds, graph capture
     out["graphed_setup_seconds"] = graphed.get("setup_seconds")
     log(f"  graphed {kw}: {out['graphed_steps_per_s']:.1f} steps/s")
+    # the same graphs without loop-chain fusion: what the graphs alone give
+    api = op2amd.api
+    api.op2h_set_fusion(0)
+    try:
+        unfused = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, renumber=renumber,
+                                     graph_steps=args.graph_steps, **kw)
+    finally:
+        api.op2h_set_fusion(1)
+    runs["graphed_unfused"] = unfused
+    out["graphed_unfused_steps_per_s"] = K / unfused["seconds"]
+    out["fusion_gain"] = out["graphed_steps_per_s"] / out["graphed_unfused_steps_per_s"]
+    log(f"  graphed without loop-chain fusion: {out['graphed_unfused_steps_per_s']:.1f} steps/s")
     out["graph_gain_vs_tuned_eager"] = out["graphed_steps_per_s"] / out["tuned_eager_steps_per_s"]
     if plain_too:
         out["speedup_vs_plain"] = out["graphed_steps_per_s"] / out["plain_steps_per_s"]

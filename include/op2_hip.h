@@ -22,10 +22,16 @@
 #ifndef OP2_HIP_H
 #define OP2_HIP_H
 
+/* the C headers the reference's op_lib_core.h brings into every OP2
+ * program (op_lib_core.h:43-50), so that programs relying on them build */
+#include <math.h>
+#include <stdarg.h>
 #include <stdbool.h>
 #include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
+#include <strings.h>
 
 /* access codes, argument types, staging modes: op_lib_core.h:108-125 */
 #define OP_READ 0
@@ -361,7 +367,12 @@ const char *op2h_version(void);
 #ifdef __cplusplus
 } /* extern "C" */
 
+#include <cstddef>
 #include <cstring>
+#include <iostream>
+#include <limits>
+#include <stdexcept>
+#include <utility>  // op_seq.h:8, op_lib_core.h:65-68
 
 /* ---- C++ templates (op_lib_cpp.h:127-203) ---- */
 namespace op2h_detail {

@@ -1341,14 +1341,15 @@ const BoundaryArgs *boundary_args(const op2h_owner_plan *p, op_map pbecell, op_m
   // boundary edges per cell, in order
   const int nbe = pbecell->from->size, ncell = p->to_size;
   const int *key_be = op2h::old_index_host(pbecell->from);
+  const int *becell = op2h::host_map(pbecell);  // in the device numbering
   std::vector<int> bo(ncell + 1, 0), order(nbe);
-  for (int e = 0; e < nbe; ++e) bo[pbecell->map[e] + 1]++;
+  for (int e = 0; e < nbe; ++e) bo[becell[e] + 1]++;
   for (int c = 0; c < ncell; ++c) bo[c + 1] += bo[c];
   for (int e = 0; e < nbe; ++e) order[e] = e;
   if (key_be)
     std::sort(order.begin(), order.end(), [&](int a, int b) { return key_be[a] < key_be[b]; });
   std::vector<int> fill(bo.begin(), bo.end() - 1), bl(std::max(nbe, 1));
-  for (int e : order) bl[fill[pbecell->map[e]]++] = e;
+  for (int e : order) bl[fill[becell[e]]++] = e;
   std::vector<ushort4> rec(std::max(no, 1));
   std::vector<int> offs(nb + 1, 0), edges;
   for (int b = 0; b < nb; ++b) {

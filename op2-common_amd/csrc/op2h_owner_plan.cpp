@@ -291,7 +291,7 @@ std::unique_ptr<OwnerPlanRec> build_owner_plan(op_set set, int block_cells,
   OwnerLists L;
   // under transparent renumbering the application's element order fixes the
   // order of each target's increments
-  if (!owner_lists(nto, nel, dim, own->map, ninc, inc_idx.data(), block_cells, L, err,
+  if (!owner_lists(nto, nel, dim, host_map(own), ninc, inc_idx.data(), block_cells, L, err,
                    old_index_host(set), key_instances))
     return nullptr;
   rec->key_instances = key_instances;

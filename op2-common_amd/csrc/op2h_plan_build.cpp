@@ -91,7 +91,7 @@ LoopShape loop_shape(op_set set, int nargs, const op_arg *args) {
 namespace {
 inline int mapv(const LoopShape &sh, int m, int e) {
   const auto &am = sh.arg_map[m];
-  return am.first->map[(size_t)e * am.first->dim + am.second];
+  return host_map(am.first)[(size_t)e * am.first->dim + am.second];
 }
 }  // namespace
 
@@ -197,7 +197,7 @@ std::unique_ptr<StagedPlanRec> build_staged_plan(op_set set, int part_size,
       for (int l = 0; l < nl; ++l)
         if (sh.locs[l].first == mp)
           for (int e = lo; e < hi; ++e)
-            tmp.push_back(mp->map[(size_t)e * mp->dim + sh.locs[l].second]);
+            tmp.push_back(host_map(mp)[(size_t)e * mp->dim + sh.locs[l].second]);
       std::sort(tmp.begin(), tmp.end());
       tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
       if (tmp.size() > 65535) {
@@ -214,7 +214,7 @@ std::unique_ptr<StagedPlanRec> build_staged_plan(op_set set, int part_size,
       for (int l = 0; l < nl; ++l)
         if (sh.locs[l].first == mp)
           for (int e = lo; e < hi; ++e)
-            loc_map[l][e] = (unsigned short)inv[s][mp->map[(size_t)e * mp->dim +
+            loc_map[l][e] = (unsigned short)inv[s][host_map(mp)[(size_t)e * mp->dim +
                                                            sh.locs[l].second]];
     }
   }
@@ -342,7 +342,7 @@ std::unique_ptr<GatherPlanRec> build_gather_plan(op_map map, hipStream_t stream)
   P.to_size = to;
   P.dim = dim;
   std::vector<int> count(to + 1, 0);
-  for (size_t i = 0; i < (size_t)from * dim; ++i) count[map->map[i] + 1]++;
+  for (size_t i = 0; i < (size_t)from * dim; ++i) count[host_map(map)[i] + 1]++;
   std::vector<int> targets;
   for (int t = 0; t < to; ++t)
     if (count[t + 1]) targets.push_back(t);
@@ -356,7 +356,7 @@ std::unique_ptr<GatherPlanRec> build_gather_plan(op_map map, hipStream_t stream)
   if (key) std::sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
   for (int e : order)
     for (int k = 0; k < dim; ++k) {
-      int t = map->map[(size_t)e * dim + k];
+      int t = host_map(map)[(size_t)e * dim + k];
       entries[fill[t]++] = e * dim + k;
     }
   std::vector<int> offsets(targets.size() + 1);

@@ -66,6 +66,11 @@ LoopShape loop_shape(op_set set, int nargs, const op_arg *args);
 // sum increments in element order use it to keep the application's order.
 const int *old_index_host(op_set set);
 
+// A map's table in the device numbering, which plans index: under
+// transparent renumbering the runtime's renumbered copy (the op_map's own
+// table keeps the application's numbering), else the op_map's table.
+const int *host_map(op_map m);
+
 // Greedy first-fit colouring in element order with 32 colours per pass
 // (op_rt_support.cpp:684-754 for elements, :798-884 for blocks).
 // Returns colours per item and the number of colours.

@@ -355,7 +355,7 @@ std::vector<op2h_check_access> accesses(op_set set, int nargs, const op_arg *arg
   for (int m = 0; m < nargs; ++m) {
     if (sh.dat_slot[m] < 0) continue;
     const op_map mp = args[m].map;
-    acc.push_back({mp->map, mp->dim, args[m].idx, mp->to->size, sh.dat_slot[m],
+    acc.push_back({host_map(mp), mp->dim, args[m].idx, mp->to->size, sh.dat_slot[m],
                    sh.conflict[m] ? 1 : 0});
   }
   return acc;
@@ -396,7 +396,7 @@ void check_staged_or_exit(const StagedPlanRec &rec, const char *name, op_set set
     long long tot = 0;
     for (int b = 0; b < nb; ++b) tot += is[s][b];
     im[s] = fetch(P.ind_map[s], (size_t)tot);
-    h.map_table[s] = sh.maps[s]->map;
+    h.map_table[s] = host_map(sh.maps[s]);
     h.map_dim[s] = sh.maps[s]->dim;
     h.map_to_size[s] = sh.maps[s]->to->size;
     h.ind_offs[s] = io[s].data();
@@ -438,7 +438,7 @@ void check_owner_or_exit(const OwnerPlanRec &rec, const char *name, int nargs,
   h.ninc = P.ninc;
   for (int m = 0; m < nargs; ++m)
     if (P.inc_of_arg[m] >= 0) {
-      h.map_table = args[m].map->map;
+      h.map_table = host_map(args[m].map);
       h.map_dim = args[m].map->dim;
       h.inc_idx[P.inc_of_arg[m]] = args[m].idx;
     }
@@ -467,7 +467,7 @@ void check_gather_or_exit(const GatherPlanRec &rec, op_map map) {
   auto off = fetch(P.offsets, (size_t)P.ntargets + 1);
   auto ent = fetch(P.entries, (size_t)P.from_size * P.dim);
   auto tg = fetch(P.targets, (size_t)P.ntargets);
-  verdict(op2h_check_gather_plan(P.from_size, P.to_size, P.dim, map->map, P.ntargets,
+  verdict(op2h_check_gather_plan(P.from_size, P.to_size, P.dim, host_map(map), P.ntargets,
                                  off.data(), ent.data(), tg.data(), old_index_host(map->from)),
           "gather", map->name);
 }

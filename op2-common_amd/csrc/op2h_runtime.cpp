@@ -168,6 +168,9 @@ struct Runtime {
   // sets are renumbered as op_renumber does, but the application keeps its
   // own numbering: fetches, dumps and element indices are translated
   bool transparent = false, transparent_done = false;
+  // under transparent renumbering: each map's table in the device numbering,
+  // by map index (the op_map's own table keeps the application's)
+  std::vector<std::vector<int>> dev_maps;
   std::string transparent_map;                    // base map named by the token
   std::vector<std::vector<int>> set_old_of_new;   // per renumbered set
   std::vector<int *> old_index_d;                 // device copies of those
@@ -270,15 +273,26 @@ void upload_dat(op_dat d) {
     // so a null-stream memset could land after the copy below
     HIPCHECK(hipMemsetAsync(d->data_d, 0, std::max(bytes, (size_t)64), R.stream));
   }
+  // the host rows are in the application's numbering; under transparent
+  // renumbering row i goes to the device's row p[i]
+  const int *p = device_number(s);
+  std::vector<char> tmp;
   if (!dat_is_soa(d)) {
-    HIPCHECK(hipMemcpyAsync(d->data_d, d->data, (size_t)d->size * n,
-                            hipMemcpyHostToDevice, R.stream));
+    const char *src = d->data;
+    if (p) {
+      tmp.resize((size_t)d->size * n);
+      for (size_t i = 0; i < n; ++i)
+        std::memcpy(&tmp[(size_t)p[i] * d->size], d->data + i * d->size, d->size);
+      src = tmp.data();
+    }
+    HIPCHECK(hipMemcpyAsync(d->data_d, src, (size_t)d->size * n, hipMemcpyHostToDevice,
+                            R.stream));
   } else {
     int eb = d->size / d->dim;
-    std::vector<char> tmp(bytes);
+    tmp.resize(bytes);
     for (size_t i = 0; i < n; ++i)
       for (int k = 0; k < d->dim; ++k)
-        std::memcpy(&tmp[((size_t)k * s->stride + i) * eb],
+        std::memcpy(&tmp[((size_t)k * s->stride + (p ? (size_t)p[i] : i)) * eb],
                     d->data + (i * d->dim + k) * eb, eb);
     HIPCHECK(hipMemcpyAsync(d->data_d, tmp.data(), bytes, hipMemcpyHostToDevice,
                             R.stream));
@@ -292,10 +306,18 @@ void download_dat(op_dat d) {
   if (!d->data_d) return;
   op_set s = d->set;
   size_t n = (size_t)s->size;
-  if (!dat_is_soa(d)) {
+  const int *p = device_number(s);  // device row of the application's row i
+  if (!dat_is_soa(d) && !p) {
     HIPCHECK(hipMemcpyAsync(d->data, d->data_d, (size_t)d->size * n,
                             hipMemcpyDeviceToHost, R.stream));
     HIPCHECK(hipStreamSynchronize(R.stream));
+  } else if (!dat_is_soa(d)) {
+    std::vector<char> tmp((size_t)d->size * n);
+    HIPCHECK(hipMemcpyAsync(tmp.data(), d->data_d, tmp.size(), hipMemcpyDeviceToHost,
+                            R.stream));
+    HIPCHECK(hipStreamSynchronize(R.stream));
+    for (size_t i = 0; i < n; ++i)
+      std::memcpy(d->data + i * d->size, &tmp[(size_t)p[i] * d->size], d->size);
   } else {
     int eb = d->size / d->dim;
     size_t bytes = (size_t)d->size * s->stride;
@@ -306,7 +328,7 @@ void download_dat(op_dat d) {
     for (size_t i = 0; i < n; ++i)
       for (int k = 0; k < d->dim; ++k)
         std::memcpy(d->data + (i * d->dim + k) * eb,
-                    &tmp[((size_t)k * s->stride + i) * eb], eb);
+                    &tmp[((size_t)k * s->stride + (p ? (size_t)p[i] : i)) * eb], eb);
   }
   d->dirty_hd = 0;
 }
@@ -481,6 +503,7 @@ extern "C" void op_exit(void) {
     if (p) (void)hipFree(p);
   R.old_index_d.clear();
   R.set_old_of_new.clear();
+  R.dev_maps.clear();
   R.transparent = R.transparent_done = false;
   R.transparent_map.clear();
   R.tune.clear();
@@ -552,19 +575,28 @@ extern "C" op_map op_decl_map(op_set from, op_set to, int dim, idx_l_t *imap,
              m->map[i]);
       exit(-1);
     }
-  // declared after transparent renumbering: from the application's numbering
-  // to the device's (entries through the target set's, rows through the
-  // source set's)
-  if (const int *pt = device_number(to))
-    for (size_t i = 0; i < n; ++i) m->map[i] = pt[m->map[i]];
-  if (const int *pf = device_number(from)) {
-    std::vector<int> tmp(m->map, m->map + n);
-    for (int i = 0; i < from->size; ++i)
-      std::memcpy(m->map + (size_t)pf[i] * dim, tmp.data() + (size_t)i * dim, dim * sizeof(int));
+  // declared after transparent renumbering: the device's table goes from
+  // the application's numbering to the device's (entries through the target
+  // set's, rows through the source set's); the map's own table stays as given
+  Runtime &R = rt();
+  if (R.dev_maps.size() <= (size_t)m->index) R.dev_maps.resize(m->index + 1);
+  const int *pt = device_number(to), *pf = device_number(from);
+  if (pt || pf) {
+    std::vector<int> ent(m->map, m->map + n);
+    if (pt)
+      for (size_t i = 0; i < n; ++i) ent[i] = pt[ent[i]];
+    std::vector<int> &dev = R.dev_maps[m->index];
+    if (pf) {
+      dev.resize(n);
+      for (int i = 0; i < from->size; ++i)
+        std::memcpy(&dev[(size_t)pf[i] * dim], &ent[(size_t)i * dim], dim * sizeof(int));
+    } else {
+      dev.swap(ent);
+    }
   }
   HIPCHECK(hipMalloc(&m->map_d, std::max(n, (size_t)1) * sizeof(int)));
-  HIPCHECK(hipMemcpy(m->map_d, m->map, n * sizeof(int), hipMemcpyHostToDevice));
-  rt().maps.push_back(m);
+  HIPCHECK(hipMemcpy(m->map_d, op2h::host_map(m), n * sizeof(int), hipMemcpyHostToDevice));
+  R.maps.push_back(m);
   return m;
 }
 
@@ -596,14 +628,8 @@ op_dat decl_dat(op_set set, int dim, char const *type, int size, char *data,
   d->size = dim * size;
   size_t bytes = (size_t)d->size * set->size;
   d->data = (char *)std::calloc(std::max(bytes, (size_t)1), 1);
-  if (data) {
-    if (const int *p = device_number(set)) {  // rows to the device numbering
-      for (int i = 0; i < set->size; ++i)
-        std::memcpy(d->data + (size_t)p[i] * d->size, data + (size_t)i * d->size, d->size);
-    } else {
-      std::memcpy(d->data, data, bytes);
-    }
-  }
+  // the application's rows (uploads move them to the device numbering)
+  if (data) std::memcpy(d->data, data, bytes);
   d->name = copy_str(name);
   d->type = copy_str(type);
   d->dirtybit = 1;
@@ -1102,14 +1128,15 @@ void renumber_impl(op_map base) {
   adj.reserve(off[n]);
   for (int v = 0; v < n; ++v) adj.insert(adj.end(), nbr[v].begin(), nbr[v].end());
   auto bandwidth = [&](double *avg) {
+    const int *t = op2h::host_map(base);
     long long sum = 0;
     int mx = 0;
     for (int e = 0; e < base->from->size; ++e) {
       int dist = 0;
       for (int a = 0; a < base->dim; ++a)
         for (int b = 0; b < base->dim; ++b)
-          dist = std::max(dist, std::abs(base->map[(size_t)e * base->dim + a] -
-                                         base->map[(size_t)e * base->dim + b]));
+          dist = std::max(dist, std::abs(t[(size_t)e * base->dim + a] -
+                                         t[(size_t)e * base->dim + b]));
       mx = std::max(mx, dist);
       sum += dist;
     }
@@ -1137,23 +1164,37 @@ void renumber_impl(op_map base) {
   rn.propagate(S, S);
 
   // reorder_set (op_renumber.cpp:153-198): rows of maps from the set and of
-  // dats on it move; entries of maps into the set are renumbered
+  // dats on it move; entries of maps into the set are renumbered.  Under
+  // transparent renumbering the application's tables and rows stay as they
+  // are: the maps' device tables are renumbered copies, and dats move to the
+  // device numbering as they are uploaded.
+  const bool keep = R.transparent;
+  if (keep) {
+    R.dev_maps.resize(R.maps.size());
+    for (op_map m : R.maps)
+      if (R.dev_maps[m->index].empty())
+        R.dev_maps[m->index].assign(m->map, m->map + (size_t)m->from->size * m->dim);
+  }
+  auto table = [&](op_map m) { return keep ? R.dev_maps[m->index].data() : m->map; };
   for (op_set s : R.sets) {
     const std::vector<int> &p = rn.perm[s->index];
     if (p.empty() || s->size == 0) continue;
     for (op_map m : R.maps) {
       if (m->from == s) {
-        std::vector<int> tmp(m->map, m->map + (size_t)s->size * m->dim);
+        int *t = table(m);
+        std::vector<int> tmp(t, t + (size_t)s->size * m->dim);
         for (int i = 0; i < s->size; ++i)
           std::copy(tmp.begin() + (size_t)i * m->dim, tmp.begin() + (size_t)(i + 1) * m->dim,
-                    m->map + (size_t)p[i] * m->dim);
+                    t + (size_t)p[i] * m->dim);
       }
     }
     for (op_map m : R.maps)
-      if (m->to == s)
-        for (size_t i = 0; i < (size_t)m->from->size * m->dim; ++i) m->map[i] = p[m->map[i]];
+      if (m->to == s) {
+        int *t = table(m);
+        for (size_t i = 0; i < (size_t)m->from->size * m->dim; ++i) t[i] = p[t[i]];
+      }
     for (op_dat d : R.dats) {
-      if (!d || d->set != s) continue;
+      if (!d || d->set != s || keep) continue;
       std::vector<char> tmp(d->data, d->data + (size_t)d->size * s->size);
       for (int i = 0; i < s->size; ++i)
         std::memcpy(d->data + (size_t)p[i] * d->size, tmp.data() + (size_t)i * d->size,
@@ -1168,7 +1209,7 @@ void renumber_impl(op_map base) {
     }
   }
   for (op_map m : R.maps)
-    HIPCHECK(hipMemcpy(m->map_d, m->map, (size_t)m->from->size * m->dim * sizeof(int),
+    HIPCHECK(hipMemcpy(m->map_d, table(m), (size_t)m->from->size * m->dim * sizeof(int),
                        hipMemcpyHostToDevice));
   for (op_dat d : R.dats)
     if (d) upload_dat(d);
@@ -1232,19 +1273,10 @@ void transparent_renumber() {
   }
 }
 
-// Host data of a dat in the application's numbering: the dat's own rows, or,
-// under transparent renumbering, a permuted copy in `buf`.
-const char *app_rows(op_dat dat, std::vector<char> &buf) {
-  Runtime &R = rt();
-  if (!R.transparent || (size_t)dat->set->index >= R.set_new_of_old.size() ||
-      R.set_new_of_old[dat->set->index].empty())
-    return dat->data;
-  const std::vector<int> &p = R.set_new_of_old[dat->set->index];
-  buf.resize((size_t)dat->size * dat->set->size);
-  for (int i = 0; i < dat->set->size; ++i)
-    std::memcpy(&buf[(size_t)i * dat->size], dat->data + (size_t)p[i] * dat->size, dat->size);
-  return buf.data();
-}
+// Host data of a dat in the application's numbering: its own rows, which
+// stay in that numbering under transparent renumbering too (uploads and
+// downloads move rows between the numberings).
+const char *app_rows(op_dat dat, std::vector<char> &) { return dat->data; }
 }  // namespace
 
 extern "C" void op_renumber(op_map base) {
@@ -1263,6 +1295,13 @@ extern "C" const int *op2h_set_old_index(op_set set) {
 }
 
 namespace op2h {
+const int *host_map(op_map m) {
+  Runtime &R = rt();
+  if ((size_t)m->index < R.dev_maps.size() && !R.dev_maps[m->index].empty())
+    return R.dev_maps[m->index].data();
+  return m->map;
+}
+
 const int *old_index_host(op_set set) {
   Runtime &R = rt();
   if (!R.transparent || (size_t)set->index >= R.set_old_of_new.size() ||
@@ -1520,8 +1559,8 @@ ModelLoop describe_loop(const char *name, op_set set, int nargs, const op_arg *a
       int mx = 0;
       for (int a : cols)
         for (int b : cols)
-          mx = std::max(mx, std::abs(main->map[(size_t)e * main->dim + a] -
-                                     main->map[(size_t)e * main->dim + b]));
+          mx = std::max(mx, std::abs(op2h::host_map(main)[(size_t)e * main->dim + a] -
+                                     op2h::host_map(main)[(size_t)e * main->dim + b]));
       sum += mx;
     }
     d.bandwidth = set->size ? sum / set->size : 0.0;

@@ -4,6 +4,7 @@ the application's constants, read from C++ sources.
 The reference's translator-v2 builds the same model with libclang
 (translator-v2/op2-translator/cpp/parser.py:90-240, op.py); this one reads the
 call sites lexically, which is enough for OP2's fixed call forms."""
+import ast
 import os
 import re
 from dataclasses import dataclass, field
@@ -108,12 +109,48 @@ def _type(s):
 
 
 def _int(s, consts, what):
+    """An integer constant expression: literals, the program's integer
+    constants and macros, + - * / % and parentheses (C semantics: division
+    truncates toward zero), as in op_decl_dat(edges, map_dim + 1, ...)."""
     s = s.strip()
     if re.fullmatch(r"-?\d+", s):
         return int(s)
     if s in consts:
         return consts[s]
-    raise OpError(f"{what} must be an integer constant, got {s!r}")
+    try:
+        v = _eval_int(ast.parse(s, mode="eval").body, consts)
+    except (SyntaxError, ValueError, ZeroDivisionError):
+        v = None
+    if v is None:
+        raise OpError(f"{what} must be an integer constant, got {s!r}")
+    return v
+
+
+def _eval_int(node, consts):
+    if isinstance(node, ast.Constant) and type(node.value) is int:
+        return node.value
+    if isinstance(node, ast.Name):
+        if node.id not in consts:
+            raise ValueError(node.id)
+        return consts[node.id]
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+        v = _eval_int(node.operand, consts)
+        return -v if isinstance(node.op, ast.USub) else v
+    if isinstance(node, ast.BinOp):
+        a, b = _eval_int(node.left, consts), _eval_int(node.right, consts)
+        if isinstance(node.op, ast.Add):
+            return a + b
+        if isinstance(node.op, ast.Sub):
+            return a - b
+        if isinstance(node.op, ast.Mult):
+            return a * b
+        if isinstance(node.op, (ast.Div, ast.FloorDiv)):
+            q = abs(a) // abs(b)
+            return q if (a >= 0) == (b >= 0) else -q
+        if isinstance(node.op, ast.Mod):
+            r = abs(a) % abs(b)
+            return r if a >= 0 else -r
+    raise ValueError("not an integer constant expression")
 
 
 def _map(s):

@@ -35,6 +35,12 @@ REF_PROGRAMS = {
     "airfoil_ref_sp": "apps/c/airfoil/airfoil_plain/sp/airfoil.cpp",
     # res, adt and qold as temporary dats declared and freed every iteration
     "airfoil_ref_tempdats": "apps/c/airfoil/airfoil_tempdats/dp/airfoil.cpp",
+    # the reference's own functional tests (self-checking; exit 1 on a failure)
+    "ref_gbl": "tests/functional/gbl/gbl_tests.cpp",
+    "ref_reduc": "tests/functional/dat_reductions/reduc_tests.cpp",
+    "ref_idx": "tests/functional/idx/idx_tests.cpp",
+    "ref_strides": "tests/functional/strides/stride_tests.cpp",
+    "ref_const": "tests/functional/const/const_tests.cpp",
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
