@@ -449,14 +449,17 @@ __device__ __forceinline__ void owner_apply(T *acc, const T *lf, int ent) {
 //     evaluated by both neighbouring blocks), from global memory, into LDS;
 //  2. each owned cell adds its edges' fluxes to res in ascending edge order
 //     (+f where it is ecell[0], -f where it is ecell[1]), as the sequential
-//     executor does, so res is bit-identical to it.
+//     executor does, so res is bit-identical to it.  zero_acc: res is known
+//     to hold +0.0 in every owned cell (the update before it zeroed it), so
+//     the sums start from +0.0 instead of reading it: the same bits.
 // lf: LDS for the block's fluxes, [max_e][4].
 template <bool SOA, bool C4>
 __device__ __forceinline__ void owner_res_block(int b, const double *__restrict x,
                                                 const double *__restrict q,
                                                 const double *__restrict adt,
                                                 double *__restrict res, const OwnerArgs &P,
-                                                int sx, int sq, double *lf) {
+                                                int sx, int sq, double *lf,
+                                                bool zero_acc = false) {
   const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
   const int2 bb = P.base[b];
   for (int j = threadIdx.x; j < ne; j += blockDim.x) {
@@ -472,8 +475,8 @@ __device__ __forceinline__ void owner_res_block(int b, const double *__restrict 
   const int o0 = P.owned_offs[b], no = P.owned_offs[b + 1] - o0;
   for (int i = threadIdx.x; i < no; i += blockDim.x) {
     const int cell = P.owned[o0 + i];
-    double acc[4];
-    ld4<SOA>(res, cell, sq, acc);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!zero_acc) ld4<SOA>(res, cell, sq, acc);
     if (C4) {
       const ushort4 e = P.csr4[o0 + i];
       if (e.x != 0xFFFF) owner_apply(acc, lf, e.x);
@@ -509,7 +512,8 @@ __device__ __forceinline__ void owner_res_bres_block(int b, const double *__rest
                                                      const double *__restrict adt,
                                                      double *__restrict res,
                                                      const OwnerArgs &P, const BoundaryArgs &B,
-                                                     int sx, int sq, double *lf) {
+                                                     int sx, int sq, double *lf,
+                                                     bool zero_acc = false) {
   const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
   const int2 bb = P.base[b];
   for (int j = threadIdx.x; j < ne; j += blockDim.x) {
@@ -526,8 +530,8 @@ __device__ __forceinline__ void owner_res_bres_block(int b, const double *__rest
   const int bb0 = B.offs[b];
   for (int i = threadIdx.x; i < no; i += blockDim.x) {
     const int cell = P.owned[o0 + i];
-    double acc[4];
-    ld4<SOA>(res, cell, sq, acc);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!zero_acc) ld4<SOA>(res, cell, sq, acc);
     const ushort4 e = B.rec[o0 + i];
     const unsigned short ent[4] = {e.x, e.y, e.z, e.w};
 #pragma unroll

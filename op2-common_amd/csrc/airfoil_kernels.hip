@@ -544,9 +544,9 @@ template <bool SOA, bool C4>
 __global__ void __launch_bounds__(1024)
     k_res_owner(const double *__restrict x, const double *__restrict q,
                 const double *__restrict adt, double *__restrict res,
-                OwnerArgs P, int sx, int sq) {
+                OwnerArgs P, int sx, int sq, int zero_acc) {
   extern __shared__ double lf[];
-  owner_res_block<SOA, C4>(xcd_block(kXcds), x, q, adt, res, P, sx, sq, lf);
+  owner_res_block<SOA, C4>(xcd_block(kXcds), x, q, adt, res, P, sx, sq, lf, zero_acc != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1154,10 +1154,13 @@ __global__ void __launch_bounds__(1024)
                    double *adt, double *qold_out, const double *__restrict x,
                    const int *__restrict pcell, int n, int s, int sx,
                    double *__restrict part_sum, double *__restrict part_max,
-                   int *__restrict part_loc, const int *__restrict oldidx, int chunk) {
+                   int *__restrict part_loc, const int *__restrict oldidx, int chunk,
+                   int res_dead) {
   // qold (read by update, written by save_soln) and adt (read by update,
   // written by adt_calc) are read before they are written, by the same
-  // thread for the same cell: not restrict-qualified
+  // thread for the same cell: not restrict-qualified.  res_dead: the update's
+  // zeros in res are not stored (the res_calc that follows in the same step
+  // starts its sums from +0.0 instead of reading them)
   __shared__ double s_sum[16];
   __shared__ double s_max[16];
   __shared__ int s_loc[16];
@@ -1177,7 +1180,7 @@ __global__ void __launch_bounds__(1024)
       ml.v = mx;
       ml.loc = loc;
       st4<SOA>(q, c, s, qn);
-      st4<SOA>(res, c, s, r);
+      if (!res_dead) st4<SOA>(res, c, s, r);
     } else {
       ld4<SOA>(q, c, s, qn);
     }
@@ -1220,9 +1223,9 @@ template <bool SOA>
 __global__ void __launch_bounds__(1024)
     k_res_bres_owner(const double *__restrict x, const double *__restrict q,
                      const double *__restrict adt, double *__restrict res, OwnerArgs P,
-                     BoundaryArgs B, int sx, int sq) {
+                     BoundaryArgs B, int sx, int sq, int zero_acc) {
   extern __shared__ double lf[];
-  owner_res_bres_block<SOA>(xcd_block(kXcds), x, q, adt, res, P, B, sx, sq, lf);
+  owner_res_bres_block<SOA>(xcd_block(kXcds), x, q, adt, res, P, B, sx, sq, lf, zero_acc != 0);
 }
 
 // A pending run of direct loops over one cell set: [update] [save_soln].
@@ -1274,6 +1277,35 @@ void flush_direct(void *) {
              P.st, (const double2 *)P.save_src, (double2 *)P.save_dst, P.save_n2);
 }
 
+// A pending fused direct chain ([update] [save_soln] adt_calc per cell).
+using ChainKernel = void (*)(const double *, double *, double *, double *, double *,
+                             const double *, const int *, int, int, int, double *, double *,
+                             int *, const int *, int, int);
+struct PendingChain {
+  ChainKernel kern = nullptr;
+  PendingDirect P;
+  int g = 0, bs = 0, chunk = 0, n = 0, sq = 0, sx = 0;
+  double *q = nullptr, *adt = nullptr;
+  const double *x = nullptr;
+  const int *pcell = nullptr;
+  hipStream_t st = nullptr;
+};
+PendingChain g_chain;
+
+// res_dead: the update's zeros in res are not stored (the owner res_calc
+// launched right after it sums from +0.0 over every cell of the set)
+void launch_chain(bool res_dead) {
+  const PendingChain C = g_chain;
+  g_chain = PendingChain();
+  const PendingDirect &P = C.P;
+  launch_k(false, nullptr, nullptr, C.kern, C.g, C.bs, 0, C.st, P.qold, C.q, P.res, C.adt,
+           P.save_dst, C.x, C.pcell, C.n, C.sq, C.sx, P.part_sum, P.part_max, P.part_loc,
+           P.oldidx, C.chunk, (int)(res_dead && P.upd));
+  KCHECK(hipGetLastError());
+}
+
+void flush_chain(void *) { launch_chain(false); }
+
 // A pending owner-plan res_calc.
 struct PendingRes {
   bool soa = false;
@@ -1285,6 +1317,7 @@ struct PendingRes {
   int sx = 0, sq = 0, bs = 0;
   size_t lds = 0;
   op_map pecell = nullptr;
+  bool zero = false;  // res holds the update's zeros: sums start from +0.0
 };
 PendingRes g_res;
 PendingRes g_last_res;  // the latest owner-plan res_calc, pending or not
@@ -1294,7 +1327,8 @@ void launch_res_owner(const PendingRes &P) {
     launch_k(false, nullptr, nullptr,
              P.A.csr4 ? (P.soa ? k_res_owner<true, true> : k_res_owner<false, true>)
                       : (P.soa ? k_res_owner<true, false> : k_res_owner<false, false>),
-             P.plan->nblocks, P.bs, P.lds, P.st, P.x, P.q, P.adt, P.res, P.A, P.sx, P.sq);
+             P.plan->nblocks, P.bs, P.lds, P.st, P.x, P.q, P.adt, P.res, P.A, P.sx, P.sq,
+             (int)P.zero);
 }
 
 void flush_res(void *) {
@@ -1487,13 +1521,26 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
         // (its partials), else adt_calc's
         const int gg = P.upd ? P.g : g, bb = P.upd ? P.bs : ctx.block_size;
         const int chunk = P.upd ? P.chunk : 0;
-        auto kern = P.upd ? (P.save ? (soa ? k_direct_chain<true, true, true> : k_direct_chain<false, true, true>)
-                                    : (soa ? k_direct_chain<true, true, false> : k_direct_chain<false, true, false>))
-                          : (soa ? k_direct_chain<true, false, true> : k_direct_chain<false, false, true>);
-        launch_k(false, nullptr, nullptr, kern, gg, bb, 0, st, P.qold, P.q ? P.q : (double *)q,
-                 P.res, adt, P.save_dst, x, pcell, set->size, sq, sx, P.part_sum, P.part_max,
-                 P.part_loc, P.oldidx, chunk);
-        KCHECK(hipGetLastError());
+        PendingChain &C = g_chain;
+        C = PendingChain();
+        C.kern = P.upd ? (P.save ? (soa ? k_direct_chain<true, true, true> : k_direct_chain<false, true, true>)
+                                 : (soa ? k_direct_chain<true, true, false> : k_direct_chain<false, true, false>))
+                       : (soa ? k_direct_chain<true, false, true> : k_direct_chain<false, false, true>);
+        C.P = P;
+        C.g = gg;
+        C.bs = bb;
+        C.chunk = chunk;
+        C.q = P.q ? P.q : (double *)q;
+        C.adt = adt;
+        C.x = x;
+        C.pcell = pcell;
+        C.n = set->size;
+        C.sq = sq;
+        C.sx = sx;
+        C.st = st;
+        // held for the res_calc that follows, which may take the update's
+        // zeros in res as known (launched as it is by anything else)
+        op2h_defer("chain", flush_chain, nullptr);
         account(1, set, 6, args);
         op2h_loop_end(&ctx);
         return;
@@ -1529,8 +1576,18 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
                                                op_arg arg6, op_arg arg7) {
   op_arg args[8] = {arg0, arg1, arg2, arg3, arg4, arg5, arg6, arg7};
   op2h_loop_ctx ctx;
+  op2h_accept_deferred("chain");
   op2h_loop_begin(&ctx, 2, name, set, 8, args, OP2H_PLAN_STAGED, 256);
   hipStream_t st = (hipStream_t)ctx.stream;
+  // a fused direct chain held for this loop: launched here, before it, with
+  // or without the update's zeros in res (only an owner plan covering every
+  // cell can take them as known)
+  bool chain = op2h_take_deferred("chain", nullptr) != 0;
+  if (chain && (!op2h_fusion_enabled() || set->size == 0 || ctx.plan != OP2H_PLAN_OWNER ||
+                is_float(args[6]))) {
+    flush_chain(nullptr);
+    chain = false;
+  }
   if (set->size > 0) {
     int dummy;
     bool soa = loop_soa(args, 8, &dummy);
@@ -1663,6 +1720,13 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
           printf("op2hip: res_calc owner plan needs %zu bytes of LDS; use a smaller block\n", lds);
           exit(-1);
         }
+        // res is dead between the chain's update (which sets every cell's
+        // res to +0.0) and this loop when the plan owns every cell: the
+        // zeros are neither stored nor read
+        const bool zero = chain && g_chain.P.upd && g_chain.P.res == res &&
+                          p->n_owned == args[6].dat->set->size;
+        if (chain) launch_chain(zero);
+        chain = false;
         PendingRes R;
         R.soa = soa;
         R.st = st;
@@ -1677,6 +1741,7 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
         R.bs = bs;
         R.lds = lds;
         R.pecell = args[6].map;
+        R.zero = zero;
         g_last_res = R;
         if (op2h_fusion_enabled()) {  // held for the bres_calc that follows
           g_res = R;
@@ -1688,7 +1753,7 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
           launch_k(tm.ext, tm.e0, tm.e1,
                    A.csr4 ? (soa ? k_res_owner<true, true> : k_res_owner<false, true>)
                           : (soa ? k_res_owner<true, false> : k_res_owner<false, false>),
-                   p->nblocks, bs, lds, st, x, q, adt, res, A, sx, sq);
+                   p->nblocks, bs, lds, st, x, q, adt, res, A, sx, sq, (int)zero);
         tm.done();
         break;
       }
@@ -1732,6 +1797,7 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
     }
     KCHECK(hipGetLastError());
   }
+  if (chain) flush_chain(nullptr);  // (every path above has launched it)
   account(2, set, 8, args);
   op2h_loop_end(&ctx);
 }
@@ -1777,7 +1843,8 @@ extern "C" void op_par_loop_airfoil_4_bres_calc(char const *name, op_set set,
         g_res = PendingRes();
         if (P.plan->nblocks)
           launch_k(false, nullptr, nullptr, soa ? k_res_bres_owner<true> : k_res_bres_owner<false>,
-                   P.plan->nblocks, P.bs, P.lds, st, x, q, adt, res, P.A, *B, sx, sq);
+                   P.plan->nblocks, P.bs, P.lds, st, x, q, adt, res, P.A, *B, sx, sq,
+                   (int)P.zero);
         KCHECK(hipGetLastError());
         account(3, set, 6, args);
         op2h_loop_end(&ctx);
