@@ -19,6 +19,7 @@
 #include <hip/hip_ext.h>
 
 #include <tuple>
+#include <type_traits>
 #include <utility>
 
 #include <algorithm>
@@ -957,7 +958,7 @@ template <bool SOA, bool C4>
 __global__ void __launch_bounds__(1024)
     k_res_owner_mixed(const double *__restrict x, const double *__restrict q,
                       const float *__restrict adt, float *__restrict res,
-                      OwnerArgs P, int sx, int sq) {
+                      OwnerArgs P, int sx, int sq, int zero_acc) {
   extern __shared__ float lff[];
   const int b = xcd_block(kXcds);
   const int e0 = P.elem_offs[b], ne = P.elem_offs[b + 1] - e0;
@@ -975,8 +976,8 @@ __global__ void __launch_bounds__(1024)
   const int o0 = P.owned_offs[b], no = P.owned_offs[b + 1] - o0;
   for (int i = threadIdx.x; i < no; i += blockDim.x) {
     const int cell = P.owned[o0 + i];
-    float acc[4];
-    ld4s<SOA>(res, cell, sq, acc);
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    if (!zero_acc) ld4s<SOA>(res, cell, sq, acc);
     if (C4) {
       const ushort4 e = P.csr4[o0 + i];
       if (e.x != 0xFFFF) owner_apply(acc, lff, e.x);
@@ -1048,14 +1049,14 @@ __global__ void __launch_bounds__(1024)
     k_update_mixed(const double *__restrict qold, double *__restrict q,
                    float *__restrict res, const float *__restrict adt, int n, int s,
                    double *__restrict part_sum, double *__restrict part_max,
-                   int *__restrict part_loc, const int *__restrict oldidx) {
+                   int *__restrict part_loc, const int *__restrict oldidx, int chunk) {
   __shared__ double s_sum[16];
   __shared__ double s_max[16];
   __shared__ int s_loc[16];
   double rms = 0.0;
   MaxLoc ml{-INFINITY, 0x7fffffff};
-  const int stride = gridDim.x * blockDim.x;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+  const CellRange cr = cell_range(n, chunk);
+  for (int c = cr.c; c < cr.end; c += cr.step) {
     double qo[4], qn[4], r[4], a = (double)adt[c], mx = ml.v;
     float rf[4];
     int loc = ml.loc;
@@ -1088,9 +1089,9 @@ __global__ void __launch_bounds__(1024)
       bs += s_sum[w];
       bm = maxloc_combine(bm, MaxLoc{s_max[w], s_loc[w]});
     }
-    part_sum[blockIdx.x] = bs;
-    part_max[blockIdx.x] = bm.v;
-    part_loc[blockIdx.x] = bm.loc;
+    part_sum[cr.L] = bs;
+    part_max[cr.L] = bm.v;
+    part_loc[cr.L] = bm.loc;
   }
 }
 
@@ -1148,10 +1149,13 @@ void account(int kernel_idx, op_set set, int nargs, const op_arg *args) {
 // ---------------------------------------------------------------------------
 namespace {
 
-template <bool SOA, bool UPD, bool SAVE>
+// R: the type of res and adt, double, or float in mixed mode (the update in
+// FP64 from FP32 res and adt, adt_calc in FP32 from x and q rounded, as
+// k_update_mixed and k_adt_calc_mixed).
+template <class R, bool SOA, bool UPD, bool SAVE>
 __global__ void __launch_bounds__(1024)
-    k_direct_chain(const double *qold, double *__restrict q, double *__restrict res,
-                   double *adt, double *qold_out, const double *__restrict x,
+    k_direct_chain(const double *qold, double *__restrict q, R *__restrict res,
+                   R *adt, double *qold_out, const double *__restrict x,
                    const int *__restrict pcell, int n, int s, int sx,
                    double *__restrict part_sum, double *__restrict part_max,
                    int *__restrict part_loc, const int *__restrict oldidx, int chunk,
@@ -1161,6 +1165,7 @@ __global__ void __launch_bounds__(1024)
   // thread for the same cell: not restrict-qualified.  res_dead: the update's
   // zeros in res are not stored (the res_calc that follows in the same step
   // starts its sums from +0.0 instead of reading them)
+  constexpr bool MIX = std::is_same<R, float>::value;
   __shared__ double s_sum[16];
   __shared__ double s_max[16];
   __shared__ int s_loc[16];
@@ -1171,28 +1176,52 @@ __global__ void __launch_bounds__(1024)
   for (int c = cr.c; c < cr.end; c += cr.step) {
     double qn[4];
     if (UPD) {
-      double qo[4], r[4], a = adt[c], mx = ml.v;
+      double qo[4], r[4], a = (double)adt[c], mx = ml.v;
       int loc = ml.loc;
       ld4<SOA>(qold, c, s, qo);
-      ld4<SOA>(res, c, s, r);
+      if constexpr (MIX) {
+        float rf[4];
+        ld4s<SOA>(res, c, s, rf);
+        for (int d = 0; d < 4; ++d) r[d] = (double)rf[d];
+      } else {
+        ld4<SOA>(res, c, s, r);
+      }
       int ci = oldidx ? oldidx[c] : c;
       op2_k::update(qo, qn, r, &a, &rms, &mx, &ci, &loc);
       ml.v = mx;
       ml.loc = loc;
       st4<SOA>(q, c, s, qn);
-      if (!res_dead) st4<SOA>(res, c, s, r);
+      if (!res_dead) {
+        if constexpr (MIX) {
+          const float z[4] = {0.f, 0.f, 0.f, 0.f};
+          st4s<SOA>(res, c, s, z);
+        } else {
+          st4<SOA>(res, c, s, r);
+        }
+      }
     } else {
       ld4<SOA>(q, c, s, qn);
     }
     if (SAVE) st4<SOA>(qold_out, c, s, qn);
     const int4 cn = reinterpret_cast<const int4 *>(pcell)[c];
-    double x1[2], x2[2], x3[2], x4[2], a2;
-    ld2<SOA>(x, cn.x, sx, x1);
-    ld2<SOA>(x, cn.y, sx, x2);
-    ld2<SOA>(x, cn.z, sx, x3);
-    ld2<SOA>(x, cn.w, sx, x4);
-    op2_k::adt_calc(x1, x2, x3, x4, qn, &a2);
-    adt[c] = a2;
+    if constexpr (MIX) {
+      float x1[2], x2[2], x3[2], x4[2], qf[4], a2;
+      ld2f<SOA>(x, cn.x, sx, x1);
+      ld2f<SOA>(x, cn.y, sx, x2);
+      ld2f<SOA>(x, cn.z, sx, x3);
+      ld2f<SOA>(x, cn.w, sx, x4);
+      for (int d = 0; d < 4; ++d) qf[d] = (float)qn[d];
+      op2_kf::adt_calc(x1, x2, x3, x4, qf, &a2);
+      adt[c] = a2;
+    } else {
+      double x1[2], x2[2], x3[2], x4[2], a2;
+      ld2<SOA>(x, cn.x, sx, x1);
+      ld2<SOA>(x, cn.y, sx, x2);
+      ld2<SOA>(x, cn.z, sx, x3);
+      ld2<SOA>(x, cn.w, sx, x4);
+      op2_k::adt_calc(x1, x2, x3, x4, qn, &a2);
+      adt[c] = a2;
+    }
   }
   if (!UPD) return;
   // the partials of k_update, by the same tree
@@ -1253,8 +1282,10 @@ struct PendingDirect {
   int n = 0, s = 0;
   // update (its partials handed to the graph's history kernel)
   int g = 0, bs = 0, chunk = 0;
+  bool mixed = false;  // res and adt FP32 (res_f, adt_f), else FP64 (res, adt)
   const double *qold = nullptr;
   double *q = nullptr, *res = nullptr, *adt = nullptr;
+  float *res_f = nullptr, *adt_f = nullptr;
   double *part_sum = nullptr, *part_max = nullptr;
   int *part_loc = nullptr;
   const int *oldidx = nullptr;
@@ -1268,7 +1299,11 @@ PendingDirect g_direct;
 void flush_direct(void *) {
   PendingDirect P = g_direct;
   g_direct = PendingDirect();
-  if (P.upd)
+  if (P.upd && P.mixed)
+    launch_k(false, nullptr, nullptr, P.soa ? k_update_mixed<true> : k_update_mixed<false>, P.g,
+             P.bs, 0, P.st, P.qold, P.q, P.res_f, (const float *)P.adt_f, P.n, P.s, P.part_sum,
+             P.part_max, P.part_loc, P.oldidx, P.chunk);
+  else if (P.upd)
     launch_k(false, nullptr, nullptr, P.soa ? k_update<true> : k_update<false>, P.g, P.bs, 0,
              P.st, P.qold, P.q, P.res, (const double *)P.adt, P.n, P.s, P.part_sum,
              P.part_max, P.part_loc, P.oldidx, P.chunk);
@@ -1278,11 +1313,14 @@ void flush_direct(void *) {
 }
 
 // A pending fused direct chain ([update] [save_soln] adt_calc per cell).
-using ChainKernel = void (*)(const double *, double *, double *, double *, double *,
-                             const double *, const int *, int, int, int, double *, double *,
-                             int *, const int *, int, int);
+template <class R>
+using ChainKernel = void (*)(const double *, double *, R *, R *, double *, const double *,
+                             const int *, int, int, int, double *, double *, int *,
+                             const int *, int, int);
 struct PendingChain {
-  ChainKernel kern = nullptr;
+  ChainKernel<double> kern = nullptr;  // FP64 res and adt
+  ChainKernel<float> kern_f = nullptr;  // mixed mode
+  float *adt_f = nullptr;
   PendingDirect P;
   int g = 0, bs = 0, chunk = 0, n = 0, sq = 0, sx = 0;
   double *q = nullptr, *adt = nullptr;
@@ -1298,9 +1336,14 @@ void launch_chain(bool res_dead) {
   const PendingChain C = g_chain;
   g_chain = PendingChain();
   const PendingDirect &P = C.P;
-  launch_k(false, nullptr, nullptr, C.kern, C.g, C.bs, 0, C.st, P.qold, C.q, P.res, C.adt,
-           P.save_dst, C.x, C.pcell, C.n, C.sq, C.sx, P.part_sum, P.part_max, P.part_loc,
-           P.oldidx, C.chunk, (int)(res_dead && P.upd));
+  if (C.kern_f)
+    launch_k(false, nullptr, nullptr, C.kern_f, C.g, C.bs, 0, C.st, P.qold, C.q, P.res_f,
+             C.adt_f, P.save_dst, C.x, C.pcell, C.n, C.sq, C.sx, P.part_sum, P.part_max,
+             P.part_loc, P.oldidx, C.chunk, (int)(res_dead && P.upd));
+  else
+    launch_k(false, nullptr, nullptr, C.kern, C.g, C.bs, 0, C.st, P.qold, C.q, P.res, C.adt,
+             P.save_dst, C.x, C.pcell, C.n, C.sq, C.sx, P.part_sum, P.part_max, P.part_loc,
+             P.oldidx, C.chunk, (int)(res_dead && P.upd));
   KCHECK(hipGetLastError());
 }
 
@@ -1514,8 +1557,10 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
     if (held) {
       PendingDirect P = g_direct;
       const double *qsrc = P.upd ? P.q : P.save_src;
+      const bool mixed = is_float(args[5]);
       if (P.set == set && P.soa == soa && (soa ? sq == P.s : true) && qsrc == q &&
-          !is_float(args[5]) && (!P.upd || P.adt == adt)) {
+          (!P.upd || (P.mixed == mixed &&
+                      (mixed ? (void *)P.adt_f == args[5].data_d : P.adt == adt)))) {
         g_direct = PendingDirect();
         // [update] [save_soln] adt_calc per cell: the update's launch shape
         // (its partials), else adt_calc's
@@ -1523,9 +1568,16 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
         const int chunk = P.upd ? P.chunk : 0;
         PendingChain &C = g_chain;
         C = PendingChain();
-        C.kern = P.upd ? (P.save ? (soa ? k_direct_chain<true, true, true> : k_direct_chain<false, true, true>)
-                                 : (soa ? k_direct_chain<true, true, false> : k_direct_chain<false, true, false>))
-                       : (soa ? k_direct_chain<true, false, true> : k_direct_chain<false, false, true>);
+        if (mixed) {
+          C.kern_f = P.upd ? (P.save ? (soa ? k_direct_chain<float, true, true, true> : k_direct_chain<float, false, true, true>)
+                                     : (soa ? k_direct_chain<float, true, true, false> : k_direct_chain<float, false, true, false>))
+                           : (soa ? k_direct_chain<float, true, false, true> : k_direct_chain<float, false, false, true>);
+          C.adt_f = (float *)args[5].data_d;
+        } else {
+          C.kern = P.upd ? (P.save ? (soa ? k_direct_chain<double, true, true, true> : k_direct_chain<double, false, true, true>)
+                                   : (soa ? k_direct_chain<double, true, true, false> : k_direct_chain<double, false, true, false>))
+                         : (soa ? k_direct_chain<double, true, false, true> : k_direct_chain<double, false, false, true>);
+        }
         C.P = P;
         C.g = gg;
         C.bs = bb;
@@ -1583,8 +1635,7 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
   // or without the update's zeros in res (only an owner plan covering every
   // cell can take them as known)
   bool chain = op2h_take_deferred("chain", nullptr) != 0;
-  if (chain && (!op2h_fusion_enabled() || set->size == 0 || ctx.plan != OP2H_PLAN_OWNER ||
-                is_float(args[6]))) {
+  if (chain && (!op2h_fusion_enabled() || set->size == 0 || ctx.plan != OP2H_PLAN_OWNER)) {
     flush_chain(nullptr);
     chain = false;
   }
@@ -1610,11 +1661,16 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
       float *resf = (float *)args[6].data_d;
       const op2h_owner_plan *owner_p = nullptr;
       OwnerArgs OA{};
+      bool zero = false;  // as in FP64 below: res dead after the chain's update
       if (ctx.plan == OP2H_PLAN_OWNER) {  // built before the timed region
         ctx.part_size = bs;
         owner_p = op2h_plan_owner_local(&ctx);
         OA = owner_args_res(owner_p, args, st);
+        zero = chain && g_chain.P.upd && g_chain.P.mixed && g_chain.P.res_f == resf &&
+               owner_p->n_owned == args[6].dat->set->size;
       }
+      if (chain) launch_chain(zero);
+      chain = false;
       op2h_kernel_timer_start(&ctx);
       if (ctx.plan == OP2H_PLAN_ATOMIC) {
         int g = grid_for(set->size, bs);
@@ -1631,7 +1687,7 @@ extern "C" void op_par_loop_airfoil_3_res_calc(char const *name, op_set set,
           auto kern = OA.csr4 ? (soa ? k_res_owner_mixed<true, true> : k_res_owner_mixed<false, true>)
                               : (soa ? k_res_owner_mixed<true, false> : k_res_owner_mixed<false, false>);
           launch_k(false, nullptr, nullptr, kern, nblocks, bs, lds, st, x, q, adtf, resf, OA,
-                   sx, sq);
+                   sx, sq, (int)zero);
         }
       } else {
         const op2h_gather_plan *p = op2h_plan_gather(args[6].map);
@@ -1988,12 +2044,13 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
     printf("op2hip: update needs res and adt both FP32 (mixed mode) or both FP64\n");
     exit(-1);
   }
-  if (deferred && !is_float(args[2]) && op2h_fusion_enabled() && set->size > 0) {
+  if (deferred && op2h_fusion_enabled() && set->size > 0) {
     // held for the save_soln / adt_calc that follow (its partials are
     // finished by the graph's history kernel either way)
     PendingDirect &P = g_direct;
     P = PendingDirect();
     P.upd = true;
+    P.mixed = is_float(args[2]);
     P.set = set;
     P.soa = soa;
     P.st = st;
@@ -2004,8 +2061,13 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
     P.chunk = shape.chunk;
     P.qold = (const double *)args[0].data_d;
     P.q = (double *)args[1].data_d;
-    P.res = (double *)args[2].data_d;
-    P.adt = (double *)args[3].data_d;
+    if (P.mixed) {
+      P.res_f = (float *)args[2].data_d;
+      P.adt_f = (float *)args[3].data_d;
+    } else {
+      P.res = (double *)args[2].data_d;
+      P.adt = (double *)args[3].data_d;
+    }
     P.part_sum = part_sum;
     P.part_max = part_max;
     P.part_loc = part_loc;
@@ -2020,7 +2082,7 @@ extern "C" void op_par_loop_airfoil_5_update(char const *name, op_set set,
     launch_k(tm.ext, tm.e0, nullptr, soa ? k_update_mixed<true> : k_update_mixed<false>, g, bs,
              0, st, (const double *)args[0].data_d, (double *)args[1].data_d,
              (float *)args[2].data_d, (const float *)args[3].data_d, set->size, s, part_sum,
-             part_max, part_loc, op2h_set_old_index(set));
+             part_max, part_loc, op2h_set_old_index(set), shape.chunk);
   } else {
     launch_k(tm.ext, tm.e0, nullptr, soa ? k_update<true> : k_update<false>, g, bs, 0, st,
              (const double *)args[0].data_d, (double *)args[1].data_d,

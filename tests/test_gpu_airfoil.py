@@ -265,3 +265,29 @@ def test_streaming_shapes_same_bits(kw):
         assert np.array_equal(r["q"], g["q"])
         assert np.array_equal(r["errloc"], g["errloc"])
         assert pointwise_rel_err(r["rms"], g["rms"]) <= TOL
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(soa=True)], ids=["aos", "soa"])
+def test_fused_loop_chains_mixed_mode_equal_the_unfused_loops(kw):
+    """Mixed mode (FP32 res and adt) inside captured steps: the direct loops
+    fused per cell (the FP64 update from FP32 res and adt, adt_calc in FP32)
+    and res_calc summing from +0.0 where the update's zeros are dead give
+    every iteration's rms, maxerr and errloc and the final q of the unfused
+    graph bit for bit."""
+    import op2amd.api as api
+    g = golden("airfoil_120x60_random_a1")
+    # bres_calc on the gather plan: atomics would add corner cells' two
+    # boundary fluxes in whichever order they land
+    base = dict(mode="graphed", plan_res="owner", block_res=128, plan_bres="gather",
+                graph_steps=10, warmup=10, mixed=True)
+    base.update(kw)
+    n0 = api.op2h_fused_count()
+    fused = op2amd.airfoil.run(mesh_of(g), 400, **base)
+    assert api.op2h_fused_count() - n0 >= 4 * fused["graph_steps"]
+    api.op2h_set_fusion(0)
+    try:
+        unfused = op2amd.airfoil.run(mesh_of(g), 400, **base)
+    finally:
+        api.op2h_set_fusion(1)
+    for k in ("rms", "rms_k0", "maxerr", "errloc", "q"):
+        assert np.array_equal(fused[k], unfused[k]), k
