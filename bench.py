@@ -311,20 +311,26 @@ def workload_large(op2amd, args):
 
 
 def workload_ensemble(op2amd, args):
-    """256 uneven meshes (40x20 .. 640x320, per-member Mach and alpha):
-    one launch per loop over all members, graph-captured, block sizes picked
-    by the ensemble's autotuner over the 5 warm-up steps, vs one launch per
-    loop per member."""
+    """256 uneven meshes (40x20 .. 640x320, per-member Mach and alpha): each
+    loop one persistent launch over all members' work items, the step
+    graph-captured, workgroup sizes picked by the ensemble's autotuner over
+    the 5 warm-up steps, vs one launch per loop per member.  Per-member step
+    times: inside the batched step from device-clock stamps of every work
+    item (a separate, shorter run), in the looped run from HIP events."""
     sp = op2amd.ensemble.spec(nmesh=args.ensemble_size)
     n, W = args.ensemble_iters, 5
     batched = op2amd.ensemble.run(sp, n, warmup=W, batched=True, graphed=True, autotune=True)
     looped = op2amd.ensemble.run(sp, min(n, 30), warmup=W, batched=False)
     timing = op2amd.ensemble.run(sp, 12, warmup=2, batched=False, member_timing=True)
+    btiming = op2amd.ensemble.run(sp, 12, warmup=W + 1, batched=True, graphed=True,
+                                  autotune=True, member_timing=True)
     nc = batched["total_cells"]
     b_ups = nc * (n - W) / batched["seconds"]
     l_ups = nc * (min(n, 30) - W) / looped["seconds"]
     per = timing["member_step_ms"]
+    lat, busy = btiming["member_step_ms"], btiming["member_busy_ms"]
     same = bool(np.array_equal(batched["rms"][:min(n, 30)], looped["rms"][:min(n, 30)]))
+    pct = lambda v, p: round(float(np.percentile(v, p)), 4)
     line = {
         "metric": "ensemble cell-updates/s (256 uneven airfoil meshes, FP64)",
         "value": round(b_ups, 1), "unit": "cell-steps/s", "n_gpus": 1, "steps": n - W,
@@ -337,11 +343,17 @@ def workload_ensemble(op2amd, args):
                    "speedup_batched_vs_looped": b_ups / l_ups,
                    "batched_step_ms": 1e3 * batched["seconds"] / (n - W),
                    "looped_step_ms": 1e3 * looped["seconds"] / (min(n, 30) - W),
-                   "looped_member_step_ms_p50": float(np.percentile(per, 50)),
-                   "looped_member_step_ms_p99": float(np.percentile(per, 99)),
+                   "batched_member_step_ms_p50": pct(lat, 50),
+                   "batched_member_step_ms_p99": pct(lat, 99),
+                   "batched_member_busy_ms_p50": pct(busy, 50),
+                   "batched_member_busy_ms_p99": pct(busy, 99),
+                   "looped_member_step_ms_p50": pct(per, 50),
+                   "looped_member_step_ms_p99": pct(per, 99),
                    "looped_member_step_ms_sum": float(per.sum()),
                    "rms_identical_batched_vs_looped": same,
-                   "batched_block_sizes": batched["blocks"]},
+                   "batched_workgroup_sizes": batched["blocks"],
+                   "batched_persistent_grids": batched["grids"],
+                   "dead_res": batched["dead_res"]},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

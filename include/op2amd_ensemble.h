@@ -3,12 +3,18 @@
  *
  * An ensemble is many airfoil problems of different sizes and free streams
  * advanced in lock step.  All member meshes are concatenated into one set of
- * nodes/cells/edges/bedges with per-member offset tables; each airfoil loop
- * is then ONE launch over every member (no edge crosses members, so the
- * concatenation is exact), the per-member rms/maxerr/errloc reductions run
- * through a tile table whose tiles never straddle two members, and the whole
- * time step is captured in one HIP graph.  The baseline launches every loop
- * once per member.  The arithmetic per element is the single-mesh airfoil's
+ * nodes/cells/edges/bedges; each loop's work is a table of items, runs of one
+ * member's elements (tiles of <= 1024 cells, the owner patches of res_calc,
+ * <= 64 boundary cells of bres_calc), with per-member offsets into it.  Each
+ * loop of the time step is ONE persistent launch over every member's items: a
+ * grid of as many workgroups as the GPU holds at once, each walking the item
+ * table (no edge crosses members, so the concatenation is exact).  update,
+ * save_soln and adt_calc of a cell run in one pass (the update of the second
+ * half step does the next step's save_soln and adt_calc; one save_soln and
+ * adt_calc launch precedes the first step), the per-member rms/maxerr/errloc
+ * reductions run through per-tile partials, and the whole time step is
+ * captured in one HIP graph.  The baseline launches every loop once per
+ * member.  The arithmetic per element is the single-mesh airfoil's
  * (owner-compute res_calc/bres_calc), so each member's trajectory equals its
  * own sequential run (airfoil.cpp:221-297) bit for bit in q.
  *
@@ -54,11 +60,16 @@ typedef struct {
   int niter, warmup;
   int batched;     /* 1: one launch per loop for all members; 0: per member */
   int graphed;     /* batched only: time step replayed from a HIP graph */
-  int member_timing; /* per-member step time with HIP events (looped mode) */
-  int autotune;      /* batched only: block sizes of the save_soln, adt_calc
-                        and bres_calc launches picked by timing the first
-                        five steps (one candidate of 64..1024 per step; the
-                        results do not depend on the block sizes) */
+  int member_timing; /* per-member step times: looped mode, HIP events around
+                        each member's launches; batched mode, device-clock
+                        stamps of every work item inside the batched step */
+  int autotune;      /* batched only: workgroup sizes of the update+save_soln+
+                        adt_calc and bres_calc launches picked by timing the
+                        first five steps (one candidate of 64..1024 per step;
+                        the results do not depend on them) */
+  int max_workgroups; /* > 0: at most this many workgroups per launch (each
+                         then walks more work items); 0: as many as the GPU
+                         holds at once */
 } op2amd_ensemble_opts;
 
 typedef struct {
@@ -67,14 +78,25 @@ typedef struct {
   double *maxerr;       /* [niter][nmesh] */
   int *errloc;          /* [niter][nmesh] cell index within the member */
   double *q_out;        /* [total cells][4], members in order */
-  double *member_step_ms; /* [nmesh] mean step time per member (looped mode
-                             with member_timing; batched: the whole step) */
+  double *member_step_ms; /* [nmesh] mean step time per member: looped mode
+                             with member_timing, the member's launches;
+                             batched mode with member_timing, from the step's
+                             first work item start to the member's last work
+                             item end; batched otherwise: the whole step */
+  double *member_busy_ms; /* [nmesh] batched mode with member_timing: summed
+                             over the step's launches, the member's last item
+                             end minus its first item start */
   /* filled in */
   double seconds;       /* wall time of iterations warmup+1..niter */
   double setup_seconds;
   long long total_cells, total_edges;
-  int blocks[4];        /* block sizes used: save_soln, adt_calc, res_calc
-                           (its owner patch size, 128), bres_calc */
+  int blocks[3];        /* workgroup sizes: update+save_soln+adt_calc,
+                           res_calc (its owner patch size, 128), bres_calc */
+  int grids[3];         /* persistent grids (workgroups) of the same launches
+                           over all members */
+  int dead_res;         /* 1: every cell is owned by a res_calc patch, so the
+                           update does not zero res and res_calc does not
+                           read it */
 } op2amd_ensemble_result;
 
 int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,

@@ -1,10 +1,13 @@
 // Batched airfoil ensembles (include/op2amd_ensemble.h).
 //
 // Members are concatenated into one mesh; every loop of the time step is one
-// launch over all members, except in the looped baseline, which launches each
-// loop once per member over that member's ranges.  res_calc and bres_calc use
-// the owner-compute plan (each cell sums its edges' fluxes in edge order), so
-// results do not depend on which launch structure computed them.
+// persistent launch over all members' work items (runs of one member's
+// elements, found through per-member offset tables), except in the looped
+// baseline, which launches each loop once per member over that member's
+// items.  res_calc and bres_calc use the owner-compute plan (each cell sums
+// its edges' fluxes in edge order), and the reductions' partials are formed
+// per tile, so results do not depend on the launch structure or the
+// workgroup sizes.
 #include "op2amd_ensemble.h"
 
 #include <hip/hip_runtime.h>
@@ -57,38 +60,45 @@ double now() {
 
 // ---- kernels ------------------------------------------------------------
 
-__global__ void k_ens_copy(const double2 *__restrict q, double2 *__restrict qold,
-                           long long lo2, long long hi2) {
-  long long i = lo2 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < hi2) qold[i] = q[i];
+// Device-clock stamps of work items, for per-member step times inside the
+// batched step: item i of the launch at offset `off` of the step writes
+// {begin, end} (wall_clock64 ticks) at t[2 * (slot * per_step + off + i)],
+// slot = steps completed so far (the history counter) mod capacity.  t ==
+// nullptr: no stamps.
+struct Stamps {
+  unsigned long long *t;
+  const unsigned long long *count;
+  int per_step, capacity, off;
+};
+
+__device__ __forceinline__ unsigned long long stamp_now(const Stamps &S) {
+  return S.t ? (unsigned long long)wall_clock64() : 0ull;
 }
 
-__global__ void k_ens_adt(const double *__restrict x, const double *__restrict q,
-                          double *__restrict adt, const int4 *__restrict pcell,
-                          int lo, int hi) {
-  int c = lo + xcd_block(kXcds) * blockDim.x + threadIdx.x;
-  if (c >= hi) return;
-  int4 cn = pcell[c];
-  double x1[2], x2[2], x3[2], x4[2], qc[4], a;
-  ld2<false>(x, cn.x, 0, x1);
-  ld2<false>(x, cn.y, 0, x2);
-  ld2<false>(x, cn.z, 0, x3);
-  ld2<false>(x, cn.w, 0, x4);
-  ld4<false>(q, c, 0, qc);
-  op2_k::adt_calc(x1, x2, x3, x4, qc, &a);
-  adt[c] = a;
+__device__ __forceinline__ void stamp(const Stamps &S, int item, unsigned long long t0) {
+  const unsigned long long t1 = (unsigned long long)wall_clock64();
+  const size_t k = (size_t)(*S.count % (unsigned long long)S.capacity) * S.per_step + S.off + item;
+  S.t[2 * k] = t0;
+  S.t[2 * k + 1] = t1;
 }
 
-// res_calc over blocks [blo, blo + gridDim.x) of the members' owner
-// partitions: each edge evaluated once per patch it touches, each cell's
-// increments summed in edge order (op2_dev::owner_res_block), so every member
-// gets its sequential executor's bits.  LDS: fluxes [max_e][4].
+// res_calc, persistent over the owner patches [blo, bhi) of the members'
+// owner partitions: each edge evaluated once per patch it touches, each
+// cell's increments summed in edge order (op2_dev::owner_res_block), so every
+// member gets its sequential executor's bits.  zero: every cell is owned and
+// the update before zeroed res, so the sums start from +0.0 without reading
+// it.  LDS: fluxes [max_e][4].
 __global__ void __launch_bounds__(1024)
     k_ens_res_owner(const double *__restrict x, const double *__restrict q,
                     const double *__restrict adt, double *__restrict res, OwnerArgs P,
-                    int blo) {
+                    int blo, int bhi, int zero, Stamps S) {
   extern __shared__ double lf[];
-  owner_res_block<false, true>(blo + xcd_block(kXcds), x, q, adt, res, P, 0, 0, lf);
+  for (int b = blo + xcd_block(kXcds); b < bhi; b += gridDim.x) {
+    const unsigned long long t0 = stamp_now(S);
+    owner_res_block<false, true>(b, x, q, adt, res, P, 0, 0, lf, zero != 0);
+    __syncthreads();  // the next patch rewrites lf
+    if (S.t && threadIdx.x == 0) stamp(S, b, t0);
+  }
 }
 
 // bres_calc (bres_calc.h:1-35) with the member's own free stream `qi`
@@ -121,31 +131,42 @@ __device__ __forceinline__ void bres_calc_qi(const double *x1, const double *x2,
   }
 }
 
-// one thread per boundary cell; its boundary edges in edge order
+// bres_calc, persistent over items [ilo, ihi) of <= 64 boundary cells of one
+// member ({first, end} target), one item per wave: one lane per boundary cell
+// adds its boundary edges in edge order.
 __global__ void k_ens_bres(const double *__restrict x, const double *__restrict q,
                            const double *__restrict adt, double *res,
                            const int2 *__restrict pbedge, const int *__restrict bound,
                            const int *__restrict bmesh, const double *__restrict qinf,
                            const int *__restrict offsets, const int *__restrict entries,
-                           const int *__restrict targets, int tlo, int thi) {
-  int t = tlo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= thi) return;
-  int cell = targets[t];
-  double acc[4], q1[4];
-  ld4<false>(res, cell, 0, acc);
-  ld4<false>(q, cell, 0, q1);
-  double a1 = adt[cell];
-  for (int k = offsets[t]; k < offsets[t + 1]; ++k) {
-    int e = entries[k];
-    int2 bn = pbedge[e];
-    double x1[2], x2[2], r[4] = {0.0, 0.0, 0.0, 0.0};
-    ld2<false>(x, bn.x, 0, x1);
-    ld2<false>(x, bn.y, 0, x2);
-    int bd = bound[e];
-    bres_calc_qi(x1, x2, q1, &a1, r, &bd, qinf + 4 * (size_t)bmesh[e]);
-    for (int d = 0; d < 4; ++d) acc[d] += r[d];
+                           const int *__restrict targets, const int2 *__restrict items,
+                           int ilo, int ihi, Stamps S) {
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  const int stride = gridDim.x * nw;
+  for (int i = ilo + xcd_block(kXcds) * nw + (threadIdx.x >> 6); i < ihi; i += stride) {
+    const unsigned long long t0 = stamp_now(S);
+    const int2 it = items[i];
+    const int t = it.x + lane;
+    if (t < it.y) {
+      const int cell = targets[t];
+      double acc[4], q1[4];
+      ld4<false>(res, cell, 0, acc);
+      ld4<false>(q, cell, 0, q1);
+      const double a1 = adt[cell];
+      for (int k = offsets[t]; k < offsets[t + 1]; ++k) {
+        const int e = entries[k];
+        const int2 bn = pbedge[e];
+        double x1[2], x2[2], r[4] = {0.0, 0.0, 0.0, 0.0};
+        ld2<false>(x, bn.x, 0, x1);
+        ld2<false>(x, bn.y, 0, x2);
+        const int bd = bound[e];
+        bres_calc_qi(x1, x2, q1, &a1, r, &bd, qinf + 4 * (size_t)bmesh[e]);
+        for (int d = 0; d < 4; ++d) acc[d] += r[d];
+      }
+      st4<false>(res, cell, 0, acc);
+    }
+    if (S.t && lane == 0) stamp(S, i, t0);
   }
-  st4<false>(res, cell, 0, acc);
 }
 
 struct MaxLoc {
@@ -173,55 +194,89 @@ __device__ __forceinline__ MaxLoc wave_maxloc(MaxLoc m) {
   return m;
 }
 
-// Persistent update over the tile table: workgroup w processes tiles
-// w, w + gridDim.x, ... of [tlo, thi); a tile is <= kTile cells of one member.
+// update, save_soln and adt_calc of each cell in one pass, persistent over
+// tiles [tlo, thi) of <= kTile cells of one member ({first cell, member}):
+//  UPD   q = update(qold, res, adt) (update.h), the member's rms/maxerr/errloc
+//        partials per tile when RED;
+//  SAVE  qold = q (save_soln.h), the next time step's;
+//  then  adt = adt_calc(x, q) (adt_calc.h) for the next half step.
+// DEAD: the update's zeros in res are not stored (res_calc, which comes next,
+// starts every cell's sum from +0.0).  A workgroup of any multiple of 64
+// threads covers a tile in kTile / blockDim.x rounds; partials are summed per
+// 64-cell group (one wave sum) and the tile's 16 groups in order, so the bits
+// do not depend on the workgroup size.  qold and adt are read before they are
+// written, by the same thread for the same cell: not restrict-qualified.
+template <bool UPD, bool SAVE, bool RED, bool DEAD>
 __global__ void __launch_bounds__(kTile)
-    k_ens_update(const double *__restrict qold, double *__restrict q,
-                 double *__restrict res, const double *__restrict adt,
-                 const int *__restrict tile_start, const int *__restrict tile_len,
-                 const int *__restrict tile_base, int tlo, int thi,
-                 double *__restrict part_sum, double *__restrict part_max,
-                 int *__restrict part_loc) {
+    k_ens_chain(const double *__restrict x, double *qold, double *__restrict q,
+                double *__restrict res, double *adt, const int4 *__restrict pcell,
+                const int2 *__restrict tiles, const int *__restrict coff, int tlo, int thi,
+                double *__restrict part_sum, double *__restrict part_max,
+                int *__restrict part_loc, Stamps S) {
   __shared__ double s_sum[kTile / 64];
   __shared__ double s_max[kTile / 64];
   __shared__ int s_loc[kTile / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int t = tlo + blockIdx.x; t < thi; t += gridDim.x) {
-    double rms = 0.0;
-    MaxLoc ml{-INFINITY, 0x7fffffff};
-    if ((int)threadIdx.x < tile_len[t]) {
-      int c = tile_start[t] + threadIdx.x;
-      int local = c - tile_base[t];  // index within the member
-      double qo[4], qn[4], r[4], a = adt[c], mx = ml.v;
-      int loc = ml.loc;
-      ld4<false>(qold, c, 0, qo);
-      ld4<false>(res, c, 0, r);
-      op2_k::update(qo, qn, r, &a, &rms, &mx, &local, &loc);
-      ml.v = mx;
-      ml.loc = loc;
-      st4<false>(q, c, 0, qn);
-      st4<false>(res, c, 0, r);
-    }
-    rms = wave_sum(rms);
-    ml = wave_maxloc(ml);
-    if (lane == 0) {
-      s_sum[wave] = rms;
-      s_max[wave] = ml.v;
-      s_loc[wave] = ml.loc;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double bs = 0.0;
-      MaxLoc bm{-INFINITY, 0x7fffffff};
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-        bs += s_sum[w];
-        bm = maxloc_combine(bm, MaxLoc{s_max[w], s_loc[w]});
+  for (int t = tlo + xcd_block(kXcds); t < thi; t += gridDim.x) {
+    const unsigned long long t0 = stamp_now(S);
+    const int2 tl = tiles[t];
+    const int base = coff[tl.y], end = min(tl.x + kTile, coff[tl.y + 1]);
+    for (int r = 0; r < kTile; r += blockDim.x) {
+      const int c = tl.x + r + (int)threadIdx.x;
+      double rms = 0.0;
+      MaxLoc ml{-INFINITY, 0x7fffffff};
+      if (c < end) {
+        double qn[4];
+        if (UPD) {
+          double qo[4], rr[4], a = adt[c], mx = ml.v;
+          int loc = ml.loc, local = c - base;  // index within the member
+          ld4<false>(qold, c, 0, qo);
+          ld4<false>(res, c, 0, rr);
+          op2_k::update(qo, qn, rr, &a, &rms, &mx, &local, &loc);
+          ml.v = mx;
+          ml.loc = loc;
+          st4<false>(q, c, 0, qn);
+          if (!DEAD) st4<false>(res, c, 0, rr);
+        } else {
+          ld4<false>(q, c, 0, qn);
+        }
+        if (SAVE) st4<false>(qold, c, 0, qn);
+        const int4 cn = pcell[c];
+        double x1[2], x2[2], x3[2], x4[2], a2;
+        ld2<false>(x, cn.x, 0, x1);
+        ld2<false>(x, cn.y, 0, x2);
+        ld2<false>(x, cn.z, 0, x3);
+        ld2<false>(x, cn.w, 0, x4);
+        op2_k::adt_calc(x1, x2, x3, x4, qn, &a2);
+        adt[c] = a2;
       }
-      part_sum[t] = bs;
-      part_max[t] = bm.v;
-      part_loc[t] = bm.loc;
+      if (RED) {
+        rms = wave_sum(rms);
+        ml = wave_maxloc(ml);
+        if (lane == 0) {
+          const int g = (r >> 6) + wave;
+          s_sum[g] = rms;
+          s_max[g] = ml.v;
+          s_loc[g] = ml.loc;
+        }
+      }
     }
-    __syncthreads();
+    if (RED) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double bs = 0.0;
+        MaxLoc bm{-INFINITY, 0x7fffffff};
+        for (int g = 0; g < kTile / 64; ++g) {
+          bs += s_sum[g];
+          bm = maxloc_combine(bm, MaxLoc{s_max[g], s_loc[g]});
+        }
+        part_sum[t] = bs;
+        part_max[t] = bm.v;
+        part_loc[t] = bm.loc;
+      }
+    }
+    if (RED || S.t) __syncthreads();  // s_* are rewritten by the next tile
+    if (S.t && threadIdx.x == 0) stamp(S, t, t0);
   }
 }
 
@@ -229,12 +284,13 @@ __global__ void __launch_bounds__(kTile)
 __global__ void __launch_bounds__(256)
     k_ens_finish(const double *__restrict part_sum, const double *__restrict part_max,
                  const int *__restrict part_loc, const int *__restrict mesh_tile_off,
-                 int mlo, double *__restrict slots) {
+                 int mlo, double *__restrict slots, Stamps S) {
   __shared__ double s_sum[4];
   __shared__ double s_max[4];
   __shared__ int s_loc[4];
   const int m = mlo + blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const unsigned long long t0 = stamp_now(S);
   double tot = 0.0;
   MaxLoc tm{-INFINITY, 0x7fffffff};
   for (int t = mesh_tile_off[m] + tid; t < mesh_tile_off[m + 1]; t += 256) {
@@ -259,6 +315,7 @@ __global__ void __launch_bounds__(256)
     slots[3 * (size_t)m] = t;
     slots[3 * (size_t)m + 1] = mm.v;
     slots[3 * (size_t)m + 2] = (double)mm.loc;
+    if (S.t) stamp(S, m, t0);
   }
 }
 
@@ -326,7 +383,8 @@ extern "C" int op2amd_ensemble_describe(const op2amd_ensemble_spec *spec,
 extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
                                    const op2amd_ensemble_opts *o,
                                    op2amd_ensemble_result *res) {
-  if (!spec || !o || !res || o->niter < 1 || o->warmup < 0 || o->warmup >= o->niter)
+  if (!spec || !o || !res || o->niter < 1 || o->warmup < 0 || o->warmup >= o->niter ||
+      o->max_workgroups < 0)
     return 1;
   double t_setup = now();
   const int M = spec->nmesh;
@@ -456,17 +514,22 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   for (int m = 0; m <= M; ++m)
     mesh_btarget_off[m] = (int)(std::lower_bound(btargets.begin(), btargets.end(), coff[m]) -
                                 btargets.begin());
-  // update tiles: <= kTile cells, never straddling two members
-  std::vector<int> tile_start, tile_len, tile_base, mesh_tile_off{0};
+  // work items, runs of one member's elements, with per-member offsets into
+  // each table: tiles of <= kTile cells for update+save_soln+adt_calc, the
+  // owner patches (mblk) for res_calc, <= 64 boundary cells for bres_calc
+  std::vector<int2> tiles, bitems;  // {first cell, member}, {first, end target}
+  std::vector<int> mesh_tile_off{0}, mesh_bitem_off{0};
   for (int m = 0; m < M; ++m) {
-    for (int c = coff[m]; c < coff[m + 1]; c += kTile) {
-      tile_start.push_back(c);
-      tile_len.push_back(std::min(kTile, coff[m + 1] - c));
-      tile_base.push_back(coff[m]);
-    }
-    mesh_tile_off.push_back((int)tile_start.size());
+    for (int c = coff[m]; c < coff[m + 1]; c += kTile) tiles.push_back(make_int2(c, m));
+    mesh_tile_off.push_back((int)tiles.size());
+    for (int t = mesh_btarget_off[m]; t < mesh_btarget_off[m + 1]; t += 64)
+      bitems.push_back(make_int2(t, std::min(t + 64, mesh_btarget_off[m + 1])));
+    mesh_bitem_off.push_back((int)bitems.size());
   }
-  const int ntiles = (int)tile_start.size();
+  const int ntiles = (int)tiles.size(), npatch = mblk[M], nbitem = (int)bitems.size();
+  // every cell owned by a patch: res_calc writes all of res, so the update
+  // need not zero it (dead res) and res_calc need not read it
+  const bool dead = (int)owned.size() == Nc;
 
   // ---- device data ----
   std::vector<void *> keep;
@@ -478,7 +541,7 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       for (int k = 0; k < 4; ++k) q0[4 * (size_t)c + k] = mem[m].qinf[k];
   double *d_x = dev_upload(x, keep);
   double *d_q = dev_upload(q0, keep);
-  double *d_qold = dev_upload(q0, keep);
+  double *d_qold = dev_alloc<double>(4 * (size_t)Nc, keep);
   double *d_res = dev_alloc<double>(4 * (size_t)Nc, keep);
   double *d_adt = dev_alloc<double>(Nc, keep);
   int4 *d_cell = dev_upload(cell, keep);
@@ -499,9 +562,9 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   int *d_btargets = dev_upload(btargets, keep);
   int *d_boffsets = dev_upload(boffsets, keep);
   int *d_bentries = dev_upload(bentries, keep);
-  int *d_tstart = dev_upload(tile_start, keep);
-  int *d_tlen = dev_upload(tile_len, keep);
-  int *d_tbase = dev_upload(tile_base, keep);
+  int2 *d_bitems = dev_upload(bitems, keep);
+  int2 *d_tiles = dev_upload(tiles, keep);
+  int *d_coff = dev_upload(coff, keep);
   int *d_mtoff = dev_upload(mesh_tile_off, keep);
   double *d_psum = dev_alloc<double>(ntiles, keep);
   double *d_pmax = dev_alloc<double>(ntiles, keep);
@@ -511,6 +574,15 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   const int capacity = o->niter;
   double *d_hist = dev_alloc<double>((size_t)hist_words * capacity, keep);
   unsigned long long *d_count = dev_alloc<unsigned long long>(1, keep);
+  // per-step stamp layout, in launch order: res_calc, bres_calc, the chain,
+  // twice, then the per-member finish
+  const int off_res[2] = {0, npatch + nbitem + ntiles};
+  const int off_bres[2] = {npatch, 2 * npatch + nbitem + ntiles};
+  const int off_chain[2] = {npatch + nbitem, 2 * (npatch + nbitem) + ntiles};
+  const int off_fin = 2 * (npatch + nbitem + ntiles), per_step = off_fin + M;
+  const bool stamps = o->batched && o->member_timing;
+  unsigned long long *d_stamp =
+      stamps ? dev_alloc<unsigned long long>(2 * (size_t)per_step * capacity, keep) : nullptr;
   {
     // constants of airfoil.cpp:149-152 in this translation unit's copies
     double gam = 1.4f, gm1 = gam - 1.0f, cfl = 0.9f, eps = 0.05f;
@@ -525,47 +597,85 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
 
   int nsm = 0;
   ECHECK(hipDeviceGetAttribute(&nsm, hipDeviceAttributeMultiprocessorCount, 0));
-  // block sizes of save_soln, adt_calc, res_calc, bres_calc (the defaults, or
-  // the autotuner's picks); every kernel's result is independent of them.
-  // res_calc's is its owner patch size, fixed by the partition.
-  int blk[4] = {256, 1024, kPatch, 128};
-  // with `ev`, each launch is bracketed by events ev[2j], ev[2j+1] in launch
-  // order: copy, then adt, res, bres of each half step
+  using ChainFn = void (*)(const double *, double *, double *, double *, double *,
+                           const int4 *, const int2 *, const int *, int, int, double *,
+                           double *, int *, Stamps);
+  const ChainFn k_prologue = k_ens_chain<false, true, false, false>;
+  const ChainFn k_chain0 = dead ? k_ens_chain<true, false, false, true>
+                                : k_ens_chain<true, false, false, false>;
+  const ChainFn k_chain1 = dead ? k_ens_chain<true, true, true, true>
+                                : k_ens_chain<true, true, true, false>;
+  // persistent grids: as many workgroups as the GPU holds at once for the
+  // kernel and workgroup size (occupancy), capped by the launch's items
+  auto resident = [&](const void *fn, int block, size_t lds) {
+    int nb = 0;
+    ECHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, lds));
+    if (nb < 1) {
+      printf("op2hip ensemble: a kernel does not fit a compute unit (block %d, LDS %zu)\n",
+             block, lds);
+      exit(-1);
+    }
+    return o->max_workgroups > 0 ? std::min(nb * nsm, o->max_workgroups) : nb * nsm;
+  };
+  // workgroup sizes of the chain (update+save_soln+adt_calc) and bres_calc
+  // (the defaults, or the autotuner's picks), res_calc's (its patch size,
+  // fixed by the partition); no kernel's result depends on them
+  int blk[3] = {256, kPatch, 128};
+  int cap[3] = {0, 0, 0};
+  auto set_caps = [&]() {
+    cap[0] = std::min(resident((const void *)k_chain0, blk[0], 0),
+                      resident((const void *)k_chain1, blk[0], 0));
+    cap[1] = resident((const void *)k_ens_res_owner, blk[1], res_lds);
+    cap[2] = resident((const void *)k_ens_bres, blk[2], 0) * (blk[2] / 64);  // waves
+  };
+  set_caps();
+  const int res_zero = dead ? 1 : 0;
+  auto S_at = [&](int off) {
+    return Stamps{stamps ? d_stamp : nullptr, d_count, per_step, capacity, off};
+  };
+  // save_soln and adt_calc of members [mlo, mhi), once before the first step
+  // (each step's last chain does both for the next step)
+  auto prologue = [&](int mlo, int mhi) {
+    const int t0 = mesh_tile_off[mlo], t1 = mesh_tile_off[mhi];
+    if (t1 > t0)
+      k_prologue<<<std::min(t1 - t0, cap[0]), blk[0], 0, st>>>(
+          d_x, d_qold, d_q, d_res, d_adt, d_cell, d_tiles, d_coff, t0, t1, d_psum, d_pmax,
+          d_ploc, Stamps{nullptr, d_count, per_step, capacity, 0});
+  };
+  // one time step of members [mlo, mhi): twice res_calc, bres_calc and the
+  // chain, then the per-member reductions; with `ev`, each of the six loop
+  // launches is bracketed by events ev[2j], ev[2j+1] in launch order
   auto step_range = [&](int mlo, int mhi, hipEvent_t *ev = nullptr) {
-    // save_soln, then twice adt_calc, res_calc, bres_calc, update
-    const int c0 = coff[mlo], c1 = coff[mhi];
-    long long lo2 = 2LL * c0, hi2 = 2LL * c1;
+    const int p0 = mblk[mlo], p1 = mblk[mhi];
+    const int b0 = mesh_bitem_off[mlo], b1 = mesh_bitem_off[mhi];
+    const int t0 = mesh_tile_off[mlo], t1 = mesh_tile_off[mhi];
     int e = 0;
     auto mark = [&]() {
       if (ev) ECHECK(hipEventRecord(ev[e++], st));
     };
-    mark();
-    k_ens_copy<<<grid_for(hi2 - lo2, blk[0]), blk[0], 0, st>>>((const double2 *)d_q,
-                                                              (double2 *)d_qold, lo2, hi2);
-    mark();
     for (int k = 0; k < 2; ++k) {
       mark();
-      k_ens_adt<<<grid_for(c1 - c0, blk[1]), blk[1], 0, st>>>(d_x, d_q, d_adt, d_cell, c0, c1);
+      if (p1 > p0)
+        k_ens_res_owner<<<std::min(p1 - p0, cap[1]), blk[1], res_lds, st>>>(
+            d_x, d_q, d_adt, d_res, OA, p0, p1, res_zero, S_at(off_res[k]));
       mark();
       mark();
-      if (mblk[mhi] > mblk[mlo])
-        k_ens_res_owner<<<mblk[mhi] - mblk[mlo], kPatch, res_lds, st>>>(d_x, d_q, d_adt, d_res,
-                                                                        OA, mblk[mlo]);
+      if (b1 > b0) {
+        const int wpb = blk[2] / 64, g = (std::min(b1 - b0, cap[2]) + wpb - 1) / wpb;
+        k_ens_bres<<<g, blk[2], 0, st>>>(d_x, d_q, d_adt, d_res, d_bedge, d_bound, d_bmesh,
+                                         d_qinf, d_boffsets, d_bentries, d_btargets,
+                                         d_bitems, b0, b1, S_at(off_bres[k]));
+      }
       mark();
-      const int t0 = mesh_btarget_off[mlo], t1 = mesh_btarget_off[mhi];
       mark();
       if (t1 > t0)
-        k_ens_bres<<<grid_for(t1 - t0, blk[3]), blk[3], 0, st>>>(
-            d_x, d_q, d_adt, d_res, d_bedge, d_bound, d_bmesh, d_qinf, d_boffsets,
-            d_bentries, d_btargets, t0, t1);
+        (k ? k_chain1 : k_chain0)<<<std::min(t1 - t0, cap[0]), blk[0], 0, st>>>(
+            d_x, d_qold, d_q, d_res, d_adt, d_cell, d_tiles, d_coff, t0, t1, d_psum, d_pmax,
+            d_ploc, S_at(off_chain[k]));
       mark();
-      const int tl0 = mesh_tile_off[mlo], tl1 = mesh_tile_off[mhi];
-      const int grid = std::max(1, std::min(tl1 - tl0, 4 * nsm));
-      k_ens_update<<<grid, kTile, 0, st>>>(d_qold, d_q, d_res, d_adt, d_tstart, d_tlen,
-                                            d_tbase, tl0, tl1, d_psum, d_pmax, d_ploc);
-      k_ens_finish<<<mhi - mlo, 256, 0, st>>>(d_psum, d_pmax, d_ploc, d_mtoff, mlo,
-                                              d_slots);
     }
+    k_ens_finish<<<mhi - mlo, 256, 0, st>>>(d_psum, d_pmax, d_ploc, d_mtoff, mlo, d_slots,
+                                            S_at(off_fin));
   };
   auto history = [&]() {
     k_ens_history<<<1, 256, 0, st>>>(d_slots, hist_words, d_hist, d_count, capacity);
@@ -584,15 +694,16 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       ECHECK(hipEventCreate(&ev_b[m]));
     }
   }
-  // autotuning: step i (i < 5) runs the save_soln, adt_calc and bres_calc
-  // kernels with candidate i; the fastest total per kernel over its launches
-  // wins
+  // autotuning: step i (i < 5) runs the chain and bres_calc with workgroup
+  // size candidate i; the fastest total per kernel over its launches wins
   const int cand[5] = {64, 128, 256, 512, 1024};
   const int ntune = (o->batched && o->autotune) ? std::min(5, o->niter) : 0;
-  double tune_ms[4][5] = {};
-  hipEvent_t tev[14];
+  double tune_ms[3][5] = {};
+  hipEvent_t tev[12];
   if (ntune)
     for (auto &e : tev) ECHECK(hipEventCreate(&e));
+  prologue(0, M);
+  ECHECK(hipGetLastError());
   double t0 = 0.0;
   for (int it = 1; it <= o->niter; ++it) {
     if (it == o->warmup + 1) {
@@ -601,24 +712,27 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       t0 = now();
     }
     if (it <= ntune) {
-      blk[0] = blk[1] = blk[3] = cand[it - 1];
+      blk[0] = blk[2] = cand[it - 1];
+      set_caps();
       step_range(0, M, tev);
       history();
       ECHECK(hipStreamSynchronize(st));
-      // event pairs: copy; then (adt, res, bres) twice
-      const int kind[7] = {0, 1, 2, 3, 1, 2, 3};
-      for (int j = 0; j < 7; ++j) {
+      // event pairs: (res, bres, chain) twice
+      const int kind[6] = {1, 2, 0, 1, 2, 0};
+      for (int j = 0; j < 6; ++j) {
         float ms = 0.f;
         ECHECK(hipEventElapsedTime(&ms, tev[2 * j], tev[2 * j + 1]));
         tune_ms[kind[j]][it - 1] += ms;
       }
-      if (it == ntune)
-        for (int k : {0, 1, 3}) {
+      if (it == ntune) {
+        for (int k : {0, 2}) {
           int best = 0;
           for (int c = 1; c < ntune; ++c)
             if (tune_ms[k][c] < tune_ms[k][best]) best = c;
           blk[k] = cand[best];
         }
+        set_caps();
+      }
     } else if (o->batched) {
       if (o->graphed) {
         if (!gexec) {
@@ -675,13 +789,68 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
     }
   if (res->q_out)
     ECHECK(hipMemcpy(res->q_out, d_q, 4 * (size_t)Nc * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> busy_ms(M, 0.0);
+  if (stamps) {
+    // per timed step: member m's latency runs from the step's first item
+    // start to m's last item end; its busy time sums, over the step's
+    // launches, m's last item end minus its first item start
+    std::vector<unsigned long long> ts(2 * (size_t)per_step * capacity);
+    ECHECK(hipMemcpy(ts.data(), d_stamp, ts.size() * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+    int rate_khz = 0;
+    ECHECK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0));
+    const double tick_ms = 1.0 / (double)std::max(rate_khz, 1);
+    // launch segments: offset, member offset table, items
+    struct Seg {
+      int off;
+      const std::vector<int> *moff;
+    };
+    const Seg segs[7] = {{off_res[0], &mblk},          {off_bres[0], &mesh_bitem_off},
+                         {off_chain[0], &mesh_tile_off}, {off_res[1], &mblk},
+                         {off_bres[1], &mesh_bitem_off}, {off_chain[1], &mesh_tile_off},
+                         {off_fin, nullptr}};
+    std::vector<double> lat(M, 0.0);
+    for (int it = o->warmup; it < o->niter; ++it) {
+      const unsigned long long *T = &ts[2 * (size_t)it * per_step];
+      unsigned long long s0 = ~0ull;
+      for (int k = 0; k < per_step; ++k) s0 = std::min(s0, T[2 * k]);
+      for (int m = 0; m < M; ++m) {
+        unsigned long long last = 0;
+        for (const Seg &g : segs) {
+          const int i0 = g.moff ? (*g.moff)[m] : m, i1 = g.moff ? (*g.moff)[m + 1] : m + 1;
+          if (i1 <= i0) continue;
+          unsigned long long b = ~0ull, e = 0;
+          for (int i = i0; i < i1; ++i) {
+            b = std::min(b, T[2 * (size_t)(g.off + i)]);
+            e = std::max(e, T[2 * (size_t)(g.off + i) + 1]);
+          }
+          busy_ms[m] += (double)(e - b) * tick_ms;
+          last = std::max(last, e);
+        }
+        lat[m] += (double)(last - s0) * tick_ms;
+      }
+      ++timed_steps;
+    }
+    for (int m = 0; m < M; ++m) {
+      member_ms[m] = lat[m];
+      if (timed_steps) busy_ms[m] /= timed_steps;
+    }
+  }
   if (res->member_step_ms) {
     for (int m = 0; m < M; ++m)
       res->member_step_ms[m] =
-          o->batched ? 1e3 * res->seconds / std::max(1, o->niter - o->warmup)
-                     : (timed_steps ? member_ms[m] / timed_steps : 0.0);
+          o->member_timing ? (timed_steps ? member_ms[m] / timed_steps : 0.0)
+                           : (o->batched ? 1e3 * res->seconds / std::max(1, o->niter - o->warmup)
+                                         : 0.0);
   }
-  for (int k = 0; k < 4; ++k) res->blocks[k] = blk[k];
+  if (res->member_busy_ms)
+    for (int m = 0; m < M; ++m) res->member_busy_ms[m] = stamps ? busy_ms[m] : 0.0;
+  for (int k = 0; k < 3; ++k) {
+    res->blocks[k] = blk[k];
+    res->grids[k] = cap[k];
+  }
+  res->grids[2] = (cap[2] + blk[2] / 64 - 1) / (blk[2] / 64);
+  res->dead_res = dead ? 1 : 0;
   if (ntune)
     for (auto e : tev) (void)hipEventDestroy(e);
   if (gexec) (void)hipGraphExecDestroy(gexec);

@@ -23,15 +23,20 @@ class Member(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("niter", C.c_int), ("warmup", C.c_int), ("batched", C.c_int),
-                ("graphed", C.c_int), ("member_timing", C.c_int), ("autotune", C.c_int)]
+                ("graphed", C.c_int), ("member_timing", C.c_int), ("autotune", C.c_int),
+                ("max_workgroups", C.c_int)]
 
 
 class Result(C.Structure):
     _fields_ = [("rms", C.POINTER(C.c_double)), ("maxerr", C.POINTER(C.c_double)),
                 ("errloc", C.POINTER(C.c_int)), ("q_out", C.POINTER(C.c_double)),
-                ("member_step_ms", C.POINTER(C.c_double)), ("seconds", C.c_double),
+                ("member_step_ms", C.POINTER(C.c_double)),
+                ("member_busy_ms", C.POINTER(C.c_double)), ("seconds", C.c_double),
                 ("setup_seconds", C.c_double), ("total_cells", C.c_longlong),
-                ("total_edges", C.c_longlong), ("blocks", C.c_int * 4)]
+                ("total_edges", C.c_longlong), ("blocks", C.c_int * 3),
+                ("grids", C.c_int * 3), ("dead_res", C.c_int)]
+
+LOOPS = ("update_save_adt", "res_calc", "bres_calc")
 
 
 lib.op2amd_ensemble_describe.restype = C.c_int
@@ -55,7 +60,7 @@ def describe(sp):
 
 
 def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
-        want_q=False, autotune=False):
+        want_q=False, autotune=False, max_workgroups=0):
     members = describe(sp)
     M = sp.nmesh
     total = sum(m["ncell"] for m in members)
@@ -63,6 +68,7 @@ def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
     maxerr = np.zeros((niter, M))
     errloc = np.zeros((niter, M), np.int32)
     step_ms = np.zeros(M)
+    busy_ms = np.zeros(M)
     q = np.zeros((total, 4)) if want_q else None
     r = Result()
     r.rms = rms.ctypes.data_as(C.POINTER(C.c_double))
@@ -70,13 +76,15 @@ def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
     r.errloc = errloc.ctypes.data_as(C.POINTER(C.c_int))
     r.q_out = q.ctypes.data_as(C.POINTER(C.c_double)) if want_q else C.POINTER(C.c_double)()
     r.member_step_ms = step_ms.ctypes.data_as(C.POINTER(C.c_double))
-    o = Opts(niter, warmup, int(batched), int(graphed), int(member_timing), int(autotune))
+    r.member_busy_ms = busy_ms.ctypes.data_as(C.POINTER(C.c_double))
+    o = Opts(niter, warmup, int(batched), int(graphed), int(member_timing), int(autotune),
+             int(max_workgroups))
     rc = lib.op2amd_ensemble_run(C.byref(sp), C.byref(o), C.byref(r))
     if rc:
         raise RuntimeError(f"ensemble run failed ({rc})")
     return {"members": members, "rms": rms, "maxerr": maxerr, "errloc": errloc, "q": q,
-            "member_step_ms": step_ms, "seconds": r.seconds,
+            "member_step_ms": step_ms, "member_busy_ms": busy_ms, "seconds": r.seconds,
             "setup_seconds": r.setup_seconds, "total_cells": r.total_cells,
             "total_edges": r.total_edges,
-            "blocks": dict(zip(("save_soln", "adt_calc", "res_calc", "bres_calc"),
-                               list(r.blocks)))}
+            "blocks": dict(zip(LOOPS, list(r.blocks))),
+            "grids": dict(zip(LOOPS, list(r.grids))), "dead_res": bool(r.dead_res)}

@@ -56,8 +56,8 @@ def test_batched_equals_looped_and_each_member_equals_sequential_run(ordering):
 
 @pytest.mark.gpu
 def test_autotuned_block_sizes_change_no_bits():
-    """The batched run with its block sizes picked by timing the first five
-    steps (one candidate per step) gives the bits of the default launch
+    """The batched run with its workgroup sizes picked by timing the first
+    five steps (one candidate per step) gives the bits of the default launch
     shapes; the picks are candidates."""
     sp = op2amd.ensemble.spec(**SMALL)
     plain = op2amd.ensemble.run(sp, 30, batched=True, graphed=True, want_q=True)
@@ -65,7 +65,52 @@ def test_autotuned_block_sizes_change_no_bits():
                                 autotune=True)
     for key in ("rms", "maxerr", "errloc", "q"):
         assert np.array_equal(tuned[key], plain[key]), key
-    assert plain["blocks"] == {"save_soln": 256, "adt_calc": 1024, "res_calc": 128,
-                               "bres_calc": 128}
+    assert plain["blocks"] == {"update_save_adt": 256, "res_calc": 128, "bres_calc": 128}
     assert set(tuned["blocks"].values()) <= {64, 128, 256, 512, 1024}
     assert tuned["blocks"]["res_calc"] == 128
+    assert plain["dead_res"] and tuned["dead_res"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("max_workgroups", [1, 3, 7])
+def test_persistent_workgroups_walking_many_items_give_the_same_bits(max_workgroups):
+    """Grids capped far below the work items (each workgroup walks many tiles,
+    patches and boundary runs of several members) give the bits of the
+    uncapped batched run, of the looped run and of the oracle."""
+    cfg = dict(nmesh=4, min_IJ=(40, 20), max_IJ=(120, 60), seed=5, ordering="rcm")
+    sp = op2amd.ensemble.spec(**cfg)
+    niter = 12
+    ref = op2amd.ensemble.run(sp, niter, batched=False, want_q=True)
+    capped = op2amd.ensemble.run(sp, niter, batched=True, graphed=True, want_q=True,
+                                 max_workgroups=max_workgroups)
+    assert capped["grids"]["res_calc"] <= max_workgroups
+    assert capped["grids"]["update_save_adt"] <= max_workgroups
+    for key in ("rms", "maxerr", "errloc", "q"):
+        assert np.array_equal(capped[key], ref[key]), key
+    m = 1
+    mb = capped["members"][m]
+    off = capped["members"][0]["ncell"]
+    mesh = op2amd.mesh.generate(mb["I"], mb["J"], cfg["seed"] + 1 + m, "rcm")
+    want = oracle.airfoil_run_qinf(mesh, niter, mb["qinf"])
+    assert np.array_equal(capped["q"][off:off + mesh.ncell], want["q"])
+
+
+@pytest.mark.gpu
+def test_member_step_times_inside_the_batched_step():
+    """Device-clock stamps of every work item give each member's latency
+    inside the graphed batched step (from the step's first item to the
+    member's last) and its busy time (its items' spans summed over the
+    launches, which run one after another, so busy <= latency).  Stamping
+    changes no bits, and the mean step span is within the wall time per step."""
+    sp = op2amd.ensemble.spec(nmesh=12, min_IJ=(40, 20), max_IJ=(320, 160), seed=2)
+    niter, W = 10, 2
+    plain = op2amd.ensemble.run(sp, niter, warmup=W, batched=True, graphed=True, want_q=True)
+    timed = op2amd.ensemble.run(sp, niter, warmup=W, batched=True, graphed=True, want_q=True,
+                                member_timing=True)
+    for key in ("rms", "maxerr", "errloc", "q"):
+        assert np.array_equal(timed[key], plain[key]), key
+    lat, busy = timed["member_step_ms"], timed["member_busy_ms"]
+    assert np.all(lat > 0) and np.all(busy > 0)
+    assert np.all(busy <= lat * (1 + 1e-9))
+    wall_ms = 1e3 * timed["seconds"] / (niter - W)
+    assert lat.max() <= wall_ms * 1.01, (lat.max(), wall_ms)
