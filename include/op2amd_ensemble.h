@@ -70,6 +70,8 @@ typedef struct {
   int max_workgroups; /* > 0: at most this many workgroups per launch (each
                          then walks more work items); 0: as many as the GPU
                          holds at once */
+  int patch_cells;    /* cells per owner patch of res_calc, 64..1024; 0: 256
+                         (the results do not depend on it) */
 } op2amd_ensemble_opts;
 
 typedef struct {
@@ -90,9 +92,10 @@ typedef struct {
   double seconds;       /* wall time of iterations warmup+1..niter */
   double setup_seconds;
   long long total_cells, total_edges;
-  int blocks[3];        /* workgroup sizes: update+save_soln+adt_calc,
-                           res_calc (its owner patch size, 128), bres_calc */
-  int grids[3];         /* persistent grids (workgroups) of the same launches
+  int blocks[4];        /* workgroup sizes: update+adt_calc (first half
+                           step), res_calc (its owner patch size), bres_calc,
+                           update+save_soln+adt_calc (second half step) */
+  int grids[4];         /* persistent grids (workgroups) of the same launches
                            over all members */
   int dead_res;         /* 1: every cell is owned by a res_calc patch, so the
                            update does not zero res and res_calc does not

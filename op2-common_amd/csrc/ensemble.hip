@@ -37,8 +37,9 @@ namespace {
 
 using namespace op2_dev;
 
-constexpr int kTile = 1024;  // cells per update tile (one per thread)
-constexpr int kPatch = 128;  // cells per owner patch of res_calc (one per thread)
+constexpr int kTile = 1024;          // cells per tile of the update chain
+constexpr int kGroups = kTile / 64;  // reduction groups (one wave's cells) per tile
+constexpr int kPatch = 256;          // default cells per owner patch of res_calc
 
 struct SplitMix64 {
   uint64_t s;
@@ -197,15 +198,16 @@ __device__ __forceinline__ MaxLoc wave_maxloc(MaxLoc m) {
 // update, save_soln and adt_calc of each cell in one pass, persistent over
 // tiles [tlo, thi) of <= kTile cells of one member ({first cell, member}):
 //  UPD   q = update(qold, res, adt) (update.h), the member's rms/maxerr/errloc
-//        partials per tile when RED;
+//        partials when RED: one per 64-cell group (a wave's sum), at
+//        part_*[tile * kGroups + group];
 //  SAVE  qold = q (save_soln.h), the next time step's;
 //  then  adt = adt_calc(x, q) (adt_calc.h) for the next half step.
 // DEAD: the update's zeros in res are not stored (res_calc, which comes next,
 // starts every cell's sum from +0.0).  A workgroup of any multiple of 64
-// threads covers a tile in kTile / blockDim.x rounds; partials are summed per
-// 64-cell group (one wave sum) and the tile's 16 groups in order, so the bits
-// do not depend on the workgroup size.  qold and adt are read before they are
-// written, by the same thread for the same cell: not restrict-qualified.
+// threads covers a tile in kTile / blockDim.x rounds; the groups are fixed by
+// the tile, so the bits do not depend on the workgroup size.  qold and adt
+// are read before they are written, by the same thread for the same cell: not
+// restrict-qualified.
 template <bool UPD, bool SAVE, bool RED, bool DEAD>
 __global__ void __launch_bounds__(kTile)
     k_ens_chain(const double *__restrict x, double *qold, double *__restrict q,
@@ -213,9 +215,6 @@ __global__ void __launch_bounds__(kTile)
                 const int2 *__restrict tiles, const int *__restrict coff, int tlo, int thi,
                 double *__restrict part_sum, double *__restrict part_max,
                 int *__restrict part_loc, Stamps S) {
-  __shared__ double s_sum[kTile / 64];
-  __shared__ double s_max[kTile / 64];
-  __shared__ int s_loc[kTile / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int t = tlo + xcd_block(kXcds); t < thi; t += gridDim.x) {
     const unsigned long long t0 = stamp_now(S);
@@ -254,33 +253,22 @@ __global__ void __launch_bounds__(kTile)
         rms = wave_sum(rms);
         ml = wave_maxloc(ml);
         if (lane == 0) {
-          const int g = (r >> 6) + wave;
-          s_sum[g] = rms;
-          s_max[g] = ml.v;
-          s_loc[g] = ml.loc;
+          const size_t g = (size_t)t * kGroups + (r >> 6) + wave;
+          part_sum[g] = rms;
+          part_max[g] = ml.v;
+          part_loc[g] = ml.loc;
         }
       }
     }
-    if (RED) {
+    if (S.t) {
       __syncthreads();
-      if (threadIdx.x == 0) {
-        double bs = 0.0;
-        MaxLoc bm{-INFINITY, 0x7fffffff};
-        for (int g = 0; g < kTile / 64; ++g) {
-          bs += s_sum[g];
-          bm = maxloc_combine(bm, MaxLoc{s_max[g], s_loc[g]});
-        }
-        part_sum[t] = bs;
-        part_max[t] = bm.v;
-        part_loc[t] = bm.loc;
-      }
+      if (threadIdx.x == 0) stamp(S, t, t0);
     }
-    if (RED || S.t) __syncthreads();  // s_* are rewritten by the next tile
-    if (S.t && threadIdx.x == 0) stamp(S, t, t0);
   }
 }
 
-// One workgroup per member: its tiles' partials in tile order.
+// One workgroup per member: its tiles' partials, each summed over its groups
+// in order, then over the tiles.
 __global__ void __launch_bounds__(256)
     k_ens_finish(const double *__restrict part_sum, const double *__restrict part_max,
                  const int *__restrict part_loc, const int *__restrict mesh_tile_off,
@@ -294,8 +282,16 @@ __global__ void __launch_bounds__(256)
   double tot = 0.0;
   MaxLoc tm{-INFINITY, 0x7fffffff};
   for (int t = mesh_tile_off[m] + tid; t < mesh_tile_off[m + 1]; t += 256) {
-    tot += part_sum[t];
-    tm = maxloc_combine(tm, MaxLoc{part_max[t], part_loc[t]});
+    // the tile's partial: its groups in order
+    double ts = 0.0;
+    MaxLoc tg{-INFINITY, 0x7fffffff};
+    for (int g = 0; g < kGroups; ++g) {
+      const size_t k = (size_t)t * kGroups + g;
+      ts += part_sum[k];
+      tg = maxloc_combine(tg, MaxLoc{part_max[k], part_loc[k]});
+    }
+    tot += ts;
+    tm = maxloc_combine(tm, tg);
   }
   tot = wave_sum(tot);
   tm = wave_maxloc(tm);
@@ -384,8 +380,9 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
                                    const op2amd_ensemble_opts *o,
                                    op2amd_ensemble_result *res) {
   if (!spec || !o || !res || o->niter < 1 || o->warmup < 0 || o->warmup >= o->niter ||
-      o->max_workgroups < 0)
+      o->max_workgroups < 0 || (o->patch_cells != 0 && (o->patch_cells < 64 || o->patch_cells > 1024)))
     return 1;
+  const int patch = o->patch_cells ? o->patch_cells : kPatch;
   double t_setup = now();
   const int M = spec->nmesh;
   std::vector<op2amd_ensemble_member> mem(M);
@@ -433,7 +430,7 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   res->total_cells = Nc;
   res->total_edges = Ne;
 
-  // res_calc: each member's owner partition (patches of kPatch cells that
+  // res_calc: each member's owner partition (patches of `patch` cells that
   // never straddle two members), concatenated; 8-byte instance records where a
   // patch's cells and nodes span at most 65535 indices
   std::vector<int> mblk(M + 1, 0), owned_offs{0}, owned, elem_offs{0};
@@ -454,7 +451,7 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       }
       op2h::OwnerLists L;
       std::string err;
-      if (!op2h::owner_lists(coff[m + 1] - c0, nel, 2, lmap.data(), 2, inc, kPatch, L, err)) {
+      if (!op2h::owner_lists(coff[m + 1] - c0, nel, 2, lmap.data(), 2, inc, patch, L, err)) {
         printf("op2hip ensemble: member %d: %s\n", m, err.c_str());
         return 4;
       }
@@ -566,9 +563,9 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   int2 *d_tiles = dev_upload(tiles, keep);
   int *d_coff = dev_upload(coff, keep);
   int *d_mtoff = dev_upload(mesh_tile_off, keep);
-  double *d_psum = dev_alloc<double>(ntiles, keep);
-  double *d_pmax = dev_alloc<double>(ntiles, keep);
-  int *d_ploc = dev_alloc<int>(ntiles, keep);
+  double *d_psum = dev_alloc<double>((size_t)ntiles * kGroups, keep);
+  double *d_pmax = dev_alloc<double>((size_t)ntiles * kGroups, keep);
+  int *d_ploc = dev_alloc<int>((size_t)ntiles * kGroups, keep);
   double *d_slots = dev_alloc<double>(3 * (size_t)M, keep);
   const int hist_words = 3 * M;
   const int capacity = o->niter;
@@ -617,16 +614,17 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
     }
     return o->max_workgroups > 0 ? std::min(nb * nsm, o->max_workgroups) : nb * nsm;
   };
-  // workgroup sizes of the chain (update+save_soln+adt_calc) and bres_calc
-  // (the defaults, or the autotuner's picks), res_calc's (its patch size,
-  // fixed by the partition); no kernel's result depends on them
-  int blk[3] = {256, kPatch, 128};
-  int cap[3] = {0, 0, 0};
+  // workgroup sizes of the first half step's chain (update+adt_calc),
+  // res_calc (its patch size, fixed by the partition), bres_calc and the
+  // second half step's chain (update+save_soln+adt_calc): the defaults, or
+  // the autotuner's picks; no kernel's result depends on them
+  int blk[4] = {256, patch, 128, 256};
+  int cap[4] = {0, 0, 0, 0};
   auto set_caps = [&]() {
-    cap[0] = std::min(resident((const void *)k_chain0, blk[0], 0),
-                      resident((const void *)k_chain1, blk[0], 0));
+    cap[0] = resident((const void *)k_chain0, blk[0], 0);
     cap[1] = resident((const void *)k_ens_res_owner, blk[1], res_lds);
     cap[2] = resident((const void *)k_ens_bres, blk[2], 0) * (blk[2] / 64);  // waves
+    cap[3] = resident((const void *)k_chain1, blk[3], 0);
   };
   set_caps();
   const int res_zero = dead ? 1 : 0;
@@ -669,7 +667,7 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       mark();
       mark();
       if (t1 > t0)
-        (k ? k_chain1 : k_chain0)<<<std::min(t1 - t0, cap[0]), blk[0], 0, st>>>(
+        (k ? k_chain1 : k_chain0)<<<std::min(t1 - t0, cap[3 * k]), blk[3 * k], 0, st>>>(
             d_x, d_qold, d_q, d_res, d_adt, d_cell, d_tiles, d_coff, t0, t1, d_psum, d_pmax,
             d_ploc, S_at(off_chain[k]));
       mark();
@@ -694,11 +692,11 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       ECHECK(hipEventCreate(&ev_b[m]));
     }
   }
-  // autotuning: step i (i < 5) runs the chain and bres_calc with workgroup
+  // autotuning: step i (i < 5) runs both chains and bres_calc with workgroup
   // size candidate i; the fastest total per kernel over its launches wins
   const int cand[5] = {64, 128, 256, 512, 1024};
   const int ntune = (o->batched && o->autotune) ? std::min(5, o->niter) : 0;
-  double tune_ms[3][5] = {};
+  double tune_ms[4][5] = {};
   hipEvent_t tev[12];
   if (ntune)
     for (auto &e : tev) ECHECK(hipEventCreate(&e));
@@ -712,20 +710,20 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       t0 = now();
     }
     if (it <= ntune) {
-      blk[0] = blk[2] = cand[it - 1];
+      blk[0] = blk[2] = blk[3] = cand[it - 1];
       set_caps();
       step_range(0, M, tev);
       history();
       ECHECK(hipStreamSynchronize(st));
       // event pairs: (res, bres, chain) twice
-      const int kind[6] = {1, 2, 0, 1, 2, 0};
+      const int kind[6] = {1, 2, 0, 1, 2, 3};
       for (int j = 0; j < 6; ++j) {
         float ms = 0.f;
         ECHECK(hipEventElapsedTime(&ms, tev[2 * j], tev[2 * j + 1]));
         tune_ms[kind[j]][it - 1] += ms;
       }
       if (it == ntune) {
-        for (int k : {0, 2}) {
+        for (int k : {0, 2, 3}) {
           int best = 0;
           for (int c = 1; c < ntune; ++c)
             if (tune_ms[k][c] < tune_ms[k][best]) best = c;
@@ -845,7 +843,7 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   }
   if (res->member_busy_ms)
     for (int m = 0; m < M; ++m) res->member_busy_ms[m] = stamps ? busy_ms[m] : 0.0;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 4; ++k) {
     res->blocks[k] = blk[k];
     res->grids[k] = cap[k];
   }

@@ -24,7 +24,7 @@ class Member(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("niter", C.c_int), ("warmup", C.c_int), ("batched", C.c_int),
                 ("graphed", C.c_int), ("member_timing", C.c_int), ("autotune", C.c_int),
-                ("max_workgroups", C.c_int)]
+                ("max_workgroups", C.c_int), ("patch_cells", C.c_int)]
 
 
 class Result(C.Structure):
@@ -33,10 +33,10 @@ class Result(C.Structure):
                 ("member_step_ms", C.POINTER(C.c_double)),
                 ("member_busy_ms", C.POINTER(C.c_double)), ("seconds", C.c_double),
                 ("setup_seconds", C.c_double), ("total_cells", C.c_longlong),
-                ("total_edges", C.c_longlong), ("blocks", C.c_int * 3),
-                ("grids", C.c_int * 3), ("dead_res", C.c_int)]
+                ("total_edges", C.c_longlong), ("blocks", C.c_int * 4),
+                ("grids", C.c_int * 4), ("dead_res", C.c_int)]
 
-LOOPS = ("update_save_adt", "res_calc", "bres_calc")
+LOOPS = ("update_adt", "res_calc", "bres_calc", "update_save_adt")
 
 
 lib.op2amd_ensemble_describe.restype = C.c_int
@@ -60,7 +60,7 @@ def describe(sp):
 
 
 def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
-        want_q=False, autotune=False, max_workgroups=0):
+        want_q=False, autotune=False, max_workgroups=0, patch_cells=0):
     members = describe(sp)
     M = sp.nmesh
     total = sum(m["ncell"] for m in members)
@@ -78,7 +78,7 @@ def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
     r.member_step_ms = step_ms.ctypes.data_as(C.POINTER(C.c_double))
     r.member_busy_ms = busy_ms.ctypes.data_as(C.POINTER(C.c_double))
     o = Opts(niter, warmup, int(batched), int(graphed), int(member_timing), int(autotune),
-             int(max_workgroups))
+             int(max_workgroups), int(patch_cells))
     rc = lib.op2amd_ensemble_run(C.byref(sp), C.byref(o), C.byref(r))
     if rc:
         raise RuntimeError(f"ensemble run failed ({rc})")

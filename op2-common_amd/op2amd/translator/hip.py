@@ -276,18 +276,20 @@ class LoopGen:
         body.append(f"template <int LAY>")
         body.append(f"__global__ void __launch_bounds__(1024)")
         body.append(f"    {self.name}_kernel({self.name}_args A, const int *__restrict elems, int start,")
-        body.append(f"                 int end, int atomic, unsigned char *partials, int part_base) {{")
+        body.append(f"                 int end, int atomic, unsigned char *partials, int part_base,")
+        body.append(f"                 int step) {{")
         body.append(f"  const int blk = op2gen_xcd_block();  // XCD-contiguous element ranges")
-        body.append(f"  const int tid = start + blk * (int)blockDim.x + (int)threadIdx.x;")
-        body.append(f"  const bool live = tid < end;")
-        body.append(f"  const int n = live ? (elems ? elems[tid] : tid) : 0;")
         body += red_decl
-        body.append(f"  if (live) {{")
+        body.append(f"  // step > 0: the thread's elements are e, e + step, ... (a grid of fewer")
+        body.append(f"  // threads than elements, so fewer partial records); 0: one element")
+        body.append(f"  for (int e = start + blk * (int)blockDim.x + (int)threadIdx.x; e < end; e += step) {{")
+        body.append(f"    const int n = elems ? elems[e] : e;")
         body += pre
         body += track
         body.append(f"    {self.ns}::{L.kernel}({', '.join(call)});")
         body += track_after
         body += post
+        body.append(f"    if (step == 0) break;")
         body.append(f"  }}")
         if self.reds:
             # block epilogue: each wave reduces by shuffles, one barrier, then
@@ -488,6 +490,7 @@ class LoopGen:
                 fin.append(f"    if (threadIdx.x == 0) (({a.typ} *)slot{j})[k] = v;")
                 fin.append(f"  }}")
         b += decl
+        b.append(f"#pragma unroll 4")
         b.append(f"  for (int p = threadIdx.x; p < nparts; p += blockDim.x) {{")
         b.append(f"    const unsigned char *rec = partials + (size_t)p * {self.rec};")
         b += load
@@ -614,7 +617,12 @@ class LoopGen:
         s.append(f"        nparts += (cp->col_offsets_h[c + 1] - cp->col_offsets_h[c] + bs - 1) / bs;")
         s.append(f"    }} else {{")
         s.append(f"      nparts = (set->size + bs - 1) / bs;")
+        if self.reds:
+            # about three elements per thread, at most 2048 workgroups: fewer
+            # partial records for the finish to combine
+            s.append(f"      nparts = std::min((set->size + 3 * bs - 1) / (3 * bs), 2048);")
         s.append(f"    }}")
+        s.append(f"    const int step = !cp && nparts * bs < set->size ? nparts * bs : 0;")
         if self.reds:
             s.append(f"    unsigned char *partials = (unsigned char *)op2h_reduct_scratch((size_t)nparts * {self.rec});")
         else:
@@ -632,13 +640,13 @@ class LoopGen:
         s.append(f"        const int s0 = cp->col_offsets_h[c], s1 = cp->col_offsets_h[c + 1];")
         s.append(f"        const int g = (s1 - s0 + bs - 1) / bs;")
         s.append(f"        if (g) op2gen_launch(false, nullptr, nullptr, kern, g, bs, 0, st, A, cp->col_reord,")
-        s.append(f"                             s0, s1, 0, partials, base);")
+        s.append(f"                             s0, s1, 0, partials, base, 0);")
         s.append(f"        base += g;")
         s.append(f"      }}")
         s.append(f"    }} else if (nparts > 0) {{")
         stop = "nullptr" if self.reds else "t1"
         s.append(f"      op2gen_launch(ext, t0, {stop}, kern, nparts, bs, 0, st, A,")
-        s.append(f"                    nullptr, 0, set->size, 1, partials, 0);")
+        s.append(f"                    nullptr, 0, set->size, 1, partials, 0, step);")
         s.append(f"    }}")
         if self.reds:
             slots = ", ".join(f"(unsigned char *)op2h_gbl_slot(&ctx, {j}, {self.info_of.get(j, -1)})"
@@ -657,6 +665,7 @@ class LoopGen:
 PRELUDE = r'''#include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>

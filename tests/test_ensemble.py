@@ -65,9 +65,10 @@ def test_autotuned_block_sizes_change_no_bits():
                                 autotune=True)
     for key in ("rms", "maxerr", "errloc", "q"):
         assert np.array_equal(tuned[key], plain[key]), key
-    assert plain["blocks"] == {"update_save_adt": 256, "res_calc": 128, "bres_calc": 128}
+    assert plain["blocks"] == {"update_adt": 256, "res_calc": 256, "bres_calc": 128,
+                               "update_save_adt": 256}
     assert set(tuned["blocks"].values()) <= {64, 128, 256, 512, 1024}
-    assert tuned["blocks"]["res_calc"] == 128
+    assert tuned["blocks"]["res_calc"] == 256
     assert plain["dead_res"] and tuned["dead_res"]
 
 
@@ -83,8 +84,7 @@ def test_persistent_workgroups_walking_many_items_give_the_same_bits(max_workgro
     ref = op2amd.ensemble.run(sp, niter, batched=False, want_q=True)
     capped = op2amd.ensemble.run(sp, niter, batched=True, graphed=True, want_q=True,
                                  max_workgroups=max_workgroups)
-    assert capped["grids"]["res_calc"] <= max_workgroups
-    assert capped["grids"]["update_save_adt"] <= max_workgroups
+    assert max(capped["grids"].values()) <= max_workgroups
     for key in ("rms", "maxerr", "errloc", "q"):
         assert np.array_equal(capped[key], ref[key]), key
     m = 1
@@ -93,6 +93,21 @@ def test_persistent_workgroups_walking_many_items_give_the_same_bits(max_workgro
     mesh = op2amd.mesh.generate(mb["I"], mb["J"], cfg["seed"] + 1 + m, "rcm")
     want = oracle.airfoil_run_qinf(mesh, niter, mb["qinf"])
     assert np.array_equal(capped["q"][off:off + mesh.ncell], want["q"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("patch_cells", [64, 128, 1024])
+def test_res_calc_patch_size_changes_no_bits(patch_cells):
+    """Owner patches of 64, 128 or 1024 cells (one workgroup each) give the
+    bits of the default 256-cell patches: each cell sums its edges' fluxes in
+    edge order whatever patch holds it."""
+    sp = op2amd.ensemble.spec(nmesh=4, min_IJ=(40, 20), max_IJ=(160, 80), seed=7)
+    ref = op2amd.ensemble.run(sp, 10, batched=True, graphed=True, want_q=True)
+    run = op2amd.ensemble.run(sp, 10, batched=True, graphed=True, want_q=True,
+                              patch_cells=patch_cells)
+    assert run["blocks"]["res_calc"] == patch_cells and ref["blocks"]["res_calc"] == 256
+    for key in ("rms", "maxerr", "errloc", "q"):
+        assert np.array_equal(run[key], ref[key]), key
 
 
 @pytest.mark.gpu
