@@ -1,11 +1,15 @@
 // MLP cost model (include/op2amd_costmodel.h): forward, backward and Adam as
-// HIP kernels.  Sizes are small (a few thousand samples, 64 hidden units), so
-// every kernel assigns one thread per output element and reduces over samples
-// sequentially — deterministic gradients.
+// HIP kernels.  Every product of the forward and backward passes (the dense
+// layers, the weight and bias gradients, the back-propagated deltas) runs on
+// the matrix cores (v_mfma_f32_16x16x4f32, FP32 in and out); the weight
+// gradients, which reduce over the samples, are split over the samples in
+// fixed chunks whose partials are summed in chunk order, so the gradients are
+// deterministic.
 #include "op2amd_costmodel.h"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -28,7 +32,7 @@ struct op2amd_mlp {
   int cap = 0;
   float *X = nullptr, *T = nullptr, *Z1 = nullptr, *A1 = nullptr, *Z2 = nullptr,
         *A2 = nullptr, *Y = nullptr, *DY = nullptr, *D2 = nullptr, *D1 = nullptr,
-        *loss = nullptr;
+        *PK = nullptr, *loss = nullptr;  // PK: split partials of the gradients
   int adam_t = 0;
   float wd = 0.f;  // decoupled weight decay
   hipStream_t st = nullptr;
@@ -40,17 +44,77 @@ namespace {
 
 int blocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-// Z[n][m] = b[m] + sum_k X[n][k] W[k][m];  A = relu(Z) (A may be null)
-__global__ void k_dense(const float *__restrict X, const float *__restrict W,
-                        const float *__restrict b, float *__restrict Z,
-                        float *__restrict A, int N, int K, int Mo) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)N * Mo) return;
-  int n = (int)(i / Mo), m = (int)(i % Mo);
-  float z = b[m];
-  for (int k = 0; k < K; ++k) z += X[(size_t)n * K + k] * W[(size_t)k * Mo + m];
-  Z[i] = z;
-  if (A) A[i] = z > 0.f ? z : 0.f;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kSplit = 256;  // samples per split of a weight-gradient product
+
+int splits_for(int n) { return (n + kSplit - 1) / kSplit; }
+
+// C(m, n) = sum over k of a(m, k) b(k, n), k in split blockIdx.z's range
+// [z kchunk, (z + 1) kchunk) of [0, K), with
+//   a(m, k) = A[m sam + k sak] for m < ma, 1 for ma <= m < M (a row of ones
+//             gives the bias gradient next to the weight gradient),
+//   b(k, n) = B[k sbk + n sbn],
+// on the matrix cores: each wave one 16x16 tile of C, four k per
+// v_mfma_f32_16x16x4f32 (A operand: lane l holds a(l % 16, l / 16); B
+// operand: b(l / 16, l % 16); result: lane l, element v holds
+// C(4 (l / 16) + v, l % 16)).  Operands outside the matrices are zeros.
+// Epilogue: EP 0 stores split z's partial at C + z M N; EP 1 stores
+// Z = C + bias[n] and, when R is given, R = relu(Z); EP 2 stores C where
+// mask(m, n) > 0, else 0.  C, R and mask are row-major M x N.
+template <int EP>
+__global__ void __launch_bounds__(256)
+    k_gemm(const float *__restrict A, int sam, int sak, int ma, const float *__restrict B,
+           int sbk, int sbn, float *__restrict C, float *__restrict R,
+           const float *__restrict aux, int M, int N, int K, int kchunk) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i0 = (blockIdx.x * 4 + wave) * 16, j0 = blockIdx.y * 16;
+  if (i0 >= M) return;  // the whole wave: no barriers below
+  const int k_lo = blockIdx.z * kchunk, k_hi = min(K, k_lo + kchunk);
+  const int ia = i0 + (lane & 15), kk = lane >> 4, jb = j0 + (lane & 15);
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = k_lo; k0 < k_hi; k0 += 4) {
+    const int k = k0 + kk;
+    float a = 0.f, b = 0.f;
+    if (k < k_hi) {
+      if (ia < ma)
+        a = A[(size_t)ia * sam + (size_t)k * sak];
+      else if (ia < M)
+        a = 1.f;
+      if (jb < N) b = B[(size_t)k * sbk + (size_t)jb * sbn];
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  }
+  const int j = j0 + (lane & 15);
+  if (j >= N) return;
+  for (int v = 0; v < 4; ++v) {
+    const int i = i0 + 4 * (lane >> 4) + v;
+    if (i >= M) break;
+    const size_t o = (size_t)i * N + j;
+    if (EP == 0) {
+      C[(size_t)blockIdx.z * M * N + o] = acc[v];
+    } else if (EP == 1) {
+      const float z = acc[v] + aux[j];
+      C[o] = z;
+      if (R) R[o] = z > 0.f ? z : 0.f;
+    } else {
+      C[o] = aux[o] > 0.f ? acc[v] : 0.f;
+    }
+  }
+}
+
+// out[i] = sum over splits z, in order, of P[z MN + i]
+__global__ void k_split_sum(const float *__restrict P, int splits, int MN,
+                            float *__restrict out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MN) return;
+  float s = P[i];
+  for (int z = 1; z < splits; ++z) s += P[(size_t)z * MN + i];
+  out[i] = s;
+}
+
+dim3 gemm_grid(int M, int N, int splits) {
+  return dim3((unsigned)(((M + 15) / 16 + 3) / 4), (unsigned)((N + 15) / 16), (unsigned)splits);
 }
 
 // DY[n] = 2 (Y[n] - T[n]) / N
@@ -78,35 +142,6 @@ __global__ void k_loss(const float *__restrict Y, const float *__restrict T,
   if (threadIdx.x == 0) *loss = sh[0] / (float)N;
 }
 
-// dW[k][m] = sum_n A[n][k] D[n][m];  db[m] = sum_n D[n][m]  (db when k == K)
-__global__ void k_wgrad(const float *__restrict A, const float *__restrict D,
-                        float *__restrict dW, float *__restrict db, int N, int K,
-                        int Mo) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (K + 1) * Mo) return;
-  int k = i / Mo, m = i % Mo;
-  float s = 0.f;
-  if (k < K) {
-    for (int n = 0; n < N; ++n) s += A[(size_t)n * K + k] * D[(size_t)n * Mo + m];
-    dW[(size_t)k * Mo + m] = s;
-  } else {
-    for (int n = 0; n < N; ++n) s += D[(size_t)n * Mo + m];
-    db[m] = s;
-  }
-}
-
-// Din[n][k] = (sum_m D[n][m] W[k][m]) * (Z[n][k] > 0)
-__global__ void k_back(const float *__restrict D, const float *__restrict W,
-                       const float *__restrict Z, float *__restrict Din, int N, int K,
-                       int Mo) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)N * K) return;
-  int n = (int)(i / K), k = (int)(i % K);
-  float s = 0.f;
-  for (int m = 0; m < Mo; ++m) s += D[(size_t)n * Mo + m] * W[(size_t)k * Mo + m];
-  Din[i] = Z[i] > 0.f ? s : 0.f;
-}
-
 // Adam with decoupled weight decay `wd` (AdamW; wd = 0 is plain Adam).
 __global__ void k_adam(float *__restrict P, const float *__restrict G,
                        float *__restrict M, float *__restrict V, int np, float lr,
@@ -124,7 +159,8 @@ __global__ void k_adam(float *__restrict P, const float *__restrict G,
 
 void ensure(op2amd_mlp *m, int n) {
   if (n <= m->cap) return;
-  float **bufs[] = {&m->X, &m->T, &m->Z1, &m->A1, &m->Z2, &m->A2, &m->Y, &m->DY, &m->D2, &m->D1};
+  float **bufs[] = {&m->X, &m->T, &m->Z1, &m->A1, &m->Z2, &m->A2, &m->Y, &m->DY, &m->D2, &m->D1,
+                    &m->PK};
   for (float **b : bufs)
     if (*b) MCHECK(hipFree(*b));
   const size_t H = m->H;
@@ -138,26 +174,50 @@ void ensure(op2amd_mlp *m, int n) {
   MCHECK(hipMalloc(&m->DY, sizeof(float) * n));
   MCHECK(hipMalloc(&m->D2, sizeof(float) * n * H));
   MCHECK(hipMalloc(&m->D1, sizeof(float) * n * H));
+  // split partials of the largest gradient product: (max(nin, H) + 1) x H
+  MCHECK(hipMalloc(&m->PK, sizeof(float) * splits_for(n) * (std::max(m->nin, m->H) + 1) * H));
   m->cap = n;
+}
+
+// Z = A W + b (A: n x K, W: K x Mo), R = relu(Z) when R is given
+void dense(op2amd_mlp *m, const float *A, int n, int K, const float *W, const float *b,
+           int Mo, float *Z, float *R) {
+  k_gemm<1><<<gemm_grid(n, Mo, 1), 256, 0, m->st>>>(A, K, 1, n, W, Mo, 1, Z, R, b, n, Mo, K, K);
+}
+
+// [dW; db] = [A^T; 1] D over the n samples (A: n x K, D: n x Mo) into the
+// (K + 1) x Mo block G (W's gradient, then b's, as the parameters are laid out)
+void wgrad(op2amd_mlp *m, const float *A, int n, int K, const float *D, int Mo, float *G) {
+  const int sp = splits_for(n), M = K + 1;
+  k_gemm<0><<<gemm_grid(M, Mo, sp), 256, 0, m->st>>>(A, 1, K, K, D, Mo, 1, m->PK, nullptr,
+                                                       nullptr, M, Mo, n, kSplit);
+  k_split_sum<<<blocks((long long)M * Mo, 256), 256, 0, m->st>>>(m->PK, sp, M * Mo, G);
+}
+
+// Din = (D W^T) where Z > 0, else 0 (D: n x Mo, W: K x Mo, Z and Din: n x K)
+void back(op2amd_mlp *m, const float *D, int n, int Mo, const float *W, int K, const float *Z,
+          float *Din) {
+  k_gemm<2><<<gemm_grid(n, K, 1), 256, 0, m->st>>>(D, Mo, 1, n, W, 1, Mo, Din, nullptr, Z, n, K,
+                                                     Mo, Mo);
 }
 
 void forward(op2amd_mlp *m, int n) {
   const int H = m->H;
   float *P = m->P;
-  k_dense<<<blocks((long long)n * H, 256), 256, 0, m->st>>>(m->X, P + m->oW1, P + m->ob1, m->Z1, m->A1, n, m->nin, H);
-  k_dense<<<blocks((long long)n * H, 256), 256, 0, m->st>>>(m->A1, P + m->oW2, P + m->ob2, m->Z2, m->A2, n, H, H);
-  k_dense<<<blocks(n, 256), 256, 0, m->st>>>(m->A2, P + m->oW3, P + m->ob3, m->Y, nullptr, n, H, 1);
+  dense(m, m->X, n, m->nin, P + m->oW1, P + m->ob1, H, m->Z1, m->A1);
+  dense(m, m->A1, n, H, P + m->oW2, P + m->ob2, H, m->Z2, m->A2);
+  dense(m, m->A2, n, H, P + m->oW3, P + m->ob3, 1, m->Y, nullptr);
 }
 
 void backward(op2amd_mlp *m, int n) {
   const int H = m->H;
   float *P = m->P, *G = m->G;
   k_dloss<<<blocks(n, 256), 256, 0, m->st>>>(m->Y, m->T, m->DY, n);
-  k_wgrad<<<blocks((long long)(H + 1) * 1, 256), 256, 0, m->st>>>(m->A2, m->DY, G + m->oW3, G + m->ob3, n, H, 1);
-  k_back<<<blocks((long long)n * H, 256), 256, 0, m->st>>>(m->DY, P + m->oW3, m->Z2, m->D2, n, H, 1);
-  k_wgrad<<<blocks((long long)(H + 1) * H, 256), 256, 0, m->st>>>(m->A1, m->D2, G + m->oW2, G + m->ob2, n, H, H);
-  k_back<<<blocks((long long)n * H, 256), 256, 0, m->st>>>(m->D2, P + m->oW2, m->Z1, m->D1, n, H, H);
-  k_wgrad<<<blocks((long long)(m->nin + 1) * H, 256), 256, 0, m->st>>>(m->X, m->D1, G + m->oW1, G + m->ob1, n, m->nin, H);
+  wgrad(m, m->A2, n, H, m->DY, 1, G + m->oW3);
+  back(m, m->DY, n, 1, P + m->oW3, H, m->Z2, m->D2);
+  wgrad(m, m->A1, n, H, m->D2, H, G + m->oW2);
+  back(m, m->D2, n, H, P + m->oW2, H, m->Z1, m->D1);
+  wgrad(m, m->X, n, m->nin, m->D1, H, G + m->oW1);
 }
 
 void upload(op2amd_mlp *m, const float *x, const float *t, int n) {
@@ -212,7 +272,7 @@ extern "C" op2amd_mlp *op2amd_mlp_create(int nin, int hidden, uint64_t seed) {
 extern "C" void op2amd_mlp_destroy(op2amd_mlp *m) {
   if (!m) return;
   float *bufs[] = {m->P, m->G, m->M, m->V, m->loss, m->X, m->T, m->Z1, m->A1,
-                   m->Z2, m->A2, m->Y, m->DY, m->D2, m->D1};
+                   m->Z2, m->A2, m->Y, m->DY, m->D2, m->D1, m->PK};
   for (float *b : bufs)
     if (b) (void)hipFree(b);
   if (m->st) (void)hipStreamDestroy(m->st);

@@ -30,10 +30,14 @@ def test_oracle_backward_matches_finite_differences():
 
 
 @pytest.mark.gpu
-def test_hip_mlp_forward_and_gradients_match_oracle():
+@pytest.mark.parametrize("n", [1, 37, 300, 1100])
+def test_hip_mlp_forward_and_gradients_match_oracle(n):
+    """Forward pass and gradients on the matrix cores against the float64
+    oracle: one sample, sizes off the 16-row tiles and the 4-deep steps, and
+    several 256-sample splits of the weight gradients."""
     m = op2amd.costmodel.MLP(NIN, H, seed=7)
     p = m.params()
-    x, t = batch()
+    x, t = batch(n)
     y = m.forward(x)
     y_ref, _ = mlp_ref.forward(p, x, NIN, H)
     assert np.max(np.abs(y - y_ref)) <= 1e-4 * max(1.0, np.max(np.abs(y_ref)))
@@ -41,6 +45,8 @@ def test_hip_mlp_forward_and_gradients_match_oracle():
     loss_ref, g_ref = mlp_ref.loss_grad(p, x, t, NIN, H)
     assert abs(loss - loss_ref) <= 1e-5 * loss_ref
     assert np.max(np.abs(g - g_ref)) <= 1e-4 * np.max(np.abs(g_ref))
+    _, g2 = m.loss_grad(x, t)
+    assert np.array_equal(g, g2)  # split partials summed in a fixed order
 
 
 @pytest.mark.gpu
