@@ -314,15 +314,15 @@ def workload_ensemble(op2amd, args):
     """256 uneven meshes (40x20 .. 640x320, per-member Mach and alpha): each
     loop one persistent launch over all members' work items, the step
     graph-captured, workgroup sizes picked by the ensemble's autotuner over
-    the 5 warm-up steps, vs one launch per loop per member.  Per-member step
+    the 10 warm-up steps, vs one launch per loop per member.  Per-member step
     times: inside the batched step from device-clock stamps of every work
     item (a separate, shorter run), in the looped run from HIP events."""
     sp = op2amd.ensemble.spec(nmesh=args.ensemble_size)
-    n, W = args.ensemble_iters, 5
+    n, W = args.ensemble_iters, 10
     batched = op2amd.ensemble.run(sp, n, warmup=W, batched=True, graphed=True, autotune=True)
     looped = op2amd.ensemble.run(sp, min(n, 30), warmup=W, batched=False)
     timing = op2amd.ensemble.run(sp, 12, warmup=2, batched=False, member_timing=True)
-    btiming = op2amd.ensemble.run(sp, 12, warmup=W + 1, batched=True, graphed=True,
+    btiming = op2amd.ensemble.run(sp, 20, warmup=W + 1, batched=True, graphed=True,
                                   autotune=True, member_timing=True)
     nc = batched["total_cells"]
     b_ups = nc * (n - W) / batched["seconds"]
@@ -527,7 +527,8 @@ def workload_costmodel(op2amd, args):
                                     graph_steps=args.graph_steps)
     finally:
         cm.CostModel.uninstall()
-    kw = sweep(op2amd, mesh, args.sweep_iters, False)
+    kw = {k: v for k, v in sweep(op2amd, mesh, args.sweep_iters, False).items()
+          if not k.startswith("_")}  # the plans alone
     swept = op2amd.airfoil.run(mesh, n, mode="graphed", warmup=W, graph_steps=args.graph_steps,
                                **kw)
     model_rate, swept_rate = (n - W) / picked["seconds"], (n - W) / swept["seconds"]

@@ -63,10 +63,11 @@ typedef struct {
   int member_timing; /* per-member step times: looped mode, HIP events around
                         each member's launches; batched mode, device-clock
                         stamps of every work item inside the batched step */
-  int autotune;      /* batched only: workgroup sizes of the update+save_soln+
-                        adt_calc and bres_calc launches picked by timing the
-                        first five steps (one candidate of 64..1024 per step;
-                        the results do not depend on them) */
+  int autotune;      /* batched only: workgroup sizes of the two update+
+                        adt_calc launches and of bres_calc picked by timing
+                        the first ten steps (candidates 64..1024, one per
+                        step, each twice; a candidate's time is its faster
+                        step; the results do not depend on them) */
   int max_workgroups; /* > 0: at most this many workgroups per launch (each
                          then walks more work items); 0: as many as the GPU
                          holds at once */

@@ -692,11 +692,15 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       ECHECK(hipEventCreate(&ev_b[m]));
     }
   }
-  // autotuning: step i (i < 5) runs both chains and bres_calc with workgroup
-  // size candidate i; the fastest total per kernel over its launches wins
+  // autotuning: steps 1..10 run both chains and bres_calc with workgroup
+  // size candidate (i - 1) mod 5, so each candidate twice; per kernel, a
+  // candidate's time is its faster step (its launches in the step summed),
+  // and the fastest candidate wins
   const int cand[5] = {64, 128, 256, 512, 1024};
-  const int ntune = (o->batched && o->autotune) ? std::min(5, o->niter) : 0;
-  double tune_ms[4][5] = {};
+  const int ntune = (o->batched && o->autotune) ? std::min(10, o->niter) : 0;
+  double tune_ms[4][5];
+  for (auto &row : tune_ms)
+    for (double &v : row) v = INFINITY;
   hipEvent_t tev[12];
   if (ntune)
     for (auto &e : tev) ECHECK(hipEventCreate(&e));
@@ -710,22 +714,25 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       t0 = now();
     }
     if (it <= ntune) {
-      blk[0] = blk[2] = blk[3] = cand[it - 1];
+      const int ci = (it - 1) % 5;
+      blk[0] = blk[2] = blk[3] = cand[ci];
       set_caps();
       step_range(0, M, tev);
       history();
       ECHECK(hipStreamSynchronize(st));
       // event pairs: (res, bres, chain) twice
       const int kind[6] = {1, 2, 0, 1, 2, 3};
+      double step_ms[4] = {0.0, 0.0, 0.0, 0.0};
       for (int j = 0; j < 6; ++j) {
         float ms = 0.f;
         ECHECK(hipEventElapsedTime(&ms, tev[2 * j], tev[2 * j + 1]));
-        tune_ms[kind[j]][it - 1] += ms;
+        step_ms[kind[j]] += ms;
       }
+      for (int k = 0; k < 4; ++k) tune_ms[k][ci] = std::min(tune_ms[k][ci], step_ms[k]);
       if (it == ntune) {
         for (int k : {0, 2, 3}) {
           int best = 0;
-          for (int c = 1; c < ntune; ++c)
+          for (int c = 1; c < 5; ++c)
             if (tune_ms[k][c] < tune_ms[k][best]) best = c;
           blk[k] = cand[best];
         }

@@ -651,7 +651,9 @@ class LoopGen:
         if self.reds:
             slots = ", ".join(f"(unsigned char *)op2h_gbl_slot(&ctx, {j}, {self.info_of.get(j, -1)})"
                               for j in self.reds)
-            s.append(f"    op2gen_launch(ext, nullptr, t1, {self.name}_finish, 1, 1024, 0, st, partials,")
+            # one workgroup of 256: at most 2048 records (8 per thread) outside
+            # colouring; fewer waves make its barriers and wave combines cheaper
+            s.append(f"    op2gen_launch(ext, nullptr, t1, {self.name}_finish, 1, 256, 0, st, partials,")
             s.append(f"                  nparts, {slots});")
         s.append(f"    if (!ext) op2h_kernel_timer_stop(&ctx);")
         s.append(f"    OP2GEN_CHECK(hipGetLastError());")

@@ -57,7 +57,7 @@ def test_batched_equals_looped_and_each_member_equals_sequential_run(ordering):
 @pytest.mark.gpu
 def test_autotuned_block_sizes_change_no_bits():
     """The batched run with its workgroup sizes picked by timing the first
-    five steps (one candidate per step) gives the bits of the default launch
+    ten steps (one candidate per step, each twice) gives the bits of the default launch
     shapes; the picks are candidates."""
     sp = op2amd.ensemble.spec(**SMALL)
     plain = op2amd.ensemble.run(sp, 30, batched=True, graphed=True, want_q=True)
