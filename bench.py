@@ -353,7 +353,7 @@ def workload_ensemble(op2amd, args):
                    "rms_identical_batched_vs_looped": same,
                    "batched_workgroup_sizes": batched["blocks"],
                    "batched_persistent_grids": batched["grids"],
-                   "dead_res": batched["dead_res"]},
+                   "dead_res": batched["dead_res"], "fused_bres": batched["fused_bres"]},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

@@ -73,6 +73,9 @@ typedef struct {
                          holds at once */
   int patch_cells;    /* cells per owner patch of res_calc, 64..1024; 0: 256
                          (the results do not depend on it) */
+  int separate_bres;  /* 1: bres_calc in launches of its own; 0: inside
+                         res_calc's launch where every boundary cell is owned
+                         (the same bits) */
 } op2amd_ensemble_opts;
 
 typedef struct {
@@ -94,13 +97,17 @@ typedef struct {
   double setup_seconds;
   long long total_cells, total_edges;
   int blocks[4];        /* workgroup sizes: update+adt_calc (first half
-                           step), res_calc (its owner patch size), bres_calc,
-                           update+save_soln+adt_calc (second half step) */
+                           step), res_calc (its owner patch size), bres_calc
+                           (0: fused into res_calc), update+save_soln+adt_calc
+                           (second half step) */
   int grids[4];         /* persistent grids (workgroups) of the same launches
                            over all members */
   int dead_res;         /* 1: every cell is owned by a res_calc patch, so the
                            update does not zero res and res_calc does not
                            read it */
+  int fused_bres;       /* 1: bres_calc runs inside res_calc's launch, each
+                           boundary cell adding its boundary edges after its
+                           interior ones (the order of the two loops) */
 } op2amd_ensemble_result;
 
 int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,

@@ -105,6 +105,38 @@ __device__ __forceinline__ void res_flux(const double *x1, const double *x2,
   f[3] = 0.5 * (vol1 * (q1[3] + p1) + vol2 * (q2[3] + p2)) + mu * (q1[3] - q2[3]);
 }
 
+// bres_calc with the free stream qi (qinf of the program, or a member's own)
+__device__ __forceinline__ void bres_calc_q(const double *x1, const double *x2,
+                                            const double *q1, const double *adt1,
+                                            double *res1, const int *bound,
+                                            const double *qi) {
+  double dx, dy, mu, ri, p1, vol1, p2, vol2, f;
+  dx = x1[0] - x2[0];
+  dy = x1[1] - x2[1];
+  ri = 1.0 / q1[0];
+  p1 = gm1_d * (q1[3] - 0.5 * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
+  if (*bound == 1) {
+    res1[1] += +p1 * dy;
+    res1[2] += -p1 * dx;
+  } else {
+    vol1 = ri * (q1[1] * dy - q1[2] * dx);
+    ri = 1.0 / qi[0];
+    p2 = gm1_d * (qi[3] - 0.5 * ri * (qi[1] * qi[1] + qi[2] * qi[2]));
+    vol2 = ri * (qi[1] * dy - qi[2] * dx);
+    mu = (*adt1) * eps_d;
+    f = 0.5 * (vol1 * q1[0] + vol2 * qi[0]) + mu * (q1[0] - qi[0]);
+    res1[0] += f;
+    f = 0.5 * (vol1 * q1[1] + p1 * dy + vol2 * qi[1] + p2 * dy) + mu * (q1[1] - qi[1]);
+    res1[1] += f;
+    f = 0.5 * (vol1 * q1[2] - p1 * dx + vol2 * qi[2] - p2 * dx) + mu * (q1[2] - qi[2]);
+    res1[2] += f;
+    f = 0.5 * (vol1 * (q1[3] + p1) + vol2 * (qi[3] + p2)) + mu * (q1[3] - qi[3]);
+    res1[3] += f;
+  }
+}
+
+// bres_calc.h with the program's qinf (kept apart from bres_calc_q so that
+// qinf stays in the constant bank: scalar loads, no vector registers)
 __device__ __forceinline__ void bres_calc(const double *x1, const double *x2,
                                           const double *q1, const double *adt1,
                                           double *res1, const int *bound) {
@@ -504,9 +536,15 @@ struct BoundaryArgs {
   const int *edges;     // boundary edge of each (block, j)
   const int2 *pbedge;   // boundary edge -> its two nodes
   const int *bound;     // boundary edge -> 1 wall, else far field
+  // ensembles (QI): boundary edge -> its member, and the members' free
+  // streams [member][4]
+  const int *bmesh;
+  const double *qinf;
 };
 
-template <bool SOA>
+// QI: each boundary edge with its member's free stream (B.bmesh, B.qinf),
+// else the program's qinf.
+template <bool SOA, bool QI = false>
 __device__ __forceinline__ void owner_res_bres_block(int b, const double *__restrict x,
                                                      const double *__restrict q,
                                                      const double *__restrict adt,
@@ -546,7 +584,10 @@ __device__ __forceinline__ void owner_res_bres_block(int b, const double *__rest
         ld4<SOA>(q, cell, sq, q1);
         const double a1 = adt[cell];
         const int bnd = B.bound[be];
-        op2_k::bres_calc(x1, x2, q1, &a1, acc, &bnd);
+        if constexpr (QI)
+          op2_k::bres_calc_q(x1, x2, q1, &a1, acc, &bnd, B.qinf + 4 * (size_t)B.bmesh[be]);
+        else
+          op2_k::bres_calc(x1, x2, q1, &a1, acc, &bnd);
       } else {
         owner_apply(acc, lf, (int)ent[k]);
       }

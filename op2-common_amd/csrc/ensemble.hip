@@ -102,33 +102,21 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-// bres_calc (bres_calc.h:1-35) with the member's own free stream `qi`
-__device__ __forceinline__ void bres_calc_qi(const double *x1, const double *x2,
-                                             const double *q1, const double *adt1,
-                                             double *res1, const int *bound,
-                                             const double *qi) {
-  double dx, dy, mu, ri, p1, vol1, p2, vol2, f;
-  dx = x1[0] - x2[0];
-  dy = x1[1] - x2[1];
-  ri = 1.0 / q1[0];
-  p1 = gm1_d * (q1[3] - 0.5 * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
-  if (*bound == 1) {
-    res1[1] += +p1 * dy;
-    res1[2] += -p1 * dx;
-  } else {
-    vol1 = ri * (q1[1] * dy - q1[2] * dx);
-    ri = 1.0 / qi[0];
-    p2 = gm1_d * (qi[3] - 0.5 * ri * (qi[1] * qi[1] + qi[2] * qi[2]));
-    vol2 = ri * (qi[1] * dy - qi[2] * dx);
-    mu = (*adt1) * eps_d;
-    f = 0.5 * (vol1 * q1[0] + vol2 * qi[0]) + mu * (q1[0] - qi[0]);
-    res1[0] += f;
-    f = 0.5 * (vol1 * q1[1] + p1 * dy + vol2 * qi[1] + p2 * dy) + mu * (q1[1] - qi[1]);
-    res1[1] += f;
-    f = 0.5 * (vol1 * q1[2] - p1 * dx + vol2 * qi[2] - p2 * dx) + mu * (q1[2] - qi[2]);
-    res1[2] += f;
-    f = 0.5 * (vol1 * (q1[3] + p1) + vol2 * (qi[3] + p2)) + mu * (q1[3] - qi[3]);
-    res1[3] += f;
+// res_calc and bres_calc in one pass, persistent over the owner patches
+// [blo, bhi): each owned cell sums its interior edges' fluxes in edge order,
+// then its boundary edges' (bres_calc with the member's own free stream) in
+// boundary-edge order, the order of res_calc then bres_calc
+// (op2_dev::owner_res_bres_block).  zero as for k_ens_res_owner.
+__global__ void __launch_bounds__(1024)
+    k_ens_res_bres_owner(const double *__restrict x, const double *__restrict q,
+                         const double *__restrict adt, double *__restrict res, OwnerArgs P,
+                         BoundaryArgs B, int blo, int bhi, int zero, Stamps S) {
+  extern __shared__ double lf[];
+  for (int b = blo + xcd_block(kXcds); b < bhi; b += gridDim.x) {
+    const unsigned long long t0 = stamp_now(S);
+    owner_res_bres_block<false, true>(b, x, q, adt, res, P, B, 0, 0, lf, zero != 0);
+    __syncthreads();  // the next patch rewrites lf
+    if (S.t && threadIdx.x == 0) stamp(S, b, t0);
   }
 }
 
@@ -161,7 +149,7 @@ __global__ void k_ens_bres(const double *__restrict x, const double *__restrict 
         ld2<false>(x, bn.x, 0, x1);
         ld2<false>(x, bn.y, 0, x2);
         const int bd = bound[e];
-        bres_calc_qi(x1, x2, q1, &a1, r, &bd, qinf + 4 * (size_t)bmesh[e]);
+        op2_k::bres_calc_q(x1, x2, q1, &a1, r, &bd, qinf + 4 * (size_t)bmesh[e]);
         for (int d = 0; d < 4; ++d) acc[d] += r[d];
       }
       st4<false>(res, cell, 0, acc);
@@ -525,10 +513,41 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       bitems.push_back(make_int2(t, std::min(t + 64, mesh_btarget_off[m + 1])));
     mesh_bitem_off.push_back((int)bitems.size());
   }
-  const int ntiles = (int)tiles.size(), npatch = mblk[M], nbitem = (int)bitems.size();
+  const int ntiles = (int)tiles.size(), npatch = mblk[M];
   // every cell owned by a patch: res_calc writes all of res, so the update
   // need not zero it (dead res) and res_calc need not read it
   const bool dead = (int)owned.size() == Nc;
+  // bres_calc fused into res_calc: per owned cell its interior increments,
+  // then its boundary edges as 0x8000 | j, j indexing the patch's boundary
+  // edges (in cell order, each cell's in bedge order); possible when every
+  // boundary cell is owned and no cell has more than four entries
+  std::vector<ushort4> brec(owned.size());
+  std::vector<int> bedge_offs(npatch + 1, 0), bedge_list;
+  bool fused = dead && 2 * max_e < 0x8000 && !o->separate_bres;
+  for (int b = 0; fused && b < npatch; ++b) {
+    bedge_offs[b] = (int)bedge_list.size();
+    for (int i = owned_offs[b]; fused && i < owned_offs[b + 1]; ++i) {
+      unsigned short ent[4] = {csr4[i].x, csr4[i].y, csr4[i].z, csr4[i].w};
+      int k = 0;
+      while (k < 4 && ent[k] != 0xFFFF) ++k;
+      const int t = bpos[owned[i]];
+      if (t >= 0)
+        for (int j = boffsets[t]; j < boffsets[t + 1]; ++j) {
+          const int local = (int)bedge_list.size() - bedge_offs[b];
+          if (k == 4 || local >= 0x7FFF) {
+            fused = false;
+            break;
+          }
+          ent[k++] = (unsigned short)(0x8000 | local);
+          bedge_list.push_back(bentries[j]);
+        }
+      brec[i] = make_ushort4(ent[0], ent[1], ent[2], ent[3]);
+    }
+  }
+  bedge_offs[npatch] = (int)bedge_list.size();
+  fused = fused && (int)bedge_list.size() == Nb;
+  // bres_calc's own items when it is not fused
+  const int nbitem = fused ? 0 : (int)bitems.size();
 
   // ---- device data ----
   std::vector<void *> keep;
@@ -562,6 +581,16 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   int *d_boffsets = dev_upload(boffsets, keep);
   int *d_bentries = dev_upload(bentries, keep);
   int2 *d_bitems = dev_upload(bitems, keep);
+  BoundaryArgs BA{};
+  if (fused) {
+    BA.rec = dev_upload(brec, keep);
+    BA.offs = dev_upload(bedge_offs, keep);
+    BA.edges = dev_upload(bedge_list, keep);
+    BA.pbedge = d_bedge;
+    BA.bound = d_bound;
+    BA.bmesh = d_bmesh;
+    BA.qinf = d_qinf;
+  }
   int2 *d_tiles = dev_upload(tiles, keep);
   int *d_coff = dev_upload(coff, keep);
   int *d_mtoff = dev_upload(mesh_tile_off, keep);
@@ -624,7 +653,8 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   int cap[4] = {0, 0, 0, 0};
   auto set_caps = [&]() {
     cap[0] = resident((const void *)k_chain0, blk[0], 0);
-    cap[1] = resident((const void *)k_ens_res_owner, blk[1], res_lds);
+    cap[1] = resident(fused ? (const void *)k_ens_res_bres_owner : (const void *)k_ens_res_owner,
+                      blk[1], res_lds);
     cap[2] = resident((const void *)k_ens_bres, blk[2], 0) * (blk[2] / 64);  // waves
     cap[3] = resident((const void *)k_chain1, blk[3], 0);
   };
@@ -642,9 +672,11 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
           d_x, d_qold, d_q, d_res, d_adt, d_cell, d_tiles, d_coff, t0, t1, d_psum, d_pmax,
           d_ploc, Stamps{nullptr, d_count, per_step, capacity, 0});
   };
-  // one time step of members [mlo, mhi): twice res_calc, bres_calc and the
-  // chain, then the per-member reductions; with `ev`, each of the six loop
-  // launches is bracketed by events ev[2j], ev[2j+1] in launch order
+  // one time step of members [mlo, mhi): twice res_calc and bres_calc (one
+  // launch when fused) and the chain, then the per-member reductions; with
+  // `ev`, res_calc, bres_calc and the chain of each half step are bracketed by
+  // events ev[2j], ev[2j+1] in that order (bres_calc's pair brackets nothing
+  // when fused)
   auto step_range = [&](int mlo, int mhi, hipEvent_t *ev = nullptr) {
     const int p0 = mblk[mlo], p1 = mblk[mhi];
     const int b0 = mesh_bitem_off[mlo], b1 = mesh_bitem_off[mhi];
@@ -655,12 +687,15 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
     };
     for (int k = 0; k < 2; ++k) {
       mark();
-      if (p1 > p0)
+      if (p1 > p0 && fused)
+        k_ens_res_bres_owner<<<std::min(p1 - p0, cap[1]), blk[1], res_lds, st>>>(
+            d_x, d_q, d_adt, d_res, OA, BA, p0, p1, res_zero, S_at(off_res[k]));
+      else if (p1 > p0)
         k_ens_res_owner<<<std::min(p1 - p0, cap[1]), blk[1], res_lds, st>>>(
             d_x, d_q, d_adt, d_res, OA, p0, p1, res_zero, S_at(off_res[k]));
       mark();
       mark();
-      if (b1 > b0) {
+      if (b1 > b0 && !fused) {
         const int wpb = blk[2] / 64, g = (std::min(b1 - b0, cap[2]) + wpb - 1) / wpb;
         k_ens_bres<<<g, blk[2], 0, st>>>(d_x, d_q, d_adt, d_res, d_bedge, d_bound, d_bmesh,
                                          d_qinf, d_boffsets, d_bentries, d_btargets,
@@ -733,6 +768,7 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       for (int k = 0; k < 4; ++k) tune_ms[k][ci] = std::min(tune_ms[k][ci], step_ms[k]);
       if (it == ntune) {
         for (int k : {0, 2, 3}) {
+          if (k == 2 && fused) continue;
           int best = 0;
           for (int c = 1; c < 5; ++c)
             if (tune_ms[k][c] < tune_ms[k][best]) best = c;
@@ -812,9 +848,11 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       int off;
       const std::vector<int> *moff;
     };
-    const Seg segs[7] = {{off_res[0], &mblk},          {off_bres[0], &mesh_bitem_off},
+    const std::vector<int> no_items(M + 1, 0);  // bres_calc fused: no items of its own
+    const std::vector<int> *bmoff = fused ? &no_items : &mesh_bitem_off;
+    const Seg segs[7] = {{off_res[0], &mblk},          {off_bres[0], bmoff},
                          {off_chain[0], &mesh_tile_off}, {off_res[1], &mblk},
-                         {off_bres[1], &mesh_bitem_off}, {off_chain[1], &mesh_tile_off},
+                         {off_bres[1], bmoff},           {off_chain[1], &mesh_tile_off},
                          {off_fin, nullptr}};
     std::vector<double> lat(M, 0.0);
     for (int it = o->warmup; it < o->niter; ++it) {
@@ -857,6 +895,8 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
     res->grids[k] = cap[k];
   }
   res->grids[2] = (cap[2] + blk[2] / 64 - 1) / (blk[2] / 64);
+  if (fused) res->blocks[2] = res->grids[2] = 0;
+  res->fused_bres = fused ? 1 : 0;
   res->dead_res = dead ? 1 : 0;
   if (ntune)
     for (auto e : tev) (void)hipEventDestroy(e);

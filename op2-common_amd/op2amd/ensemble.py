@@ -24,7 +24,8 @@ class Member(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("niter", C.c_int), ("warmup", C.c_int), ("batched", C.c_int),
                 ("graphed", C.c_int), ("member_timing", C.c_int), ("autotune", C.c_int),
-                ("max_workgroups", C.c_int), ("patch_cells", C.c_int)]
+                ("max_workgroups", C.c_int), ("patch_cells", C.c_int),
+                ("separate_bres", C.c_int)]
 
 
 class Result(C.Structure):
@@ -34,7 +35,7 @@ class Result(C.Structure):
                 ("member_busy_ms", C.POINTER(C.c_double)), ("seconds", C.c_double),
                 ("setup_seconds", C.c_double), ("total_cells", C.c_longlong),
                 ("total_edges", C.c_longlong), ("blocks", C.c_int * 4),
-                ("grids", C.c_int * 4), ("dead_res", C.c_int)]
+                ("grids", C.c_int * 4), ("dead_res", C.c_int), ("fused_bres", C.c_int)]
 
 LOOPS = ("update_adt", "res_calc", "bres_calc", "update_save_adt")
 
@@ -60,7 +61,7 @@ def describe(sp):
 
 
 def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
-        want_q=False, autotune=False, max_workgroups=0, patch_cells=0):
+        want_q=False, autotune=False, max_workgroups=0, patch_cells=0, separate_bres=False):
     members = describe(sp)
     M = sp.nmesh
     total = sum(m["ncell"] for m in members)
@@ -78,7 +79,7 @@ def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
     r.member_step_ms = step_ms.ctypes.data_as(C.POINTER(C.c_double))
     r.member_busy_ms = busy_ms.ctypes.data_as(C.POINTER(C.c_double))
     o = Opts(niter, warmup, int(batched), int(graphed), int(member_timing), int(autotune),
-             int(max_workgroups), int(patch_cells))
+             int(max_workgroups), int(patch_cells), int(separate_bres))
     rc = lib.op2amd_ensemble_run(C.byref(sp), C.byref(o), C.byref(r))
     if rc:
         raise RuntimeError(f"ensemble run failed ({rc})")
@@ -87,4 +88,5 @@ def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
             "setup_seconds": r.setup_seconds, "total_cells": r.total_cells,
             "total_edges": r.total_edges,
             "blocks": dict(zip(LOOPS, list(r.blocks))),
-            "grids": dict(zip(LOOPS, list(r.grids))), "dead_res": bool(r.dead_res)}
+            "grids": dict(zip(LOOPS, list(r.grids))), "dead_res": bool(r.dead_res),
+            "fused_bres": bool(r.fused_bres)}

@@ -65,9 +65,11 @@ def test_autotuned_block_sizes_change_no_bits():
                                 autotune=True)
     for key in ("rms", "maxerr", "errloc", "q"):
         assert np.array_equal(tuned[key], plain[key]), key
-    assert plain["blocks"] == {"update_adt": 256, "res_calc": 256, "bres_calc": 128,
+    # bres_calc runs inside res_calc's launch: no workgroup size of its own
+    assert plain["blocks"] == {"update_adt": 256, "res_calc": 256, "bres_calc": 0,
                                "update_save_adt": 256}
-    assert set(tuned["blocks"].values()) <= {64, 128, 256, 512, 1024}
+    assert plain["fused_bres"] and tuned["fused_bres"]
+    assert set(tuned["blocks"].values()) <= {0, 64, 128, 256, 512, 1024}
     assert tuned["blocks"]["res_calc"] == 256
     assert plain["dead_res"] and tuned["dead_res"]
 
@@ -93,6 +95,26 @@ def test_persistent_workgroups_walking_many_items_give_the_same_bits(max_workgro
     mesh = op2amd.mesh.generate(mb["I"], mb["J"], cfg["seed"] + 1 + m, "rcm")
     want = oracle.airfoil_run_qinf(mesh, niter, mb["qinf"])
     assert np.array_equal(capped["q"][off:off + mesh.ncell], want["q"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("max_workgroups", [0, 5])
+def test_bres_calc_fused_into_res_calc_gives_the_same_bits(max_workgroups):
+    """bres_calc fused into res_calc's owner launch (each boundary cell adds
+    its boundary edges after its interior ones) and bres_calc in launches of
+    its own, one wave per run of boundary cells, give the same bits, with the
+    tuned workgroup sizes and with grids capped so that workgroups walk many
+    items."""
+    sp = op2amd.ensemble.spec(nmesh=5, min_IJ=(40, 20), max_IJ=(160, 80), seed=9)
+    kw = dict(batched=True, graphed=True, want_q=True, max_workgroups=max_workgroups,
+              autotune=max_workgroups == 0)
+    fused = op2amd.ensemble.run(sp, 20, **kw)
+    apart = op2amd.ensemble.run(sp, 20, separate_bres=True, **kw)
+    assert fused["fused_bres"] and not apart["fused_bres"]
+    assert fused["blocks"]["bres_calc"] == 0
+    assert apart["blocks"]["bres_calc"] in (64, 128, 256, 512, 1024)
+    for key in ("rms", "maxerr", "errloc", "q"):
+        assert np.array_equal(fused[key], apart[key]), key
 
 
 @pytest.mark.gpu
