@@ -4,8 +4,9 @@
  * An ensemble is many airfoil problems of different sizes and free streams
  * advanced in lock step.  All member meshes are concatenated into one set of
  * nodes/cells/edges/bedges; each loop's work is a table of items, runs of one
- * member's elements (tiles of <= 1024 cells, the owner patches of res_calc,
- * <= 64 boundary cells of bres_calc), with per-member offsets into it.  Each
+ * member's elements (tiles of <= 1024 cells, the owner patches of res_calc
+ * with bres_calc inside them, or <= 64 boundary cells when bres_calc runs
+ * apart), with per-member offsets into it.  Each
  * loop of the time step is ONE persistent launch over every member's items: a
  * grid of as many workgroups as the GPU holds at once, each walking the item
  * table (no edge crosses members, so the concatenation is exact).  update,
