@@ -62,6 +62,19 @@ def describe(sp):
 
 def run(sp, niter, warmup=0, batched=True, graphed=True, member_timing=False,
         want_q=False, autotune=False, max_workgroups=0, patch_cells=0, separate_bres=False):
+    """Run `niter` time steps of the ensemble `sp` (op2amd_ensemble_run).
+
+    batched: each loop one persistent launch over every member's work items
+    (graphed: the step replayed from a HIP graph), else one launch per loop
+    per member.  autotune: workgroup sizes picked over the first ten steps.
+    member_timing: per-member step times (batched: device-clock stamps of
+    every work item; looped: HIP events), in member_step_ms (latency inside
+    the step) and member_busy_ms (the member's items' spans summed).
+    max_workgroups caps the persistent grids, patch_cells sets res_calc's
+    owner patch size, separate_bres keeps bres_calc out of res_calc's launch;
+    none of them changes a bit of the results.  Returns per-iteration,
+    per-member rms/maxerr/errloc ([niter][nmesh]), q when want_q, timings
+    and the launch shapes used."""
     members = describe(sp)
     M = sp.nmesh
     total = sum(m["ncell"] for m in members)
