@@ -86,9 +86,9 @@ __device__ __forceinline__ void stamp(const Stamps &S, int item, unsigned long l
 // res_calc, persistent over the owner patches [blo, bhi) of the members'
 // owner partitions: each edge evaluated once per patch it touches, each
 // cell's increments summed in edge order (op2_dev::owner_res_block), so every
-// member gets its sequential executor's bits.  zero: every cell is owned and
-// the update before zeroed res, so the sums start from +0.0 without reading
-// it.  LDS: fluxes [max_e][4].
+// member gets its sequential executor's bits.  zero: every cell is owned, so
+// res holds the update's zeros in every cell (not stored: dead) and each sum
+// starts from +0.0 without reading it.  LDS: fluxes [max_e][4].
 __global__ void __launch_bounds__(1024)
     k_ens_res_owner(const double *__restrict x, const double *__restrict q,
                     const double *__restrict adt, double *__restrict res, OwnerArgs P,
