@@ -29,6 +29,7 @@
 #include <cstring>
 
 #include <map>
+#include <vector>
 
 #include "op2_hip_airfoil.h"
 #include "op2_hip_rt.h"
@@ -1156,7 +1157,8 @@ template <class R, bool SOA, bool UPD, bool SAVE>
 __global__ void __launch_bounds__(1024)
     k_direct_chain(const double *qold, double *__restrict q, R *__restrict res,
                    R *adt, double *qold_out, const double *__restrict x,
-                   const int *__restrict pcell, int n, int s, int sx,
+                   const int *__restrict pcell, const uint2 *__restrict prow,
+                   const int *__restrict pbase, int n, int s, int sx,
                    double *__restrict part_sum, double *__restrict part_max,
                    int *__restrict part_loc, const int *__restrict oldidx, int chunk,
                    int res_dead) {
@@ -1203,7 +1205,15 @@ __global__ void __launch_bounds__(1024)
       ld4<SOA>(q, c, s, qn);
     }
     if (SAVE) st4<SOA>(qold_out, c, s, qn);
-    const int4 cn = reinterpret_cast<const int4 *>(pcell)[c];
+    int4 cn;
+    if (prow) {  // compact rows: four 16-bit offsets from the cell group's base
+      const uint2 u = prow[c];
+      const int b = pbase[c >> 6];
+      cn = make_int4(b + (int)(u.x & 0xFFFFu), b + (int)(u.x >> 16), b + (int)(u.y & 0xFFFFu),
+                     b + (int)(u.y >> 16));
+    } else {
+      cn = reinterpret_cast<const int4 *>(pcell)[c];
+    }
     if constexpr (MIX) {
       float x1[2], x2[2], x3[2], x4[2], qf[4], a2;
       ld2f<SOA>(x, cn.x, sx, x1);
@@ -1315,8 +1325,8 @@ void flush_direct(void *) {
 // A pending fused direct chain ([update] [save_soln] adt_calc per cell).
 template <class R>
 using ChainKernel = void (*)(const double *, double *, R *, R *, double *, const double *,
-                             const int *, int, int, int, double *, double *, int *,
-                             const int *, int, int);
+                             const int *, const uint2 *, const int *, int, int, int, double *,
+                             double *, int *, const int *, int, int);
 struct PendingChain {
   ChainKernel<double> kern = nullptr;  // FP64 res and adt
   ChainKernel<float> kern_f = nullptr;  // mixed mode
@@ -1326,6 +1336,8 @@ struct PendingChain {
   double *q = nullptr, *adt = nullptr;
   const double *x = nullptr;
   const int *pcell = nullptr;
+  const uint2 *prow = nullptr;  // compact rows of pcell, or null
+  const int *pbase = nullptr;
   hipStream_t st = nullptr;
 };
 PendingChain g_chain;
@@ -1338,12 +1350,12 @@ void launch_chain(bool res_dead) {
   const PendingDirect &P = C.P;
   if (C.kern_f)
     launch_k(false, nullptr, nullptr, C.kern_f, C.g, C.bs, 0, C.st, P.qold, C.q, P.res_f,
-             C.adt_f, P.save_dst, C.x, C.pcell, C.n, C.sq, C.sx, P.part_sum, P.part_max,
-             P.part_loc, P.oldidx, C.chunk, (int)(res_dead && P.upd));
+             C.adt_f, P.save_dst, C.x, C.pcell, C.prow, C.pbase, C.n, C.sq, C.sx, P.part_sum,
+             P.part_max, P.part_loc, P.oldidx, C.chunk, (int)(res_dead && P.upd));
   else
     launch_k(false, nullptr, nullptr, C.kern, C.g, C.bs, 0, C.st, P.qold, C.q, P.res, C.adt,
-             P.save_dst, C.x, C.pcell, C.n, C.sq, C.sx, P.part_sum, P.part_max, P.part_loc,
-             P.oldidx, C.chunk, (int)(res_dead && P.upd));
+             P.save_dst, C.x, C.pcell, C.prow, C.pbase, C.n, C.sq, C.sx, P.part_sum,
+             P.part_max, P.part_loc, P.oldidx, C.chunk, (int)(res_dead && P.upd));
   KCHECK(hipGetLastError());
 }
 
@@ -1473,6 +1485,64 @@ const BoundaryArgs *boundary_args(const op2h_owner_plan *p, op_map pbecell, op_m
   return &B;
 }
 
+// Compact rows of a four-node map (cells -> nodes) for the fused direct
+// chain: per cell its nodes as 16-bit offsets from the lowest node of its
+// 64-cell group, 8 bytes instead of 16, where every group's nodes lie within
+// 65535 of that base (a locally numbered mesh; else null).  `build` false:
+// only a record built before (inside a capture nothing may be uploaded).
+struct CompactRows {
+  const uint2 *rec = nullptr;
+  const int *base = nullptr;
+};
+
+const CompactRows *compact_rows(op_map m, hipStream_t st, bool build) {
+  struct Slot {
+    unsigned epoch = 0;
+    bool ok = false;
+    CompactRows R;
+  };
+  static std::map<int, Slot> cache;
+  auto it = cache.find(m->index);
+  if (it != cache.end() && it->second.epoch == op2h_plan_epoch())
+    return it->second.ok ? &it->second.R : nullptr;
+  if (!build) return nullptr;
+  Slot &slot = cache[m->index];
+  slot = Slot();
+  slot.epoch = op2h_plan_epoch();
+  const int n = m->from->size, ng = (n + 63) / 64;
+  if (m->dim != 4 || n < 1) return nullptr;
+  const int *hm = op2h::host_map(m);  // in the device numbering
+  std::vector<uint2> rec(n);
+  std::vector<int> base(ng);
+  for (int g = 0; g < ng; ++g) {
+    const int c0 = 64 * g, c1 = std::min(n, c0 + 64);
+    int lo = INT_MAX, hi = INT_MIN;
+    for (size_t i = 4 * (size_t)c0; i < 4 * (size_t)c1; ++i) {
+      lo = std::min(lo, hm[i]);
+      hi = std::max(hi, hm[i]);
+    }
+    if ((long long)hi - lo > 0xFFFF) return nullptr;
+    base[g] = lo;
+    for (int c = c0; c < c1; ++c) {
+      const int *r = hm + 4 * (size_t)c;
+      rec[c] = make_uint2((unsigned)(r[0] - lo) | ((unsigned)(r[1] - lo) << 16),
+                          (unsigned)(r[2] - lo) | ((unsigned)(r[3] - lo) << 16));
+    }
+  }
+  char name[64];
+  snprintf(name, sizeof name, "compact_rows.%d", m->index);
+  const size_t b_rec = (8 * (size_t)n + 255) / 256 * 256;
+  int fresh = 0;
+  char *buf = (char *)op2h_plan_user(name, b_rec + 4 * (size_t)ng, &fresh);
+  KCHECK(hipMemcpyAsync(buf, rec.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st));
+  KCHECK(hipMemcpyAsync(buf + b_rec, base.data(), 4 * (size_t)ng, hipMemcpyHostToDevice, st));
+  KCHECK(hipStreamSynchronize(st));  // rec and base are locals
+  slot.R.rec = (const uint2 *)buf;
+  slot.R.base = (const int *)(buf + b_rec);
+  slot.ok = true;
+  return &slot.R;
+}
+
 }  // namespace
 
 extern "C" void op_par_loop_airfoil_1_save_soln(char const *name, op_set set,
@@ -1586,6 +1656,10 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
         C.adt = adt;
         C.x = x;
         C.pcell = pcell;
+        if (const CompactRows *cr = compact_rows(args[0].map, st, false)) {
+          C.prow = cr->rec;
+          C.pbase = cr->base;
+        }
         C.n = set->size;
         C.sq = sq;
         C.sx = sx;
@@ -1599,6 +1673,9 @@ extern "C" void op_par_loop_airfoil_2_adt_calc(char const *name, op_set set,
       }
       flush_direct(nullptr);
     }
+    // the chains fused inside a captured step read the compact rows, built
+    // here (outside capture) at the first eager call
+    if (!ctx.capturing) (void)compact_rows(args[0].map, st, true);
     LoopTimer tm(&ctx);
     if (is_float(args[5])) {
       float *adtf = (float *)args[5].data_d;
