@@ -191,7 +191,8 @@ __device__ __forceinline__ MaxLoc wave_maxloc(MaxLoc m) {
 //  SAVE  qold = q (save_soln.h), the next time step's;
 //  then  adt = adt_calc(x, q) (adt_calc.h) for the next half step.
 // DEAD: the update's zeros in res are not stored (res_calc, which comes next,
-// starts every cell's sum from +0.0).  A workgroup of any multiple of 64
+// starts every cell's sum from +0.0).  prow/pbase (or null): the cells' nodes
+// as 16-bit offsets from the lowest node of their tile's 64-cell group.  A workgroup of any multiple of 64
 // threads covers a tile in kTile / blockDim.x rounds; the groups are fixed by
 // the tile, so the bits do not depend on the workgroup size.  qold and adt
 // are read before they are written, by the same thread for the same cell: not
@@ -200,6 +201,7 @@ template <bool UPD, bool SAVE, bool RED, bool DEAD>
 __global__ void __launch_bounds__(kTile)
     k_ens_chain(const double *__restrict x, double *qold, double *__restrict q,
                 double *__restrict res, double *adt, const int4 *__restrict pcell,
+                const uint2 *__restrict prow, const int *__restrict pbase,
                 const int2 *__restrict tiles, const int *__restrict coff, int tlo, int thi,
                 double *__restrict part_sum, double *__restrict part_max,
                 int *__restrict part_loc, Stamps S) {
@@ -228,7 +230,15 @@ __global__ void __launch_bounds__(kTile)
           ld4<false>(q, c, 0, qn);
         }
         if (SAVE) st4<false>(qold, c, 0, qn);
-        const int4 cn = pcell[c];
+        int4 cn;
+        if (prow) {  // compact rows: 16-bit offsets from the 64-cell group's base
+          const uint2 u = prow[c];
+          const int b = pbase[(size_t)t * kGroups + ((r + (int)threadIdx.x) >> 6)];
+          cn = make_int4(b + (int)(u.x & 0xFFFFu), b + (int)(u.x >> 16),
+                         b + (int)(u.y & 0xFFFFu), b + (int)(u.y >> 16));
+        } else {
+          cn = pcell[c];
+        }
         double x1[2], x2[2], x3[2], x4[2], a2;
         ld2<false>(x, cn.x, 0, x1);
         ld2<false>(x, cn.y, 0, x2);
@@ -514,6 +524,31 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
     mesh_bitem_off.push_back((int)bitems.size());
   }
   const int ntiles = (int)tiles.size(), npatch = mblk[M];
+  // compact cell rows for the chains: per 64-cell group of a tile its lowest
+  // node, per cell four 16-bit offsets from it (8 bytes instead of 16), when
+  // every group's nodes span at most 65535
+  std::vector<uint2> crow(Nc);
+  std::vector<int> cbase((size_t)ntiles * kGroups, 0);
+  bool compact = true;
+  for (int t = 0; compact && t < ntiles; ++t) {
+    const int c0 = tiles[t].x, c1 = std::min(c0 + kTile, coff[tiles[t].y + 1]);
+    for (int g = 0; compact && c0 + 64 * g < c1; ++g) {
+      const int g0 = c0 + 64 * g, g1 = std::min(c1, g0 + 64);
+      int lo = INT_MAX, hi = INT_MIN;
+      for (int c = g0; c < g1; ++c) {
+        const int4 v = cell[c];
+        lo = std::min(std::min(lo, std::min(v.x, v.y)), std::min(v.z, v.w));
+        hi = std::max(std::max(hi, std::max(v.x, v.y)), std::max(v.z, v.w));
+      }
+      if ((long long)hi - lo > 0xFFFF) compact = false;
+      cbase[(size_t)t * kGroups + g] = lo;
+      for (int c = g0; c < g1; ++c) {
+        const int4 v = cell[c];
+        crow[c] = make_uint2((unsigned)(v.x - lo) | ((unsigned)(v.y - lo) << 16),
+                             (unsigned)(v.z - lo) | ((unsigned)(v.w - lo) << 16));
+      }
+    }
+  }
   // every cell owned by a patch: res_calc writes all of res, so the update
   // need not zero it (dead res) and res_calc need not read it
   const bool dead = (int)owned.size() == Nc;
@@ -563,6 +598,8 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   double *d_res = dev_alloc<double>(4 * (size_t)Nc, keep);
   double *d_adt = dev_alloc<double>(Nc, keep);
   int4 *d_cell = dev_upload(cell, keep);
+  const uint2 *d_crow = compact ? dev_upload(crow, keep) : nullptr;
+  const int *d_cbase = compact ? dev_upload(cbase, keep) : nullptr;
   OwnerArgs OA{};
   OA.inst = dev_upload(inst, keep);
   OA.base = dev_upload(pbase, keep);
@@ -626,8 +663,8 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
   int nsm = 0;
   ECHECK(hipDeviceGetAttribute(&nsm, hipDeviceAttributeMultiprocessorCount, 0));
   using ChainFn = void (*)(const double *, double *, double *, double *, double *,
-                           const int4 *, const int2 *, const int *, int, int, double *,
-                           double *, int *, Stamps);
+                           const int4 *, const uint2 *, const int *, const int2 *, const int *,
+                           int, int, double *, double *, int *, Stamps);
   const ChainFn k_prologue = k_ens_chain<false, true, false, false>;
   const ChainFn k_chain0 = dead ? k_ens_chain<true, false, false, true>
                                 : k_ens_chain<true, false, false, false>;
@@ -669,8 +706,8 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
     const int t0 = mesh_tile_off[mlo], t1 = mesh_tile_off[mhi];
     if (t1 > t0)
       k_prologue<<<std::min(t1 - t0, cap[0]), blk[0], 0, st>>>(
-          d_x, d_qold, d_q, d_res, d_adt, d_cell, d_tiles, d_coff, t0, t1, d_psum, d_pmax,
-          d_ploc, Stamps{nullptr, d_count, per_step, capacity, 0});
+          d_x, d_qold, d_q, d_res, d_adt, d_cell, d_crow, d_cbase, d_tiles, d_coff, t0, t1,
+          d_psum, d_pmax, d_ploc, Stamps{nullptr, d_count, per_step, capacity, 0});
   };
   // one time step of members [mlo, mhi): twice res_calc and bres_calc (one
   // launch when fused) and the chain, then the per-member reductions; with
@@ -705,8 +742,8 @@ extern "C" int op2amd_ensemble_run(const op2amd_ensemble_spec *spec,
       mark();
       if (t1 > t0)
         (k ? k_chain1 : k_chain0)<<<std::min(t1 - t0, cap[3 * k]), blk[3 * k], 0, st>>>(
-            d_x, d_qold, d_q, d_res, d_adt, d_cell, d_tiles, d_coff, t0, t1, d_psum, d_pmax,
-            d_ploc, S_at(off_chain[k]));
+            d_x, d_qold, d_q, d_res, d_adt, d_cell, d_crow, d_cbase, d_tiles, d_coff, t0, t1,
+            d_psum, d_pmax, d_ploc, S_at(off_chain[k]));
       mark();
     }
     k_ens_finish<<<mhi - mlo, 256, 0, st>>>(d_psum, d_pmax, d_ploc, d_mtoff, mlo, d_slots,
